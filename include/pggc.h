@@ -1,0 +1,163 @@
+/*
+ * pggc.h -- C ABI of the MI355X-native PGGC Louvain/Leiden hot path.
+ *
+ * Replaces the reference's C++ template boundary
+ *   jet_community::clustering_methods::leiden_part / louvain_part
+ *   (interface/clustering_methods.h:20-24)
+ * and its internal seams (src/header/local_mover.h:21-25,
+ * src/header/contract.h:20-24, src/header/leidenR.h:26-31,
+ * src/header/ordering.h:21), with plain pointers and sizes.
+ *
+ * Conventions
+ *  - Every array argument is DEVICE memory (HBM of the context's GPU) unless
+ *    its comment says "host".
+ *  - Every function returns PGGC_OK (0) or a negative status; the message of
+ *    the last failure is available from pggc_last_error().
+ *  - A context owns one HIP stream and a caching device allocator.  It is not
+ *    thread-safe; different contexts are independent.
+ *  - Graphs are symmetric CSR without self loops: row_ptr int64[n+1],
+ *    col int32[nnz], at most one of w (int32[nnz]) / w64 (int64[nnz]) set
+ *    (neither = unit weights), vtx_w int64[n].
+ *  - Results are deterministic for a given seed.
+ */
+#ifndef PGGC_H
+#define PGGC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGGC_OK 0
+#define PGGC_EINVAL -1   /* bad argument */
+#define PGGC_ENOMEM -2   /* device allocation failed */
+#define PGGC_EHIP -3     /* HIP runtime error */
+#define PGGC_ECAPACITY -4/* output or label capacity exceeded */
+#define PGGC_EINTERNAL -5
+
+/* objectives (app/io/parse_args.cpp:10-16) */
+#define PGGC_OBJ_MOD 0   /* modularity, unit edge weights, degree vertex weights */
+#define PGGC_OBJ_WMOD 1  /* weighted modularity, weighted-degree vertex weights */
+#define PGGC_OBJ_CPM 2   /* constant Potts model, unit vertex weights */
+#define PGGC_OBJ_NLCC 3  /* normalized LambdaCC */
+#define PGGC_OBJ_LCC 4   /* base LambdaCC */
+
+/* algorithms (app/CMakeLists.txt:3-7: plouvain, pleiden, pleiden+) */
+#define PGGC_LOUVAIN 0
+#define PGGC_LEIDEN 1
+#define PGGC_LEIDEN_PLUS 2
+
+typedef struct pggc_ctx pggc_ctx;
+
+/* weighted_graph (interface/weighted_graph.h:10-19) */
+typedef struct pggc_graph {
+    int64_t n;
+    int64_t nnz;
+    const int64_t* row_ptr;
+    const int32_t* col;
+    const int32_t* w;    /* int32 weights or NULL */
+    const int64_t* w64;  /* int64 weights or NULL */
+    const int64_t* vtx_w;
+} pggc_graph;
+
+/* cluster_data (interface/cluster_data.hpp:18-28) plus the objective's
+ * normalisation: objective() = obj * scale + offset.  vol is a device array
+ * of vol_cap >= n entries owned by the caller. */
+typedef struct pggc_state {
+    int64_t* vol;
+    int64_t vol_cap;
+    int64_t uncut;
+    int64_t top_nnz;
+    double obj;
+    double lambda;
+    double scale;
+    double offset;
+    int32_t label_count;
+    int32_t reserved;
+} pggc_state;
+
+int pggc_version(void);
+const char* pggc_last_error(void);
+
+/* stream: a hipStream_t (NULL = default stream) */
+int pggc_ctx_create(pggc_ctx** ctx, void* stream);
+int pggc_ctx_destroy(pggc_ctx* ctx);
+/* seed stream used by the Leiden refinement (stands in for the reference's
+ * std::random_device, src/leidenR.cpp:289,291,399,479) */
+int pggc_ctx_set_seed(pggc_ctx* ctx, uint64_t seed, uint64_t draws);
+int pggc_ctx_get_draws(pggc_ctx* ctx, uint64_t* draws);
+/* bytes held by the context allocator (host out) */
+int pggc_ctx_memory(pggc_ctx* ctx, int64_t* live, int64_t* cached);
+
+/* get_vtx_weights (app/io/vertex_weighting.cpp:27-44): degree (MOD),
+ * weighted degree (WMOD), unit otherwise. */
+int pggc_vertex_weights(pggc_ctx* ctx, const pggc_graph* g, int objective, int64_t* vtx_w_out);
+
+/* get_objective (app/objective_helpers.hpp:9-22): sets lambda and the
+ * normalisation, vol = vtx_w, singleton clustering, obj updated. */
+int pggc_state_init(pggc_ctx* ctx, const pggc_graph* g, int objective, double lambda_mult, pggc_state* st);
+
+/* generate_orderings (src/ordering.cpp:131-164).  offs (host) receives
+ * {offset_large, offset_large2, offset_mid, offset_mid2}. */
+int pggc_orderings(pggc_ctx* ctx, const pggc_graph* g, int32_t* order1, int32_t* order2, int64_t* offs);
+
+/* modularity evaluator (app/verify_objective.cpp:64-89 with
+ * interface/cluster_data.hpp:76-87): labels in [0,L); vol_out[L] (device);
+ * uncut_out and sq_out[2] = {high, low} words of sum(vol^2) on the host. */
+int pggc_evaluate(pggc_ctx* ctx, const pggc_graph* g, const int32_t* labels, int64_t L, int64_t* vol_out,
+                  int64_t* uncut_out, uint64_t* sq_out);
+
+/* own-cluster weight of every vertex (the pvals_clone contract,
+ * src/local_mover.cpp:1354-1358) */
+int pggc_own_weight(pggc_ctx* ctx, const pggc_graph* g, const int32_t* labels, int64_t* out);
+
+/* build_coarse_graph (src/contract.cpp:234-336): coarse CSR of the labelling
+ * vcmap[n] in [0,nc), self loops dropped, rows sorted by column.  out_rp has
+ * nc+1 entries; out_col/out_w hold cap entries.  *nnz_out receives the coarse
+ * nnz; PGGC_ECAPACITY if it exceeds cap (nothing written to out_col/out_w). */
+int pggc_contract(pggc_ctx* ctx, const pggc_graph* g, const int32_t* vcmap, int64_t nc, int64_t* out_rp,
+                  int32_t* out_col, int64_t* out_w, int64_t cap, int64_t* nnz_out);
+
+/* One local-move sweep from a fixed state (candidates_and_destinations +
+ * afterburner_filter, src/local_mover.cpp:567-634,374-433): dest and gain are
+ * the in/out destination cache and gains; raw != 0 scores the raw adjacency
+ * (first sweep of an initial call); constraint may be NULL.  moves_out (n)
+ * receives the accepted moves in order1 order; counts on the host. */
+int pggc_sweep(pggc_ctx* ctx, const pggc_graph* g, const int32_t* part, const int64_t* vol, int64_t L,
+               double lambda, float ratio, int raw, const int32_t* constraint, int32_t* dest, float* gain,
+               int32_t* moves_out, int64_t* nmoves, int64_t* ncand);
+
+/* local_move / local_move_strict (src/local_mover.cpp:1360-1483) on one level,
+ * unconstrained: part_inout is relabelled contiguously; st in/out. */
+int pggc_local_move(pggc_ctx* ctx, const pggc_graph* g, int32_t* part_inout, pggc_state* st, int is_initial,
+                    int strict, int simulated_annealing);
+
+/* coarsen_leidenR (src/leidenR.cpp:263-500) of the Louvain labels louv with
+ * volumes vol[L]; uniform != 0 uses the unit-weight neighbour rule.  The two
+ * seeds are draws draw0 and draw0+1 of the stream `seed`. */
+int pggc_leiden_refine(pggc_ctx* ctx, const pggc_graph* g, const int32_t* louv, const int64_t* vol, int64_t L,
+                       double lambda, int uniform, uint64_t seed, uint64_t draw0, int32_t* vcmap_out,
+                       int64_t* nc_out);
+
+/* louvain_part<constrained> (interface/clustering_methods.h:23-24): constraint
+ * (n entries, mutated like the reference) is used when constrained != 0. */
+int pggc_louvain(pggc_ctx* ctx, const pggc_graph* g, pggc_state* st, int constrained, int32_t* constraint,
+                 int32_t* part_out);
+
+/* leiden_part<plus, improve> (interface/clustering_methods.h:20-21) */
+int pggc_leiden(pggc_ctx* ctx, const pggc_graph* g, pggc_state* st, int plus, int improve, const int32_t* input,
+                int32_t* part_out);
+
+/* run_clustering (app/cluster_main.cpp:116-165) for one trial: the first
+ * clustering plus the ex_iters loop (-1 = until the objective stops rising).
+ * objective_out (host) = get_objective(); info (host, 8 entries) =
+ * {label_count, uncut, levels, sweeps, moves, outer iterations, candidates, 0}. */
+int pggc_cluster(pggc_ctx* ctx, const pggc_graph* g, int algo, int objective, double lambda_mult, int ex_iters,
+                 uint64_t seed, int32_t* part_out, double* objective_out, int64_t* info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PGGC_H */

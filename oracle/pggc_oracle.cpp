@@ -1,0 +1,1101 @@
+// PGGC CPU oracle: a serial restatement of the reference's Louvain/Leiden hot
+// path, used ONLY as test infrastructure (parity checker and CPU baseline).
+// Nothing in the product library links or calls this file.
+//
+// Every function cites the reference file:line whose behaviour it restates.
+// Where the reference is nondeterministic (CAS-built hash tables, team
+// reductions in FP32, random_device seeds, unordered coarse columns) this file
+// fixes ONE deterministic choice; the same choice is implemented by the HIP
+// library, and the choice is listed at each site ("canonical rule").
+//
+// Integer widening: offsets, volumes, uncut and coarse edge weights are int64
+// here (the reference uses int32 everywhere, interface/core_types.h:9-14);
+// values agree whenever the reference does not overflow.
+//
+// Floating point: compiled with -ffp-contract=off; the four expressions that
+// the CUDA compiler contracts into an FMA in the reference (p_conn, j_conn,
+// the afterburner term q and the weighted leidenR score) use fmaf explicitly.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <unordered_map>
+#include <vector>
+
+namespace orc {
+
+using i32 = int32_t;
+using i64 = int64_t;
+using u32 = uint32_t;
+using u64 = uint64_t;
+using u128 = unsigned __int128;
+
+// src/local_mover.cpp:74-80
+constexpr float OBJ_MIN = -FLT_MAX;
+constexpr i32 NULL_PART = -1;
+constexpr i32 NO_MOVE = -3;
+constexpr i32 NEW_PART = -4;
+constexpr float AFTERBURNER_EPS = 0.1f;
+// src/header/ordering.h:17-19
+constexpr i64 MID_CUTOFF = 64;
+constexpr i64 LARGE_CUTOFF = 128;
+constexpr i64 MASSIVE_CUTOFF = 15000;
+// src/leidenR.cpp:38-39
+constexpr i32 ORD_MAX = std::numeric_limits<i32>::max();
+constexpr i32 SPLIT = 1000000000;
+
+// xorshift32, src/local_mover.cpp:82-87 (also src/contract.cpp:91-97)
+static inline u32 xorshift32(u32 x) {
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return x;
+}
+
+// Kokkos::pod_hash<int32> (src/leidenR.cpp:37) = MurmurHash3_x86_32 of the
+// 4 key bytes with seed 0 (public-domain algorithm by A. Appleby).
+static inline u32 rotl32(u32 x, int r) { return (x << r) | (x >> (32 - r)); }
+static inline u32 murmur3_i32(i32 key) {
+    u32 k1 = (u32)key;
+    u32 h1 = 0;
+    k1 *= 0xcc9e2d51u;
+    k1 = rotl32(k1, 15);
+    k1 *= 0x1b873593u;
+    h1 ^= k1;
+    h1 = rotl32(h1, 13);
+    h1 = h1 * 5u + 0xe6546b64u;
+    h1 ^= 4u;
+    h1 ^= h1 >> 16;
+    h1 *= 0x85ebca6bu;
+    h1 ^= h1 >> 13;
+    h1 *= 0xc2b2ae35u;
+    h1 ^= h1 >> 16;
+    return h1;
+}
+
+// Seeded stand-in for std::random_device (src/leidenR.cpp:289,291,399,479):
+// the k-th draw of a run with seed s is the low 32 bits of splitmix64(s + k).
+static inline u32 seed_draw(u64 seed, u64 k) {
+    u64 z = seed + (k + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (u32)z;
+}
+
+struct Graph {
+    i64 n = 0;
+    std::vector<i64> rp;
+    std::vector<i32> col;
+    std::vector<i64> w;   // empty when uniform
+    std::vector<i64> vw;  // vertex weights
+    bool uniform = true;
+    i64 nnz() const { return rp[n]; }
+    i64 deg(i64 i) const { return rp[i + 1] - rp[i]; }
+    i64 wt(i64 j) const { return uniform ? 1 : w[j]; }
+};
+
+// cluster_data (interface/cluster_data.hpp:18-88) with the objective's
+// normalisation (modularity :109-133, constant_potts :135-156,
+// normalized_lcc :158-182, base cluster_data).
+struct State {
+    std::vector<i64> vol;  // total_deg
+    i64 uncut = 0;
+    i64 top_nnz = 0;
+    double obj = -1.0;
+    i32 label_count = 0;
+    double lambda = 1.0;
+    double scale = 1.0;   // get_objective() = obj*scale + offset
+    double offset = 0.0;
+
+    void update_objective() {  // interface/cluster_data.hpp:76-87
+        u128 sq = 0;
+        for (i32 l = 0; l < label_count; l++) {
+            i64 v = vol[l];
+            u64 a = (u64)(v < 0 ? -v : v);
+            sq += (u128)a * (u128)a;
+        }
+        double m = (double)uncut;
+        m -= lambda * (double)sq;
+        obj = m;
+    }
+    double objective() const { return obj * scale + offset; }
+    void copy_from(const State& r) {  // interface/cluster_data.hpp:56-65
+        if (vol.size() < (size_t)r.label_count) vol.resize(r.vol.size());
+        std::copy(r.vol.begin(), r.vol.begin() + r.label_count, vol.begin());
+        top_nnz = r.top_nnz;
+        uncut = r.uncut;
+        obj = r.obj;
+        label_count = r.label_count;
+        lambda = r.lambda;
+    }
+};
+
+// ---------------------------------------------------------------- A1 ordering
+// src/ordering.cpp:131-164 (stable degree bucketing into order1 / order2)
+struct Orders {
+    std::vector<i32> o1, o2;
+    i64 off_large = 0, off_large2 = 0, off_mid = 0, off_mid2 = 0;
+};
+static Orders make_orders(const Graph& g) {
+    Orders o;
+    i64 n = g.n;
+    o.o1.resize(n);
+    o.o2.resize(n);
+    i64 b[5] = {0, 0, 0, 0, 0};
+    for (i64 i = 0; i < n; i++) {
+        i64 d = g.deg(i);
+        if (d < LARGE_CUTOFF) b[0]++;
+        else if (d < MASSIVE_CUTOFF) b[1]++;
+        else b[4]++;
+        if (d < MID_CUTOFF) b[2]++;
+        else if (d < MASSIVE_CUTOFF) b[3]++;
+    }
+    i64 off[5] = {0, b[0], 0, b[2], b[0] + b[1]};
+    o.off_large = off[1];
+    o.off_mid = off[3];
+    o.off_large2 = off[4];
+    o.off_mid2 = off[4];
+    i64 c[5] = {0, 0, 0, 0, 0};
+    for (i64 i = 0; i < n; i++) {
+        i64 d = g.deg(i);
+        if (d < LARGE_CUTOFF) o.o1[off[0] + c[0]++] = (i32)i;
+        else if (d < MASSIVE_CUTOFF) o.o1[off[1] + c[1]++] = (i32)i;
+        else {
+            o.o1[off[4] + c[4]] = (i32)i;
+            o.o2[off[4] + c[4]] = (i32)i;
+            c[4]++;
+        }
+        if (d < MID_CUTOFF) o.o2[off[2] + c[2]++] = (i32)i;
+        else if (d < MASSIVE_CUTOFF) o.o2[off[3] + c[3]++] = (i32)i;
+    }
+    return o;
+}
+
+// --------------------------------------------------------------- A12 evaluator
+// app/verify_objective.cpp:64-89 + interface/cluster_data.hpp:76-87
+static void evaluate(const Graph& g, const i32* labels, i64 L, std::vector<i64>& vol, i64& uncut) {
+    vol.assign(L, 0);
+    uncut = 0;
+    for (i64 i = 0; i < g.n; i++) {
+        i32 l = labels[i];
+        for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++)
+            if (labels[g.col[j]] == l) uncut += g.wt(j);
+        vol[l] += g.vw[i];
+    }
+}
+
+// ------------------------------------------------------------- A10 contraction
+// src/contract.cpp:234-336: coarse vertex u collects sum of w(i,j) over fine
+// edges with vcmap(i)=u, vcmap(j)=v != u (self loops dropped :191,209).
+// Canonical rule: coarse columns sorted ascending (the reference emits them
+// in hash-slot order, :302-328).
+static Graph contract(const Graph& g, const std::vector<i32>& vcmap, i64 nc) {
+    std::vector<i64> cnt(nc + 1, 0);
+    for (i64 i = 0; i < g.n; i++) cnt[vcmap[i] + 1]++;
+    for (i64 u = 0; u < nc; u++) cnt[u + 1] += cnt[u];
+    std::vector<i32> members(g.n);
+    {
+        std::vector<i64> pos(cnt.begin(), cnt.end() - 1);
+        for (i64 i = 0; i < g.n; i++) members[pos[vcmap[i]]++] = (i32)i;
+    }
+    Graph c;
+    c.n = nc;
+    c.uniform = false;
+    c.rp.assign(nc + 1, 0);
+    std::vector<i64> acc(nc, 0);
+    std::vector<i32> touched;
+    std::vector<char> seen(nc, 0);
+    for (i64 u = 0; u < nc; u++) {
+        touched.clear();
+        for (i64 k = cnt[u]; k < cnt[u + 1]; k++) {
+            i32 i = members[k];
+            for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++) {
+                i32 v = vcmap[g.col[j]];
+                if (v == u) continue;
+                if (!seen[v]) { seen[v] = 1; touched.push_back(v); }
+                acc[v] += g.wt(j);
+            }
+        }
+        std::sort(touched.begin(), touched.end());
+        for (i32 v : touched) {
+            c.col.push_back(v);
+            c.w.push_back(acc[v]);
+            acc[v] = 0;
+            seen[v] = 0;
+        }
+        c.rp[u + 1] = (i64)c.col.size();
+    }
+    return c;
+}
+
+// ---------------------------------------------------------- local-move state
+struct Mover {
+    const Graph& g;
+    Orders ord;
+    std::vector<i32> part, dest;
+    std::vector<float> gain;
+    bool tables = false;  // cdata.init (src/local_mover.cpp:97)
+    Mover(const Graph& gg) : g(gg) { ord = make_orders(g); }
+};
+
+// select_destinations (src/local_mover.cpp:435-563) for one vertex.
+// Canonical rule: among eligible clusters keep the max j_conn, ties -> the
+// smallest label; NEW_PART (-4) competes with j_conn 0 when p_conn < 0, so a
+// real cluster beats it only with j_conn > 0 (the team path's min-reduce,
+// :553-561, applied with one table slot per thread).
+static void score_vertex(Mover& M, State& st, i64 i, float lam, float ratio, bool constrained,
+                         const i32* constraint, std::unordered_map<i32, i64>& agg) {
+    const Graph& g = M.g;
+    if (M.dest[i] == NO_MOVE) return;  // :477-479
+    const bool raw = !M.tables;        // first sweep scores the raw adjacency :1391-1395
+    i64 w = g.vw[i];
+    float multi = (float)w * lam;
+    i32 p = M.part[i];
+    agg.clear();
+    i64 pv = 0;
+    for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++) {
+        i32 v = g.col[j];
+        i32 key = raw ? v : M.part[v];
+        if (!raw && key == p) pv += g.wt(j);
+        else agg[key] += g.wt(j);
+    }
+    if (raw) pv = 0;  // pvals are zero before the first table build (:1349)
+    float p_conn = fmaf(-(float)(st.vol[p] - w), multi, (float)pv);
+    float b = (float)((double)p_conn * (1.0 - (double)ratio));
+    if (p_conn < 0) b = 0;
+    bool have = false;
+    float bj = 0;
+    i32 bc = 0;
+    for (auto& kv : agg) {
+        i32 c = kv.first;
+        i64 s = kv.second;
+        if (!(s > 0 && (float)s >= b)) continue;
+        if (constrained && constraint[c] != constraint[p]) continue;
+        float jc = fmaf(-(float)st.vol[c], multi, (float)s);
+        if (jc >= b) {
+            if (!have || jc > bj || (jc == bj && c < bc)) { have = true; bj = jc; bc = c; }
+        }
+    }
+    if (p_conn < 0 && (!have || !(bj > 0.0f))) {  // NEW_PART wins ties at 0
+        M.gain[i] = 0.0f - p_conn;
+        M.dest[i] = NEW_PART;
+        return;
+    }
+    if (have) {
+        M.gain[i] = bj - p_conn;
+        M.dest[i] = bc;
+    } else {
+        M.gain[i] = OBJ_MIN;
+        M.dest[i] = NO_MOVE;
+    }
+}
+
+// candidates_and_destinations (src/local_mover.cpp:567-634): score every
+// vertex, then list the ones with dest != NO_MOVE in order1 order.
+static std::vector<i32> candidates(Mover& M, State& st, float ratio, bool constrained, const i32* constraint) {
+    float lam = (float)st.lambda;
+    std::unordered_map<i32, i64> agg;
+    for (i64 i = 0; i < M.g.n; i++) score_vertex(M, st, i, lam, ratio, constrained, constraint, agg);
+    std::vector<i32> out;
+    for (i64 x = 0; x < M.g.n; x++) {
+        i32 i = M.ord.o1[x];
+        if (M.dest[i] != NO_MOVE) out.push_back(i);
+    }
+    return out;
+}
+
+// afterburner_kernel (src/local_mover.cpp:280-370). Canonical FP32 order of
+// the change sum: degree < 128 -> one sequential fold in adjacency order
+// (the reference's serial path); 128 <= degree < 15000 -> 64 strided partial
+// folds joined by a halving tree; degree >= 15000 -> 1024 strided partial
+// folds joined by a halving tree (the reference's team reductions, :347-363,
+// whose join order CUDA leaves unspecified).
+static inline void ab_term(const Mover& M, i64 j, i32 p, i32 best, float multi,
+                           float igain, i32 hi, float& acc) {
+    const Graph& g = M.g;
+    i32 v = g.col[j];
+    float vg = M.gain[v];
+    if ((vg - igain) >= AFTERBURNER_EPS ||
+        (std::fabs(vg - igain) < AFTERBURNER_EPS && (i32)xorshift32((u32)v) < hi)) {
+        i32 vpart = M.dest[v];
+        float q = fmaf(-multi, (float)g.vw[v], (float)g.wt(j));
+        acc = acc - ((vpart == p) ? q : 0.0f);
+        acc = acc + ((vpart == best && best != NEW_PART) ? q : 0.0f);
+        vpart = M.part[v];
+        acc = acc + ((vpart == p) ? q : 0.0f);
+        acc = acc - ((vpart == best) ? q : 0.0f);
+    }
+}
+
+static float tree_fold(const Mover& M, i64 i, i32 p, i32 best, float multi, float igain,
+                       i32 hi, int T) {
+    std::vector<float> part(T, 0.0f);
+    i64 s = M.g.rp[i], e = M.g.rp[i + 1];
+    for (int t = 0; t < T; t++)
+        for (i64 j = s + t; j < e; j += T) ab_term(M, j, p, best, multi, igain, hi, part[t]);
+    for (int off = T / 2; off >= 1; off /= 2)
+        for (int t = 0; t < off; t++) part[t] = part[t] + part[t + off];
+    return part[0];
+}
+
+static float ab_change(const Mover& M, i64 i, float lam) {
+    const Graph& g = M.g;
+    i32 best = M.dest[i];
+    i32 p = M.part[i];
+    float multi = (float)g.vw[i] * lam;
+    float igain = M.gain[i];
+    i32 hi = (i32)xorshift32((u32)i);
+    i64 d = g.deg(i);
+    if (d < LARGE_CUTOFF) {
+        float acc = 0.0f;
+        for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++) ab_term(M, j, p, best, multi, igain, hi, acc);
+        return acc;
+    }
+    return tree_fold(M, i, p, best, multi, igain, hi, d < MASSIVE_CUTOFF ? 64 : 1024);
+}
+
+static std::vector<i32> afterburner(Mover& M, State& st, const std::vector<i32>& cands) {
+    float lam = (float)st.lambda;
+    std::vector<i32> out;
+    for (i32 i : cands) {
+        float ch = ab_change(M, i, lam);
+        if (M.gain[i] + ch >= 0) out.push_back(i);  // :332-334
+    }
+    return out;
+}
+
+// set_new_cluster_ids (src/local_mover.cpp:215-274)
+static void set_new_cluster_ids(Mover& M, State& st, const std::vector<i32>& moves, bool constrained, i32* constraint) {
+    std::vector<i32> nv;
+    for (i32 i : moves)
+        if (M.dest[i] == NEW_PART) nv.push_back(i);
+    if (nv.empty()) return;
+    i32 curr = st.label_count;
+    std::vector<char> has(curr, 0);
+    for (i64 i = 0; i < M.g.n; i++) has[M.part[i]] = 1;
+    i64 k = 0;
+    for (i32 l = 0; l < curr && k < (i64)nv.size(); l++) {
+        if (!has[l]) {
+            i32 x = nv[k++];
+            M.dest[x] = l;
+            if (constrained) constraint[l] = constraint[M.part[x]];
+        }
+    }
+    i64 avail = k;
+    if (avail < (i64)nv.size()) {
+        i64 needed = (i64)nv.size() - avail;
+        st.label_count += (i32)needed;
+        if ((i64)st.vol.size() < st.label_count) st.vol.resize(st.label_count, 0);
+        for (i64 x = avail; x < (i64)nv.size(); x++) {
+            i32 i = nv[x];
+            i32 d = (i32)(x - avail) + curr;
+            M.dest[i] = d;
+            if (constrained) constraint[d] = constraint[M.part[i]];
+        }
+        for (i32 l = curr; l < st.label_count; l++) st.vol[l] = 0;
+    }
+}
+
+// perform_moves (src/local_mover.cpp:1241-1279). The uncut update equals
+// sum(pvals_after) - sum(pvals_before) (:1274-1277), computed here directly
+// from the moved vertices' adjacency; the dest cache is reset to NULL_PART for
+// every vertex on a full table build (:1264-1266) and otherwise for the movers
+// and their neighbours (find_affected* :717,798; update_small :1072,1121).
+static void perform_moves(Mover& M, State& st, const std::vector<i32>& moves) {
+    const Graph& g = M.g;
+    std::vector<char> mv(g.n, 0);
+    for (i32 i : moves) mv[i] = 1;
+    i64 delta = 0;
+    for (i32 i : moves) {
+        i32 oi = M.part[i], ni = M.dest[i];
+        for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++) {
+            i32 v = g.col[j];
+            i32 ov = M.part[v], nvv = mv[v] ? M.dest[v] : M.part[v];
+            i64 d = (i64)(ni == nvv) - (i64)(oi == ov);
+            delta += d * g.wt(j) * (mv[v] ? 1 : 2);
+        }
+    }
+    for (i32 i : moves) {
+        st.vol[M.part[i]] -= g.vw[i];
+        st.vol[M.dest[i]] += g.vw[i];
+    }
+    for (i32 i : moves) M.part[i] = M.dest[i];
+    if (!M.tables) {
+        M.tables = true;
+        std::fill(M.dest.begin(), M.dest.end(), NULL_PART);
+    } else {
+        for (i32 i : moves) {
+            M.dest[i] = NULL_PART;
+            for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++) M.dest[g.col[j]] = NULL_PART;
+        }
+    }
+    st.uncut += delta;
+    st.update_objective();
+}
+
+// relabel_contiguously (src/local_mover.cpp:100-129)
+static void relabel(std::vector<i32>& labels, State& st) {
+    i32 L = st.label_count;
+    std::vector<i32> used(L, 0);
+    for (i32 l : labels) used[l] = 1;
+    i32 t = 0;
+    std::vector<i64> nvol;
+    for (i32 l = 0; l < L; l++) {
+        if (used[l]) {
+            used[l] = t++;
+            nvol.push_back(st.vol[l]);
+        }
+    }
+    for (i32 l = 0; l < t; l++) st.vol[l] = nvol[l];
+    for (auto& x : labels) x = used[x];
+    st.label_count = t;
+}
+
+struct Stats {
+    i64 sweeps = 0, moves = 0, cands = 0;
+};
+
+// local_move (src/local_mover.cpp:1360-1417)
+static void local_move(const Graph& g, std::vector<i32>& best_part, State& best, bool is_initial, bool constrained,
+                       i32* constraint, bool SA, Stats* stats) {
+    State curr = best;
+    curr.copy_from(best);
+    Mover M(g);
+    M.part = best_part;
+    M.dest.assign(g.n, NULL_PART);
+    M.gain.assign(g.n, 0.0f);
+    M.tables = !is_initial;
+    const float ratios[2] = {0.75f, 0.25f};
+    const int limits[2] = {4, 2};
+    int iter = 0;
+    for (int x = 0; x < 2; x++) {
+        float fr = SA ? ratios[x] : 0.0f;
+        for (int c = 0; c < limits[x]; c++) {
+            iter++;
+            std::vector<i32> cands = candidates(M, curr, fr, constrained, constraint);
+            if (stats) { stats->sweeps++; stats->cands += (i64)cands.size(); }
+            if (cands.empty()) break;
+            std::vector<i32> moves = afterburner(M, curr, cands);
+            if (moves.empty()) break;
+            if (iter > 1 || !is_initial) set_new_cluster_ids(M, curr, moves, constrained, constraint);
+            perform_moves(M, curr, moves);
+            if (stats) stats->moves += (i64)moves.size();
+            if (curr.obj > best.obj) {
+                best.copy_from(curr);
+                best_part = M.part;
+            }
+        }
+    }
+    relabel(best_part, best);
+}
+
+// afterburner_filter_strict (src/local_mover.cpp:131-213). Canonical FP32
+// order: per move a sequential fold in adjacency order then over the earlier
+// moves (as written, :154-177); the prefix sum is one sequential FP32 scan in
+// move-list order (Kokkos parallel_scan order is unspecified in the reference).
+static std::vector<i32> strict_filter(Mover& M, State& st, const std::vector<i32>& moves, double& diff) {
+    const Graph& g = M.g;
+    float lam = (float)st.lambda;
+    i64 k = (i64)moves.size();
+    std::vector<i64> prio(g.n, g.n + 1);
+    for (i64 x = 0; x < k; x++) prio[moves[x]] = x;
+    std::vector<float> sg(k);
+    for (i64 x = 0; x < k; x++) {
+        i32 i = moves[x];
+        i32 best = M.dest[i], p = M.part[i];
+        float multi = (float)g.vw[i] * lam;
+        float oc = M.gain[i];
+        for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++) {
+            i32 v = g.col[j];
+            i64 wgt = g.wt(j);
+            if (prio[v] < prio[i]) {
+                i32 vp = M.dest[v];
+                oc = oc - (float)((vp == p) ? wgt : 0);
+                oc = oc + (float)((vp == best && best != NEW_PART) ? wgt : 0);
+                vp = M.part[v];
+                oc = oc + (float)((vp == p) ? wgt : 0);
+                oc = oc - (float)((vp == best) ? wgt : 0);
+            }
+        }
+        for (i64 jx = 0; jx < x; jx++) {
+            i32 v = moves[jx];
+            i32 vp = M.dest[v];
+            float wg = -multi * (float)g.vw[v];
+            oc = oc - ((vp == p) ? wg : 0.0f);
+            oc = oc + ((vp == best && best != NEW_PART) ? wg : 0.0f);
+            vp = M.part[v];
+            oc = oc + ((vp == p) ? wg : 0.0f);
+            oc = oc - ((vp == best) ? wg : 0.0f);
+        }
+        sg[x] = oc;
+    }
+    float run = 0.0f;
+    for (i64 x = 0; x < k; x++) {
+        run = run + sg[x];
+        sg[x] = run;
+    }
+    float bv = -1.0f;
+    i64 bl = g.n;
+    for (i64 x = 0; x < k; x++)
+        if (sg[x] > bv) { bv = sg[x]; bl = x; }
+    diff = 0;
+    i64 trunc = 0;
+    if (bl < k && bv > 0) { trunc = bl + 1; diff = (double)bv * 2; }
+    for (i64 x = 0; x < k; x++) M.gain[moves[x]] = sg[x];
+    return std::vector<i32>(moves.begin(), moves.begin() + trunc);
+}
+
+// local_move_strict (src/local_mover.cpp:1419-1483)
+static void local_move_strict(const Graph& g, std::vector<i32>& best_part, State& best, bool is_initial,
+                              bool constrained, i32* constraint, Stats* stats) {
+    State curr = best;
+    curr.copy_from(best);
+    Mover M(g);
+    M.part = best_part;
+    M.dest.assign(g.n, NULL_PART);
+    M.gain.assign(g.n, 0.0f);
+    M.tables = !is_initial;
+    for (int it = 0; it < 6; it++) {
+        std::vector<i32> moves = candidates(M, curr, 0.0f, constrained, constraint);
+        if (stats) { stats->sweeps++; stats->cands += (i64)moves.size(); }
+        if (moves.empty()) break;
+        if (it > 0 || !is_initial) set_new_cluster_ids(M, curr, moves, constrained, constraint);
+        double diff = 0;
+        moves = strict_filter(M, curr, moves, diff);
+        if (moves.empty()) break;
+        perform_moves(M, curr, moves);
+        if (stats) stats->moves += (i64)moves.size();
+        if (curr.obj > best.obj) {
+            best.copy_from(curr);
+            best_part = M.part;
+        }
+        std::fill(M.dest.begin(), M.dest.end(), NULL_PART);  // :1478-1480
+    }
+    relabel(best_part, best);
+}
+
+// own-cluster weight of every vertex under labels (the pvals_clone contract,
+// src/local_mover.cpp:1354-1358, consumed by src/leidenR.cpp:279)
+static std::vector<i64> own_weight(const Graph& g, const std::vector<i32>& labels) {
+    std::vector<i64> pv(g.n, 0);
+    for (i64 i = 0; i < g.n; i++)
+        for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++)
+            if (labels[g.col[j]] == labels[i]) pv[i] += g.wt(j);
+    return pv;
+}
+
+// ----------------------------------------------------------------- A11 leidenR
+struct Rng {
+    u64 seed;
+    u64 k = 0;
+    u32 draw() { return seed_draw(seed, k++); }
+};
+
+// find_trees (src/leidenR.cpp:212-260)
+static void find_trees(std::vector<i32>& vcmap, i64 n, const std::vector<i32>& hn, std::vector<i32>& order) {
+    for (i64 i = 0; i < n; i++) {
+        i32 now = (i32)i, then = hn[i];
+        if (now == then) { vcmap[i] = (i32)i; continue; }
+        while (order[now] > order[then]) { now = then; then = hn[then]; }
+        if (now != i) { vcmap[i] = now; vcmap[now] = now; }
+    }
+    for (i64 i = 0; i < n; i++)
+        if (vcmap[i] == ORD_MAX) order[i] = 2 * SPLIT - order[i];
+    // part 2 is race-free in effect: every chase from an unassigned vertex
+    // follows the same pointer path, so sequential order gives the same labels
+    for (i64 i = 0; i < n; i++) {
+        if (vcmap[i] != ORD_MAX) continue;
+        i32 now = (i32)i, then = hn[i];
+        while (vcmap[now] == ORD_MAX && order[now] > order[then]) { now = then; then = hn[then]; }
+        vcmap[i] = vcmap[now];
+    }
+}
+
+// ensure_gamma_connectivity (src/leidenR.cpp:41-187). Canonical rule: the
+// vertices of one tree are sorted by (order, vertex id); the reference's
+// KokkosKernels sort leaves ties to its bucket insertion order.
+static void ensure_gamma(const Graph& g, std::vector<i32>& vcmap, const std::vector<i32>& constraint,
+                         const std::vector<i32>& order, const std::vector<i64>& total_deg,
+                         const std::vector<i64>& pvals, float gamma) {
+    i64 n = g.n;
+    std::vector<i32> ids(n);
+    std::iota(ids.begin(), ids.end(), 0);
+    std::sort(ids.begin(), ids.end(), [&](i32 a, i32 b) {
+        if (vcmap[a] != vcmap[b]) return vcmap[a] < vcmap[b];
+        if (order[a] != order[b]) return order[a] < order[b];
+        return a < b;
+    });
+    std::vector<i64> cbeg(n + 1, 0);
+    for (i64 i = 0; i < n; i++) cbeg[vcmap[i] + 1]++;
+    for (i64 c = 0; c < n; c++) cbeg[c + 1] += cbeg[c];
+    std::vector<i64> tsize(n);  // exclusive prefix of vertex weight, global
+    {
+        i64 run = 0;
+        for (i64 x = 0; x < n; x++) { tsize[x] = run; run += g.vw[ids[x]]; }
+    }
+    std::vector<i64> inner(n, 0);
+    for (i64 i = 0; i < n; i++) {
+        i64 r = 0;
+        for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++) {
+            i32 v = g.col[j];
+            if (vcmap[i] == vcmap[v] && order[v] < order[i]) r += g.wt(j);
+        }
+        inner[i] = r;
+    }
+    std::vector<i64> breakers(n, n);
+    for (i64 x = 0; x < n; x++) {
+        i32 i = ids[x];
+        i32 c = vcmap[i];
+        i64 prev = tsize[x] - tsize[cbeg[c]];
+        if ((float)inner[i] < gamma * (float)g.vw[i] * (float)prev) breakers[c] = std::min(breakers[c], x);
+    }
+    std::vector<i64> outer(n);
+    {
+        i64 run = 0;
+        for (i64 x = 0; x < n; x++) {
+            i32 i = ids[x];
+            run += pvals[i] - 2 * inner[i];
+            outer[i] = run;
+        }
+    }
+    for (i64 x = 0; x < n; x++) {
+        i32 i = ids[x];
+        i32 c = vcmap[i];
+        i64 cb = cbeg[c];
+        i64 cur = tsize[x] + g.vw[i] - tsize[cb];
+        i64 other = total_deg[constraint[i]] - cur;
+        i64 out = outer[i];
+        if (cb > 0) out -= outer[ids[cb - 1]];
+        if ((float)out < gamma * (float)cur * (float)other) breakers[c] = std::min(breakers[c], x + 1);
+    }
+    for (i64 x = 0; x < n; x++) {
+        i32 i = ids[x];
+        if (x >= breakers[vcmap[i]]) vcmap[i] = i;
+    }
+}
+
+// contigitize_clusters (src/leidenR.cpp:189-210)
+static i64 contigitize(std::vector<i32>& vcmap, i64 n) {
+    std::vector<i32> rank(n, -1);
+    i64 nc = 0;
+    for (i64 u = 0; u < n; u++)
+        if (vcmap[u] == u) rank[u] = (i32)nc++;
+    for (i64 u = 0; u < n; u++) vcmap[u] = rank[vcmap[u]];
+    return nc;
+}
+
+// coarsen_leidenR (src/leidenR.cpp:263-500); `uniform` selects the unit-weight
+// neighbour rule used only on the top level of unit-weight graphs (:173).
+static std::vector<i32> leiden_refine(const Graph& g, const std::vector<i32>& constraint, const State& rfd,
+                                      const std::vector<i64>& pvals, bool uniform, Rng& rng, i64& nc) {
+    i64 n = g.n;
+    float gamma = (float)rfd.lambda;
+    std::vector<i32> vcmap(n, ORD_MAX), hn(n), well(n);
+    for (i64 i = 0; i < n; i++) {  // :281-287
+        i64 wd = g.vw[i];
+        well[i] = ((float)pvals[i] >= gamma * (float)wd * (float)(rfd.vol[constraint[i]] - wd)) ? 1 : 0;
+    }
+    if (uniform) {
+        // :296-393: first satisfactory neighbour cyclically from a hashed start
+        i32 seed = (i32)rng.draw();
+        for (i64 i = 0; i < n; i++) {
+            i64 s = g.rp[i], e = g.rp[i + 1], width = e - s;
+            float multi = gamma * (float)g.vw[i];
+            u32 h = murmur3_i32((i32)((u32)seed + (u32)i));
+            i32 jx32 = (i32)h;
+            if (jx32 % 1000 == 0 || well[i] == 0 || width == 0) { hn[i] = (i32)i; continue; }
+            i64 jx = s + (i64)(h % (u32)width);
+            i32 pick = (i32)i;
+            bool found = false;
+            for (i64 j = jx; j < e && !found; j++) {
+                i32 v = g.col[j];
+                if (constraint[i] != constraint[v] || well[v] == 0) continue;
+                if ((float)g.wt(j) >= multi * (float)g.vw[v]) { pick = v; found = true; }
+            }
+            for (i64 j = s; j < jx && !found; j++) {
+                i32 v = g.col[j];
+                if (constraint[i] != constraint[v] || well[v] == 0) continue;
+                if ((float)g.wt(j) >= multi * (float)g.vw[v]) { pick = v; found = true; }
+            }
+            hn[i] = pick;
+        }
+    } else {
+        // :395-474: weighted random argmax of rand * (w - gamma w_i w_v)^4.
+        // Canonical rule: degree < 128 keeps the last maximum (serial path),
+        // degree >= 128 keeps the first maximum (MaxFirstLoc team path).
+        // (w - ...)^4 is evaluated as ((double)val^2)^2 in double.
+        i32 seed = (i32)rng.draw();
+        for (i64 i = 0; i < n; i++) {
+            i64 s = g.rp[i], e = g.rp[i + 1];
+            float multi = gamma * (float)g.vw[i];
+            if (murmur3_i32((i32)((u32)seed + (u32)i)) % 10000u == 0 || well[i] == 0) { hn[i] = (i32)i; continue; }
+            bool last = g.deg(i) < LARGE_CUTOFF;
+            float m = 0.0f;
+            i64 am = e;
+            for (i64 j = s; j < e; j++) {
+                i32 v = g.col[j];
+                if (constraint[i] != constraint[v] || well[v] == 0) continue;
+                float val = fmaf(-multi, (float)g.vw[v], (float)g.wt(j));
+                if (val >= 0) {
+                    float smp = (float)murmur3_i32((i32)((u32)seed + (u32)j)) / (float)ORD_MAX;
+                    double v2 = (double)val * (double)val;
+                    smp = (float)((double)smp * (v2 * v2));
+                    if (last ? (smp >= m) : (smp > m || am == e)) { m = smp; am = j; }
+                }
+            }
+            hn[i] = (am >= s && am < e) ? g.col[am] : (i32)i;
+        }
+    }
+    std::vector<i32> order(n);
+    {
+        i32 seed = (i32)rng.draw();
+        for (i64 i = 0; i < n; i++) {  // :482-486
+            i32 o = (i32)murmur3_i32((i32)((u32)i + (u32)seed));
+            i32 r = o % SPLIT;
+            order[i] = r < 0 ? -r : r;
+        }
+    }
+    for (i64 i = 0; i < n; i++)  // :487-494
+        if (order[i] == order[hn[i]]) hn[i] = (i32)i;
+    find_trees(vcmap, n, hn, order);
+    ensure_gamma(g, vcmap, constraint, order, rfd.vol, pvals, gamma);
+    nc = contigitize(vcmap, n);
+    return vcmap;
+}
+
+// ------------------------------------------------------------- A13 drivers
+static void coarsen_vtx_w(const Graph& g, const std::vector<i32>& map, Graph& c) {  // src/clustering_methods.cpp:33-38
+    c.vw.assign(c.n, 0);
+    for (i64 i = 0; i < g.n; i++) c.vw[map[i]] += g.vw[i];
+}
+
+// Canonical rule for downsample (src/clustering_methods.cpp:40-45) when the
+// members of one cluster carry different values: the highest vertex id wins.
+static void downsample(const std::vector<i32>& in, std::vector<i32>& out, const std::vector<i32>& map) {
+    for (size_t i = 0; i < in.size(); i++) out[map[i]] = in[i];
+}
+
+struct Run {
+    Stats stats;
+    i64 levels = 0;
+    i64 total_nnz = 0;
+};
+
+// leiden_part<plus, improve> (src/clustering_methods.cpp:47-137)
+static std::vector<i32> leiden_part(const Graph& top, State& rfd, bool plus, bool improve,
+                                    const std::vector<i32>* input, Rng& rng, Run& run) {
+    std::vector<Graph> levels;
+    std::vector<std::vector<i32>> parts;
+    levels.push_back(top);
+    std::vector<i32> part(top.n);
+    if (!improve) std::iota(part.begin(), part.end(), 0);
+    else part = *input;
+    while (true) {
+        const Graph& c = levels.back();
+        double old_obj = rfd.obj;
+        bool init = !improve && levels.size() == 1;
+        local_move(c, part, rfd, init, false, nullptr, true, &run.stats);
+        if (c.nnz() > 100000 && old_obj == rfd.obj) local_move(c, part, rfd, init, false, nullptr, false, &run.stats);
+        if (old_obj == rfd.obj) local_move_strict(c, part, rfd, init, false, nullptr, &run.stats);
+        if (rfd.label_count == c.n) { parts.push_back(part); break; }
+        std::vector<i32> louv = part;
+        std::vector<i64> pv = own_weight(c, louv);
+        i64 nc = 0;
+        bool uni = levels.size() == 1 && c.uniform;
+        std::vector<i32> cmap = leiden_refine(c, louv, rfd, pv, uni, rng, nc);
+        parts.push_back(cmap);
+        if (nc < c.n) {
+            Graph next = contract(c, cmap, nc);
+            coarsen_vtx_w(c, cmap, next);
+            part.assign(nc, 0);
+            downsample(louv, part, cmap);
+            levels.push_back(std::move(next));
+        } else {
+            Graph same = c;
+            levels.push_back(std::move(same));
+        }
+    }
+    run.levels = (i64)levels.size();
+    for (auto& l : levels) run.total_nnz += l.nnz();
+    for (int i = (int)levels.size() - 2; i >= 0; i--) {
+        const Graph& c = levels[i];
+        std::vector<i32>& cp = parts[i + 1];
+        std::vector<i32>& fp = parts[i];
+        for (i64 x = 0; x < c.n; x++) fp[x] = cp[fp[x]];
+        if (plus) local_move(c, fp, rfd, false, false, nullptr, true, &run.stats);
+    }
+    return parts[0];
+}
+
+// louvain_part<constrained> (src/clustering_methods.cpp:139-222)
+static std::vector<i32> louvain_part(const Graph& top, State& rfd, bool constrained, std::vector<i32> constraint,
+                                     Run& run) {
+    if (constrained) {  // cluster_data::reset (interface/cluster_data.hpp:49-55)
+        std::copy(top.vw.begin(), top.vw.end(), rfd.vol.begin());
+        rfd.uncut = 0;
+        rfd.label_count = (i32)top.n;
+        rfd.update_objective();
+    }
+    std::vector<Graph> levels;
+    std::vector<std::vector<i32>> parts;
+    levels.push_back(top);
+    bool drop = constrained;
+    while (true) {
+        const Graph& c = levels.back();
+        std::vector<i32> part(c.n);
+        std::iota(part.begin(), part.end(), 0);
+        i32* cons = constrained ? constraint.data() : nullptr;
+        local_move(c, part, rfd, true, constrained, cons, true, &run.stats);
+        if (constrained && rfd.label_count == c.n) local_move_strict(c, part, rfd, true, constrained, cons, &run.stats);
+        parts.push_back(part);
+        if (rfd.label_count < c.n) {
+            Graph next = contract(c, part, rfd.label_count);
+            next.vw.assign(rfd.vol.begin(), rfd.vol.begin() + rfd.label_count);
+            if (constrained) {
+                std::vector<i32> nc(rfd.label_count, 0);
+                downsample(constraint, nc, part);
+                constraint = nc;
+            }
+            levels.push_back(std::move(next));
+        } else if (drop) {
+            std::fill(constraint.begin(), constraint.end(), 0);
+            parts.pop_back();
+            drop = false;
+        } else {
+            break;
+        }
+    }
+    run.levels = (i64)levels.size();
+    for (auto& l : levels) run.total_nnz += l.nnz();
+    if (levels.size() > 1) { levels.pop_back(); parts.pop_back(); }
+    for (int i = (int)levels.size() - 2; i >= 0; i--) {
+        const Graph& c = levels[i];
+        std::vector<i32>& cp = parts[i + 1];
+        std::vector<i32>& fp = parts[i];
+        for (i64 x = 0; x < c.n; x++) fp[x] = cp[fp[x]];
+        local_move(c, fp, rfd, false, false, nullptr, true, &run.stats);
+    }
+    return parts[0];
+}
+
+}  // namespace orc
+
+// =====================================================================
+// C ABI for the tests (host arrays only)
+// =====================================================================
+using namespace orc;
+
+static Graph make_graph(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w64, const i64* vw) {
+    Graph g;
+    g.n = n;
+    g.rp.assign(rp, rp + n + 1);
+    g.col.assign(col, col + rp[n]);
+    if (w32) { g.uniform = false; g.w.assign(w32, w32 + rp[n]); }
+    else if (w64) { g.uniform = false; g.w.assign(w64, w64 + rp[n]); }
+    if (vw) g.vw.assign(vw, vw + n);
+    else g.vw.assign(n, 1);
+    return g;
+}
+
+extern "C" {
+
+u32 or_murmur3_i32(i32 key) { return murmur3_i32(key); }
+u32 or_xorshift32(u32 x) { return xorshift32(x); }
+u32 or_seed_draw(u64 seed, u64 k) { return seed_draw(seed, k); }
+
+// A1: order1/order2 plus offsets {large, large2, mid, mid2}
+void or_orderings(i64 n, const i64* rp, i32* o1, i32* o2, i64* offs) {
+    Graph g;
+    g.n = n;
+    g.rp.assign(rp, rp + n + 1);
+    Orders o = make_orders(g);
+    std::copy(o.o1.begin(), o.o1.end(), o1);
+    std::copy(o.o2.begin(), o.o2.end(), o2);
+    offs[0] = o.off_large; offs[1] = o.off_large2; offs[2] = o.off_mid; offs[3] = o.off_mid2;
+}
+
+// A12: vol[L], uncut, exact sum of squares as (hi, lo) u64 words
+void or_evaluate(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* vw, const i32* labels, i64 L,
+                 i64* vol_out, i64* uncut_out, u64* sq_out) {
+    Graph g = make_graph(n, rp, col, w32, nullptr, vw);
+    std::vector<i64> vol;
+    i64 uncut;
+    evaluate(g, labels, L, vol, uncut);
+    u128 sq = 0;
+    for (i64 l = 0; l < L; l++) sq += (u128)(u64)vol[l] * (u128)(u64)vol[l];
+    std::copy(vol.begin(), vol.end(), vol_out);
+    *uncut_out = uncut;
+    sq_out[0] = (u64)(sq >> 64);
+    sq_out[1] = (u64)sq;
+}
+
+// A10: returns coarse nnz; fills out arrays if capacity suffices (else -needed)
+i64 or_contract(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w64, const i32* vcmap, i64 nc,
+                i64* out_rp, i32* out_col, i64* out_w, i64 cap) {
+    Graph g = make_graph(n, rp, col, w32, w64, nullptr);
+    std::vector<i32> m(vcmap, vcmap + n);
+    Graph c = contract(g, m, nc);
+    if (c.nnz() > cap) return -c.nnz();
+    std::copy(c.rp.begin(), c.rp.end(), out_rp);
+    std::copy(c.col.begin(), c.col.end(), out_col);
+    std::copy(c.w.begin(), c.w.end(), out_w);
+    return c.nnz();
+}
+
+// One sweep from a fixed state (candidates + afterburner) with the dest cache
+// in/out: returns the number of accepted moves, written (order1 order) to
+// moves_out; dest/gain are updated in place.  raw != 0 scores the raw
+// adjacency (the first sweep of an initial call).
+i64 or_sweep(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w64, const i64* vw, const i32* part,
+             const i64* vol, i64 L, double lambda, float ratio, int raw, const i32* constraint, i32* dest, float* gain,
+             i32* moves_out, i64* ncand_out) {
+    Graph g = make_graph(n, rp, col, w32, w64, vw);
+    State st;
+    st.vol.assign(vol, vol + L);
+    st.label_count = (i32)L;
+    st.lambda = lambda;
+    Mover M(g);
+    M.part.assign(part, part + n);
+    M.dest.assign(dest, dest + n);
+    M.gain.assign(gain, gain + n);
+    M.tables = !raw;
+    std::vector<i32> c = candidates(M, st, ratio, constraint != nullptr, constraint);
+    std::vector<i32> mv = c.empty() ? std::vector<i32>() : afterburner(M, st, c);
+    std::copy(M.dest.begin(), M.dest.end(), dest);
+    std::copy(M.gain.begin(), M.gain.end(), gain);
+    std::copy(mv.begin(), mv.end(), moves_out);
+    *ncand_out = (i64)c.size();
+    return (i64)mv.size();
+}
+
+// own-cluster weight per vertex
+void or_own_weight(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w64, const i32* labels,
+                   i64* out) {
+    Graph g = make_graph(n, rp, col, w32, w64, nullptr);
+    std::vector<i32> l(labels, labels + n);
+    std::vector<i64> pv = own_weight(g, l);
+    std::copy(pv.begin(), pv.end(), out);
+}
+
+// objective set-up: app/objective_helpers.hpp:9-29, interface/cluster_data.hpp
+// kind: 0 Mod, 1 WMod, 2 CPM, 3 NLCC, 4 LCC (base)
+static State make_state(const Graph& g, int kind, double mult) {
+    State s;
+    s.vol.assign(g.vw.begin(), g.vw.end());
+    s.top_nnz = g.nnz();
+    s.label_count = (i32)g.n;
+    s.uncut = 0;
+    s.lambda = 1.0;
+    if (kind == 0 || kind == 1) {
+        i64 gd = 0;
+        if (kind == 0 || g.uniform) gd = g.nnz();
+        else for (i64 v : g.vw) gd += v;
+        double inv = 1.0 / (double)gd;
+        s.lambda = mult * inv;
+        s.scale = inv;
+    } else if (kind == 2) {
+        s.lambda = mult * 0.5;
+        s.offset = s.lambda * (double)g.n;
+    } else if (kind == 3) {
+        i64 gd = 0;
+        if (g.uniform) gd = g.nnz();
+        else for (i64 v : g.w) gd += v;
+        u64 vt = 0;
+        for (i64 v : g.vw) vt += (u64)v;
+        double denom = (double)(vt * vt);
+        s.lambda = mult * (double)gd / denom;
+        s.scale = 1.0 / (double)gd;
+    } else {
+        s.lambda = mult;
+    }
+    s.update_objective();
+    return s;
+}
+
+// Full clustering as app/cluster_main.cpp:116-165 (run_clustering) for one
+// trial.  algo: 0 pLouvain, 1 pLeiden, 2 pLeiden+.  Returns the objective;
+// part_out[n]; info[0..5] = label_count, uncut, levels, sweeps, moves, outer
+// iterations.
+double or_cluster(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* vw, int kind, double mult,
+                  int algo, int ex_iters, u64 seed, i32* part_out, i64* info) {
+    Graph g = make_graph(n, rp, col, w32, nullptr, vw);
+    State rfd = make_state(g, kind, mult);
+    Rng rng{seed};
+    Run run;
+    std::vector<i32> part;
+    if (algo == 0) part = louvain_part(g, rfd, false, {}, run);
+    else part = leiden_part(g, rfd, algo == 2, false, nullptr, rng, run);
+    double last = -std::numeric_limits<double>::infinity();
+    int counter = 0;
+    i64 outer = 1;
+    while (true) {  // iteration_termination_condition (app/cluster_main.cpp:101-114)
+        bool term = true;
+        if (ex_iters < 0) {
+            if (rfd.objective() > last) term = false;
+            last = rfd.objective();
+        } else if (counter++ < ex_iters) {
+            term = false;
+        }
+        if (term) break;
+        std::vector<i32> cons = part;
+        if (algo == 0) part = louvain_part(g, rfd, true, cons, run);
+        else part = leiden_part(g, rfd, algo == 2, true, &cons, rng, run);
+        outer++;
+    }
+    std::copy(part.begin(), part.end(), part_out);
+    info[0] = rfd.label_count;
+    info[1] = rfd.uncut;
+    info[2] = run.levels;
+    info[3] = run.stats.sweeps;
+    info[4] = run.stats.moves;
+    info[5] = outer;
+    return rfd.objective();
+}
+
+// local_move on one level from a given state (for parity of A8 on its own)
+// vol_inout sized >= n; returns label_count; part_inout relabelled.
+i64 or_local_move(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w64, const i64* vw,
+                  i32* part_inout, i64* vol_inout, i64 L, i64* uncut_inout, double lambda, int is_initial,
+                  int strict, int SA, double* obj_inout) {
+    Graph g = make_graph(n, rp, col, w32, w64, vw);
+    State st;
+    st.vol.assign(vol_inout, vol_inout + n);
+    st.label_count = (i32)L;
+    st.lambda = lambda;
+    st.uncut = *uncut_inout;
+    st.top_nnz = g.nnz();
+    st.obj = *obj_inout;
+    std::vector<i32> part(part_inout, part_inout + n);
+    if (strict) local_move_strict(g, part, st, is_initial != 0, false, nullptr, nullptr);
+    else local_move(g, part, st, is_initial != 0, false, nullptr, SA != 0, nullptr);
+    std::copy(part.begin(), part.end(), part_inout);
+    std::copy(st.vol.begin(), st.vol.begin() + st.label_count, vol_inout);
+    *uncut_inout = st.uncut;
+    *obj_inout = st.obj;
+    return st.label_count;
+}
+
+// leidenR on one level: louv = Louvain labels (constraint), vol[L] its volumes
+i64 or_leiden_refine(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w64, const i64* vw,
+                     const i32* louv, const i64* vol, i64 L, double lambda, int uniform, u64 seed, u64 draw0,
+                     i32* vcmap_out) {
+    Graph g = make_graph(n, rp, col, w32, w64, vw);
+    State st;
+    st.vol.assign(vol, vol + L);
+    st.label_count = (i32)L;
+    st.lambda = lambda;
+    std::vector<i32> lv(louv, louv + n);
+    std::vector<i64> pv = own_weight(g, lv);
+    Rng rng{seed, draw0};
+    i64 nc = 0;
+    std::vector<i32> m = leiden_refine(g, lv, st, pv, uniform != 0, rng, nc);
+    std::copy(m.begin(), m.end(), vcmap_out);
+    return nc;
+}
+
+}  // extern "C"

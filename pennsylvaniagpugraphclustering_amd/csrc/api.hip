@@ -1,0 +1,605 @@
+// Multilevel drivers (src/clustering_methods.cpp), the run_clustering outer
+// loop (app/cluster_main.cpp:101-165) and the C ABI declared in pggc.h.
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <vector>
+
+#include "device.h"
+#include "leiden.h"
+#include "localmove.h"
+#include "pggc.h"
+
+using namespace pggc;
+
+struct pggc_ctx {
+    Ctx c;
+    MoveScratch ms;
+};
+
+namespace pggc {
+
+struct RunStats {
+    MoveStats mv;
+    i64 levels = 0;
+    i64 total_nnz = 0;
+};
+
+// ---------------------------------------------------------------- kernels
+__global__ void k_project(i64 n, const i32* coarse, i32* fine) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x)
+        fine[x] = coarse[fine[x]];
+}
+__global__ void k_down_winner(i64 n, const i32* map, i32* win) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        atomicMax(&win[map[i]], (i32)i);
+}
+__global__ void k_down_gather(i64 nc, const i32* win, const i32* in, i32* out) {
+    for (i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += (i64)gridDim.x * blockDim.x)
+        if (win[c] >= 0) out[c] = in[win[c]];
+}
+__global__ void k_downsample(i64 n, const i32* map, const i32* in, i32* out) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        out[map[i]] = in[i];
+}
+template <class WT>
+__global__ void __launch_bounds__(256) k_wdeg(const i64* rp, WT W, i64 n, i64* out) {
+    const int lane = threadIdx.x & 63;
+    for (i64 i = (i64)blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += (i64)gridDim.x * 4) {
+        i64 acc = 0;
+        for (i64 j = rp[i] + lane; j < rp[i + 1]; j += 64) acc += W(j);
+        acc = wave_sum_i64(acc);
+        if (lane == 0) out[i] = acc;
+    }
+}
+__global__ void k_deg(const i64* rp, i64 n, i64* out) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        out[i] = rp[i + 1] - rp[i];
+}
+
+static void project(Ctx& c, i64 n, const i32* coarse, i32* fine) {
+    k_project<<<grid_for(n, 1024, 8192), 256, 0, c.stream>>>(n, coarse, fine);
+    PGGC_LAUNCH_CHECK();
+}
+
+// downsample (src/clustering_methods.cpp:40-45); when members of a cluster
+// carry different values the highest vertex id wins.
+static void downsample_max(Ctx& c, i64 n, const i32* map, i64 nc, const i32* in, i32* out) {
+    Buf<i32> win(c, std::max<i64>(nc, 1));
+    fill_i32(c, win, nc, -1);
+    k_down_winner<<<grid_for(n, 1024, 8192), 256, 0, c.stream>>>(n, map, win);
+    k_down_gather<<<grid_for(nc, 1024, 8192), 256, 0, c.stream>>>(nc, win, in, out);
+    PGGC_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- drivers
+struct Level {
+    DGraph g;
+    std::unique_ptr<OwnedGraph> own;
+    std::unique_ptr<LevelPlan> plan;
+};
+
+static Level top_level(const DGraph& g) {
+    Level l;
+    l.g = g;
+    l.plan.reset(new LevelPlan());
+    return l;
+}
+static Level owned_level(std::unique_ptr<OwnedGraph> og) {
+    Level l;
+    l.g = og->view();
+    l.own = std::move(og);
+    l.plan.reset(new LevelPlan());
+    return l;
+}
+
+// louvain_part<constrained> (src/clustering_methods.cpp:139-222)
+static void louvain_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& rfd, bool constrained,
+                         i32* constraint, i32* part_out, RunStats& rs, LevelPlan* top_plan) {
+    if (constrained) {  // cluster_data::reset (interface/cluster_data.hpp:49-55)
+        PGGC_HIP(hipMemcpyAsync(rfd.vol, top.vw, top.n * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
+        rfd.uncut = 0;
+        rfd.label_count = (i32)top.n;
+        update_objective(c, rfd);
+    }
+    std::vector<Level> levels;
+    std::vector<Buf<i32>> parts;
+    std::vector<Buf<i32>> cons_store;
+    levels.push_back(top_level(top));
+    if (top_plan) levels[0].plan.reset();
+    i32* cons = constraint;
+    bool drop = constrained;
+    while (true) {
+        Level& L = levels.back();
+        LevelPlan& lp = (levels.size() == 1 && top_plan) ? *top_plan : *L.plan;
+        const DGraph g = L.g;
+        Buf<i32> part(c, std::max<i64>(g.n, 1));
+        iota_i32(c, part, g.n);
+        local_move(c, ms, lp, g, part, rfd, true, constrained, cons, true, &rs.mv);
+        if (constrained && rfd.label_count == g.n) local_move_strict(c, ms, lp, g, part, rfd, true, constrained, cons, &rs.mv);
+        const i32* pp = part.p;
+        parts.push_back(std::move(part));
+        if (rfd.label_count < g.n) {
+            const i64 nc = rfd.label_count;
+            std::unique_ptr<OwnedGraph> og(new OwnedGraph());
+            contract(c, g, pp, nc, *og);
+            og->vw = Buf<i64>(c, std::max<i64>(nc, 1));
+            PGGC_HIP(hipMemcpyAsync(og->vw, rfd.vol, nc * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
+            if (constrained) {
+                Buf<i32> nxt(c, std::max<i64>(nc, 1));
+                downsample_max(c, g.n, pp, nc, cons, nxt);
+                cons = nxt.p;
+                cons_store.push_back(std::move(nxt));
+            }
+            levels.push_back(owned_level(std::move(og)));
+        } else if (drop) {
+            fill_i32(c, cons, g.n, 0);
+            parts.pop_back();
+            drop = false;
+        } else {
+            break;
+        }
+    }
+    rs.levels = (i64)levels.size();
+    for (auto& l : levels) rs.total_nnz += l.g.nnz;
+    if (levels.size() > 1) {
+        levels.pop_back();
+        parts.pop_back();
+    }
+    for (int i = (int)levels.size() - 2; i >= 0; i--) {
+        const DGraph g = levels[i].g;
+        LevelPlan& lp = (i == 0 && top_plan) ? *top_plan : *levels[i].plan;
+        project(c, g.n, parts[i + 1], parts[i]);
+        local_move(c, ms, lp, g, parts[i], rfd, false, false, nullptr, true, &rs.mv);
+    }
+    PGGC_HIP(hipMemcpyAsync(part_out, parts[0].p, top.n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+    c.sync();
+}
+
+// leiden_part<plus, improve> (src/clustering_methods.cpp:47-137)
+static void leiden_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& rfd, bool plus, bool improve,
+                        const i32* input, i32* part_out, RunStats& rs, LevelPlan* top_plan) {
+    std::vector<Level> levels;
+    std::vector<Buf<i32>> parts;
+    levels.push_back(top_level(top));
+    Buf<i32> part(c, std::max<i64>(top.n, 1));
+    if (!improve) iota_i32(c, part, top.n);
+    else PGGC_HIP(hipMemcpyAsync(part, input, top.n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+    while (true) {
+        const size_t li = levels.size() - 1;
+        const DGraph g = levels[li].g;
+        LevelPlan& lp = (li == 0 && top_plan) ? *top_plan : *levels[li].plan;
+        const double old_obj = rfd.obj;
+        const bool init = !improve && levels.size() == 1;
+        local_move(c, ms, lp, g, part, rfd, init, false, nullptr, true, &rs.mv);
+        if (g.nnz > 100000 && old_obj == rfd.obj) local_move(c, ms, lp, g, part, rfd, init, false, nullptr, false, &rs.mv);
+        if (old_obj == rfd.obj) local_move_strict(c, ms, lp, g, part, rfd, init, false, nullptr, &rs.mv);
+        if (rfd.label_count == g.n) {
+            parts.push_back(std::move(part));
+            break;
+        }
+        Buf<i32> louv = std::move(part);
+        Buf<i64> pv(c, std::max<i64>(g.n, 1));
+        own_weight(c, g, louv, pv);
+        const bool uni = levels.size() == 1 && g.wk == W_UNIT;
+        Buf<i32> cmap(c, std::max<i64>(g.n, 1));
+        const i64 nc = leiden_refine(c, g, lp.ord, louv, rfd.vol, rfd.lambda, pv, uni, cmap);
+        const i32* cm = cmap.p;
+        parts.push_back(std::move(cmap));
+        if (nc < g.n) {
+            std::unique_ptr<OwnedGraph> og(new OwnedGraph());
+            contract(c, g, cm, nc, *og);
+            og->vw = Buf<i64>(c, std::max<i64>(nc, 1));
+            coarsen_vertex_weights(c, g, cm, nc, og->vw);
+            part = Buf<i32>(c, std::max<i64>(nc, 1));
+            k_downsample<<<grid_for(g.n, 1024, 8192), 256, 0, c.stream>>>(g.n, cm, louv, part);
+            PGGC_LAUNCH_CHECK();
+            levels.push_back(owned_level(std::move(og)));
+        } else {
+            part = std::move(louv);
+            Level same;
+            same.g = g;
+            same.plan.reset(new LevelPlan());
+            levels.push_back(std::move(same));
+        }
+    }
+    rs.levels = (i64)levels.size();
+    for (auto& l : levels) rs.total_nnz += l.g.nnz;
+    for (int i = (int)levels.size() - 2; i >= 0; i--) {
+        const DGraph g = levels[i].g;
+        LevelPlan& lp = (i == 0 && top_plan) ? *top_plan : *levels[i].plan;
+        project(c, g.n, parts[i + 1], parts[i]);
+        if (plus) local_move(c, ms, lp, g, parts[i], rfd, false, false, nullptr, true, &rs.mv);
+    }
+    PGGC_HIP(hipMemcpyAsync(part_out, parts[0].p, top.n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+    c.sync();
+}
+
+// get_objective (app/objective_helpers.hpp:9-22, interface/cluster_data.hpp)
+static void state_init(Ctx& c, const DGraph& g, int objective, double mult, RefineData& rd) {
+    PGGC_HIP(hipMemcpyAsync(rd.vol, g.vw, g.n * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
+    rd.top_nnz = g.nnz;
+    rd.label_count = (i32)g.n;
+    rd.uncut = 0;
+    rd.lambda = 1.0;
+    rd.scale = 1.0;
+    rd.offset = 0.0;
+    Buf<i64> tmp(c, 2);
+    if (objective == PGGC_OBJ_MOD || objective == PGGC_OBJ_WMOD) {
+        i64 gd;
+        if (objective == PGGC_OBJ_MOD || g.wk == W_UNIT) gd = g.nnz;
+        else {
+            sum_i64(c, g.vw, g.n, tmp);
+            gd = c.read1(tmp);
+        }
+        double inv = 1.0 / (double)gd;
+        rd.lambda = mult * inv;
+        rd.scale = inv;
+    } else if (objective == PGGC_OBJ_CPM) {
+        rd.lambda = mult * 0.5;
+        rd.offset = rd.lambda * (double)g.n;
+    } else if (objective == PGGC_OBJ_NLCC) {
+        i64 gd = g.nnz;
+        if (g.wk != W_UNIT) {
+            Buf<i64> wd(c, std::max<i64>(g.n, 1));
+            PGGC_WDISPATCH(g, (k_wdeg<<<grid_for(g.n, 4, 1 << 16), 256, 0, c.stream>>>(g.rp, W, g.n, wd)));
+            PGGC_LAUNCH_CHECK();
+            sum_i64(c, wd, g.n, tmp);
+            gd = c.read1(tmp);
+        }
+        sum_i64(c, g.vw, g.n, tmp);
+        u64 vt = (u64)c.read1(tmp);
+        double denom = (double)(vt * vt);
+        rd.lambda = mult * (double)gd / denom;
+        rd.scale = 1.0 / (double)gd;
+    } else if (objective == PGGC_OBJ_LCC) {
+        rd.lambda = mult;
+    } else {
+        throw Error(-1, "unknown objective");
+    }
+    update_objective(c, rd);
+}
+
+}  // namespace pggc
+
+// =================================================================== C ABI
+static thread_local std::string g_err;
+
+template <class F>
+static int guard(F f) {
+    try {
+        f();
+        return PGGC_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return PGGC_EINTERNAL;
+    }
+}
+
+static DGraph view_of(const pggc_graph* g) {
+    if (!g || g->n < 0 || g->nnz < 0 || (g->n > 0 && !g->row_ptr) || (g->nnz > 0 && !g->col))
+        throw Error(PGGC_EINVAL, "invalid graph");
+    if (g->w && g->w64) throw Error(PGGC_EINVAL, "graph has both int32 and int64 weights");
+    if (g->n >= 0x7fffffff) throw Error(PGGC_EINVAL, "vertex count exceeds int32 ids");
+    DGraph d;
+    d.n = g->n;
+    d.nnz = g->nnz;
+    d.rp = g->row_ptr;
+    d.col = g->col;
+    d.vw = g->vtx_w;
+    if (g->w) {
+        d.w = g->w;
+        d.wk = W_I32;
+    } else if (g->w64) {
+        d.w = g->w64;
+        d.wk = W_I64;
+    }
+    return d;
+}
+
+static RefineData rd_of(const pggc_state* s) {
+    if (!s || !s->vol) throw Error(PGGC_EINVAL, "invalid state");
+    RefineData r;
+    r.vol = s->vol;
+    r.cap = s->vol_cap;
+    r.uncut = s->uncut;
+    r.top_nnz = s->top_nnz;
+    r.obj = s->obj;
+    r.label_count = s->label_count;
+    r.lambda = s->lambda;
+    r.scale = s->scale;
+    r.offset = s->offset;
+    return r;
+}
+static void rd_store(const RefineData& r, pggc_state* s) {
+    s->uncut = r.uncut;
+    s->top_nnz = r.top_nnz;
+    s->obj = r.obj;
+    s->label_count = r.label_count;
+    s->lambda = r.lambda;
+    s->scale = r.scale;
+    s->offset = r.offset;
+}
+
+#define CTX_CHECK(ctx) \
+    if (!(ctx)) throw Error(PGGC_EINVAL, "null context")
+
+extern "C" {
+
+int pggc_version(void) { return 100; }
+const char* pggc_last_error(void) { return g_err.c_str(); }
+
+int pggc_ctx_create(pggc_ctx** out, void* stream) {
+    return guard([&] {
+        if (!out) throw Error(PGGC_EINVAL, "null output");
+        std::unique_ptr<pggc_ctx> x(new pggc_ctx());
+        x->c.stream = (hipStream_t)stream;
+        PGGC_HIP(hipGetDevice(&x->c.device));
+        PGGC_HIP(hipHostMalloc((void**)&x->c.pinned, 64 * sizeof(i64), hipHostMallocDefault));
+        *out = x.release();
+    });
+}
+
+int pggc_ctx_destroy(pggc_ctx* ctx) {
+    return guard([&] {
+        if (!ctx) return;
+        (void)hipStreamSynchronize(ctx->c.stream);
+        ctx->ms = MoveScratch();
+        for (auto& kv : ctx->c.live) (void)hipFree(kv.first);
+        ctx->c.live.clear();
+        ctx->c.trim();
+        if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);
+        delete ctx;
+    });
+}
+
+int pggc_ctx_set_seed(pggc_ctx* ctx, uint64_t seed, uint64_t draws) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        ctx->c.seed = seed;
+        ctx->c.draws = draws;
+    });
+}
+int pggc_ctx_get_draws(pggc_ctx* ctx, uint64_t* draws) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        *draws = ctx->c.draws;
+    });
+}
+int pggc_ctx_memory(pggc_ctx* ctx, int64_t* live, int64_t* cached) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (live) *live = (int64_t)ctx->c.bytes_live;
+        if (cached) *cached = (int64_t)ctx->c.bytes_cached;
+    });
+}
+
+int pggc_vertex_weights(pggc_ctx* ctx, const pggc_graph* gg, int objective, int64_t* out) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        DGraph g = view_of(gg);
+        Ctx& c = ctx->c;
+        if (g.n == 0) return;
+        if (objective == PGGC_OBJ_MOD || (objective == PGGC_OBJ_WMOD && g.wk == W_UNIT)) {
+            k_deg<<<grid_for(g.n, 1024, 8192), 256, 0, c.stream>>>(g.rp, g.n, out);
+        } else if (objective == PGGC_OBJ_WMOD) {
+            PGGC_WDISPATCH(g, (k_wdeg<<<grid_for(g.n, 4, 1 << 16), 256, 0, c.stream>>>(g.rp, W, g.n, out)));
+        } else {
+            fill_i64(c, out, g.n, 1);
+        }
+        PGGC_LAUNCH_CHECK();
+        c.sync();
+    });
+}
+
+int pggc_state_init(pggc_ctx* ctx, const pggc_graph* gg, int objective, double mult, pggc_state* st) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        DGraph g = view_of(gg);
+        RefineData rd = rd_of(st);
+        if (rd.cap < g.n) throw Error(PGGC_ECAPACITY, "state volume capacity below n");
+        state_init(ctx->c, g, objective, mult, rd);
+        rd_store(rd, st);
+    });
+}
+
+int pggc_orderings(pggc_ctx* ctx, const pggc_graph* gg, int32_t* order1, int32_t* order2, int64_t* offs) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        DGraph g = view_of(gg);
+        Ctx& c = ctx->c;
+        Orders o;
+        make_orders(c, g, o);
+        if (g.n > 0) {
+            PGGC_HIP(hipMemcpyAsync(order1, o.o1, g.n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+            PGGC_HIP(hipMemcpyAsync(order2, o.o2, g.n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+        }
+        c.sync();
+        offs[0] = o.off_large;
+        offs[1] = o.off_large2;
+        offs[2] = o.off_mid;
+        offs[3] = o.off_mid2;
+    });
+}
+
+int pggc_evaluate(pggc_ctx* ctx, const pggc_graph* gg, const int32_t* labels, int64_t L, int64_t* vol_out,
+                  int64_t* uncut_out, uint64_t* sq_out) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        DGraph g = view_of(gg);
+        Ctx& c = ctx->c;
+        if (!g.vw) throw Error(PGGC_EINVAL, "vertex weights required");
+        Buf<i64> scal(c, 4);
+        evaluate(c, g, labels, L, vol_out, scal);
+        sum_sq_u128(c, vol_out, L, scal.p + 1);
+        const i64* h = c.read(scal, 3);
+        *uncut_out = h[0];
+        sq_out[0] = (u64)h[1];
+        sq_out[1] = (u64)h[2];
+    });
+}
+
+int pggc_own_weight(pggc_ctx* ctx, const pggc_graph* gg, const int32_t* labels, int64_t* out) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        DGraph g = view_of(gg);
+        own_weight(ctx->c, g, labels, out);
+        ctx->c.sync();
+    });
+}
+
+int pggc_contract(pggc_ctx* ctx, const pggc_graph* gg, const int32_t* vcmap, int64_t nc, int64_t* out_rp,
+                  int32_t* out_col, int64_t* out_w, int64_t cap, int64_t* nnz_out) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        DGraph g = view_of(gg);
+        if (nc < 0) throw Error(PGGC_EINVAL, "negative cluster count");
+        Ctx& c = ctx->c;
+        OwnedGraph og;
+        contract(c, g, vcmap, nc, og);
+        *nnz_out = og.nnz;
+        PGGC_HIP(hipMemcpyAsync(out_rp, og.rp, (nc + 1) * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
+        if (og.nnz > cap) {
+            c.sync();
+            throw Error(PGGC_ECAPACITY, "coarse nnz exceeds output capacity");
+        }
+        if (og.nnz > 0) {
+            PGGC_HIP(hipMemcpyAsync(out_col, og.col, og.nnz * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+            PGGC_HIP(hipMemcpyAsync(out_w, og.w, og.nnz * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
+        }
+        c.sync();
+    });
+}
+
+int pggc_sweep(pggc_ctx* ctx, const pggc_graph* gg, const int32_t* part, const int64_t* vol, int64_t L,
+               double lambda, float ratio, int raw, const int32_t* constraint, int32_t* dest, float* gain,
+               int32_t* moves_out, int64_t* nmoves, int64_t* ncand) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        DGraph g = view_of(gg);
+        LevelPlan lp;
+        *nmoves = sweep_once(ctx->c, ctx->ms, lp, g, part, vol, L, lambda, ratio, raw != 0, constraint, dest, gain,
+                             moves_out, ncand);
+    });
+}
+
+int pggc_local_move(pggc_ctx* ctx, const pggc_graph* gg, int32_t* part, pggc_state* st, int is_initial, int strict,
+                    int SA) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        DGraph g = view_of(gg);
+        RefineData rd = rd_of(st);
+        LevelPlan lp;
+        if (strict) local_move_strict(ctx->c, ctx->ms, lp, g, part, rd, is_initial != 0, false, nullptr, nullptr);
+        else local_move(ctx->c, ctx->ms, lp, g, part, rd, is_initial != 0, false, nullptr, SA != 0, nullptr);
+        ctx->c.sync();
+        rd_store(rd, st);
+    });
+}
+
+int pggc_leiden_refine(pggc_ctx* ctx, const pggc_graph* gg, const int32_t* louv, const int64_t* vol, int64_t L,
+                       double lambda, int uniform, uint64_t seed, uint64_t draw0, int32_t* vcmap_out,
+                       int64_t* nc_out) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        DGraph g = view_of(gg);
+        Ctx& c = ctx->c;
+        (void)L;
+        u64 s0 = c.seed, d0 = c.draws;
+        c.seed = seed;
+        c.draws = draw0;
+        Orders o;
+        make_orders(c, g, o);
+        Buf<i64> pv(c, std::max<i64>(g.n, 1));
+        own_weight(c, g, louv, pv);
+        *nc_out = leiden_refine(c, g, o, louv, vol, lambda, pv, uniform != 0, vcmap_out);
+        c.sync();
+        c.seed = s0;
+        c.draws = d0;
+    });
+}
+
+int pggc_louvain(pggc_ctx* ctx, const pggc_graph* gg, pggc_state* st, int constrained, int32_t* constraint,
+                 int32_t* part_out) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        DGraph g = view_of(gg);
+        RefineData rd = rd_of(st);
+        RunStats rs;
+        louvain_part(ctx->c, ctx->ms, g, rd, constrained != 0, constraint, part_out, rs, nullptr);
+        rd_store(rd, st);
+    });
+}
+
+int pggc_leiden(pggc_ctx* ctx, const pggc_graph* gg, pggc_state* st, int plus, int improve, const int32_t* input,
+                int32_t* part_out) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        DGraph g = view_of(gg);
+        RefineData rd = rd_of(st);
+        RunStats rs;
+        leiden_part(ctx->c, ctx->ms, g, rd, plus != 0, improve != 0, input, part_out, rs, nullptr);
+        rd_store(rd, st);
+    });
+}
+
+int pggc_cluster(pggc_ctx* ctx, const pggc_graph* gg, int algo, int objective, double mult, int ex_iters,
+                 uint64_t seed, int32_t* part_out, double* objective_out, int64_t* info) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        DGraph g = view_of(gg);
+        if (!g.vw) throw Error(PGGC_EINVAL, "vertex weights required");
+        // Mod and CPM treat edge weights as unit (app/objective_helpers.hpp:24-29)
+        if (objective == PGGC_OBJ_MOD || objective == PGGC_OBJ_CPM) {
+            g.w = nullptr;
+            g.wk = W_UNIT;
+        }
+        Ctx& c = ctx->c;
+        c.seed = seed;
+        c.draws = 0;
+        Buf<i64> vol(c, g.n + 1);
+        RefineData rd;
+        rd.vol = vol;
+        rd.cap = g.n;
+        state_init(c, g, objective, mult, rd);
+        RunStats rs;
+        LevelPlan top;
+        if (algo == PGGC_LOUVAIN) louvain_part(c, ctx->ms, g, rd, false, nullptr, part_out, rs, &top);
+        else leiden_part(c, ctx->ms, g, rd, algo == PGGC_LEIDEN_PLUS, false, nullptr, part_out, rs, &top);
+        double last = -std::numeric_limits<double>::infinity();
+        int counter = 0;
+        i64 outer = 1;
+        Buf<i32> cons(c, g.n + 1);
+        while (true) {  // iteration_termination_condition (app/cluster_main.cpp:101-114)
+            bool term = true;
+            if (ex_iters < 0) {
+                if (rd.objective() > last) term = false;
+                last = rd.objective();
+            } else if (counter++ < ex_iters) {
+                term = false;
+            }
+            if (term) break;
+            PGGC_HIP(hipMemcpyAsync(cons, part_out, g.n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+            if (algo == PGGC_LOUVAIN) louvain_part(c, ctx->ms, g, rd, true, cons, part_out, rs, &top);
+            else leiden_part(c, ctx->ms, g, rd, algo == PGGC_LEIDEN_PLUS, true, cons, part_out, rs, &top);
+            outer++;
+        }
+        c.sync();
+        *objective_out = rd.objective();
+        if (info) {
+            info[0] = rd.label_count;
+            info[1] = rd.uncut;
+            info[2] = rs.levels;
+            info[3] = rs.mv.sweeps;
+            info[4] = rs.mv.moves;
+            info[5] = outer;
+            info[6] = rs.mv.cands;
+            info[7] = rs.total_nnz;
+        }
+    });
+}
+
+}  // extern "C"

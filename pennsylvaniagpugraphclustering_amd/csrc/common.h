@@ -1,0 +1,148 @@
+// Shared definitions for the PGGC HIP library (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace pggc {
+
+using i32 = int32_t;
+using i64 = int64_t;
+using u32 = uint32_t;
+using u64 = uint64_t;
+
+// Sentinel labels of the local mover (src/local_mover.cpp:74-80).
+constexpr i32 NULL_PART = -1;
+constexpr i32 NO_MOVE = -3;
+constexpr i32 NEW_PART = -4;
+constexpr float AFTERBURNER_EPS = 0.1f;
+// Degree cutoffs of the ordering pass (src/header/ordering.h:17-19).
+constexpr i64 MID_CUTOFF = 64;
+constexpr i64 LARGE_CUTOFF = 128;
+constexpr i64 MASSIVE_CUTOFF = 15000;
+// leidenR constants (src/leidenR.cpp:38-39).
+constexpr i32 ORD_MAX = 0x7fffffff;
+constexpr i32 SPLIT = 1000000000;
+
+constexpr int WAVE = 64;
+
+// Error raised inside the library; converted to a status code at the C ABI.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define PGGC_HIP(call)                                                                         \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw ::pggc::Error(-3, std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+#define PGGC_LAUNCH_CHECK() PGGC_HIP(hipGetLastError())
+
+// Edge-weight accessors: unit (uniform graphs), int32 (input graphs) and
+// int64 (coarse graphs, whose summed weights can exceed int32).
+struct WUnit {
+    __device__ __forceinline__ i64 operator()(i64) const { return 1; }
+};
+struct W32 {
+    const i32* p;
+    __device__ __forceinline__ i64 operator()(i64 j) const { return p[j]; }
+};
+struct W64 {
+    const i64* p;
+    __device__ __forceinline__ i64 operator()(i64 j) const { return p[j]; }
+};
+
+enum WKind : int { W_UNIT = 0, W_I32 = 1, W_I64 = 2 };
+
+// One level of the multilevel hierarchy, resident in HBM.
+struct DGraph {
+    i64 n = 0;
+    i64 nnz = 0;
+    const i64* rp = nullptr;   // row offsets [n+1]
+    const i32* col = nullptr;  // column ids [nnz]
+    const void* w = nullptr;   // weights [nnz] (null when W_UNIT)
+    int wk = W_UNIT;
+    const i64* vw = nullptr;   // vertex weights [n]
+};
+
+#define PGGC_WDISPATCH(G, ...)                               \
+    do {                                                      \
+        if ((G).wk == ::pggc::W_UNIT) {                       \
+            ::pggc::WUnit W{};                                \
+            __VA_ARGS__;                                      \
+        } else if ((G).wk == ::pggc::W_I32) {                 \
+            ::pggc::W32 W{(const ::pggc::i32*)(G).w};         \
+            __VA_ARGS__;                                      \
+        } else {                                              \
+            ::pggc::W64 W{(const ::pggc::i64*)(G).w};         \
+            __VA_ARGS__;                                      \
+        }                                                     \
+    } while (0)
+
+// xorshift32 of the local mover (src/local_mover.cpp:82-87).
+__host__ __device__ __forceinline__ u32 xorshift32(u32 x) {
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return x;
+}
+
+// MurmurHash3_x86_32 of a 4-byte key with seed 0 (the hash behind the
+// reference's Kokkos::pod_hash<int32>, src/leidenR.cpp:37).
+__host__ __device__ __forceinline__ u32 murmur3_i32(i32 key) {
+    u32 k1 = (u32)key;
+    u32 h1 = 0;
+    k1 *= 0xcc9e2d51u;
+    k1 = (k1 << 15) | (k1 >> 17);
+    k1 *= 0x1b873593u;
+    h1 ^= k1;
+    h1 = (h1 << 13) | (h1 >> 19);
+    h1 = h1 * 5u + 0xe6546b64u;
+    h1 ^= 4u;
+    h1 ^= h1 >> 16;
+    h1 *= 0x85ebca6bu;
+    h1 ^= h1 >> 13;
+    h1 *= 0xc2b2ae35u;
+    h1 ^= h1 >> 16;
+    return h1;
+}
+
+// k-th seed draw of a run (stand-in for std::random_device in leidenR).
+__host__ __device__ __forceinline__ u32 seed_draw(u64 seed, u64 k) {
+    u64 z = seed + (k + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (u32)z;
+}
+
+// Monotone map float -> u32 (for packed argmax keys).
+__device__ __forceinline__ u32 ord_f32(float f) {
+    u32 b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float unord_f32(u32 o) {
+    u32 b = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    return __uint_as_float(b);
+}
+
+// Hash slot for a cluster id in a power-of-two table.
+__device__ __forceinline__ u32 slot_hash(i32 key, u32 mask) {
+    return ((u32)key * 0x9E3779B1u) >> 7 & mask;
+}
+
+inline i64 ceil_div(i64 a, i64 b) { return (a + b - 1) / b; }
+
+inline int grid_for(i64 work, int per_block, int cap = 1 << 20) {
+    i64 g = ceil_div(work, per_block);
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+}  // namespace pggc

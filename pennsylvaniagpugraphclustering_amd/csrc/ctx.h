@@ -1,0 +1,82 @@
+// Device context: stream, caching allocator, pinned readback slots, and the
+// primitive operations (scan, compaction, radix sort, reductions) shared by
+// the hot-path kernels.
+#pragma once
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "common.h"
+
+namespace pggc {
+
+struct Ctx {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::string last_error;
+    // caching allocator: blocks are kept for reuse until the context ends
+    std::multimap<size_t, void*> free_blocks;
+    std::unordered_map<void*, size_t> live;
+    size_t bytes_live = 0, bytes_cached = 0;
+    i64* pinned = nullptr;  // 64 pinned int64 readback slots
+    u64 seed = 0;           // leidenR seed stream
+    u64 draws = 0;
+
+    void* alloc(size_t bytes);
+    void release(void* p);
+    void trim();
+    void sync() { PGGC_HIP(hipStreamSynchronize(stream)); }
+    // copy `count` int64 device values to the pinned slots and wait
+    const i64* read(const i64* dev, int count);
+    i64 read1(const i64* dev) { return read(dev, 1)[0]; }
+    u32 next_draw() { return seed_draw(seed, draws++); }
+};
+
+// RAII device buffer from the context's caching allocator.
+template <class T>
+struct Buf {
+    Ctx* c = nullptr;
+    T* p = nullptr;
+    size_t n = 0;
+    Buf() = default;
+    Buf(Ctx& ctx, size_t count) : c(&ctx), n(count) { p = (T*)ctx.alloc(count * sizeof(T) + 16); }
+    Buf(const Buf&) = delete;
+    Buf& operator=(const Buf&) = delete;
+    Buf(Buf&& o) noexcept : c(o.c), p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    Buf& operator=(Buf&& o) noexcept {
+        if (this != &o) {
+            reset();
+            c = o.c; p = o.p; n = o.n;
+            o.p = nullptr; o.n = 0;
+        }
+        return *this;
+    }
+    ~Buf() { reset(); }
+    void reset() {
+        if (p && c) c->release(p);
+        p = nullptr;
+        n = 0;
+    }
+    T* get() const { return p; }
+    operator T*() const { return p; }
+};
+
+// ---- primitives (prims.hip) ----
+void fill_i32(Ctx& c, i32* p, i64 n, i32 v);
+void fill_i64(Ctx& c, i64* p, i64 n, i64 v);
+void iota_i32(Ctx& c, i32* p, i64 n);
+// out[0..n] = exclusive scan of in[0..n); out[n] = total.  in may alias out.
+void scan_i64(Ctx& c, const i64* in, i64* out, i64 n);
+void scan_i32_to_i64(Ctx& c, const i32* in, i64* out, i64 n);
+// Ordered compaction of the indices x in [0,n) with flag[x] != 0, writing
+// src ? src[x] : x; returns the count (read back to the host).
+i64 compact_flags(Ctx& c, const i32* flag, const i32* src, i64 n, i32* out, i64* count_dev);
+// Stable LSD radix sort of (key, val) pairs on the low `bits` key bits.
+void radix_sort_pairs(Ctx& c, u64* keys, i32* vals, i64 n, int bits);
+// sum of int64 values (device result in *out)
+void sum_i64(Ctx& c, const i64* in, i64 n, i64* out);
+// exact sum of squares of int64 values: out[0] = high word, out[1] = low word
+void sum_sq_u128(Ctx& c, const i64* in, i64 n, i64* out);
+
+}  // namespace pggc

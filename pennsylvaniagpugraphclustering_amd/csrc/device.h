@@ -1,0 +1,105 @@
+// Device-side building blocks: wave and block reductions and scans (wave64).
+#pragma once
+#include "common.h"
+
+namespace pggc {
+
+__device__ __forceinline__ i64 wave_sum_i64(i64 v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ u64 wave_max_u64(u64 v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        u64 y = __shfl_xor(v, o);
+        v = y > v ? y : v;
+    }
+    return v;
+}
+
+// group reductions over `G` consecutive lanes (G a power of two <= 64)
+template <int G>
+__device__ __forceinline__ i64 group_sum_i64(i64 v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+    return v;
+}
+template <int G>
+__device__ __forceinline__ u64 group_max_u64(u64 v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        u64 y = __shfl_xor(v, o, G);
+        v = y > v ? y : v;
+    }
+    return v;
+}
+template <int G>
+__device__ __forceinline__ i64 group_max_i64(i64 v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        i64 y = __shfl_xor(v, o, G);
+        v = y > v ? y : v;
+    }
+    return v;
+}
+
+// Sum over the whole block; every thread receives the total.
+__device__ __forceinline__ i64 block_sum_i64(i64 v) {
+    __shared__ i64 ws[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    v = wave_sum_i64(v);
+    if (lane == 0) ws[w] = v;
+    __syncthreads();
+    i64 t = 0;
+    for (int k = 0; k < nw; k++) t += ws[k];
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ u64 block_max_u64(u64 v) {
+    __shared__ u64 ws[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    v = wave_max_u64(v);
+    if (lane == 0) ws[w] = v;
+    __syncthreads();
+    u64 t = 0;
+    for (int k = 0; k < nw; k++) t = ws[k] > t ? ws[k] : t;
+    __syncthreads();
+    return t;
+}
+
+// Exclusive prefix sum over the block in thread order; *total = block sum.
+__device__ __forceinline__ i64 block_excl_scan_i64(i64 v, i64* total) {
+    __shared__ i64 ws[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    i64 x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        i64 y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) ws[w] = x;
+    __syncthreads();
+    i64 pre = 0, tot = 0;
+    for (int k = 0; k < nw; k++) {
+        if (k < w) pre += ws[k];
+        tot += ws[k];
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + x - v;
+}
+
+// Row lookup helper: largest r in [lo, hi) with rp[r] <= j.
+__device__ __forceinline__ i64 upper_row(const i64* rp, i64 lo, i64 hi, i64 j) {
+    while (hi - lo > 1) {
+        i64 m = (lo + hi) >> 1;
+        if (rp[m] <= j) lo = m;
+        else hi = m;
+    }
+    return lo;
+}
+
+}  // namespace pggc

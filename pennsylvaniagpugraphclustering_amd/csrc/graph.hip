@@ -1,0 +1,511 @@
+// Degree ordering, evaluator, own-cluster weights and contraction kernels.
+#include <algorithm>
+
+#include "device.h"
+#include "graph.h"
+#include "hash.h"
+
+namespace pggc {
+
+// ------------------------------------------------------ stable K-way partition
+constexpr int PT = 256, PI = 8, PTILE = PT * PI;
+
+template <int K>
+__global__ void __launch_bounds__(PT) k_part_count(const i32* cls, i64 n, i64* counts, i64 nb) {
+    i64 base = (i64)blockIdx.x * PTILE + (i64)threadIdx.x * PI;
+    i64 c[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) c[k] = 0;
+#pragma unroll
+    for (int q = 0; q < PI; q++) {
+        i64 x = base + q;
+        if (x < n) {
+            int k = cls[x];
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) c[kk] += (k == kk);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        i64 s = block_sum_i64(c[k]);
+        if (threadIdx.x == 0) counts[(i64)k * nb + blockIdx.x] = s;
+    }
+}
+
+template <int K>
+__global__ void __launch_bounds__(PT) k_part_write(const i32* cls, i64 n, const i64* offs, i64 nb, i32* out) {
+    i64 base = (i64)blockIdx.x * PTILE + (i64)threadIdx.x * PI;
+    int ck[PI];
+    i64 c[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) c[k] = 0;
+#pragma unroll
+    for (int q = 0; q < PI; q++) {
+        i64 x = base + q;
+        ck[q] = x < n ? cls[x] : -1;
+#pragma unroll
+        for (int kk = 0; kk < K; kk++) c[kk] += (ck[q] == kk);
+    }
+    i64 pos[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        i64 tot;
+        pos[k] = block_excl_scan_i64(c[k], &tot) + offs[(i64)k * nb + blockIdx.x];
+    }
+#pragma unroll
+    for (int q = 0; q < PI; q++) {
+#pragma unroll
+        for (int kk = 0; kk < K; kk++)
+            if (ck[q] == kk) out[pos[kk]++] = (i32)(base + q);
+    }
+}
+
+__global__ void k_gather_starts(const i64* offs, i64 nb, int K, i64* starts) {
+    int k = threadIdx.x;
+    if (k <= K) starts[k] = offs[(i64)k * nb];
+}
+
+template <int K>
+static void partition_k(Ctx& c, const i32* cls, i64 n, i32* out, i64* starts) {
+    i64 nb = ceil_div(std::max<i64>(n, 1), PTILE);
+    Buf<i64> counts(c, K * nb + 1);
+    k_part_count<K><<<(int)nb, PT, 0, c.stream>>>(cls, n, counts, nb);
+    PGGC_LAUNCH_CHECK();
+    scan_i64(c, counts, counts, K * nb);
+    k_part_write<K><<<(int)nb, PT, 0, c.stream>>>(cls, n, counts, nb, out);
+    PGGC_LAUNCH_CHECK();
+    Buf<i64> st(c, K + 1);
+    k_gather_starts<<<1, 64, 0, c.stream>>>(counts, nb, K, st);
+    PGGC_LAUNCH_CHECK();
+    const i64* h = c.read(st, K + 1);
+    for (int k = 0; k <= K; k++) starts[k] = h[k];
+}
+
+void partition_classes(Ctx& c, const i32* cls, i64 n, int K, i32* out, i64* starts) {
+    switch (K) {
+        case 2: partition_k<2>(c, cls, n, out, starts); break;
+        case 3: partition_k<3>(c, cls, n, out, starts); break;
+        case 4: partition_k<4>(c, cls, n, out, starts); break;
+        case 8: partition_k<8>(c, cls, n, out, starts); break;
+        default: throw Error(-1, "partition_classes: unsupported class count");
+    }
+}
+
+// ------------------------------------------------------------------ orderings
+__global__ void k_order_classes(const i64* rp, i64 n, i32* c1, i32* c2) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        i64 d = rp[i + 1] - rp[i];
+        c1[i] = d < LARGE_CUTOFF ? 0 : (d < MASSIVE_CUTOFF ? 1 : 2);
+        c2[i] = d < MID_CUTOFF ? 0 : (d < MASSIVE_CUTOFF ? 1 : 2);
+    }
+}
+
+void make_orders(Ctx& c, const DGraph& g, Orders& o) {
+    i64 n = g.n;
+    if (o.o1.n < (size_t)n) {
+        o.o1 = Buf<i32>(c, std::max<i64>(n, 1));
+        o.o2 = Buf<i32>(c, std::max<i64>(n, 1));
+    }
+    Buf<i32> c1(c, std::max<i64>(n, 1)), c2(c, std::max<i64>(n, 1));
+    k_order_classes<<<grid_for(n, 256 * 4, 8192), 256, 0, c.stream>>>(g.rp, n, c1, c2);
+    PGGC_LAUNCH_CHECK();
+    i64 s1[4], s2[4];
+    partition_k<3>(c, c1, n, o.o1, s1);
+    partition_k<3>(c, c2, n, o.o2, s2);
+    o.off_large = s1[1];
+    o.off_large2 = s1[2];
+    o.off_mid = s2[1];
+    o.off_mid2 = s2[2];
+}
+
+// ------------------------------------------------- edge-chunk row reductions
+// Each block owns a contiguous range of EC edges; each thread owns EPT
+// consecutive edges and walks their rows with a monotone row pointer.
+constexpr int EB = 256, EPT = 16, EC = EB * EPT;
+
+struct ChunkRows {
+    i64 eb, ee, rlo, rhi;
+};
+__device__ __forceinline__ void chunk_rows(const i64* rp, i64 n, i64 nnz, ChunkRows& cr) {
+    __shared__ i64 s_lo, s_hi;
+    cr.eb = (i64)blockIdx.x * EC;
+    cr.ee = cr.eb + EC < nnz ? cr.eb + EC : nnz;
+    if (threadIdx.x == 0) {
+        s_lo = upper_row(rp, 0, n, cr.eb);
+        s_hi = upper_row(rp, 0, n, cr.ee - 1) + 1;
+    }
+    __syncthreads();
+    cr.rlo = s_lo;
+    cr.rhi = s_hi;
+}
+
+template <class WT>
+__global__ void __launch_bounds__(EB) k_eval_uncut(const i64* rp, const i32* col, WT W, const i32* lab, i64 n,
+                                                    i64 nnz, i64* uncut) {
+    ChunkRows cr;
+    chunk_rows(rp, n, nnz, cr);
+    i64 j0 = cr.eb + (i64)threadIdx.x * EPT;
+    i64 acc = 0;
+    if (j0 < cr.ee) {
+        i64 r = upper_row(rp, cr.rlo, cr.rhi, j0);
+        i64 nxt = rp[r + 1];
+        i32 lr = lab[r];
+        for (int k = 0; k < EPT; k++) {
+            i64 j = j0 + k;
+            if (j >= cr.ee) break;
+            while (j >= nxt) {
+                r++;
+                nxt = rp[r + 1];
+                lr = lab[r];
+            }
+            if (lab[col[j]] == lr) acc += W(j);
+        }
+    }
+    acc = block_sum_i64(acc);
+    if (threadIdx.x == 0 && acc) atomicAdd((unsigned long long*)uncut, (unsigned long long)acc);
+}
+
+__global__ void k_eval_vol(const i32* lab, const i64* vw, i64 n, i64* vol) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        atomicAdd((unsigned long long*)&vol[lab[i]], (unsigned long long)vw[i]);
+}
+
+void evaluate(Ctx& c, const DGraph& g, const i32* labels, i64 L, i64* vol, i64* dev_uncut) {
+    fill_i64(c, vol, L, 0);
+    fill_i64(c, dev_uncut, 1, 0);
+    if (g.nnz > 0) {
+        PGGC_WDISPATCH(g, (k_eval_uncut<<<(int)ceil_div(g.nnz, EC), EB, 0, c.stream>>>(g.rp, g.col, W, labels, g.n,
+                                                                                       g.nnz, dev_uncut)));
+        PGGC_LAUNCH_CHECK();
+    }
+    if (g.n > 0) {
+        k_eval_vol<<<grid_for(g.n, 256 * 4, 8192), 256, 0, c.stream>>>(labels, g.vw, g.n, vol);
+        PGGC_LAUNCH_CHECK();
+    }
+}
+
+template <class WT>
+__global__ void __launch_bounds__(EB) k_own_weight(const i64* rp, const i32* col, WT W, const i32* lab, i64 n,
+                                                    i64 nnz, i64* out) {
+    ChunkRows cr;
+    chunk_rows(rp, n, nnz, cr);
+    i64 j0 = cr.eb + (i64)threadIdx.x * EPT;
+    if (j0 >= cr.ee) return;
+    i64 r = upper_row(rp, cr.rlo, cr.rhi, j0);
+    i64 nxt = rp[r + 1];
+    i32 lr = lab[r];
+    i64 acc = 0;
+    for (int k = 0; k < EPT; k++) {
+        i64 j = j0 + k;
+        if (j >= cr.ee) break;
+        while (j >= nxt) {
+            if (acc) atomicAdd((unsigned long long*)&out[r], (unsigned long long)acc);
+            acc = 0;
+            r++;
+            nxt = rp[r + 1];
+            lr = lab[r];
+        }
+        if (lab[col[j]] == lr) acc += W(j);
+    }
+    if (acc) atomicAdd((unsigned long long*)&out[r], (unsigned long long)acc);
+}
+
+void own_weight(Ctx& c, const DGraph& g, const i32* labels, i64* out) {
+    fill_i64(c, out, g.n, 0);
+    if (g.nnz == 0) return;
+    PGGC_WDISPATCH(g, (k_own_weight<<<(int)ceil_div(g.nnz, EC), EB, 0, c.stream>>>(g.rp, g.col, W, labels, g.n,
+                                                                                   g.nnz, out)));
+    PGGC_LAUNCH_CHECK();
+}
+
+__global__ void k_coarsen_vw(const i32* map, const i64* vw, i64 n, i64* out) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        atomicAdd((unsigned long long*)&out[map[i]], (unsigned long long)vw[i]);
+}
+
+void coarsen_vertex_weights(Ctx& c, const DGraph& g, const i32* map, i64 nc, i64* out) {
+    fill_i64(c, out, nc, 0);
+    if (g.n == 0) return;
+    k_coarsen_vw<<<grid_for(g.n, 256 * 4, 8192), 256, 0, c.stream>>>(map, g.vw, g.n, out);
+    PGGC_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- contraction
+__global__ void k_ccount(const i64* rp, const i32* vcmap, i64 n, i64* cnt, i64* bound) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        i32 u = vcmap[i];
+        atomicAdd((unsigned long long*)&cnt[u], 1ull);
+        atomicAdd((unsigned long long*)&bound[u], (unsigned long long)(rp[i + 1] - rp[i]));
+    }
+}
+
+__global__ void k_cscatter(const i32* vcmap, i64 n, i64* pos, i32* members) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        i32 u = vcmap[i];
+        i64 k = (i64)atomicAdd((unsigned long long*)&pos[u], 1ull);
+        members[k] = (i32)i;
+    }
+}
+
+constexpr i64 C_SMALL = 256;   // bound <= C_SMALL: 64-thread block, 512-slot LDS table
+constexpr i64 C_MID = 2048;    // bound <= C_MID: 256-thread block, 4096-slot LDS table
+
+__global__ void k_cclass(const i64* bound, i64 nc, i32* cls) {
+    for (i64 u = (i64)blockIdx.x * blockDim.x + threadIdx.x; u < nc; u += (i64)gridDim.x * blockDim.x) {
+        i64 b = bound[u];
+        cls[u] = b == 0 ? 3 : (b <= C_SMALL ? 0 : (b <= C_MID ? 1 : 2));
+    }
+}
+
+__host__ __device__ __forceinline__ i64 table_slots(i64 bound, i64 nc) {
+    i64 cap = bound < nc ? bound : nc;
+    i64 s = 2;
+    while (s < 2 * cap) s <<= 1;
+    return s;
+}
+
+__global__ void k_goff_sizes(const i32* list, i64 m, const i64* bound, i64 nc, i64* sz) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (i64)gridDim.x * blockDim.x)
+        sz[x] = table_slots(bound[list[x]], nc);
+}
+
+// One block per coarse vertex: hash-merge the members' edges by coarse
+// neighbour, then write the distinct (neighbour, weight) pairs to the
+// temporary row at toff[u] (sorted when the table lives in LDS).
+template <int BS, int SLOTS, bool GLOBAL, class WT>
+__global__ void __launch_bounds__(BS) k_cagg(const i64* rp, const i32* col, WT W, const i32* vcmap, i64 nc,
+                                              const i32* list, i64 nlist, const i64* mstart, const i32* members,
+                                              const i64* bound, i32* gkeys, i64* gvals, const i64* goff,
+                                              const i64* toff, i32* tcol, i64* tw, i64* ccnt) {
+    __shared__ i32 skeys[GLOBAL ? 1 : SLOTS];
+    __shared__ i64 svals[GLOBAL ? 1 : SLOTS];
+    __shared__ i64 pre[BS];
+    __shared__ i64 rs[BS];
+    __shared__ int s_cnt;
+    const int t = threadIdx.x;
+    for (i64 li = blockIdx.x; li < nlist; li += gridDim.x) {
+        const i32 u = list[li];
+        const i64 S = table_slots(bound[u], nc);
+        const u32 mask = (u32)(S - 1);
+        i32* keys = GLOBAL ? gkeys + goff[li] : skeys;
+        i64* vals = GLOBAL ? gvals + goff[li] : svals;
+        for (i64 s = t; s < S; s += BS) {
+            keys[s] = EMPTY_KEY;
+            vals[s] = 0;
+        }
+        if (t == 0) s_cnt = 0;
+        __syncthreads();
+        const i64 m0 = mstart[u], m1 = mstart[u + 1];
+        for (i64 mb = m0; mb < m1; mb += BS) {
+            i64 mi = mb + t;
+            i64 d = 0, st = 0;
+            if (mi < m1) {
+                i32 v = members[mi];
+                st = rp[v];
+                d = rp[v + 1] - st;
+            }
+            i64 tot;
+            i64 ex = block_excl_scan_i64(d, &tot);
+            pre[t] = ex;
+            rs[t] = st;
+            __syncthreads();
+            for (i64 e = t; e < tot; e += BS) {
+                int lo = 0, hi = BS;
+                while (hi - lo > 1) {
+                    int md = (lo + hi) >> 1;
+                    if (pre[md] <= e) lo = md;
+                    else hi = md;
+                }
+                i64 j = rs[lo] + (e - pre[lo]);
+                i32 v = vcmap[col[j]];
+                if (v != u) {
+                    if (GLOBAL) table_add_global(keys, vals, mask, v, W(j));
+                    else table_add_lds(keys, vals, mask, v, W(j));
+                }
+            }
+            __syncthreads();
+        }
+        if (!GLOBAL) {
+            for (i64 s = t; s < S; s += BS)
+                if (keys[s] == EMPTY_KEY) keys[s] = 0x7fffffff;
+            __syncthreads();
+            bitonic_sort_lds<BS>(keys, vals, (int)S);
+            int c = 0;
+            for (i64 s = t; s < S; s += BS) c += keys[s] != 0x7fffffff;
+            c = (int)block_sum_i64(c);
+            i64 base = toff[u];
+            for (i64 s = t; s < c; s += BS) {
+                tcol[base + s] = keys[s];
+                tw[base + s] = vals[s];
+            }
+            if (t == 0) ccnt[u] = c;
+        } else {
+            __threadfence();
+            __syncthreads();
+            i64 base = toff[u];
+            for (i64 s = t; s < S; s += BS) {
+                i32 k = __hip_atomic_load(&keys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (k != EMPTY_KEY) {
+                    int p = atomicAdd(&s_cnt, 1);
+                    tcol[base + p] = k;
+                    tw[base + p] = __hip_atomic_load(&vals[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+            __syncthreads();
+            if (t == 0) ccnt[u] = s_cnt;
+        }
+        __syncthreads();
+    }
+}
+
+// Copy temporary rows into the compact coarse CSR.
+__global__ void __launch_bounds__(EB) k_crow_copy(const i64* crp, i64 nc, i64 nnzc, const i64* toff, const i32* tcol,
+                                                   const i64* tw, i32* ocol, i64* ow) {
+    ChunkRows cr;
+    chunk_rows(crp, nc, nnzc, cr);
+    i64 j0 = cr.eb + (i64)threadIdx.x * EPT;
+    if (j0 >= cr.ee) return;
+    i64 r = upper_row(crp, cr.rlo, cr.rhi, j0);
+    i64 nxt = crp[r + 1];
+    for (int k = 0; k < EPT; k++) {
+        i64 j = j0 + k;
+        if (j >= cr.ee) break;
+        while (j >= nxt) {
+            r++;
+            nxt = crp[r + 1];
+        }
+        i64 src = toff[r] + (j - crp[r]);
+        ocol[j] = tcol[src];
+        ow[j] = tw[src];
+    }
+}
+
+// Sorting of rows merged in global tables: flatten, radix sort by
+// (row rank, column), scatter back.
+__global__ void k_big_count(const i32* list, i64 m, const i64* crp, i64* cnt) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (i64)gridDim.x * blockDim.x) {
+        i32 u = list[x];
+        cnt[x] = crp[u + 1] - crp[u];
+    }
+}
+__global__ void k_big_flatten(const i32* list, i64 m, const i64* crp, const i64* boff, const i32* ocol, u64* key,
+                              i32* val, i64* target) {
+    for (i64 x = blockIdx.x; x < m; x += gridDim.x) {
+        i32 u = list[x];
+        i64 s = crp[u], e = crp[u + 1], b = boff[x];
+        for (i64 j = s + threadIdx.x; j < e; j += blockDim.x) {
+            i64 f = b + (j - s);
+            key[f] = ((u64)x << 32) | (u32)ocol[j];
+            val[f] = (i32)f;
+            target[f] = j;
+        }
+    }
+}
+__global__ void k_big_gather(const u64* key, const i32* val, const i64* target, i64 E, const i64* ow_in,
+                             i64* wtmp) {
+    for (i64 f = (i64)blockIdx.x * blockDim.x + threadIdx.x; f < E; f += (i64)gridDim.x * blockDim.x)
+        wtmp[f] = ow_in[target[val[f]]];
+}
+__global__ void k_big_scatter(const u64* key, const i64* target, i64 E, const i64* wtmp, i32* ocol, i64* ow) {
+    for (i64 f = (i64)blockIdx.x * blockDim.x + threadIdx.x; f < E; f += (i64)gridDim.x * blockDim.x) {
+        i64 j = target[f];
+        ocol[j] = (i32)(key[f] & 0xffffffffu);
+        ow[j] = wtmp[f];
+    }
+}
+
+static int bits_for(i64 v) {
+    int b = 0;
+    while (b < 62 && ((i64)1 << b) <= v) b++;
+    return b;
+}
+
+void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out) {
+    const i64 n = g.n;
+    Buf<i64> cnt(c, nc + 1), bound(c, nc + 1), mstart(c, nc + 1), toff(c, nc + 1), pos(c, nc + 1);
+    Buf<i32> members(c, std::max<i64>(n, 1));
+    fill_i64(c, cnt, nc + 1, 0);
+    fill_i64(c, bound, nc + 1, 0);
+    k_ccount<<<grid_for(n, 256 * 4, 8192), 256, 0, c.stream>>>(g.rp, vcmap, n, cnt, bound);
+    PGGC_LAUNCH_CHECK();
+    scan_i64(c, cnt, mstart, nc);
+    scan_i64(c, bound, toff, nc);
+    PGGC_HIP(hipMemcpyAsync(pos, mstart, nc * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
+    k_cscatter<<<grid_for(n, 256 * 4, 8192), 256, 0, c.stream>>>(vcmap, n, pos, members);
+    PGGC_LAUNCH_CHECK();
+    const i64 tb = c.read1(toff.p + nc);
+    Buf<i32> tcol(c, std::max<i64>(tb, 1));
+    Buf<i64> tw(c, std::max<i64>(tb, 1));
+    Buf<i64> ccnt(c, nc + 1);
+    Buf<i32> cls(c, std::max<i64>(nc, 1)), list(c, std::max<i64>(nc, 1));
+    k_cclass<<<grid_for(nc, 256 * 4, 8192), 256, 0, c.stream>>>(bound, nc, cls);
+    PGGC_LAUNCH_CHECK();
+    i64 st[5];
+    partition_k<4>(c, cls, nc, list, st);
+    fill_i64(c, ccnt, nc + 1, 0);
+    const i64 n0 = st[1] - st[0], n1 = st[2] - st[1], n2 = st[3] - st[2];
+    Buf<i64> goff;
+    Buf<i32> gkeys;
+    Buf<i64> gvals;
+    if (n2 > 0) {
+        goff = Buf<i64>(c, n2 + 1);
+        k_goff_sizes<<<grid_for(n2, 256, 4096), 256, 0, c.stream>>>(list.p + st[2], n2, bound, nc, goff);
+        PGGC_LAUNCH_CHECK();
+        scan_i64(c, goff, goff, n2);
+        i64 gs = c.read1(goff.p + n2);
+        gkeys = Buf<i32>(c, gs);
+        gvals = Buf<i64>(c, gs);
+    }
+    PGGC_WDISPATCH(g, {
+        if (n0 > 0)
+            k_cagg<64, 512, false><<<grid_for(n0, 1, 1 << 20), 64, 0, c.stream>>>(
+                g.rp, g.col, W, vcmap, nc, list.p + st[0], n0, mstart, members, bound, nullptr, nullptr, nullptr,
+                toff, tcol, tw, ccnt);
+        if (n1 > 0)
+            k_cagg<256, 4096, false><<<grid_for(n1, 1, 1 << 20), 256, 0, c.stream>>>(
+                g.rp, g.col, W, vcmap, nc, list.p + st[1], n1, mstart, members, bound, nullptr, nullptr, nullptr,
+                toff, tcol, tw, ccnt);
+        if (n2 > 0)
+            k_cagg<256, 1, true><<<grid_for(n2, 1, 1 << 16), 256, 0, c.stream>>>(
+                g.rp, g.col, W, vcmap, nc, list.p + st[2], n2, mstart, members, bound, gkeys, gvals, goff, toff,
+                tcol, tw, ccnt);
+    });
+    PGGC_LAUNCH_CHECK();
+    gkeys.reset();
+    gvals.reset();
+    out.n = nc;
+    out.rp = Buf<i64>(c, nc + 1);
+    scan_i64(c, ccnt, out.rp, nc);
+    out.nnz = c.read1(out.rp.p + nc);
+    out.col = Buf<i32>(c, std::max<i64>(out.nnz, 1));
+    out.w = Buf<i64>(c, std::max<i64>(out.nnz, 1));
+    if (out.nnz > 0) {
+        k_crow_copy<<<(int)ceil_div(out.nnz, EC), EB, 0, c.stream>>>(out.rp, nc, out.nnz, toff, tcol, tw, out.col,
+                                                                   out.w);
+        PGGC_LAUNCH_CHECK();
+    }
+    if (n2 > 0) {
+        const i32* big = list.p + st[2];
+        Buf<i64> boff(c, n2 + 1);
+        k_big_count<<<grid_for(n2, 256, 4096), 256, 0, c.stream>>>(big, n2, out.rp, boff);
+        PGGC_LAUNCH_CHECK();
+        scan_i64(c, boff, boff, n2);
+        i64 E = c.read1(boff.p + n2);
+        if (E > 0) {
+            Buf<u64> key(c, E);
+            Buf<i32> val(c, E);
+            Buf<i64> target(c, E), wtmp(c, E);
+            k_big_flatten<<<grid_for(n2, 1, 65536), 256, 0, c.stream>>>(big, n2, out.rp, boff, out.col, key, val,
+                                                                         target);
+            PGGC_LAUNCH_CHECK();
+            radix_sort_pairs(c, key, val, E, 32 + bits_for(n2));
+            k_big_gather<<<grid_for(E, 256 * 4, 8192), 256, 0, c.stream>>>(key, val, target, E, out.w, wtmp);
+            PGGC_LAUNCH_CHECK();
+            k_big_scatter<<<grid_for(E, 256 * 4, 8192), 256, 0, c.stream>>>(key, target, E, wtmp, out.col, out.w);
+            PGGC_LAUNCH_CHECK();
+        }
+    }
+    (void)n1;
+}
+
+}  // namespace pggc

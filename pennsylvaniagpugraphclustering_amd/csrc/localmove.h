@@ -1,0 +1,70 @@
+// Local mover (src/local_mover.cpp) on one level graph.
+#pragma once
+#include "graph.h"
+
+namespace pggc {
+
+// Objective state: the cluster_data mirror (interface/cluster_data.hpp:18-88).
+struct RefineData {
+    i64* vol = nullptr;   // total_deg, device, capacity `cap` labels
+    i64 cap = 0;
+    i64 uncut = 0;
+    i64 top_nnz = 0;
+    double obj = -1.0;
+    i32 label_count = 0;
+    double lambda = 1.0;
+    double scale = 1.0;   // get_objective() = obj * scale + offset
+    double offset = 0.0;
+    double objective() const { return obj * scale + offset; }
+};
+
+// obj = uncut - lambda * sum(vol^2) with the exact 128-bit sum of squares
+// given as (hi, lo) words (interface/cluster_data.hpp:76-87).
+double objective_value(i64 uncut, double lambda, u64 sq_hi, u64 sq_lo);
+// recompute rd.obj from the device volumes
+void update_objective(Ctx& c, RefineData& rd);
+
+struct MoveStats {
+    i64 sweeps = 0, moves = 0, cands = 0;
+};
+
+// Per-level cached work partitioning (reused by every local_move on a level).
+struct LevelPlan {
+    const void* key = nullptr;  // graph identity (row offsets pointer)
+    i64 n = -1;
+    Orders ord;
+    Buf<i32> sc_list;           // vertices grouped by scoring class
+    i64 sc_start[9] = {0};
+    Buf<i64> goff;              // global-table offsets for the largest class
+    i64 gslots = 0;
+};
+
+// Device scratch sized for the top level, shared by every level.
+struct MoveScratch {
+    i64 cap = 0;
+    Buf<i32> part, dest, mflag, cand, moves, flag, tmp, tmp2;
+    Buf<float> gain;
+    Buf<i64> prio, scal;
+    Buf<i32> gkeys;
+    Buf<i64> gvals;
+    i64 gcap = 0;
+    void ensure(Ctx& c, i64 n);
+    void ensure_global(Ctx& c, i64 slots);
+};
+
+void plan_level(Ctx& c, const DGraph& g, LevelPlan& p);
+
+// local_move<constrained> (src/local_mover.cpp:1360-1417) and
+// local_move_strict (:1419-1483).  best_part and cons are device arrays.
+void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* best_part, RefineData& best,
+                bool is_initial, bool constrained, i32* cons, bool SA, MoveStats* st);
+void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* best_part, RefineData& best,
+                       bool is_initial, bool constrained, i32* cons, MoveStats* st);
+
+// One scoring + afterburner sweep from a fixed state (parity entry point).
+// Returns the number of accepted moves written to moves_out (order1 order).
+i64 sweep_once(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32* part, const i64* vol, i64 L,
+               double lambda, float ratio, bool raw, const i32* cons, i32* dest, float* gain, i32* moves_out,
+               i64* ncand);
+
+}  // namespace pggc

@@ -1,0 +1,801 @@
+// Local mover: degree-classed neighbour-cluster aggregation with gain argmax,
+// the afterburner conflict filter, move application with exact uncut deltas,
+// the strict prefix filter, relabelling, and the local_move drivers.
+//
+// Determinism (shared with the CPU oracle):
+//  * destination = max gain, ties -> smallest label; NEW_PART (-4) competes
+//    with j_conn 0 when p_conn < 0 (team-path rule, src/local_mover.cpp:553-561)
+//  * afterburner FP32 sums: degree < 128 sequential in adjacency order,
+//    128..14999 64 strided partials + halving tree, >= 15000 1024 strided
+//    partials + halving tree
+//  * uncut is updated by the exact integer delta of the moved vertices
+#include <algorithm>
+#include <cfloat>
+
+#include "device.h"
+#include "hash.h"
+#include "localmove.h"
+
+namespace pggc {
+
+// ------------------------------------------------------------- objective
+double objective_value(i64 uncut, double lambda, u64 sq_hi, u64 sq_lo) {
+    unsigned __int128 sq = ((unsigned __int128)sq_hi << 64) | (unsigned __int128)sq_lo;
+    double m = (double)uncut;
+    m -= lambda * (double)sq;
+    return m;
+}
+
+void update_objective(Ctx& c, RefineData& rd) {
+    Buf<i64> sq(c, 2);
+    sum_sq_u128(c, rd.vol, rd.label_count, sq);
+    const i64* h = c.read(sq, 2);
+    rd.obj = objective_value(rd.uncut, rd.lambda, (u64)h[0], (u64)h[1]);
+}
+
+// ------------------------------------------------------------- scratch
+void MoveScratch::ensure(Ctx& c, i64 n) {
+    if (cap >= n) return;
+    cap = std::max<i64>(n, 1);
+    part = Buf<i32>(c, cap);
+    dest = Buf<i32>(c, cap);
+    mflag = Buf<i32>(c, cap);
+    cand = Buf<i32>(c, cap);
+    moves = Buf<i32>(c, cap);
+    flag = Buf<i32>(c, cap);
+    tmp = Buf<i32>(c, cap);
+    tmp2 = Buf<i32>(c, cap + 1);
+    gain = Buf<float>(c, cap);
+    prio = Buf<i64>(c, cap + 1);
+    scal = Buf<i64>(c, 16);
+    fill_i32(c, mflag, cap, 0);
+}
+
+void MoveScratch::ensure_global(Ctx& c, i64 slots) {
+    if (gcap >= slots) return;
+    gcap = slots;
+    gkeys = Buf<i32>(c, slots);
+    gvals = Buf<i64>(c, slots);
+}
+
+// ------------------------------------------------------------- level plan
+constexpr int NCLS = 8;
+__host__ __device__ __forceinline__ int score_class(i64 d) {
+    return d <= 4 ? 0 : d <= 8 ? 1 : d <= 16 ? 2 : d <= 32 ? 3 : d <= 64 ? 4 : d <= 512 ? 5 : d <= 4096 ? 6 : 7;
+}
+__host__ __device__ __forceinline__ i64 tslots(i64 d) {
+    i64 s = 2;
+    while (s < 2 * d) s <<= 1;
+    return s;
+}
+
+__global__ void k_score_classes(const i64* rp, i64 n, i32* cls) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        cls[i] = score_class(rp[i + 1] - rp[i]);
+}
+__global__ void k_global_sizes(const i64* rp, const i32* list, i64 m, i64* sz) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (i64)gridDim.x * blockDim.x) {
+        i32 i = list[x];
+        sz[x] = tslots(rp[i + 1] - rp[i]);
+    }
+}
+
+void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
+    if (p.key == (const void*)g.rp && p.n == g.n) return;
+    p.key = g.rp;
+    p.n = g.n;
+    make_orders(c, g, p.ord);
+    Buf<i32> cls(c, std::max<i64>(g.n, 1));
+    p.sc_list = Buf<i32>(c, std::max<i64>(g.n, 1));
+    k_score_classes<<<grid_for(g.n, 1024, 8192), 256, 0, c.stream>>>(g.rp, g.n, cls);
+    PGGC_LAUNCH_CHECK();
+    partition_classes(c, cls, g.n, NCLS, p.sc_list, p.sc_start);
+    i64 m7 = p.sc_start[8] - p.sc_start[7];
+    p.gslots = 0;
+    if (m7 > 0) {
+        p.goff = Buf<i64>(c, m7 + 1);
+        k_global_sizes<<<grid_for(m7, 256, 4096), 256, 0, c.stream>>>(g.rp, p.sc_list.p + p.sc_start[7], m7, p.goff);
+        PGGC_LAUNCH_CHECK();
+        scan_i64(c, p.goff, p.goff, m7);
+        p.gslots = c.read1(p.goff.p + m7);
+    }
+}
+
+// ------------------------------------------------------------- scoring
+__device__ __forceinline__ u64 pack_cand(float jc, i32 key) {
+    return ((u64)ord_f32(jc) << 32) | (u64)(0xFFFFFFFFu - (u32)(key + 4));
+}
+
+__device__ __forceinline__ void decide(i32 i, float p_conn, u64 bk, i32* dest, float* gain) {
+    bool have = bk != 0;
+    float bj = have ? unord_f32((u32)(bk >> 32)) : 0.0f;
+    i32 bc = have ? (i32)(0xFFFFFFFFu - (u32)(bk & 0xffffffffull)) - 4 : 0;
+    if (p_conn < 0.0f && (!have || !(bj > 0.0f))) {
+        gain[i] = 0.0f - p_conn;
+        dest[i] = NEW_PART;
+    } else if (have) {
+        gain[i] = bj - p_conn;
+        dest[i] = bc;
+    } else {
+        gain[i] = -FLT_MAX;
+        dest[i] = NO_MOVE;
+    }
+}
+
+// p_conn and the filter threshold (src/local_mover.cpp:481-487)
+__device__ __forceinline__ void own_terms(i64 pv, i64 volp, i64 wv, float lam, float ratio, float& multi,
+                                          float& p_conn, float& b) {
+    multi = (float)wv * lam;
+    p_conn = fmaf(-(float)(volp - wv), multi, (float)pv);
+    b = (float)((double)p_conn * (1.0 - (double)ratio));
+    if (p_conn < 0.0f) b = 0.0f;
+}
+
+// G lanes per vertex, one edge per lane (degree <= G): duplicate clusters
+// are merged with in-group shuffles.
+template <int G, class WT>
+__global__ void __launch_bounds__(256) k_score_group(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
+                                                     const i64* __restrict__ vw, const i32* __restrict__ list, i64 m,
+                                                     const i32* __restrict__ part, const i64* __restrict__ vol,
+                                                     const i32* __restrict__ cons, int raw, float lam, float ratio,
+                                                     i32* __restrict__ dest, float* __restrict__ gain) {
+    constexpr int GPB = 256 / G;
+    const int gl = threadIdx.x & (G - 1);
+    for (i64 gi = (i64)blockIdx.x * GPB + threadIdx.x / G; gi < m; gi += (i64)gridDim.x * GPB) {
+        const i32 i = list[gi];
+        if (dest[i] == NO_MOVE) continue;
+        const i64 s = rp[i];
+        const int d = (int)(rp[i + 1] - s);
+        const i32 p = part[i];
+        const i64 wv = vw[i];
+        i32 key = -5;
+        i64 wt = 0;
+        const bool has = gl < d;
+        if (has) {
+            i32 v = col[s + gl];
+            key = raw ? v : part[v];
+            wt = W(s + gl);
+        }
+        i64 agg = 0;
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            i32 kk = __shfl(key, k, G);
+            i64 ww = __shfl(wt, k, G);
+            agg += (kk == key) ? ww : 0;
+        }
+        const bool own = !raw && has && key == p;
+        const i64 pv = group_max_i64<G>(own ? agg : (i64)0);
+        float multi, p_conn, b;
+        own_terms(pv, vol[p], wv, lam, ratio, multi, p_conn, b);
+        u64 bk = 0;
+        if (has && !own && agg > 0 && (float)agg >= b && (cons == nullptr || cons[key] == cons[p])) {
+            float jc = fmaf(-(float)vol[key], multi, (float)agg);
+            if (jc >= b) bk = pack_cand(jc, key);
+        }
+        bk = group_max_u64<G>(bk);
+        if (gl == 0) decide(i, p_conn, bk, dest, gain);
+    }
+}
+
+// One block per vertex with a hash table in LDS (or HBM for the largest
+// degrees).
+template <int BS, int SLOTS, bool GLOBAL, class WT>
+__global__ void __launch_bounds__(BS) k_score_table(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
+                                                    const i64* __restrict__ vw, const i32* __restrict__ list, i64 m,
+                                                    const i32* __restrict__ part, const i64* __restrict__ vol,
+                                                    const i32* __restrict__ cons, int raw, float lam, float ratio,
+                                                    i32* __restrict__ dest, float* __restrict__ gain, i32* gkeys,
+                                                    i64* gvals, const i64* __restrict__ goff) {
+    __shared__ i32 skeys[GLOBAL ? 1 : SLOTS];
+    __shared__ i64 svals[GLOBAL ? 1 : SLOTS];
+    const int t = threadIdx.x;
+    for (i64 li = blockIdx.x; li < m; li += gridDim.x) {
+        const i32 i = list[li];
+        if (dest[i] == NO_MOVE) continue;
+        const i64 s = rp[i];
+        const i64 d = rp[i + 1] - s;
+        const i64 S = tslots(d);
+        const u32 mask = (u32)(S - 1);
+        i32* keys = GLOBAL ? gkeys + goff[li] : skeys;
+        i64* vals = GLOBAL ? gvals + goff[li] : svals;
+        for (i64 q = t; q < S; q += BS) {
+            keys[q] = EMPTY_KEY;
+            vals[q] = 0;
+        }
+        if (GLOBAL) __threadfence();
+        __syncthreads();
+        const i32 p = part[i];
+        i64 pv = 0;
+        for (i64 e = t; e < d; e += BS) {
+            i32 v = col[s + e];
+            i32 key = raw ? v : part[v];
+            i64 ww = W(s + e);
+            if (!raw && key == p) pv += ww;
+            else if (GLOBAL) table_add_global(keys, vals, mask, key, ww);
+            else table_add_lds(keys, vals, mask, key, ww);
+        }
+        if (GLOBAL) __threadfence();
+        pv = block_sum_i64(pv);
+        float multi, p_conn, b;
+        own_terms(pv, vol[p], vw[i], lam, ratio, multi, p_conn, b);
+        u64 bk = 0;
+        for (i64 q = t; q < S; q += BS) {
+            i32 key;
+            i64 agg;
+            if (GLOBAL) {
+                key = __hip_atomic_load(&keys[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (key == EMPTY_KEY) continue;
+                agg = __hip_atomic_load(&vals[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                key = keys[q];
+                if (key == EMPTY_KEY) continue;
+                agg = vals[q];
+            }
+            if (agg > 0 && (float)agg >= b && (cons == nullptr || cons[key] == cons[p])) {
+                float jc = fmaf(-(float)vol[key], multi, (float)agg);
+                if (jc >= b) {
+                    u64 k2 = pack_cand(jc, key);
+                    bk = k2 > bk ? k2 : bk;
+                }
+            }
+        }
+        bk = block_max_u64(bk);
+        if (t == 0) decide(i, p_conn, bk, dest, gain);
+        __syncthreads();
+    }
+}
+
+static int cap_grid(i64 blocks, int cap) { return (int)std::max<i64>(1, std::min<i64>(blocks, cap)); }
+
+static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32* part, const i64* vol,
+                      const i32* cons, bool raw, float lam, float ratio, i32* dest, float* gain) {
+    const i64* st = lp.sc_start;
+    const i32* L = lp.sc_list.p;
+    if (lp.gslots > 0) s.ensure_global(c, lp.gslots);
+    PGGC_WDISPATCH(g, {
+        i64 m;
+        if ((m = st[1] - st[0]) > 0)
+            k_score_group<4><<<cap_grid(ceil_div(m, 64), 1 << 16), 256, 0, c.stream>>>(
+                g.rp, g.col, W, g.vw, L + st[0], m, part, vol, cons, raw, lam, ratio, dest, gain);
+        if ((m = st[2] - st[1]) > 0)
+            k_score_group<8><<<cap_grid(ceil_div(m, 32), 1 << 16), 256, 0, c.stream>>>(
+                g.rp, g.col, W, g.vw, L + st[1], m, part, vol, cons, raw, lam, ratio, dest, gain);
+        if ((m = st[3] - st[2]) > 0)
+            k_score_group<16><<<cap_grid(ceil_div(m, 16), 1 << 16), 256, 0, c.stream>>>(
+                g.rp, g.col, W, g.vw, L + st[2], m, part, vol, cons, raw, lam, ratio, dest, gain);
+        if ((m = st[4] - st[3]) > 0)
+            k_score_group<32><<<cap_grid(ceil_div(m, 8), 1 << 16), 256, 0, c.stream>>>(
+                g.rp, g.col, W, g.vw, L + st[3], m, part, vol, cons, raw, lam, ratio, dest, gain);
+        if ((m = st[5] - st[4]) > 0)
+            k_score_group<64><<<cap_grid(ceil_div(m, 4), 1 << 16), 256, 0, c.stream>>>(
+                g.rp, g.col, W, g.vw, L + st[4], m, part, vol, cons, raw, lam, ratio, dest, gain);
+        if ((m = st[6] - st[5]) > 0)
+            k_score_table<64, 1024, false><<<cap_grid(m, 1 << 18), 64, 0, c.stream>>>(
+                g.rp, g.col, W, g.vw, L + st[5], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr, nullptr,
+                nullptr);
+        if ((m = st[7] - st[6]) > 0)
+            k_score_table<256, 8192, false><<<cap_grid(m, 1 << 18), 256, 0, c.stream>>>(
+                g.rp, g.col, W, g.vw, L + st[6], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr, nullptr,
+                nullptr);
+        if ((m = st[8] - st[7]) > 0)
+            k_score_table<256, 1, true><<<cap_grid(m, 1 << 16), 256, 0, c.stream>>>(
+                g.rp, g.col, W, g.vw, L + st[7], m, part, vol, cons, raw, lam, ratio, dest, gain, s.gkeys, s.gvals,
+                lp.goff);
+    });
+    PGGC_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------- candidates
+__global__ void k_cand_flags(const i32* o1, i64 n, const i32* dest, i32* flag) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x)
+        flag[x] = dest[o1[x]] != NO_MOVE ? 1 : 0;
+}
+// first list positions whose degree reaches LARGE_CUTOFF / MASSIVE_CUTOFF
+__global__ void k_list_bounds(const i32* list, i64 k, const i64* rp, i64* out) {
+    if (threadIdx.x >= 2) return;
+    i64 cut = threadIdx.x == 0 ? LARGE_CUTOFF : MASSIVE_CUTOFF;
+    i64 lo = 0, hi = k;
+    while (lo < hi) {
+        i64 m = (lo + hi) >> 1;
+        i32 i = list[m];
+        if (rp[i + 1] - rp[i] >= cut) hi = m;
+        else lo = m + 1;
+    }
+    out[threadIdx.x] = lo;
+}
+
+// candidates in order1 order; returns count, b[0]/b[1] = class boundaries
+static i64 gather_candidates(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32* dest, i32* out,
+                             i64* b) {
+    k_cand_flags<<<grid_for(g.n, 1024, 8192), 256, 0, c.stream>>>(lp.ord.o1, g.n, dest, s.flag);
+    PGGC_LAUNCH_CHECK();
+    i64 k = compact_flags(c, s.flag, lp.ord.o1, g.n, out, s.scal);
+    if (k > 0) {
+        k_list_bounds<<<1, 64, 0, c.stream>>>(out, k, g.rp, s.scal.p + 1);
+        PGGC_LAUNCH_CHECK();
+        const i64* h = c.read(s.scal.p + 1, 2);
+        b[0] = h[0];
+        b[1] = h[1];
+    } else {
+        b[0] = b[1] = 0;
+    }
+    return k;
+}
+
+// ------------------------------------------------------------- afterburner
+template <class WT>
+__device__ __forceinline__ void ab_term(const i32* col, WT W, const i64* vw, const i32* part, const i32* dest,
+                                        const float* gain, i64 j, i32 p, i32 best, float multi, float igain, i32 hi,
+                                        float& acc) {
+    const i32 v = col[j];
+    const float vg = gain[v];
+    if ((vg - igain) >= AFTERBURNER_EPS || (fabsf(vg - igain) < AFTERBURNER_EPS && (i32)xorshift32((u32)v) < hi)) {
+        i32 vpart = dest[v];
+        const float q = fmaf(-multi, (float)vw[v], (float)W(j));
+        acc = acc - ((vpart == p) ? q : 0.0f);
+        acc = acc + ((vpart == best && best != NEW_PART) ? q : 0.0f);
+        vpart = part[v];
+        acc = acc + ((vpart == p) ? q : 0.0f);
+        acc = acc - ((vpart == best) ? q : 0.0f);
+    }
+}
+
+template <class WT>
+__global__ void __launch_bounds__(256) k_ab_serial(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
+                                                   const i64* __restrict__ vw, const i32* __restrict__ cand, i64 k,
+                                                   const i32* __restrict__ part, const i32* __restrict__ dest,
+                                                   const float* __restrict__ gain, float lam, i32* keep, i32* mflag) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < k; x += (i64)gridDim.x * blockDim.x) {
+        const i32 i = cand[x];
+        const i32 best = dest[i], p = part[i];
+        const float multi = (float)vw[i] * lam;
+        const float igain = gain[i];
+        const i32 hi = (i32)xorshift32((u32)i);
+        float acc = 0.0f;
+        for (i64 j = rp[i]; j < rp[i + 1]; j++) ab_term(col, W, vw, part, dest, gain, j, p, best, multi, igain, hi, acc);
+        const int kp = (igain + acc >= 0.0f) ? 1 : 0;
+        keep[x] = kp;
+        if (kp) mflag[i] = 1;
+    }
+}
+
+template <class WT>
+__global__ void __launch_bounds__(256) k_ab_wave(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
+                                                 const i64* __restrict__ vw, const i32* __restrict__ cand, i64 k0,
+                                                 i64 k1, const i32* __restrict__ part, const i32* __restrict__ dest,
+                                                 const float* __restrict__ gain, float lam, i32* keep, i32* mflag) {
+    const int lane = threadIdx.x & 63;
+    for (i64 x = k0 + (i64)blockIdx.x * 4 + (threadIdx.x >> 6); x < k1; x += (i64)gridDim.x * 4) {
+        const i32 i = cand[x];
+        const i32 best = dest[i], p = part[i];
+        const float multi = (float)vw[i] * lam;
+        const float igain = gain[i];
+        const i32 hi = (i32)xorshift32((u32)i);
+        float acc = 0.0f;
+        for (i64 j = rp[i] + lane; j < rp[i + 1]; j += 64)
+            ab_term(col, W, vw, part, dest, gain, j, p, best, multi, igain, hi, acc);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_down(acc, off);
+        if (lane == 0) {
+            const int kp = (igain + acc >= 0.0f) ? 1 : 0;
+            keep[x] = kp;
+            if (kp) mflag[i] = 1;
+        }
+    }
+}
+
+template <class WT>
+__global__ void __launch_bounds__(1024) k_ab_block(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
+                                                   const i64* __restrict__ vw, const i32* __restrict__ cand, i64 k0,
+                                                   i64 k1, const i32* __restrict__ part, const i32* __restrict__ dest,
+                                                   const float* __restrict__ gain, float lam, i32* keep, i32* mflag) {
+    __shared__ float sh[1024];
+    const int t = threadIdx.x;
+    for (i64 x = k0 + blockIdx.x; x < k1; x += gridDim.x) {
+        const i32 i = cand[x];
+        const i32 best = dest[i], p = part[i];
+        const float multi = (float)vw[i] * lam;
+        const float igain = gain[i];
+        const i32 hi = (i32)xorshift32((u32)i);
+        float acc = 0.0f;
+        for (i64 j = rp[i] + t; j < rp[i + 1]; j += 1024)
+            ab_term(col, W, vw, part, dest, gain, j, p, best, multi, igain, hi, acc);
+        sh[t] = acc;
+        __syncthreads();
+        for (int off = 512; off >= 64; off >>= 1) {
+            if (t < off) sh[t] = sh[t] + sh[t + off];
+            __syncthreads();
+        }
+        if (t < 64) {
+            acc = sh[t];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_down(acc, off);
+            if (t == 0) {
+                const int kp = (igain + acc >= 0.0f) ? 1 : 0;
+                keep[x] = kp;
+                if (kp) mflag[i] = 1;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+static i64 afterburner(Ctx& c, MoveScratch& s, const DGraph& g, const i32* cand, i64 k, const i64* b, float lam,
+                       i32* moves) {
+    PGGC_WDISPATCH(g, {
+        if (b[0] > 0)
+            k_ab_serial<<<cap_grid(ceil_div(b[0], 256), 1 << 16), 256, 0, c.stream>>>(
+                g.rp, g.col, W, g.vw, cand, b[0], s.part, s.dest, s.gain, lam, s.flag, s.mflag);
+        if (b[1] > b[0])
+            k_ab_wave<<<cap_grid(ceil_div(b[1] - b[0], 4), 1 << 16), 256, 0, c.stream>>>(
+                g.rp, g.col, W, g.vw, cand, b[0], b[1], s.part, s.dest, s.gain, lam, s.flag, s.mflag);
+        if (k > b[1])
+            k_ab_block<<<cap_grid(k - b[1], 1 << 14), 1024, 0, c.stream>>>(
+                g.rp, g.col, W, g.vw, cand, b[1], k, s.part, s.dest, s.gain, lam, s.flag, s.mflag);
+    });
+    PGGC_LAUNCH_CHECK();
+    return compact_flags(c, s.flag, cand, k, moves, s.scal);
+}
+
+// ------------------------------------------------------------- new labels
+__global__ void k_new_flags(const i32* moves, i64 k, const i32* dest, i32* flag) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < k; x += (i64)gridDim.x * blockDim.x)
+        flag[x] = dest[moves[x]] == NEW_PART ? 1 : 0;
+}
+__global__ void k_mark_used(const i32* part, i64 n, i32* used) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        used[part[i]] = 1;
+}
+__global__ void k_unused_flags(const i32* used, i64 L, i32* flag) {
+    for (i64 l = (i64)blockIdx.x * blockDim.x + threadIdx.x; l < L; l += (i64)gridDim.x * blockDim.x)
+        flag[l] = used[l] == 0 ? 1 : 0;
+}
+__global__ void k_assign_new(const i32* newv, i64 K, const i32* avail, i64 A, i32 curr, i32* dest, i32* cons,
+                             const i32* part) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < K; x += (i64)gridDim.x * blockDim.x) {
+        i32 i = newv[x];
+        i32 l = x < A ? avail[x] : (i32)(x - A) + curr;
+        dest[i] = l;
+        if (cons) cons[l] = cons[part[i]];
+    }
+}
+
+// set_new_cluster_ids (src/local_mover.cpp:215-274)
+static void set_new_cluster_ids(Ctx& c, MoveScratch& s, const DGraph& g, RefineData& rd, const i32* moves, i64 k,
+                                i32* cons) {
+    k_new_flags<<<grid_for(k, 1024, 8192), 256, 0, c.stream>>>(moves, k, s.dest, s.flag);
+    PGGC_LAUNCH_CHECK();
+    i64 K = compact_flags(c, s.flag, moves, k, s.tmp, s.scal);
+    if (K == 0) return;
+    const i32 curr = rd.label_count;
+    fill_i32(c, s.flag, curr, 0);
+    k_mark_used<<<grid_for(g.n, 1024, 8192), 256, 0, c.stream>>>(s.part, g.n, s.flag);
+    PGGC_LAUNCH_CHECK();
+    Buf<i32> uf(c, std::max<i64>(curr, 1));
+    k_unused_flags<<<grid_for(curr, 1024, 8192), 256, 0, c.stream>>>(s.flag, curr, uf);
+    PGGC_LAUNCH_CHECK();
+    i64 A = compact_flags(c, uf, nullptr, curr, s.tmp2, s.scal);
+    if (K > A) {
+        i64 need = K - A;
+        if (curr + need > rd.cap) throw Error(-4, "label capacity exceeded while assigning new clusters");
+        fill_i64(c, rd.vol + curr, need, 0);
+        rd.label_count = (i32)(curr + need);
+    }
+    k_assign_new<<<grid_for(K, 1024, 8192), 256, 0, c.stream>>>(s.tmp, K, s.tmp2, A, curr, s.dest, cons, s.part);
+    PGGC_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------- perform moves
+template <class WT>
+__global__ void __launch_bounds__(256) k_delta_uncut(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
+                                                     const i32* __restrict__ moves, i64 k,
+                                                     const i32* __restrict__ part, const i32* __restrict__ dest,
+                                                     const i32* __restrict__ mflag, i64* out) {
+    const int lane = threadIdx.x & 63;
+    i64 tot = 0;
+    for (i64 x = (i64)blockIdx.x * 4 + (threadIdx.x >> 6); x < k; x += (i64)gridDim.x * 4) {
+        const i32 i = moves[x];
+        const i32 oi = part[i], ni = dest[i];
+        i64 acc = 0;
+        for (i64 j = rp[i] + lane; j < rp[i + 1]; j += 64) {
+            const i32 v = col[j];
+            const i32 ov = part[v];
+            const int mv = mflag[v];
+            const i32 nv = mv ? dest[v] : ov;
+            const i64 dd = (i64)(ni == nv) - (i64)(oi == ov);
+            acc += dd * W(j) * (mv ? 1 : 2);
+        }
+        tot += acc;
+    }
+    tot = wave_sum_i64(tot);
+    if (lane == 0 && tot != 0) atomicAdd((unsigned long long*)out, (unsigned long long)tot);
+}
+
+__global__ void k_apply_moves(const i32* moves, i64 k, const i64* vw, i32* part, const i32* dest, i64* vol) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < k; x += (i64)gridDim.x * blockDim.x) {
+        const i32 i = moves[x];
+        const i32 p = part[i], b = dest[i];
+        const i64 w = vw[i];
+        if (b < 0) continue;  // unresolved NEW_PART cannot reach here (p_conn is 0 for singletons)
+        atomicAdd((unsigned long long*)&vol[p], (unsigned long long)(-w));
+        atomicAdd((unsigned long long*)&vol[b], (unsigned long long)w);
+        part[i] = b;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_reset_affected(const i64* __restrict__ rp, const i32* __restrict__ col,
+                                                        const i32* __restrict__ moves, i64 k, i32* dest,
+                                                        i32* mflag) {
+    const int lane = threadIdx.x & 63;
+    for (i64 x = (i64)blockIdx.x * 4 + (threadIdx.x >> 6); x < k; x += (i64)gridDim.x * 4) {
+        const i32 i = moves[x];
+        for (i64 j = rp[i] + lane; j < rp[i + 1]; j += 64) dest[col[j]] = NULL_PART;
+        if (lane == 0) {
+            dest[i] = NULL_PART;
+            mflag[i] = 0;
+        }
+    }
+}
+__global__ void k_clear_mflag(const i32* moves, i64 k, i32* mflag) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < k; x += (i64)gridDim.x * blockDim.x)
+        mflag[moves[x]] = 0;
+}
+__global__ void k_set_mflag(const i32* moves, i64 k, i32* mflag) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < k; x += (i64)gridDim.x * blockDim.x)
+        mflag[moves[x]] = 1;
+}
+
+// perform_moves (src/local_mover.cpp:1241-1279)
+static void perform_moves(Ctx& c, MoveScratch& s, const DGraph& g, RefineData& curr, const i32* moves, i64 k,
+                          bool& tables) {
+    fill_i64(c, s.scal, 1, 0);
+    PGGC_WDISPATCH(g, (k_delta_uncut<<<cap_grid(ceil_div(k, 4), 1 << 16), 256, 0, c.stream>>>(
+                          g.rp, g.col, W, moves, k, s.part, s.dest, s.mflag, s.scal)));
+    PGGC_LAUNCH_CHECK();
+    k_apply_moves<<<grid_for(k, 256, 8192), 256, 0, c.stream>>>(moves, k, g.vw, s.part, s.dest, curr.vol);
+    PGGC_LAUNCH_CHECK();
+    if (!tables) {
+        tables = true;
+        fill_i32(c, s.dest, g.n, NULL_PART);
+        k_clear_mflag<<<grid_for(k, 256, 8192), 256, 0, c.stream>>>(moves, k, s.mflag);
+    } else {
+        k_reset_affected<<<cap_grid(ceil_div(k, 4), 1 << 16), 256, 0, c.stream>>>(g.rp, g.col, moves, k, s.dest,
+                                                                                  s.mflag);
+    }
+    PGGC_LAUNCH_CHECK();
+    sum_sq_u128(c, curr.vol, curr.label_count, s.scal.p + 1);
+    const i64* h = c.read(s.scal, 3);
+    curr.uncut += h[0];
+    curr.obj = objective_value(curr.uncut, curr.lambda, (u64)h[1], (u64)h[2]);
+}
+
+// ------------------------------------------------------------- relabel
+__global__ void k_relabel_vol(const i32* used, const i64* pos, i64 L, const i64* vol, i64* nvol) {
+    for (i64 l = (i64)blockIdx.x * blockDim.x + threadIdx.x; l < L; l += (i64)gridDim.x * blockDim.x)
+        if (used[l]) nvol[pos[l]] = vol[l];
+}
+__global__ void k_relabel_part(const i64* pos, i64 n, i32* part) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        part[i] = (i32)pos[part[i]];
+}
+
+// relabel_contiguously (src/local_mover.cpp:100-129)
+static void relabel(Ctx& c, MoveScratch& s, i64 n, i32* labels, RefineData& rd) {
+    const i64 L = rd.label_count;
+    fill_i32(c, s.flag, L, 0);
+    k_mark_used<<<grid_for(n, 1024, 8192), 256, 0, c.stream>>>(labels, n, s.flag);
+    PGGC_LAUNCH_CHECK();
+    Buf<i64> pos(c, L + 1), nvol(c, std::max<i64>(L, 1));
+    scan_i32_to_i64(c, s.flag, pos, L);
+    k_relabel_vol<<<grid_for(L, 1024, 8192), 256, 0, c.stream>>>(s.flag, pos, L, rd.vol, nvol);
+    PGGC_LAUNCH_CHECK();
+    i64 t = c.read1(pos.p + L);
+    PGGC_HIP(hipMemcpyAsync(rd.vol, nvol, t * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
+    k_relabel_part<<<grid_for(n, 1024, 8192), 256, 0, c.stream>>>(pos, n, labels);
+    PGGC_LAUNCH_CHECK();
+    rd.label_count = (i32)t;
+}
+
+static void copy_state(Ctx& c, RefineData& dst, const RefineData& src) {
+    if (src.label_count > dst.cap) throw Error(-4, "label capacity exceeded");
+    PGGC_HIP(hipMemcpyAsync(dst.vol, src.vol, (size_t)src.label_count * sizeof(i64), hipMemcpyDeviceToDevice,
+                            c.stream));
+    dst.top_nnz = src.top_nnz;
+    dst.uncut = src.uncut;
+    dst.obj = src.obj;
+    dst.label_count = src.label_count;
+    dst.lambda = src.lambda;
+}
+
+// ------------------------------------------------------------- drivers
+struct CurrVol {
+    Buf<i64> v;
+};
+
+static RefineData begin_curr(Ctx& c, const RefineData& best, Buf<i64>& cv, i64 n) {
+    i64 cap = std::max<i64>(best.cap, n);
+    cv = Buf<i64>(c, cap + 1);
+    RefineData curr = best;
+    curr.vol = cv;
+    curr.cap = cap;
+    copy_state(c, curr, best);
+    return curr;
+}
+
+void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* best_part, RefineData& best,
+                bool is_initial, bool constrained, i32* cons, bool SA, MoveStats* stats) {
+    const i64 n = g.n;
+    s.ensure(c, n);
+    plan_level(c, g, lp);
+    Buf<i64> cv;
+    RefineData curr = begin_curr(c, best, cv, n);
+    PGGC_HIP(hipMemcpyAsync(s.part, best_part, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+    fill_i32(c, s.dest, n, NULL_PART);
+    bool tables = !is_initial;
+    const float ratios[2] = {0.75f, 0.25f};
+    const int limits[2] = {4, 2};
+    int iter = 0;
+    const i32* ccons = constrained ? cons : nullptr;
+    for (int x = 0; x < 2; x++) {
+        const float fr = SA ? ratios[x] : 0.0f;
+        for (int cnt = 0; cnt < limits[x]; cnt++) {
+            iter++;
+            const float lam = (float)curr.lambda;
+            score_all(c, s, lp, g, s.part, curr.vol, ccons, !tables, lam, fr, s.dest, s.gain);
+            i64 b[2];
+            i64 k = gather_candidates(c, s, lp, g, s.dest, s.cand, b);
+            if (stats) {
+                stats->sweeps++;
+                stats->cands += k;
+            }
+            if (k == 0) break;
+            i64 km = afterburner(c, s, g, s.cand, k, b, lam, s.moves);
+            if (km == 0) break;
+            if (iter > 1 || !is_initial) set_new_cluster_ids(c, s, g, curr, s.moves, km, constrained ? cons : nullptr);
+            perform_moves(c, s, g, curr, s.moves, km, tables);
+            if (stats) stats->moves += km;
+            if (curr.obj > best.obj) {
+                copy_state(c, best, curr);
+                PGGC_HIP(hipMemcpyAsync(best_part, s.part, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+            }
+        }
+    }
+    relabel(c, s, n, best_part, best);
+}
+
+// afterburner_filter_strict (src/local_mover.cpp:131-213): exact sequential
+// FP32 evaluation of each move given all earlier moves in list order.
+__global__ void k_prio(const i32* moves, i64 k, i64* prio) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < k; x += (i64)gridDim.x * blockDim.x)
+        prio[moves[x]] = x;
+}
+
+template <class WT>
+__global__ void __launch_bounds__(256) k_strict_gain(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
+                                                     const i64* __restrict__ vw, const i32* __restrict__ moves,
+                                                     i64 k, const i64* __restrict__ prio,
+                                                     const i32* __restrict__ part, const i32* __restrict__ dest,
+                                                     const float* __restrict__ gain, float lam, float* sg) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < k; x += (i64)gridDim.x * blockDim.x) {
+        const i32 i = moves[x];
+        const i32 best = dest[i], p = part[i];
+        const float multi = (float)vw[i] * lam;
+        float oc = gain[i];
+        const i64 pi = prio[i];
+        for (i64 j = rp[i]; j < rp[i + 1]; j++) {
+            const i32 v = col[j];
+            const i64 wgt = W(j);
+            if (prio[v] < pi) {
+                i32 vp = dest[v];
+                oc = oc - (float)((vp == p) ? wgt : 0);
+                oc = oc + (float)((vp == best && best != NEW_PART) ? wgt : 0);
+                vp = part[v];
+                oc = oc + (float)((vp == p) ? wgt : 0);
+                oc = oc - (float)((vp == best) ? wgt : 0);
+            }
+        }
+        for (i64 jx = 0; jx < x; jx++) {
+            const i32 v = moves[jx];
+            i32 vp = dest[v];
+            const float wg = -multi * (float)vw[v];
+            oc = oc - ((vp == p) ? wg : 0.0f);
+            oc = oc + ((vp == best && best != NEW_PART) ? wg : 0.0f);
+            vp = part[v];
+            oc = oc + ((vp == p) ? wg : 0.0f);
+            oc = oc - ((vp == best) ? wg : 0.0f);
+        }
+        sg[x] = oc;
+    }
+}
+
+// sequential inclusive FP32 prefix sum and first arg-max (one thread)
+__global__ void k_strict_scan(const i32* moves, i64 k, float* sg, float* gain, i64* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float run = 0.0f, bv = -1.0f;
+    i64 bl = -1;
+    for (i64 x = 0; x < k; x++) {
+        run = run + sg[x];
+        sg[x] = run;
+        gain[moves[x]] = run;
+        if (run > bv) {
+            bv = run;
+            bl = x;
+        }
+    }
+    out[0] = (bl >= 0 && bv > 0.0f) ? bl + 1 : 0;
+}
+
+void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* best_part, RefineData& best,
+                       bool is_initial, bool constrained, i32* cons, MoveStats* stats) {
+    const i64 n = g.n;
+    s.ensure(c, n);
+    plan_level(c, g, lp);
+    Buf<i64> cv;
+    RefineData curr = begin_curr(c, best, cv, n);
+    PGGC_HIP(hipMemcpyAsync(s.part, best_part, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+    fill_i32(c, s.dest, n, NULL_PART);
+    bool tables = !is_initial;
+    const i32* ccons = constrained ? cons : nullptr;
+    Buf<float> sg(c, std::max<i64>(n, 1));
+    for (int it = 0; it < 6; it++) {
+        const float lam = (float)curr.lambda;
+        score_all(c, s, lp, g, s.part, curr.vol, ccons, !tables, lam, 0.0f, s.dest, s.gain);
+        i64 b[2];
+        i64 k = gather_candidates(c, s, lp, g, s.dest, s.moves, b);
+        if (stats) {
+            stats->sweeps++;
+            stats->cands += k;
+        }
+        if (k == 0) break;
+        if (it > 0 || !is_initial) set_new_cluster_ids(c, s, g, curr, s.moves, k, constrained ? cons : nullptr);
+        fill_i64(c, s.prio, n, n + 1);
+        k_prio<<<grid_for(k, 256, 8192), 256, 0, c.stream>>>(s.moves, k, s.prio);
+        PGGC_LAUNCH_CHECK();
+        PGGC_WDISPATCH(g, (k_strict_gain<<<grid_for(k, 256, 1 << 16), 256, 0, c.stream>>>(
+                              g.rp, g.col, W, g.vw, s.moves, k, s.prio, s.part, s.dest, s.gain, lam, sg)));
+        PGGC_LAUNCH_CHECK();
+        k_strict_scan<<<1, 64, 0, c.stream>>>(s.moves, k, sg, s.gain, s.scal);
+        PGGC_LAUNCH_CHECK();
+        i64 tr = c.read1(s.scal);
+        if (tr == 0) break;
+        k_set_mflag<<<grid_for(tr, 256, 8192), 256, 0, c.stream>>>(s.moves, tr, s.mflag);
+        PGGC_LAUNCH_CHECK();
+        perform_moves(c, s, g, curr, s.moves, tr, tables);
+        if (stats) stats->moves += tr;
+        if (curr.obj > best.obj) {
+            copy_state(c, best, curr);
+            PGGC_HIP(hipMemcpyAsync(best_part, s.part, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+        }
+        fill_i32(c, s.dest, n, NULL_PART);
+    }
+    relabel(c, s, n, best_part, best);
+}
+
+i64 sweep_once(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32* part, const i64* vol, i64 L,
+               double lambda, float ratio, bool raw, const i32* cons, i32* dest, float* gain, i32* moves_out,
+               i64* ncand) {
+    const i64 n = g.n;
+    (void)L;
+    s.ensure(c, n);
+    plan_level(c, g, lp);
+    PGGC_HIP(hipMemcpyAsync(s.part, part, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+    PGGC_HIP(hipMemcpyAsync(s.dest, dest, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+    PGGC_HIP(hipMemcpyAsync(s.gain, gain, n * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+    const float lam = (float)lambda;
+    score_all(c, s, lp, g, s.part, vol, cons, raw, lam, ratio, s.dest, s.gain);
+    i64 b[2];
+    i64 k = gather_candidates(c, s, lp, g, s.dest, s.cand, b);
+    *ncand = k;
+    i64 km = 0;
+    if (k > 0) {
+        km = afterburner(c, s, g, s.cand, k, b, lam, moves_out);
+        k_clear_mflag<<<grid_for(std::max<i64>(km, 1), 256, 8192), 256, 0, c.stream>>>(moves_out, km, s.mflag);
+        PGGC_LAUNCH_CHECK();
+    }
+    PGGC_HIP(hipMemcpyAsync(dest, s.dest, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+    PGGC_HIP(hipMemcpyAsync(gain, s.gain, n * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+    c.sync();
+    return km;
+}
+
+}  // namespace pggc

@@ -1,0 +1,364 @@
+// Context allocator and data-parallel primitives: fill, scan, ordered
+// compaction, stable LSD radix sort, exact integer reductions.
+#include <algorithm>
+
+#include "ctx.h"
+#include "device.h"
+
+namespace pggc {
+
+// ------------------------------------------------------------------ allocator
+static size_t round_bytes(size_t b) {
+    if (b < 256) return 256;
+    if (b < (1u << 20)) {
+        size_t r = 256;
+        while (r < b) r <<= 1;
+        return r;
+    }
+    const size_t mb = 1u << 20;
+    return (b + mb - 1) / mb * mb;
+}
+
+void* Ctx::alloc(size_t bytes) {
+    size_t sz = round_bytes(bytes);
+    auto it = free_blocks.lower_bound(sz);
+    if (it != free_blocks.end() && it->first <= sz + sz / 4 + 4096) {
+        void* p = it->second;
+        size_t s = it->first;
+        free_blocks.erase(it);
+        bytes_cached -= s;
+        live[p] = s;
+        bytes_live += s;
+        return p;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, sz) != hipSuccess) {
+        (void)hipGetLastError();
+        trim();
+        if (hipMalloc(&p, sz) != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(-2, "device allocation of " + std::to_string(sz) + " bytes failed");
+        }
+    }
+    live[p] = sz;
+    bytes_live += sz;
+    return p;
+}
+
+void Ctx::release(void* p) {
+    auto it = live.find(p);
+    if (it == live.end()) return;
+    free_blocks.emplace(it->second, p);
+    bytes_cached += it->second;
+    bytes_live -= it->second;
+    live.erase(it);
+}
+
+void Ctx::trim() {
+    (void)hipStreamSynchronize(stream);
+    for (auto& kv : free_blocks) (void)hipFree(kv.second);
+    free_blocks.clear();
+    bytes_cached = 0;
+}
+
+const i64* Ctx::read(const i64* dev, int count) {
+    PGGC_HIP(hipMemcpyAsync(pinned, dev, sizeof(i64) * count, hipMemcpyDeviceToHost, stream));
+    PGGC_HIP(hipStreamSynchronize(stream));
+    return pinned;
+}
+
+// ---------------------------------------------------------------------- fill
+template <class T>
+__global__ void k_fill(T* p, i64 n, T v) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x) p[x] = v;
+}
+__global__ void k_iota(i32* p, i64 n) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x) p[x] = (i32)x;
+}
+
+void fill_i32(Ctx& c, i32* p, i64 n, i32 v) {
+    if (n <= 0) return;
+    k_fill<i32><<<grid_for(n, 256 * 4, 8192), 256, 0, c.stream>>>(p, n, v);
+    PGGC_LAUNCH_CHECK();
+}
+void fill_i64(Ctx& c, i64* p, i64 n, i64 v) {
+    if (n <= 0) return;
+    k_fill<i64><<<grid_for(n, 256 * 4, 8192), 256, 0, c.stream>>>(p, n, v);
+    PGGC_LAUNCH_CHECK();
+}
+void iota_i32(Ctx& c, i32* p, i64 n) {
+    if (n <= 0) return;
+    k_iota<<<grid_for(n, 256 * 4, 8192), 256, 0, c.stream>>>(p, n);
+    PGGC_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------- scan
+constexpr int ST = 256;          // threads per scan block
+constexpr int SI = 8;            // items per thread
+constexpr int STILE = ST * SI;   // elements per tile
+
+template <class Load>
+__global__ void __launch_bounds__(ST) k_tile_sums(Load ld, i64 n, i64* bsum) {
+    i64 base = (i64)blockIdx.x * STILE;
+    i64 s = 0;
+    for (int k = 0; k < SI; k++) {
+        i64 x = base + (i64)k * ST + threadIdx.x;
+        if (x < n) s += ld(x);
+    }
+    i64 tot = block_sum_i64(s);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+template <class Load>
+__global__ void __launch_bounds__(ST) k_tile_scan(Load ld, i64 n, const i64* boff, i64* out) {
+    i64 base = (i64)blockIdx.x * STILE + (i64)threadIdx.x * SI;
+    i64 v[SI];
+    i64 s = 0;
+#pragma unroll
+    for (int k = 0; k < SI; k++) {
+        i64 x = base + k;
+        v[k] = x < n ? ld(x) : 0;
+        s += v[k];
+    }
+    i64 tot;
+    i64 pre = block_excl_scan_i64(s, &tot) + (boff ? boff[blockIdx.x] : 0);
+#pragma unroll
+    for (int k = 0; k < SI; k++) {
+        i64 x = base + k;
+        if (x < n) out[x] = pre;
+        pre += v[k];
+        if (x == n - 1) out[n] = pre;
+    }
+}
+
+struct LoadI64 {
+    const i64* p;
+    __device__ __forceinline__ i64 operator()(i64 x) const { return p[x]; }
+};
+struct LoadI32 {
+    const i32* p;
+    __device__ __forceinline__ i64 operator()(i64 x) const { return p[x]; }
+};
+struct LoadFlag {
+    const i32* p;
+    __device__ __forceinline__ i64 operator()(i64 x) const { return p[x] != 0 ? 1 : 0; }
+};
+
+template <class Load>
+static void scan_generic(Ctx& c, Load ld, i64* out, i64 n) {
+    if (n <= 0) {
+        fill_i64(c, out, 1, 0);
+        return;
+    }
+    i64 nb = ceil_div(n, STILE);
+    if (nb == 1) {
+        k_tile_scan<Load><<<1, ST, 0, c.stream>>>(ld, n, nullptr, out);
+        PGGC_LAUNCH_CHECK();
+        return;
+    }
+    Buf<i64> bs(c, nb + 1);
+    k_tile_sums<Load><<<(int)nb, ST, 0, c.stream>>>(ld, n, bs);
+    PGGC_LAUNCH_CHECK();
+    scan_i64(c, bs, bs, nb);
+    k_tile_scan<Load><<<(int)nb, ST, 0, c.stream>>>(ld, n, bs, out);
+    PGGC_LAUNCH_CHECK();
+}
+
+void scan_i64(Ctx& c, const i64* in, i64* out, i64 n) {
+    if (in == out && n > 0) {
+        // the tile scan reads a whole tile before writing it, and tiles are
+        // disjoint, so in-place operation is safe
+    }
+    scan_generic(c, LoadI64{in}, out, n);
+}
+void scan_i32_to_i64(Ctx& c, const i32* in, i64* out, i64 n) { scan_generic(c, LoadI32{in}, out, n); }
+
+// ---------------------------------------------------------- ordered compaction
+__global__ void __launch_bounds__(ST) k_compact_write(const i32* flag, const i32* src, i64 n, const i64* boff,
+                                                       i32* out, i64* count) {
+    i64 base = (i64)blockIdx.x * STILE + (i64)threadIdx.x * SI;
+    int f[SI];
+    i64 s = 0;
+#pragma unroll
+    for (int k = 0; k < SI; k++) {
+        i64 x = base + k;
+        f[k] = (x < n && flag[x] != 0) ? 1 : 0;
+        s += f[k];
+    }
+    i64 tot;
+    i64 pre = block_excl_scan_i64(s, &tot) + boff[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SI; k++) {
+        i64 x = base + k;
+        if (f[k]) out[pre++] = src ? src[x] : (i32)x;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *count = boff[blockIdx.x] + tot;
+}
+
+i64 compact_flags(Ctx& c, const i32* flag, const i32* src, i64 n, i32* out, i64* count_dev) {
+    if (n <= 0) {
+        fill_i64(c, count_dev, 1, 0);
+        return 0;
+    }
+    i64 nb = ceil_div(n, STILE);
+    Buf<i64> bs(c, nb + 1);
+    k_tile_sums<LoadFlag><<<(int)nb, ST, 0, c.stream>>>(LoadFlag{flag}, n, bs);
+    PGGC_LAUNCH_CHECK();
+    scan_i64(c, bs, bs, nb);
+    k_compact_write<<<(int)nb, ST, 0, c.stream>>>(flag, src, n, bs, out, count_dev);
+    PGGC_LAUNCH_CHECK();
+    return c.read1(count_dev);
+}
+
+// ---------------------------------------------------------------- radix sort
+constexpr int RT = 256;
+constexpr int RR = 8;  // rounds per tile
+constexpr int RTILE = RT * RR;
+
+__global__ void __launch_bounds__(RT) k_radix_hist(const u64* keys, i64 n, int shift, i64* counts, i64 nb) {
+    __shared__ u32 h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    i64 base = (i64)blockIdx.x * RTILE;
+    for (int r = 0; r < RR; r++) {
+        i64 x = base + (i64)r * RT + threadIdx.x;
+        if (x < n) atomicAdd(&h[(keys[x] >> shift) & 255], 1u);
+    }
+    __syncthreads();
+    counts[(i64)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(RT) k_radix_scatter(const u64* kin, const i32* vin, u64* kout, i32* vout, i64 n,
+                                                       int shift, const i64* offs, i64 nb) {
+    __shared__ i64 run[256];
+    __shared__ i64 bd[256];
+    __shared__ u32 wc[4][256];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    run[t] = offs[(i64)t * nb + blockIdx.x];
+    const u64 lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    i64 base = (i64)blockIdx.x * RTILE;
+    for (int r = 0; r < RR; r++) {
+        i64 x = base + (i64)r * RT + t;
+        bool valid = x < n;
+        u64 key = valid ? kin[x] : 0;
+        i32 val = valid ? vin[x] : 0;
+        u32 d = (u32)(key >> shift) & 255u;
+        wc[0][t] = 0; wc[1][t] = 0; wc[2][t] = 0; wc[3][t] = 0;
+        __syncthreads();
+        u64 peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            u64 bal = __ballot(((d >> b) & 1u) != 0);
+            peers &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        u32 rank = (u32)__popcll(peers & lt);
+        if (valid && rank == 0) wc[w][d] = (u32)__popcll(peers);
+        __syncthreads();
+        u32 c0 = wc[0][t], c1 = wc[1][t], c2 = wc[2][t], c3 = wc[3][t];
+        wc[0][t] = 0; wc[1][t] = c0; wc[2][t] = c0 + c1; wc[3][t] = c0 + c1 + c2;
+        bd[t] = run[t];
+        run[t] += (i64)(c0 + c1 + c2 + c3);
+        __syncthreads();
+        if (valid) {
+            i64 dst = bd[d] + wc[w][d] + rank;
+            kout[dst] = key;
+            vout[dst] = val;
+        }
+        __syncthreads();
+    }
+}
+
+void radix_sort_pairs(Ctx& c, u64* keys, i32* vals, i64 n, int bits) {
+    if (n <= 1) return;
+    i64 nb = ceil_div(n, RTILE);
+    Buf<u64> k2(c, n);
+    Buf<i32> v2(c, n);
+    Buf<i64> counts(c, 256 * nb + 1);
+    u64 *ka = keys, *kb = k2;
+    i32 *va = vals, *vb = v2;
+    int passes = (bits + 7) / 8;
+    for (int p = 0; p < passes; p++) {
+        int shift = p * 8;
+        k_radix_hist<<<(int)nb, RT, 0, c.stream>>>(ka, n, shift, counts, nb);
+        PGGC_LAUNCH_CHECK();
+        scan_i64(c, counts, counts, 256 * nb);
+        k_radix_scatter<<<(int)nb, RT, 0, c.stream>>>(ka, va, kb, vb, n, shift, counts, nb);
+        PGGC_LAUNCH_CHECK();
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    if (ka != keys) {
+        PGGC_HIP(hipMemcpyAsync(keys, ka, n * sizeof(u64), hipMemcpyDeviceToDevice, c.stream));
+        PGGC_HIP(hipMemcpyAsync(vals, va, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+    }
+}
+
+// ------------------------------------------------------------ exact reductions
+__global__ void __launch_bounds__(256) k_sum_i64(const i64* in, i64 n, i64* out) {
+    i64 s = 0;
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x) s += in[x];
+    s = block_sum_i64(s);
+    if (threadIdx.x == 0) atomicAdd((unsigned long long*)out, (unsigned long long)s);
+}
+
+void sum_i64(Ctx& c, const i64* in, i64 n, i64* out) {
+    fill_i64(c, out, 1, 0);
+    if (n <= 0) return;
+    k_sum_i64<<<grid_for(n, 256 * 8, 2048), 256, 0, c.stream>>>(in, n, out);
+    PGGC_LAUNCH_CHECK();
+}
+
+struct U128 {
+    u64 lo, hi;
+};
+__device__ __forceinline__ U128 add128(U128 a, U128 b) {
+    U128 r;
+    r.lo = a.lo + b.lo;
+    r.hi = a.hi + b.hi + (r.lo < a.lo ? 1 : 0);
+    return r;
+}
+
+__global__ void __launch_bounds__(256) k_sumsq(const i64* in, i64 n, U128* part) {
+    U128 s{0, 0};
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x) {
+        i64 v = in[x];
+        u64 a = (u64)(v < 0 ? -v : v);
+        U128 q{a * a, __umul64hi(a, a)};
+        s = add128(s, q);
+    }
+    __shared__ U128 sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] = add128(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(256) k_sumsq_final(const U128* part, int np, i64* out) {
+    U128 s{0, 0};
+    for (int x = threadIdx.x; x < np; x += blockDim.x) s = add128(s, part[x]);
+    __shared__ U128 sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] = add128(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = (i64)sh[0].hi;
+        out[1] = (i64)sh[0].lo;
+    }
+}
+
+void sum_sq_u128(Ctx& c, const i64* in, i64 n, i64* out) {
+    int g = grid_for(std::max<i64>(n, 1), 256 * 8, 1024);
+    Buf<U128> part(c, g);
+    k_sumsq<<<g, 256, 0, c.stream>>>(in, n, part);
+    PGGC_LAUNCH_CHECK();
+    k_sumsq_final<<<1, 256, 0, c.stream>>>(part, g, out);
+    PGGC_LAUNCH_CHECK();
+}
+
+}  // namespace pggc

@@ -1,0 +1,232 @@
+"""Synthetic graph generators and CSR helpers.
+
+Graphs are undirected, stored as symmetric CSR with no self loops and sorted
+rows: ``rp`` int64[n+1], ``col`` int32[nnz], ``w`` int32[nnz] or None (unit
+weights).  This is the layout the reference loads (app/io/io_mtx.cpp:95-201
+symmetrises, deduplicates and sorts; app/io/io_metis.cpp reads it as given).
+
+Small graphs are generated with numpy (deterministic across machines); the
+R-MAT generator can also run on a torch device for the large workloads.
+"""
+import numpy as np
+
+
+class CSR:
+    __slots__ = ("n", "rp", "col", "w")
+
+    def __init__(self, n, rp, col, w=None):
+        self.n = int(n)
+        self.rp = np.ascontiguousarray(rp, dtype=np.int64)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.w = None if w is None else np.ascontiguousarray(w, dtype=np.int32)
+
+    @property
+    def nnz(self):
+        return int(self.rp[-1])
+
+    def degrees(self):
+        return np.diff(self.rp)
+
+    def vertex_weights(self, objective="Mod"):
+        """app/io/vertex_weighting.cpp:27-44: degree for Mod, weighted degree
+        for WMod, unit otherwise."""
+        if objective == "Mod":
+            return self.degrees().astype(np.int64)
+        if objective == "WMod":
+            return _wdeg(self)
+        return np.ones(self.n, np.int64)
+
+
+def _wdeg(g):
+    out = np.zeros(g.n, np.int64)
+    if g.nnz == 0:
+        return out
+    w = np.ones(g.nnz, np.int64) if g.w is None else g.w.astype(np.int64)
+    rows = np.repeat(np.arange(g.n), np.diff(g.rp))
+    np.add.at(out, rows, w)
+    return out
+
+
+def from_edges(n, src, dst, w=None, combine="first"):
+    """Symmetrise, drop self loops, deduplicate and sort an edge list into
+    CSR.  Duplicate (u,v) pairs keep the weight of the first occurrence in
+    (u,v)-sorted order, so the result is symmetric."""
+    src = np.asarray(src, dtype=np.int64)
+    dst = np.asarray(dst, dtype=np.int64)
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    if w is not None:
+        w = np.asarray(w, dtype=np.int64)[keep]
+    # canonical undirected key (min, max) so both directions share a weight
+    lo = np.minimum(src, dst)
+    hi = np.maximum(src, dst)
+    key = lo * n + hi
+    order = np.argsort(key, kind="stable")
+    key = key[order]
+    first = np.ones(len(key), bool)
+    first[1:] = key[1:] != key[:-1]
+    key = key[first]
+    if w is not None:
+        w = w[order][first]
+    lo = key // n
+    hi = key % n
+    s = np.concatenate([lo, hi])
+    d = np.concatenate([hi, lo])
+    ww = None if w is None else np.concatenate([w, w])
+    k2 = s * n + d
+    o2 = np.argsort(k2, kind="stable")
+    s, d = s[o2], d[o2]
+    if ww is not None:
+        ww = ww[o2]
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, s + 1, 1)
+    rp = np.cumsum(rp)
+    return CSR(n, rp, d.astype(np.int32), None if ww is None else ww.astype(np.int32))
+
+
+def permute(g, perm):
+    """Relabel vertex v as perm[v] (sorted rows are kept)."""
+    n = g.n
+    rows = np.repeat(np.arange(n), np.diff(g.rp))
+    return from_edges(n, perm[rows], perm[g.col], g.w)
+
+
+def sbm(n, blocks, d_in, d_out, seed, permute_seed=None, weights=None):
+    """Planted-partition SBM: `blocks` near-equal communities, expected intra
+    degree d_in (p_in = d_in/(b-1)) and inter degree d_out
+    (p_out = d_out/(n-b)).  Intra pairs are Bernoulli-sampled exactly; inter
+    pairs are drawn uniformly (count ~ Binomial) and deduplicated."""
+    rng = np.random.default_rng(seed)
+    sizes = np.full(blocks, n // blocks, np.int64)
+    sizes[: n % blocks] += 1
+    starts = np.concatenate([[0], np.cumsum(sizes)])
+    src, dst = [], []
+    for b in range(blocks):
+        s, m = starts[b], sizes[b]
+        if m < 2:
+            continue
+        p_in = min(1.0, d_in / (m - 1))
+        iu, ju = np.triu_indices(m, 1)
+        sel = rng.random(len(iu)) < p_in
+        src.append(iu[sel] + s)
+        dst.append(ju[sel] + s)
+    block_of = np.repeat(np.arange(blocks), sizes)
+    mean_b = n / blocks
+    inter_pairs = n * (n - mean_b) / 2.0
+    p_out = d_out / max(n - mean_b, 1)
+    m_out = rng.binomial(int(inter_pairs), min(1.0, p_out)) if inter_pairs > 0 else 0
+    u = rng.integers(0, n, int(m_out * 1.1) + 16)
+    v = rng.integers(0, n, len(u))
+    ok = block_of[u] != block_of[v]
+    u, v = u[ok][:m_out], v[ok][:m_out]
+    src.append(u)
+    dst.append(v)
+    src = np.concatenate(src)
+    dst = np.concatenate(dst)
+    w = None
+    if weights is not None:
+        w = rng.integers(weights[0], weights[1] + 1, len(src))
+    if permute_seed is not None:
+        perm = np.random.default_rng(permute_seed).permutation(n)
+        src, dst = perm[src], perm[dst]
+    return from_edges(n, src, dst, w)
+
+
+def rmat_edges_numpy(scale, edge_factor, a, b, c, seed):
+    """R-MAT edge draws (Chakrabarti et al. 2004), no noise."""
+    rng = np.random.default_rng(seed)
+    m = edge_factor << scale
+    src = np.zeros(m, np.int64)
+    dst = np.zeros(m, np.int64)
+    for lvl in range(scale):
+        r = rng.random(m)
+        bit_s = r >= a + b            # quadrants c, d
+        bit_d = ((r >= a) & (r < a + b)) | (r >= a + b + c)  # quadrants b, d
+        src |= bit_s.astype(np.int64) << (scale - 1 - lvl)
+        dst |= bit_d.astype(np.int64) << (scale - 1 - lvl)
+    return src, dst
+
+
+def rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=2, weights=None):
+    n = 1 << scale
+    src, dst = rmat_edges_numpy(scale, edge_factor, a, b, c, seed)
+    w = None
+    if weights is not None:
+        w = np.random.default_rng(seed + 7919).integers(weights[0], weights[1] + 1, len(src))
+    return from_edges(n, src, dst, w)
+
+
+def rmat_torch(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=2, weights=None, device="cuda"):
+    """Large R-MAT built with torch on `device` (same construction as
+    :func:`rmat`, different random stream).  Returns numpy-free torch tensors
+    (rp int64, col int32, w int32|None) resident on the device."""
+    import torch
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    n = 1 << scale
+    m = edge_factor << scale
+    src = torch.zeros(m, dtype=torch.int64, device=device)
+    dst = torch.zeros(m, dtype=torch.int64, device=device)
+    for lvl in range(scale):
+        r = torch.rand(m, generator=gen, device=device)
+        bs = r >= a + b
+        bd = ((r >= a) & (r < a + b)) | (r >= a + b + c)
+        sh = scale - 1 - lvl
+        src |= bs.to(torch.int64) << sh
+        dst |= bd.to(torch.int64) << sh
+        del r, bs, bd
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    lo = torch.minimum(src, dst)
+    hi = torch.maximum(src, dst)
+    del src, dst, keep
+    key = torch.unique(lo * n + hi)
+    del lo, hi
+    lo = key // n
+    hi = key % n
+    del key
+    w_und = None
+    if weights is not None:
+        w_und = torch.randint(weights[0], weights[1] + 1, (lo.numel(),), generator=gen, device=device,
+                              dtype=torch.int32)
+    k2 = torch.cat([lo * n + hi, hi * n + lo])
+    ww = None if w_und is None else torch.cat([w_und, w_und])
+    del lo, hi, w_und
+    k2, order = torch.sort(k2)
+    if ww is not None:
+        ww = ww[order]
+    del order
+    s = k2 // n
+    col = (k2 % n).to(torch.int32)
+    del k2
+    cnt = torch.bincount(s, minlength=n)
+    del s
+    rp = torch.zeros(n + 1, dtype=torch.int64, device=device)
+    rp[1:] = torch.cumsum(cnt, 0)
+    return rp, col, ww
+
+
+def karate():
+    """Zachary's karate club (34 vertices, 78 edges)."""
+    edges = [
+        (0, 1), (0, 2), (0, 3), (0, 4), (0, 5), (0, 6), (0, 7), (0, 8), (0, 10), (0, 11), (0, 12), (0, 13),
+        (0, 17), (0, 19), (0, 21), (0, 31), (1, 2), (1, 3), (1, 7), (1, 13), (1, 17), (1, 19), (1, 21), (1, 30),
+        (2, 3), (2, 7), (2, 8), (2, 9), (2, 13), (2, 27), (2, 28), (2, 32), (3, 7), (3, 12), (3, 13), (4, 6),
+        (4, 10), (5, 6), (5, 10), (5, 16), (6, 16), (8, 30), (8, 32), (8, 33), (9, 33), (13, 33), (14, 32),
+        (14, 33), (15, 32), (15, 33), (18, 32), (18, 33), (19, 33), (20, 32), (20, 33), (22, 32), (22, 33),
+        (23, 25), (23, 27), (23, 29), (23, 32), (23, 33), (24, 25), (24, 27), (24, 31), (25, 31), (26, 29),
+        (26, 33), (27, 33), (28, 31), (28, 33), (29, 32), (29, 33), (30, 32), (30, 33), (31, 32), (31, 33),
+        (32, 33),
+    ]
+    e = np.array(edges)
+    return from_edges(34, e[:, 0], e[:, 1])
+
+
+def random_graph(n, avg_deg, seed, weights=None):
+    """Erdos-Renyi style multigraph draw, deduplicated."""
+    rng = np.random.default_rng(seed)
+    m = int(n * avg_deg / 2)
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    w = None if weights is None else rng.integers(weights[0], weights[1] + 1, m)
+    return from_edges(n, u, v, w)

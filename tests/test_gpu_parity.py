@@ -1,0 +1,210 @@
+"""GPU parity: every hot-path entry point of the HIP library, called through
+the C ABI, against the CPU oracle on the same seeded inputs.  Integer and
+index results must be bit-identical; objectives are compared as the exact
+same double."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from graphcases import cases, hub_graph
+from pennsylvaniagpugraphclustering_amd import graphs
+
+pytestmark = pytest.mark.gpu
+
+CASES = cases()
+
+
+def C():
+    from pennsylvaniagpugraphclustering_amd import clustering
+    return clustering
+
+
+def dgraph(g, objective):
+    return C().DeviceGraph.from_csr(g, objective)
+
+
+def weights_for(g, objective):
+    return g.w if objective not in ("Mod", "CPM") else None
+
+
+def npy(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_orderings(gpu_ctx, name):
+    g, obj = CASES[name]
+    print(f"orderings {name}", flush=True)
+    o1, o2, offs = C().generate_orderings(dgraph(g, obj))
+    r1, r2, roffs = O.orderings(g.rp)
+    assert np.array_equal(npy(o1), r1)
+    assert np.array_equal(npy(o2), r2)
+    assert list(offs) == list(roffs)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_evaluator(gpu_ctx, name):
+    g, obj = CASES[name]
+    dg = dgraph(g, obj)
+    w = weights_for(g, obj)
+    vw = g.vertex_weights(obj)
+    rng = np.random.default_rng(1)
+    for L in [1, 5, max(1, g.n // 10), g.n]:
+        print(f"evaluate {name} L={L}", flush=True)
+        lab = rng.integers(0, L, g.n).astype(np.int32)
+        vol, uncut, sq = C().evaluate(dg, lab, L)
+        rv, ru, rs = O.evaluate(g.rp, g.col, w, vw, lab, L)
+        assert np.array_equal(npy(vol), rv)
+        assert uncut == ru and sq == rs
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_own_weight(gpu_ctx, name):
+    g, obj = CASES[name]
+    lab = np.random.default_rng(2).integers(0, max(2, g.n // 50), g.n).astype(np.int32)
+    print(f"own_weight {name}", flush=True)
+    out = C().own_weight(dgraph(g, obj), lab)
+    assert np.array_equal(npy(out), O.own_weight(g.rp, g.col, weights_for(g, obj), lab))
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("frac", [0.0, 0.001, 0.05, 0.5])
+def test_contraction(gpu_ctx, name, frac):
+    """Coarse graph of a random labelling: rows, columns and summed weights
+    bit-identical (rows sorted); frac=0 puts everything in 3 clusters so the
+    coarse rows exceed the LDS tables."""
+    g, obj = CASES[name]
+    nc = 3 if frac == 0.0 else max(1, int(g.n * frac))
+    rng = np.random.default_rng(int(frac * 1000) + 7)
+    vc = rng.integers(0, nc, g.n).astype(np.int32)
+    vc[: min(nc, g.n)] = np.arange(min(nc, g.n))
+    print(f"contract {name} nc={nc}", flush=True)
+    rp, col, w = C().build_coarse_graph(dgraph(g, obj), vc, nc)
+    rrp, rcol, rw = O.contract(g.rp, g.col, weights_for(g, obj), vc, nc)
+    assert np.array_equal(npy(rp), rrp)
+    assert np.array_equal(npy(col), rcol)
+    assert np.array_equal(npy(w), rw)
+
+
+def test_contraction_of_coarse_graph(gpu_ctx):
+    """Contraction input with int64 weights (a coarse level)."""
+    g = graphs.random_graph(5000, 12, seed=3, weights=(1, 9))
+    vc = np.random.default_rng(0).integers(0, 800, g.n).astype(np.int32)
+    vc[:800] = np.arange(800)
+    rrp, rcol, rw = O.contract(g.rp, g.col, g.w, vc, 800)
+    vc2 = np.random.default_rng(1).integers(0, 40, 800).astype(np.int32)
+    vc2[:40] = np.arange(40)
+    dg = C().DeviceGraph(rrp, rcol, rw, np.ones(800, np.int64))
+    rp, col, w = C().build_coarse_graph(dg, vc2, 40)
+    erp, ecol, ew = O.contract(rrp, rcol, rw, vc2, 40)
+    assert np.array_equal(npy(rp), erp) and np.array_equal(npy(col), ecol) and np.array_equal(npy(w), ew)
+
+
+def sweep_state(g, obj, seed, L_frac):
+    rng = np.random.default_rng(seed)
+    vw = g.vertex_weights(obj)
+    L = max(2, int(g.n * L_frac))
+    part = rng.integers(0, L, g.n).astype(np.int32)
+    vol, _, _ = O.evaluate(g.rp, g.col, weights_for(g, obj), vw, part, L)
+    dest = np.where(rng.random(g.n) < 0.2, -3, -1).astype(np.int32)
+    gain = rng.standard_normal(g.n).astype(np.float32)
+    return vw, part, vol, dest, gain
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("ratio", [0.0, 0.75])
+@pytest.mark.parametrize("L_frac", [0.01, 0.3])
+def test_one_sweep_identical_moves(gpu_ctx, name, ratio, L_frac):
+    """One local-move sweep from a fixed state: destinations, FP32 gains and
+    the accepted move list (order1 order) are bit-identical."""
+    g, obj = CASES[name]
+    vw, part, vol, dest, gain = sweep_state(g, obj, 5, L_frac)
+    W = g.nnz if obj == "Mod" else int(vw.sum())
+    lam = 1.0 / W
+    print(f"sweep {name} ratio={ratio} L_frac={L_frac}", flush=True)
+    mv, d2, g2, nc = C().sweep(dgraph(g, obj), part, vol, lam, ratio, False, dest, gain)
+    rmv, rd, rg, rnc = O.sweep(g.rp, g.col, weights_for(g, obj), vw, part, vol, lam, ratio, False, dest, gain)
+    assert nc == rnc
+    assert np.array_equal(npy(d2), rd)
+    assert np.array_equal(npy(g2).view(np.int32), rg.view(np.int32))
+    assert np.array_equal(npy(mv), rmv)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_first_sweep_raw(gpu_ctx, name):
+    g, obj = CASES[name]
+    vw = g.vertex_weights(obj)
+    part = np.arange(g.n, dtype=np.int32)
+    dest = np.full(g.n, -1, np.int32)
+    gain = np.zeros(g.n, np.float32)
+    W = g.nnz if obj == "Mod" else int(vw.sum())
+    print(f"raw sweep {name}", flush=True)
+    mv, d2, g2, nc = C().sweep(dgraph(g, obj), part, vw, 1.0 / W, 0.75, True, dest, gain)
+    rmv, rd, rg, rnc = O.sweep(g.rp, g.col, weights_for(g, obj), vw, part, vw, 1.0 / W, 0.75, True, dest, gain)
+    assert nc == rnc
+    assert np.array_equal(npy(d2), rd)
+    assert np.array_equal(npy(g2).view(np.int32), rg.view(np.int32))
+    assert np.array_equal(npy(mv), rmv)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("mode", ["initial", "improve", "strict"])
+def test_local_move(gpu_ctx, name, mode):
+    g, obj = CASES[name]
+    vw = g.vertex_weights(obj)
+    w = weights_for(g, obj)
+    W = g.nnz if obj == "Mod" else int(vw.sum())
+    lam = 1.0 / W
+    if mode == "initial":
+        part = np.arange(g.n, dtype=np.int32)
+        L = g.n
+    else:
+        L = max(2, g.n // 20)
+        part = np.random.default_rng(9).integers(0, L, g.n).astype(np.int32)
+        part[:L] = np.arange(L)
+    vol, uncut, sq = O.evaluate(g.rp, g.col, w, vw, part, L)
+    obj0 = float(uncut) - lam * float(sq)
+    print(f"local_move {name} {mode}", flush=True)
+    rp_, rv, rL, ru, ro = O.local_move(g.rp, g.col, w, vw, part, vol, L, uncut, lam, obj0, mode == "initial",
+                                        strict=(mode == "strict"))
+    st, volt = C().make_state(vol, L, uncut, lam, obj0, cap=g.n)
+    out = C().local_move(dgraph(g, obj), part, st, mode == "initial", strict=(mode == "strict"))
+    assert np.array_equal(npy(out), rp_)
+    assert st.label_count == rL and st.uncut == ru and st.obj == ro
+    assert np.array_equal(npy(volt)[:rL], rv)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("seed", [1, 2])
+def test_leiden_refine(gpu_ctx, name, seed):
+    g, obj = CASES[name]
+    vw = g.vertex_weights(obj)
+    w = weights_for(g, obj)
+    _, louv, _ = O.cluster(g.rp, g.col, w, vw, obj, 1.0, "louvain", 0)
+    L = int(louv.max()) + 1
+    vol, _, _ = O.evaluate(g.rp, g.col, w, vw, louv, L)
+    W = g.nnz if obj == "Mod" else int(vw.sum())
+    uni = w is None
+    print(f"leiden_refine {name} seed={seed}", flush=True)
+    vc, nc = C().coarsen_leidenR(dgraph(g, obj), louv, vol, 1.0 / W, uni, seed)
+    rvc, rnc = O.leiden_refine(g.rp, g.col, w, vw, louv, vol, 1.0 / W, uni, seed)
+    assert nc == rnc
+    assert np.array_equal(npy(vc), rvc)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("algo", ["louvain", "leiden", "leiden+"])
+@pytest.mark.parametrize("ex_iters", [0, -1])
+def test_full_clustering(gpu_ctx, name, algo, ex_iters):
+    """Whole runs (multilevel + ex_iters loop): identical clustering and the
+    exact same objective value as the oracle."""
+    g, obj = CASES[name]
+    vw = g.vertex_weights(obj)
+    w = weights_for(g, obj)
+    print(f"cluster {name} {algo} ex_iters={ex_iters}", flush=True)
+    q, part, info = C().cluster(dgraph(g, obj), algo, obj, 1.0, ex_iters, seed=7)
+    rq, rpart, rinfo = O.cluster(g.rp, g.col, w, vw, obj, 1.0, algo, ex_iters, seed=7)
+    assert np.array_equal(npy(part), rpart)
+    assert q == rq
+    for k in ["labels", "uncut", "sweeps", "moves", "outer"]:
+        assert info[k] == rinfo[k], k

@@ -1,0 +1,283 @@
+"""CPU tests that pin the oracle against independent references: library
+routines (networkx, scipy, scikit-learn), closed forms, brute force on tiny
+graphs, invariants, and published known answers."""
+import itertools
+
+import networkx as nx
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import oracle as O
+from pennsylvaniagpugraphclustering_amd import graphs
+from graphcases import hub_graph
+
+
+def nx_graph(g):
+    G = nx.Graph()
+    G.add_nodes_from(range(g.n))
+    rows = np.repeat(np.arange(g.n), np.diff(g.rp))
+    if g.w is None:
+        G.add_edges_from((int(a), int(b)) for a, b in zip(rows, g.col) if a < b)
+    else:
+        G.add_weighted_edges_from((int(a), int(b), int(w)) for a, b, w in zip(rows, g.col, g.w) if a < b)
+    return G
+
+
+def comms(labels):
+    d = {}
+    for i, l in enumerate(labels):
+        d.setdefault(int(l), set()).add(i)
+    return list(d.values())
+
+
+def oracle_q(g, labels, objective="Mod"):
+    vw = g.vertex_weights(objective)
+    w = g.w if objective == "WMod" else None
+    L = int(labels.max()) + 1
+    vol, uncut, sq = O.evaluate(g.rp, g.col, w, vw, labels.astype(np.int32), L)
+    W = g.nnz if (objective == "Mod" or g.w is None) else int(vw.sum())
+    return uncut / W - sq / (W * W), vol, uncut, sq
+
+
+# ---------------------------------------------------------------- hashes
+def test_murmur3_matches_scikit_learn():
+    from sklearn.utils import murmurhash3_32
+    rng = np.random.default_rng(0)
+    keys = np.concatenate([[0, 1, -1, 2**31 - 1, -2**31], rng.integers(-2**31, 2**31, 2000)])
+    for k in keys:
+        assert O.murmur3(int(k)) == murmurhash3_32(int(k), seed=0, positive=True)
+
+
+def test_xorshift32_published_first_term():
+    # Marsaglia (2003) 13/17/5 xorshift: seed 1 -> 270369
+    assert O.xorshift32(1) == 270369
+    # bijection spot-check on a sample (no collisions)
+    xs = np.random.default_rng(1).integers(0, 2**32, 20000, dtype=np.uint64)
+    assert len({O.xorshift32(int(x)) for x in xs}) == len(set(xs.tolist()))
+
+
+# ---------------------------------------------------------------- evaluator
+@pytest.mark.parametrize("name", ["karate", "er", "er_w", "sbm"])
+def test_evaluator_matches_networkx(name):
+    rng = np.random.default_rng(7)
+    if name == "karate":
+        g, obj = graphs.karate(), "Mod"
+    elif name == "er":
+        g, obj = graphs.random_graph(500, 6, seed=3), "Mod"
+    elif name == "er_w":
+        g, obj = graphs.random_graph(500, 6, seed=3, weights=(1, 50)), "WMod"
+    else:
+        g, obj = graphs.sbm(1024, 16, 10, 3, seed=2), "Mod"
+    G = nx_graph(g)
+    for L in [1, 3, 17, g.n]:
+        labels = rng.integers(0, L, g.n).astype(np.int32) if L < g.n else np.arange(g.n, dtype=np.int32)
+        q, vol, uncut, sq = oracle_q(g, labels, obj)
+        qn = nx.community.modularity(G, comms(labels), weight="weight" if obj == "WMod" else None)
+        assert abs(q - qn) <= 1e-12 * max(1.0, abs(qn))
+        assert int(vol.sum()) == int(g.vertex_weights(obj).sum())
+
+
+def test_karate_known_optimum():
+    # Brandes et al., IEEE TKDE 2008: max modularity of Zachary's karate club
+    # is 0.4198 (4 communities).
+    g = graphs.karate()
+    q, part, info = O.cluster(g.rp, g.col, None, g.vertex_weights("Mod"), "Mod", 1.0, "leiden+", -1, seed=5)
+    assert abs(q - 0.4198) < 1e-4
+    assert info["labels"] == 4
+
+
+def all_partitions(n):
+    def rec(i, labels, k):
+        if i == n:
+            yield list(labels)
+            return
+        for l in range(k + 1):
+            labels.append(l)
+            yield from rec(i + 1, labels, max(k, l + 1))
+            labels.pop()
+    yield from rec(0, [], 0)
+
+
+def test_brute_force_small_graphs():
+    """The best partition found by pLeiden+ (to stable) on tiny graphs equals
+    the exhaustive maximum of modularity, and the evaluator agrees with
+    networkx on every partition."""
+    for seed in range(4):
+        g = graphs.random_graph(8, 3, seed=seed)
+        if g.nnz == 0:
+            continue
+        G = nx_graph(g)
+        best = -1
+        for lab in all_partitions(g.n):
+            lab = np.array(lab, np.int32)
+            q = oracle_q(g, lab)[0]
+            if seed == 0:
+                assert abs(q - nx.community.modularity(G, comms(lab))) < 1e-12
+            best = max(best, q)
+        q, _, _ = O.cluster(g.rp, g.col, None, g.vertex_weights("Mod"), "Mod", 1.0, "leiden+", -1, seed=1)
+        assert q <= best + 1e-12
+        assert q >= best - 0.05
+
+
+# ---------------------------------------------------------------- contraction
+@pytest.mark.parametrize("nc", [1, 7, 300, 2000])
+def test_contraction_matches_scipy(nc):
+    g = graphs.random_graph(2000, 12, seed=nc, weights=(1, 9))
+    rng = np.random.default_rng(nc)
+    vc = rng.integers(0, nc, g.n).astype(np.int32)
+    vc[:nc] = np.arange(nc)  # every label used
+    rp, col, w = O.contract(g.rp, g.col, g.w, vc, nc)
+    A = sp.csr_matrix((g.w.astype(np.int64), g.col, g.rp), shape=(g.n, g.n))
+    P = sp.csr_matrix((np.ones(g.n, np.int64), (np.arange(g.n), vc)), shape=(g.n, nc))
+    C = (P.T @ A @ P).tolil()
+    C.setdiag(0)
+    C = C.tocsr()
+    C.eliminate_zeros()
+    C.sort_indices()
+    assert np.array_equal(rp, C.indptr)
+    assert np.array_equal(col, C.indices)
+    assert np.array_equal(w, C.data)
+    # invariants: symmetric, no loops, weight conservation
+    intra = sum(int(g.w[j]) for i in range(g.n) for j in range(g.rp[i], g.rp[i + 1]) if vc[i] == vc[g.col[j]])
+    assert int(w.sum()) + intra == int(g.w.astype(np.int64).sum())
+
+
+# ---------------------------------------------------------------- ordering
+def test_orderings_stable_buckets():
+    g = hub_graph(n=20000, hub_deg=16000)
+    o1, o2, offs = O.orderings(g.rp)
+    d = np.diff(g.rp)
+    b1 = np.where(d < 128, 0, np.where(d < 15000, 1, 2))
+    b2 = np.where(d < 64, 0, np.where(d < 15000, 1, 2))
+    assert np.array_equal(o1, np.argsort(b1, kind="stable"))
+    assert np.array_equal(o2, np.argsort(b2, kind="stable"))
+    assert list(offs) == [int((b1 == 0).sum()), int((b1 < 2).sum()), int((b2 == 0).sum()), int((b2 < 2).sum())]
+
+
+# ---------------------------------------------------------------- sweep gains
+def test_sweep_gain_is_half_the_exact_objective_delta():
+    """From a fixed clustering, every candidate's FP32 gain equals half of
+    the exact change of obj = uncut - lambda*sum(vol^2) for its chosen move
+    (to FP32 rounding), and no other neighbouring cluster (or a new
+    singleton) has a larger exact change."""
+    g = graphs.random_graph(300, 8, seed=11)
+    vw = g.vertex_weights("Mod").astype(np.int64)
+    lam = 1.0 / g.nnz
+    rng = np.random.default_rng(3)
+    L = 40
+    part = rng.integers(0, L, g.n).astype(np.int32)
+    vol, uncut, sq = O.evaluate(g.rp, g.col, None, vw, part, L)
+    dest = np.full(g.n, -1, np.int32)
+    gain = np.zeros(g.n, np.float32)
+    moves, dest, gain, ncand = O.sweep(g.rp, g.col, None, vw, part, vol, lam, 0.0, False, dest, gain)
+
+    def obj_of(lab):
+        Lx = int(lab.max()) + 1
+        _, u, s = O.evaluate(g.rp, g.col, None, vw, lab.astype(np.int32), Lx)
+        return u - lam * s
+
+    base = obj_of(part)
+    checked = 0
+    for i in range(g.n):
+        if dest[i] in (-3,):
+            continue
+        nbr = set(part[g.col[g.rp[i]:g.rp[i + 1]]].tolist()) - {part[i]}
+        deltas = {}
+        for c in nbr:
+            lab = part.copy(); lab[i] = c
+            deltas[c] = obj_of(lab) - base
+        lab = part.copy(); lab[i] = L + 5
+        deltas[-4] = obj_of(lab) - base
+        tgt = int(dest[i])
+        assert tgt in deltas
+        assert abs(gain[i] - deltas[tgt] / 2) <= 1e-4 * max(1.0, abs(deltas[tgt]))
+        assert deltas[tgt] >= max(deltas.values()) - 1e-4
+        checked += 1
+    assert checked > 50
+
+
+# ---------------------------------------------------------------- leidenR
+def bfs_connected(g, members):
+    members = set(members)
+    start = next(iter(members))
+    seen = {start}
+    stack = [start]
+    while stack:
+        u = stack.pop()
+        for v in g.col[g.rp[u]:g.rp[u + 1]]:
+            v = int(v)
+            if v in members and v not in seen:
+                seen.add(v)
+                stack.append(v)
+    return len(seen) == len(members)
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_leiden_refine_invariants(weighted):
+    g = graphs.sbm(3000, 20, 10, 4, seed=9, weights=(1, 20) if weighted else None)
+    obj = "WMod" if weighted else "Mod"
+    vw = g.vertex_weights(obj)
+    w = g.w if weighted else None
+    q, louv, info = O.cluster(g.rp, g.col, w, vw, obj, 1.0, "louvain", 0)
+    L = int(louv.max()) + 1
+    vol, uncut, sq = O.evaluate(g.rp, g.col, w, vw, louv, L)
+    W = g.nnz if not weighted else int(vw.sum())
+    for seed in range(3):
+        vc, nc = O.leiden_refine(g.rp, g.col, w, vw, louv, vol, 1.0 / W, not weighted, seed)
+        assert sorted(set(vc.tolist())) == list(range(nc))
+        assert nc >= L
+        groups = {}
+        for i, c in enumerate(vc):
+            groups.setdefault(int(c), []).append(i)
+        for c, mem in groups.items():
+            assert len({int(louv[m]) for m in mem}) == 1  # refinement of louv
+            assert bfs_connected(g, mem)                   # Leiden guarantee
+
+
+# ---------------------------------------------------------------- full runs
+@pytest.mark.parametrize("algo", ["louvain", "leiden", "leiden+"])
+def test_full_runs_beat_networkx_louvain(algo):
+    g = graphs.sbm(4096, 32, 12, 4, seed=1, permute_seed=1)
+    G = nx_graph(g)
+    qn = nx.community.modularity(G, nx.community.louvain_communities(G, seed=1))
+    q, part, info = O.cluster(g.rp, g.col, None, g.vertex_weights("Mod"), "Mod", 1.0, algo, -1, seed=2)
+    assert abs(q - oracle_q(g, part)[0]) < 1e-12
+    assert q >= qn - 2e-3, (q, qn)
+
+
+def test_ex_iters_does_not_lose_quality():
+    g = graphs.rmat(11, 16, seed=4)
+    vw = g.vertex_weights("Mod")
+    q0, _, _ = O.cluster(g.rp, g.col, None, vw, "Mod", 1.0, "louvain", 0)
+    q1, _, info = O.cluster(g.rp, g.col, None, vw, "Mod", 1.0, "louvain", -1)
+    assert info["outer"] >= 2
+    assert q1 >= q0 - 1e-3
+
+
+def test_strict_reaches_node_optimality():
+    """After local_move_strict stops moving, no single-vertex move has a
+    positive exact objective change (the local-optimality condition)."""
+    g = graphs.random_graph(200, 6, seed=21)
+    vw = g.vertex_weights("Mod").astype(np.int64)
+    lam = 1.0 / g.nnz
+    part = np.arange(g.n, dtype=np.int32)
+    vol = vw.copy()
+    L, uncut, obj = g.n, 0, -lam * float((vw.astype(np.float64) ** 2).sum())
+    part, vol, L, uncut, obj = O.local_move(g.rp, g.col, None, vw, part, vol, L, uncut, lam, obj, True)
+    for _ in range(20):
+        p2, v2, L2, u2, o2 = O.local_move(g.rp, g.col, None, vw, part, vol, L, uncut, lam, obj, False, strict=True)
+        if np.array_equal(p2, part):
+            break
+        part, vol, L, uncut, obj = p2, v2, L2, u2, o2
+
+    def exact(lab):
+        _, u, s = O.evaluate(g.rp, g.col, None, vw, lab.astype(np.int32), int(lab.max()) + 1)
+        return u - lam * s
+
+    base = exact(part)
+    assert abs(base - obj) < 1e-9 * max(1, abs(obj))
+    for i in range(g.n):
+        for c in set(part[g.col[g.rp[i]:g.rp[i + 1]]].tolist()) - {part[i]}:
+            lab = part.copy(); lab[i] = c
+            assert exact(lab) - base <= 1e-6
