@@ -1,0 +1,135 @@
+"""Benchmark: end-to-end pLouvain to stable (ex_iters = -1) on the R-MAT
+scale-24 graph (edge factor 16, a/b/c/d = .57/.19/.19/.05, symmetrised,
+deduplicated, no self loops, unit weights, int64 row offsets, int32 columns,
+seed 2), one MI355X.
+
+One step = one full clustering run (multilevel local moving, contraction and
+the ex_iters loop) from the device-resident CSR to the device-resident
+clustering.  value = undirected edges / seconds per step.
+
+Prints one JSON line.  The cpu_baseline leg times the CPU oracle (a serial
+restatement, 1 core) on a bounded sample (an R-MAT graph of smaller scale)
+and reports its edges/s.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--scale", type=int, default=24)
+    p.add_argument("--algo", default="louvain", choices=["louvain", "leiden", "leiden+"])
+    p.add_argument("--ex-iters", type=int, default=-1)
+    p.add_argument("--cpu-scale", type=int, default=16)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-leiden", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline(scale, seed=2):
+    """Serial CPU oracle on an R-MAT graph of the given scale (same
+    construction, numpy stream): pLouvain to stable, edges/s."""
+    from oracle import oracle as O
+    from pennsylvaniagpugraphclustering_amd import graphs
+    g = graphs.rmat(scale, 16, seed=seed)
+    vw = g.vertex_weights("Mod")
+    t0 = time.perf_counter()
+    q, part, info = O.cluster(g.rp, g.col, None, vw, "Mod", 1.0, "louvain", -1, seed=0)
+    dt = time.perf_counter() - t0
+    return {"value": (g.nnz // 2) / dt, "unit": "edges/s", "cores": 1, "kind": "port",
+            "sample": f"R-MAT scale {scale} (n={g.n}, m={g.nnz // 2}), pLouvain ex_iters=-1, "
+                      f"{dt:.2f} s, Q={q:.6f}"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+    from pennsylvaniagpugraphclustering_amd import graphs
+    from pennsylvaniagpugraphclustering_amd import clustering as C
+
+    dev = torch.device("cuda")
+    t0 = time.perf_counter()
+    rp, col, _ = graphs.rmat_torch(a.scale, 16, seed=2, device=dev)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    ctx = C.default_context()
+    g = C.DeviceGraph(rp, col, None, None, objective="Mod", ctx=ctx)
+    m = g.nnz // 2
+    print(f"# graph R-MAT s{a.scale}: n={g.n} nnz={g.nnz} m={m} (generated in {gen_s:.1f}s)", file=sys.stderr,
+          flush=True)
+
+    def step():
+        return C.cluster(g, a.algo, "Mod", 1.0, a.ex_iters, seed=0, ctx=ctx)
+
+    for w in range(a.warmup):
+        q, part, info = step()
+        print(f"# warmup {w}: Q={q:.6f} info={info}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(a.steps):
+        q, part, info = step()
+        print(f"# step {s}: Q={q:.6f}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / max(a.steps, 1)
+
+    extra = {}
+    if not a.no_leiden:
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ql, _, infol = C.cluster(g, "leiden+", "Mod", 1.0, a.ex_iters, seed=0, ctx=ctx)
+        torch.cuda.synchronize()
+        extra["leiden_plus"] = {"seconds": time.perf_counter() - t1, "Q": ql, "labels": infol["labels"],
+                                "sweeps": infol["sweeps"]}
+        print(f"# pLeiden+: {extra['leiden_plus']}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(a.cpu_scale)
+        print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "end-to-end clustering edges/s (pLouvain to stable)",
+        "value": m / dt,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt * 1000.0,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64+fp32",
+        "data": "synthetic",
+        "config": {"workload": f"R-MAT scale {a.scale} ef16 (.57,.19,.19,.05) seed 2, unit weights, "
+                               f"{a.algo} ex_iters={a.ex_iters}, objective Mod",
+                   "n": g.n, "nnz": g.nnz, "undirected_edges": m},
+        "modularity": q,
+        "labels": info["labels"],
+        "sweeps": info["sweeps"],
+        "levels": info["levels"],
+        "outer_iterations": info["outer"],
+        "edges_per_s_per_sweep": m * info["sweeps"] / dt,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -248,8 +248,25 @@ struct Mover {
 // smallest label; NEW_PART (-4) competes with j_conn 0 when p_conn < 0, so a
 // real cluster beats it only with j_conn > 0 (the team path's min-reduce,
 // :553-561, applied with one table slot per thread).
+struct Agg {  // dense accumulator over labels with a touched list
+    std::vector<i64> acc;
+    std::vector<i32> keys;
+    std::vector<char> seen;
+    void ensure(size_t n) {
+        if (acc.size() < n) { acc.resize(n, 0); seen.resize(n, 0); }
+    }
+    void add(i32 k, i64 w) {
+        if (!seen[k]) { seen[k] = 1; keys.push_back(k); }
+        acc[k] += w;
+    }
+    void clear() {
+        for (i32 k : keys) { acc[k] = 0; seen[k] = 0; }
+        keys.clear();
+    }
+};
+
 static void score_vertex(Mover& M, State& st, i64 i, float lam, float ratio, bool constrained,
-                         const i32* constraint, std::unordered_map<i32, i64>& agg) {
+                         const i32* constraint, Agg& agg) {
     const Graph& g = M.g;
     if (M.dest[i] == NO_MOVE) return;  // :477-479
     const bool raw = !M.tables;        // first sweep scores the raw adjacency :1391-1395
@@ -262,7 +279,7 @@ static void score_vertex(Mover& M, State& st, i64 i, float lam, float ratio, boo
         i32 v = g.col[j];
         i32 key = raw ? v : M.part[v];
         if (!raw && key == p) pv += g.wt(j);
-        else agg[key] += g.wt(j);
+        else agg.add(key, g.wt(j));
     }
     if (raw) pv = 0;  // pvals are zero before the first table build (:1349)
     float p_conn = fmaf(-(float)(st.vol[p] - w), multi, (float)pv);
@@ -271,9 +288,8 @@ static void score_vertex(Mover& M, State& st, i64 i, float lam, float ratio, boo
     bool have = false;
     float bj = 0;
     i32 bc = 0;
-    for (auto& kv : agg) {
-        i32 c = kv.first;
-        i64 s = kv.second;
+    for (i32 c : agg.keys) {
+        i64 s = agg.acc[c];
         if (!(s > 0 && (float)s >= b)) continue;
         if (constrained && constraint[c] != constraint[p]) continue;
         float jc = fmaf(-(float)st.vol[c], multi, (float)s);
@@ -299,7 +315,8 @@ static void score_vertex(Mover& M, State& st, i64 i, float lam, float ratio, boo
 // vertex, then list the ones with dest != NO_MOVE in order1 order.
 static std::vector<i32> candidates(Mover& M, State& st, float ratio, bool constrained, const i32* constraint) {
     float lam = (float)st.lambda;
-    std::unordered_map<i32, i64> agg;
+    Agg agg;
+    agg.ensure(std::max<size_t>(st.vol.size(), (size_t)M.g.n) + 1);
     for (i64 i = 0; i < M.g.n; i++) score_vertex(M, st, i, lam, ratio, constrained, constraint, agg);
     std::vector<i32> out;
     for (i64 x = 0; x < M.g.n; x++) {
