@@ -4,13 +4,23 @@
 
 namespace pggc {
 
+// Cross-lane guard.  On gfx950 with this toolchain a ds_bpermute (the
+// instruction behind __shfl*) issued right after the EXEC restore that ends a
+// divergent loop or branch was observed to read stale lanes (run-to-run
+// different reductions); 16 wait states before the cross-lane sequence made
+// every reduction exact and deterministic.  Every cross-lane helper and every
+// kernel-level shuffle/ballot sequence starts with this guard.
+__device__ __forceinline__ void xlane_guard() { asm volatile("s_nop 7\n\ts_nop 7" ::: "memory"); }
+
 __device__ __forceinline__ i64 wave_sum_i64(i64 v) {
+    xlane_guard();
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
 
 __device__ __forceinline__ u64 wave_max_u64(u64 v) {
+    xlane_guard();
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         u64 y = __shfl_xor(v, o);
@@ -22,12 +32,14 @@ __device__ __forceinline__ u64 wave_max_u64(u64 v) {
 // group reductions over `G` consecutive lanes (G a power of two <= 64)
 template <int G>
 __device__ __forceinline__ i64 group_sum_i64(i64 v) {
+    xlane_guard();
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
     return v;
 }
 template <int G>
 __device__ __forceinline__ u64 group_max_u64(u64 v) {
+    xlane_guard();
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) {
         u64 y = __shfl_xor(v, o, G);
@@ -37,6 +49,7 @@ __device__ __forceinline__ u64 group_max_u64(u64 v) {
 }
 template <int G>
 __device__ __forceinline__ i64 group_max_i64(i64 v) {
+    xlane_guard();
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) {
         i64 y = __shfl_xor(v, o, G);
@@ -75,10 +88,11 @@ __device__ __forceinline__ i64 block_excl_scan_i64(i64 v, i64* total) {
     __shared__ i64 ws[16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
     i64 x = v;
+    xlane_guard();
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         i64 y = __shfl_up(x, o);
-        if (lane >= o) x += y;
+        x += (lane >= o) ? y : (i64)0;
     }
     if (lane == 63) ws[w] = x;
     __syncthreads();

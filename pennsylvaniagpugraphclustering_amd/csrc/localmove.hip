@@ -157,6 +157,7 @@ __global__ void __launch_bounds__(256) k_score_group(const i64* __restrict__ rp,
             wt = W(s + gl);
         }
         i64 agg = 0;
+        xlane_guard();
 #pragma unroll
         for (int k = 0; k < G; k++) {
             i32 kk = __shfl(key, k, G);
@@ -374,6 +375,7 @@ __global__ void __launch_bounds__(256) k_ab_wave(const i64* __restrict__ rp, con
         float acc = 0.0f;
         for (i64 j = rp[i] + lane; j < rp[i + 1]; j += 64)
             ab_term(col, W, vw, part, dest, gain, j, p, best, multi, igain, hi, acc);
+        xlane_guard();
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_down(acc, off);
         if (lane == 0) {
@@ -408,6 +410,7 @@ __global__ void __launch_bounds__(1024) k_ab_block(const i64* __restrict__ rp, c
         }
         if (t < 64) {
             acc = sh[t];
+            xlane_guard();
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_down(acc, off);
             if (t == 0) {

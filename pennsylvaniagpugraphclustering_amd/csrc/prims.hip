@@ -245,6 +245,7 @@ __global__ void __launch_bounds__(RT) k_radix_scatter(const u64* kin, const i32*
         u32 d = (u32)(key >> shift) & 255u;
         wc[0][t] = 0; wc[1][t] = 0; wc[2][t] = 0; wc[3][t] = 0;
         __syncthreads();
+        xlane_guard();
         u64 peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
