@@ -68,6 +68,7 @@ def main():
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
     ctx = C.default_context()
+    ctx.set_verbose(1)
     g = C.DeviceGraph(rp, col, None, None, objective="Mod", ctx=ctx)
     m = g.nnz // 2
     print(f"# graph R-MAT s{a.scale}: n={g.n} nnz={g.nnz} m={m} (generated in {gen_s:.1f}s)", file=sys.stderr,

@@ -87,6 +87,9 @@ int pggc_ctx_destroy(pggc_ctx* ctx);
  * std::random_device, src/leidenR.cpp:289,291,399,479) */
 int pggc_ctx_set_seed(pggc_ctx* ctx, uint64_t seed, uint64_t draws);
 int pggc_ctx_get_draws(pggc_ctx* ctx, uint64_t* draws);
+/* level > 0: pggc_cluster prints one line per outer iteration to stderr, as
+ * the reference CLI does (app/cluster_main.cpp:137,156) */
+int pggc_ctx_set_verbose(pggc_ctx* ctx, int level);
 /* bytes held by the context allocator (host out) */
 int pggc_ctx_memory(pggc_ctx* ctx, int64_t* live, int64_t* cached);
 

@@ -63,6 +63,7 @@ SIGNATURES = {
     "pggc_ctx_destroy": (c.c_int, [P]),
     "pggc_ctx_set_seed": (c.c_int, [P, c.c_uint64, c.c_uint64]),
     "pggc_ctx_get_draws": (c.c_int, [P, c.POINTER(c.c_uint64)]),
+    "pggc_ctx_set_verbose": (c.c_int, [P, c.c_int]),
     "pggc_ctx_memory": (c.c_int, [P, I64P, I64P]),
     "pggc_vertex_weights": (c.c_int, [P, c.POINTER(Graph), c.c_int, P]),
     "pggc_state_init": (c.c_int, [P, c.POINTER(Graph), c.c_int, c.c_double, c.POINTER(State)]),

@@ -59,6 +59,9 @@ class Context:
         except Exception:
             pass
 
+    def set_verbose(self, level=1):
+        check(self.lib.pggc_ctx_set_verbose(self.h, int(level)))
+
     def memory(self):
         a, b = ctypes.c_int64(), ctypes.c_int64()
         check(self.lib.pggc_ctx_memory(self.h, ctypes.byref(a), ctypes.byref(b)))

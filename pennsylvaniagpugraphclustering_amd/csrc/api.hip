@@ -1,5 +1,7 @@
 // Multilevel drivers (src/clustering_methods.cpp), the run_clustering outer
 // loop (app/cluster_main.cpp:101-165) and the C ABI declared in pggc.h.
+#include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -363,6 +365,12 @@ int pggc_ctx_set_seed(pggc_ctx* ctx, uint64_t seed, uint64_t draws) {
         ctx->c.draws = draws;
     });
 }
+int pggc_ctx_set_verbose(pggc_ctx* ctx, int level) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        ctx->c.verbose = level;
+    });
+}
 int pggc_ctx_get_draws(pggc_ctx* ctx, uint64_t* draws) {
     return guard([&] {
         CTX_CHECK(ctx);
@@ -567,8 +575,21 @@ int pggc_cluster(pggc_ctx* ctx, const pggc_graph* gg, int algo, int objective, d
         state_init(c, g, objective, mult, rd);
         RunStats rs;
         LevelPlan top;
+        // per-iteration line as printed by the reference (app/cluster_main.cpp:137,156,
+        // interface/cluster_data.hpp:98-103)
+        auto t_iter = std::chrono::steady_clock::now();
+        auto report = [&]() {
+            if (!c.verbose) return;
+            auto now = std::chrono::steady_clock::now();
+            double secs = std::chrono::duration<double>(now - t_iter).count();
+            t_iter = now;
+            fprintf(stderr, "Cluster time: %g; Cut: %lld; Objective: %.9g; Labels: %d; sweeps so far: %lld\n", secs,
+                    (long long)((rd.top_nnz - rd.uncut) / 2), rd.objective(), rd.label_count, (long long)rs.mv.sweeps);
+            fflush(stderr);
+        };
         if (algo == PGGC_LOUVAIN) louvain_part(c, ctx->ms, g, rd, false, nullptr, part_out, rs, &top);
         else leiden_part(c, ctx->ms, g, rd, algo == PGGC_LEIDEN_PLUS, false, nullptr, part_out, rs, &top);
+        report();
         double last = -std::numeric_limits<double>::infinity();
         int counter = 0;
         i64 outer = 1;
@@ -586,6 +607,7 @@ int pggc_cluster(pggc_ctx* ctx, const pggc_graph* gg, int algo, int objective, d
             if (algo == PGGC_LOUVAIN) louvain_part(c, ctx->ms, g, rd, true, cons, part_out, rs, &top);
             else leiden_part(c, ctx->ms, g, rd, algo == PGGC_LEIDEN_PLUS, true, cons, part_out, rs, &top);
             outer++;
+            report();
         }
         c.sync();
         *objective_out = rd.objective();

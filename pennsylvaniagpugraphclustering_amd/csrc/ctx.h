@@ -22,6 +22,7 @@ struct Ctx {
     i64* pinned = nullptr;  // 64 pinned int64 readback slots
     u64 seed = 0;           // leidenR seed stream
     u64 draws = 0;
+    int verbose = 0;        // >0: per-iteration progress lines on stderr
 
     void* alloc(size_t bytes);
     void release(void* p);

@@ -208,3 +208,32 @@ def test_full_clustering(gpu_ctx, name, algo, ex_iters):
     assert q == rq
     for k in ["labels", "uncut", "sweeps", "moves", "outer"]:
         assert info[k] == rinfo[k], k
+
+
+@pytest.mark.parametrize("algo", ["louvain", "leiden+"])
+def test_full_clustering_rmat16(gpu_ctx, algo):
+    """R-MAT scale 16 (65,536 vertices, 1.8M nonzeros, hubs up to degree
+    ~10^4): one full run is bit-identical to the oracle."""
+    g = graphs.rmat(16, 16, seed=2)
+    vw = g.vertex_weights("Mod")
+    print(f"cluster rmat16 {algo}", flush=True)
+    q, part, info = C().cluster(dgraph(g, "Mod"), algo, "Mod", 1.0, 0, seed=3)
+    rq, rpart, rinfo = O.cluster(g.rp, g.col, None, vw, "Mod", 1.0, algo, 0, seed=3)
+    assert np.array_equal(npy(part), rpart)
+    assert q == rq
+
+
+def test_determinism_rmat20(gpu_ctx):
+    """Size-independent property at scale 20 (1M vertices, 33M nonzeros):
+    two runs give the same clustering and objective, and the objective equals
+    a fresh evaluation of the returned clustering."""
+    import torch
+    rp, col, _ = graphs.rmat_torch(20, 16, seed=2, device="cuda")
+    dg = C().DeviceGraph(rp, col, None, None, objective="Mod")
+    print("determinism rmat20", flush=True)
+    q1, p1, i1 = C().cluster(dg, "louvain", "Mod", 1.0, 0, seed=1)
+    q2, p2, i2 = C().cluster(dg, "louvain", "Mod", 1.0, 0, seed=1)
+    assert torch.equal(p1, p2) and q1 == q2
+    q3 = C().modularity(dg, p1, "Mod")
+    assert abs(q3 - q1) <= 1e-12
+    assert i1["labels"] == int(p1.max().item()) + 1
