@@ -90,6 +90,24 @@ int pggc_ctx_get_draws(pggc_ctx* ctx, uint64_t* draws);
 /* level > 0: pggc_cluster prints one line per outer iteration to stderr, as
  * the reference CLI does (app/cluster_main.cpp:137,156) */
 int pggc_ctx_set_verbose(pggc_ctx* ctx, int level);
+/* Kernel timing (measurement hook, no reference counterpart): when on, HIP
+ * events are recorded on the context stream around the launches of each slot
+ * below, together with the algorithmic bytes of each launch.  set_timing
+ * resets the totals; kernel_stats waits for the stream and returns the
+ * launches, summed milliseconds and summed algorithmic bytes of one slot. */
+#define PGGC_KS_SCORE_GROUP 0    /* neighbour-cluster scoring, d <= 64 */
+#define PGGC_KS_SCORE_TABLE 1    /* scoring with LDS tables, 64 < d <= 2048 */
+#define PGGC_KS_SCORE_HUB 2      /* scoring of hubs, HBM tables */
+#define PGGC_KS_AFTERBURNER 3    /* conflict filter */
+#define PGGC_KS_MOVES 4          /* exact uncut delta + volume moves */
+#define PGGC_KS_CONTRACT_AGG 5   /* coarse-row merge in LDS */
+#define PGGC_KS_CONTRACT_BIG 6   /* coarse-row merge in HBM tables */
+#define PGGC_KS_CONTRACT_COPY 7  /* coarse-row copy-out */
+#define PGGC_KS_EVAL 8           /* evaluator edge pass */
+#define PGGC_KS_COUNT 9
+int pggc_ctx_set_timing(pggc_ctx* ctx, int on);
+int pggc_ctx_kernel_stats(pggc_ctx* ctx, int slot, int64_t* launches, double* ms, double* bytes);
+
 /* bytes held by the context allocator (host out) */
 int pggc_ctx_memory(pggc_ctx* ctx, int64_t* live, int64_t* cached);
 

@@ -65,6 +65,8 @@ SIGNATURES = {
     "pggc_ctx_get_draws": (c.c_int, [P, c.POINTER(c.c_uint64)]),
     "pggc_ctx_set_verbose": (c.c_int, [P, c.c_int]),
     "pggc_ctx_memory": (c.c_int, [P, I64P, I64P]),
+    "pggc_ctx_set_timing": (c.c_int, [P, c.c_int]),
+    "pggc_ctx_kernel_stats": (c.c_int, [P, c.c_int, I64P, c.POINTER(c.c_double), c.POINTER(c.c_double)]),
     "pggc_vertex_weights": (c.c_int, [P, c.POINTER(Graph), c.c_int, P]),
     "pggc_state_init": (c.c_int, [P, c.POINTER(Graph), c.c_int, c.c_double, c.POINTER(State)]),
     "pggc_orderings": (c.c_int, [P, c.POINTER(Graph), P, P, I64P]),

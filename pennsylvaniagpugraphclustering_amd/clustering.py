@@ -67,6 +67,23 @@ class Context:
         check(self.lib.pggc_ctx_memory(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    KERNEL_SLOTS = ("score_group", "score_table", "score_hub", "afterburner", "moves", "contract_agg",
+                    "contract_big", "contract_copy", "eval")
+
+    def set_timing(self, on=True):
+        """Record HIP events on the context stream around the timed kernel
+        groups (resets the totals)."""
+        check(self.lib.pggc_ctx_set_timing(self.h, 1 if on else 0))
+
+    def kernel_stats(self):
+        """{slot: (launches, milliseconds, algorithmic bytes)} since set_timing."""
+        out = {}
+        for i, name in enumerate(self.KERNEL_SLOTS):
+            n, ms, by = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+            check(self.lib.pggc_ctx_kernel_stats(self.h, i, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)))
+            out[name] = (n.value, ms.value, by.value)
+        return out
+
 
 _default = None
 

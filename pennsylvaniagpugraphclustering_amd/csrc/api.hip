@@ -354,6 +354,7 @@ int pggc_ctx_destroy(pggc_ctx* ctx) {
         ctx->c.live.clear();
         ctx->c.trim();
         if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);
+        ctx->c.kt_free();
         delete ctx;
     });
 }
@@ -371,6 +372,32 @@ int pggc_ctx_set_verbose(pggc_ctx* ctx, int level) {
         ctx->c.verbose = level;
     });
 }
+int pggc_ctx_set_timing(pggc_ctx* ctx, int on) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        ctx->c.kt_resolve();
+        KTimer& k = ctx->c.kt;
+        k.on = on != 0;
+        for (int i = 0; i < KS_COUNT; i++) {
+            k.launches[i] = 0;
+            k.ms[i] = 0.0;
+            k.bytes[i] = 0.0;
+        }
+    });
+}
+
+int pggc_ctx_kernel_stats(pggc_ctx* ctx, int slot, int64_t* launches, double* ms, double* bytes) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (slot < 0 || slot >= KS_COUNT) throw Error(PGGC_EINVAL, "unknown timing slot");
+        ctx->c.kt_resolve();
+        const KTimer& k = ctx->c.kt;
+        if (launches) *launches = k.launches[slot];
+        if (ms) *ms = k.ms[slot];
+        if (bytes) *bytes = k.bytes[slot];
+    });
+}
+
 int pggc_ctx_get_draws(pggc_ctx* ctx, uint64_t* draws) {
     return guard([&] {
         CTX_CHECK(ctx);

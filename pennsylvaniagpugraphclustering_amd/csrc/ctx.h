@@ -11,6 +11,36 @@
 
 namespace pggc {
 
+// Kernel timing slots: HIP events recorded on the context stream around the
+// launches of the kernels named here, with the algorithmic bytes each launch
+// moves (measurement hooks; off unless enabled).
+enum KSlot {
+    KS_SCORE_GROUP = 0,  // k_score_group<G>
+    KS_SCORE_TABLE,      // k_score_table (LDS tables)
+    KS_SCORE_HUB,        // k_hub_insert + k_hub_scan + k_hub_decide
+    KS_AFTERBURNER,      // k_ab_serial / k_ab_wave / k_ab_block
+    KS_MOVES,            // k_delta_uncut + k_apply_moves
+    KS_CONTRACT_AGG,     // k_cagg (LDS merge of coarse rows)
+    KS_CONTRACT_BIG,     // k_cbig_insert + k_cbig_extract
+    KS_CONTRACT_COPY,    // k_crow_copy
+    KS_EVAL,             // k_eval_uncut
+    KS_COUNT
+};
+
+struct KTimer {
+    struct Pending {
+        int slot;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    bool on = false;
+    std::vector<Pending> pend;
+    std::vector<hipEvent_t> pool;
+    i64 launches[KS_COUNT] = {0};
+    double ms[KS_COUNT] = {0};
+    double bytes[KS_COUNT] = {0};
+};
+
 struct Ctx {
     hipStream_t stream = nullptr;
     int device = 0;
@@ -23,6 +53,7 @@ struct Ctx {
     u64 seed = 0;           // leidenR seed stream
     u64 draws = 0;
     int verbose = 0;        // >0: per-iteration progress lines on stderr
+    KTimer kt;
 
     void* alloc(size_t bytes);
     void release(void* p);
@@ -32,6 +63,25 @@ struct Ctx {
     const i64* read(const i64* dev, int count);
     i64 read1(const i64* dev) { return read(dev, 1)[0]; }
     u32 next_draw() { return seed_draw(seed, draws++); }
+    // kernel timing: kt_start/kt_stop bracket launches on `stream`;
+    // kt_resolve waits for the stream and folds the pending pairs into the
+    // per-slot totals.
+    hipEvent_t kt_event();
+    int kt_start(int slot);
+    void kt_stop(int token, double bytes);
+    void kt_resolve();
+    void kt_free();
+};
+
+// Brackets the launches of one timed region (no-op when timing is off).
+struct KScope {
+    Ctx& c;
+    int tok;
+    double bytes;
+    KScope(Ctx& ctx, int slot, double b) : c(ctx), tok(ctx.kt.on ? ctx.kt_start(slot) : -1), bytes(b) {}
+    ~KScope() {
+        if (tok >= 0) c.kt_stop(tok, bytes);
+    }
 };
 
 // RAII device buffer from the context's caching allocator.

@@ -116,4 +116,21 @@ __device__ __forceinline__ i64 upper_row(const i64* rp, i64 lo, i64 hi, i64 j) {
     return lo;
 }
 
+// Advance row r (with nxt = rp[r+1]) until it contains edge j: a few linear
+// steps, then a binary search, so runs of empty rows cost O(log) instead of
+// their length.
+__device__ __forceinline__ i64 advance_row(const i64* rp, i64 r, i64 rhi, i64 j, i64& nxt) {
+    int steps = 0;
+    while (j >= nxt) {
+        if (++steps > 4) {
+            r = upper_row(rp, r, rhi, j);
+            nxt = rp[r + 1];
+            return r;
+        }
+        r++;
+        nxt = rp[r + 1];
+    }
+    return r;
+}
+
 }  // namespace pggc

@@ -153,9 +153,8 @@ __global__ void __launch_bounds__(EB) k_eval_uncut(const i64* rp, const i32* col
         for (int k = 0; k < EPT; k++) {
             i64 j = j0 + k;
             if (j >= cr.ee) break;
-            while (j >= nxt) {
-                r++;
-                nxt = rp[r + 1];
+            if (j >= nxt) {
+                r = advance_row(rp, r, cr.rhi, j, nxt);
                 lr = lab[r];
             }
             if (lab[col[j]] == lr) acc += W(j);
@@ -198,11 +197,10 @@ __global__ void __launch_bounds__(EB) k_own_weight(const i64* rp, const i32* col
     for (int k = 0; k < EPT; k++) {
         i64 j = j0 + k;
         if (j >= cr.ee) break;
-        while (j >= nxt) {
+        if (j >= nxt) {
             if (acc) atomicAdd((unsigned long long*)&out[r], (unsigned long long)acc);
             acc = 0;
-            r++;
-            nxt = rp[r + 1];
+            r = advance_row(rp, r, cr.rhi, j, nxt);
             lr = lab[r];
         }
         if (lab[col[j]] == lr) acc += W(j);
@@ -358,6 +356,91 @@ __global__ void __launch_bounds__(BS) k_cagg(const i64* rp, const i32* col, WT W
     }
 }
 
+// Big coarse rows (bound > C_MID): the flattened member-edge range of each
+// such row is split into work items of BCH edges, each item a block that
+// inserts into the row's HBM hash table; a second pass splits each table into
+// slot items and compacts the occupied slots.
+constexpr int BCH = 4096;  // edges (or slots) per work item
+
+__global__ void k_member_degs(const i32* members, const i64* rp, i64 n, i64* mdp) {
+    for (i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (i64)gridDim.x * blockDim.x) {
+        i32 v = members[k];
+        mdp[k] = rp[v + 1] - rp[v];
+    }
+}
+__global__ void k_items(const i32* list, i64 m, const i64* bound, const i64* goff, i64 nc, int slots, i64* items) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (i64)gridDim.x * blockDim.x) {
+        i64 len = slots ? goff[x + 1] - goff[x] : bound[list[x]];
+        items[x] = (len + BCH - 1) / BCH;
+    }
+}
+
+template <class WT>
+__global__ void __launch_bounds__(256) k_cbig_insert(const i64* rp, const i32* col, WT W, const i32* vcmap,
+                                                      const i32* members, const i64* mstart, const i64* mdp,
+                                                      const i32* big, i64 nbig, const i64* items, const i64* goff,
+                                                      i32* gkeys, i64* gvals) {
+    const i64 b = blockIdx.x;
+    const i64 ui = upper_row(items, 0, nbig, b);
+    const i32 u = big[ui];
+    const i64 m0 = mstart[u], m1 = mstart[u + 1];
+    const i64 e_lo = mdp[m0] + (b - items[ui]) * BCH;
+    const i64 e_end = mdp[m1];
+    const i64 e_hi = e_lo + BCH < e_end ? e_lo + BCH : e_end;
+    i32* keys = gkeys + goff[ui];
+    i64* vals = gvals + goff[ui];
+    const u32 mask = (u32)(goff[ui + 1] - goff[ui] - 1);
+    const i64 e0 = e_lo + (i64)threadIdx.x * 16;
+    if (e0 >= e_hi) return;
+    i64 k = upper_row(mdp, m0, m1, e0);
+    i64 nxt = mdp[k + 1];
+    for (int q = 0; q < 16; q++) {
+        const i64 e = e0 + q;
+        if (e >= e_hi) break;
+        if (e >= nxt) k = advance_row(mdp, k, m1, e, nxt);
+        const i32 v = members[k];
+        const i64 j = rp[v] + (e - mdp[k]);
+        const i32 cv = vcmap[col[j]];
+        if (cv != u) table_add_global(keys, vals, mask, cv, W(j));
+    }
+}
+
+__global__ void __launch_bounds__(256) k_cbig_extract(const i32* big, i64 nbig, const i64* items, const i64* goff,
+                                                       const i32* gkeys, const i64* gvals, const i64* toff,
+                                                       i32* tcol, i64* tw, i64* ccnt) {
+    __shared__ int s_cnt;
+    __shared__ i64 s_base;
+    const i64 b = blockIdx.x;
+    const i64 ui = upper_row(items, 0, nbig, b);
+    const i32 u = big[ui];
+    const i64 s_lo = goff[ui] + (b - items[ui]) * BCH;
+    const i64 s_end = goff[ui + 1];
+    const i64 s_hi = s_lo + BCH < s_end ? s_lo + BCH : s_end;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    i32 kk[16];
+    int pos[16];
+    xlane_guard();
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const i64 sl = s_lo + threadIdx.x + (i64)q * 256;
+        kk[q] = sl < s_hi ? gkeys[sl] : EMPTY_KEY;
+        pos[q] = kk[q] != EMPTY_KEY ? atomicAdd(&s_cnt, 1) : -1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_base = (i64)atomicAdd((unsigned long long*)&ccnt[u], (unsigned long long)s_cnt);
+    __syncthreads();
+    const i64 base = toff[u] + s_base;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        if (pos[q] >= 0) {
+            const i64 sl = s_lo + threadIdx.x + (i64)q * 256;
+            tcol[base + pos[q]] = kk[q];
+            tw[base + pos[q]] = gvals[sl];
+        }
+    }
+}
+
 // Copy temporary rows into the compact coarse CSR.
 __global__ void __launch_bounds__(EB) k_crow_copy(const i64* crp, i64 nc, i64 nnzc, const i64* toff, const i32* tcol,
                                                    const i64* tw, i32* ocol, i64* ow) {
@@ -370,10 +453,7 @@ __global__ void __launch_bounds__(EB) k_crow_copy(const i64* crp, i64 nc, i64 nn
     for (int k = 0; k < EPT; k++) {
         i64 j = j0 + k;
         if (j >= cr.ee) break;
-        while (j >= nxt) {
-            r++;
-            nxt = crp[r + 1];
-        }
+        if (j >= nxt) r = advance_row(crp, r, cr.rhi, j, nxt);
         i64 src = toff[r] + (j - crp[r]);
         ocol[j] = tcol[src];
         ow[j] = tw[src];
@@ -444,18 +524,6 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
     partition_k<4>(c, cls, nc, list, st);
     fill_i64(c, ccnt, nc + 1, 0);
     const i64 n0 = st[1] - st[0], n1 = st[2] - st[1], n2 = st[3] - st[2];
-    Buf<i64> goff;
-    Buf<i32> gkeys;
-    Buf<i64> gvals;
-    if (n2 > 0) {
-        goff = Buf<i64>(c, n2 + 1);
-        k_goff_sizes<<<grid_for(n2, 256, 4096), 256, 0, c.stream>>>(list.p + st[2], n2, bound, nc, goff);
-        PGGC_LAUNCH_CHECK();
-        scan_i64(c, goff, goff, n2);
-        i64 gs = c.read1(goff.p + n2);
-        gkeys = Buf<i32>(c, gs);
-        gvals = Buf<i64>(c, gs);
-    }
     PGGC_WDISPATCH(g, {
         if (n0 > 0)
             k_cagg<64, 512, false><<<grid_for(n0, 1, 1 << 20), 64, 0, c.stream>>>(
@@ -465,14 +533,38 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
             k_cagg<256, 4096, false><<<grid_for(n1, 1, 1 << 20), 256, 0, c.stream>>>(
                 g.rp, g.col, W, vcmap, nc, list.p + st[1], n1, mstart, members, bound, nullptr, nullptr, nullptr,
                 toff, tcol, tw, ccnt);
-        if (n2 > 0)
-            k_cagg<256, 1, true><<<grid_for(n2, 1, 1 << 16), 256, 0, c.stream>>>(
-                g.rp, g.col, W, vcmap, nc, list.p + st[2], n2, mstart, members, bound, gkeys, gvals, goff, toff,
-                tcol, tw, ccnt);
     });
     PGGC_LAUNCH_CHECK();
-    gkeys.reset();
-    gvals.reset();
+    if (n2 > 0) {
+        const i32* big = list.p + st[2];
+        Buf<i64> goff(c, n2 + 1), items(c, n2 + 1), mdp(c, n + 1);
+        k_goff_sizes<<<grid_for(n2, 256, 4096), 256, 0, c.stream>>>(big, n2, bound, nc, goff);
+        PGGC_LAUNCH_CHECK();
+        scan_i64(c, goff, goff, n2);
+        const i64 gs = c.read1(goff.p + n2);
+        Buf<i32> gkeys(c, gs);
+        Buf<i64> gvals(c, gs);
+        fill_i32(c, gkeys, gs, EMPTY_KEY);
+        fill_i64(c, gvals, gs, 0);
+        k_member_degs<<<grid_for(n, 1024, 8192), 256, 0, c.stream>>>(members, g.rp, n, mdp);
+        PGGC_LAUNCH_CHECK();
+        scan_i64(c, mdp, mdp, n);
+        k_items<<<grid_for(n2, 256, 4096), 256, 0, c.stream>>>(big, n2, bound, goff, nc, 0, items);
+        PGGC_LAUNCH_CHECK();
+        scan_i64(c, items, items, n2);
+        const i64 I = c.read1(items.p + n2);
+        if (I > 0) {
+            PGGC_WDISPATCH(g, (k_cbig_insert<<<(unsigned)I, 256, 0, c.stream>>>(
+                                  g.rp, g.col, W, vcmap, members, mstart, mdp, big, n2, items, goff, gkeys, gvals)));
+            PGGC_LAUNCH_CHECK();
+        }
+        k_items<<<grid_for(n2, 256, 4096), 256, 0, c.stream>>>(big, n2, bound, goff, nc, 1, items);
+        PGGC_LAUNCH_CHECK();
+        scan_i64(c, items, items, n2);
+        const i64 I2 = c.read1(items.p + n2);
+        k_cbig_extract<<<(unsigned)I2, 256, 0, c.stream>>>(big, n2, items, goff, gkeys, gvals, toff, tcol, tw, ccnt);
+        PGGC_LAUNCH_CHECK();
+    }
     out.n = nc;
     out.rp = Buf<i64>(c, nc + 1);
     scan_i64(c, ccnt, out.rp, nc);

@@ -35,8 +35,13 @@ struct LevelPlan {
     Orders ord;
     Buf<i32> sc_list;           // vertices grouped by scoring class
     i64 sc_start[9] = {0};
-    Buf<i64> goff;              // global-table offsets for the largest class
-    i64 gslots = 0;
+    // hubs (largest degree class): HBM hash tables, self-cleaning (every
+    // scan resets the slots it reads), edge and slot work items per hub
+    Buf<i64> goff, items, sitems, hub_pv;
+    Buf<u64> hub_best;
+    Buf<i32> gkeys;
+    Buf<i64> gvals;
+    i64 gslots = 0, nitems = 0, nsitems = 0;
 };
 
 // Device scratch sized for the top level, shared by every level.
@@ -45,11 +50,7 @@ struct MoveScratch {
     Buf<i32> part, dest, mflag, cand, moves, flag, tmp, tmp2;
     Buf<float> gain;
     Buf<i64> prio, scal;
-    Buf<i32> gkeys;
-    Buf<i64> gvals;
-    i64 gcap = 0;
     void ensure(Ctx& c, i64 n);
-    void ensure_global(Ctx& c, i64 slots);
 };
 
 void plan_level(Ctx& c, const DGraph& g, LevelPlan& p);

@@ -51,18 +51,12 @@ void MoveScratch::ensure(Ctx& c, i64 n) {
     fill_i32(c, mflag, cap, 0);
 }
 
-void MoveScratch::ensure_global(Ctx& c, i64 slots) {
-    if (gcap >= slots) return;
-    gcap = slots;
-    gkeys = Buf<i32>(c, slots);
-    gvals = Buf<i64>(c, slots);
-}
-
 // ------------------------------------------------------------- level plan
 constexpr int NCLS = 8;
 __host__ __device__ __forceinline__ int score_class(i64 d) {
-    return d <= 4 ? 0 : d <= 8 ? 1 : d <= 16 ? 2 : d <= 32 ? 3 : d <= 64 ? 4 : d <= 512 ? 5 : d <= 4096 ? 6 : 7;
+    return d <= 4 ? 0 : d <= 8 ? 1 : d <= 16 ? 2 : d <= 32 ? 3 : d <= 64 ? 4 : d <= 512 ? 5 : d <= 2048 ? 6 : 7;
 }
+constexpr int HCH = 4096;  // hub edges (or table slots) per work item
 __host__ __device__ __forceinline__ i64 tslots(i64 d) {
     i64 s = 2;
     while (s < 2 * d) s <<= 1;
@@ -73,11 +67,17 @@ __global__ void k_score_classes(const i64* rp, i64 n, i32* cls) {
     for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
         cls[i] = score_class(rp[i + 1] - rp[i]);
 }
-__global__ void k_global_sizes(const i64* rp, const i32* list, i64 m, i64* sz) {
+__global__ void k_global_sizes(const i64* rp, const i32* list, i64 m, i64* sz, i64* items) {
     for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (i64)gridDim.x * blockDim.x) {
         i32 i = list[x];
-        sz[x] = tslots(rp[i + 1] - rp[i]);
+        i64 d = rp[i + 1] - rp[i];
+        sz[x] = tslots(d);
+        items[x] = (d + HCH - 1) / HCH;
     }
+}
+__global__ void k_slot_items(const i64* goff, i64 m, i64* sitems) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (i64)gridDim.x * blockDim.x)
+        sitems[x] = (goff[x + 1] - goff[x] + HCH - 1) / HCH;
 }
 
 void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
@@ -94,10 +94,25 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
     p.gslots = 0;
     if (m7 > 0) {
         p.goff = Buf<i64>(c, m7 + 1);
-        k_global_sizes<<<grid_for(m7, 256, 4096), 256, 0, c.stream>>>(g.rp, p.sc_list.p + p.sc_start[7], m7, p.goff);
+        p.items = Buf<i64>(c, m7 + 1);
+        p.sitems = Buf<i64>(c, m7 + 1);
+        p.hub_pv = Buf<i64>(c, m7);
+        p.hub_best = Buf<u64>(c, m7);
+        k_global_sizes<<<grid_for(m7, 256, 4096), 256, 0, c.stream>>>(g.rp, p.sc_list.p + p.sc_start[7], m7, p.goff,
+                                                                        p.items);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, p.goff, p.goff, m7);
+        scan_i64(c, p.items, p.items, m7);
+        k_slot_items<<<grid_for(m7, 256, 4096), 256, 0, c.stream>>>(p.goff, m7, p.sitems);
+        PGGC_LAUNCH_CHECK();
+        scan_i64(c, p.sitems, p.sitems, m7);
         p.gslots = c.read1(p.goff.p + m7);
+        p.nitems = c.read1(p.items.p + m7);
+        p.nsitems = c.read1(p.sitems.p + m7);
+        p.gkeys = Buf<i32>(c, p.gslots);
+        p.gvals = Buf<i64>(c, p.gslots);
+        fill_i32(c, p.gkeys, p.gslots, EMPTY_KEY);
+        fill_i64(c, p.gvals, p.gslots, 0);
     }
 }
 
@@ -246,13 +261,90 @@ __global__ void __launch_bounds__(BS) k_score_table(const i64* __restrict__ rp, 
     }
 }
 
+// Hubs: edge work items of HCH edges insert into the hub's HBM table and add
+// the own-cluster weight; slot work items then scan (and reset) the table and
+// fold the packed best candidate with atomicMax; a last pass decides.
+template <class WT>
+__global__ void __launch_bounds__(256) k_hub_insert(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
+                                                    const i32* __restrict__ hubs, i64 m, const i64* __restrict__ items,
+                                                    const i64* __restrict__ goff, const i32* __restrict__ part,
+                                                    const i32* __restrict__ dest, int raw, i32* gkeys, i64* gvals,
+                                                    i64* hub_pv) {
+    const i64 b = blockIdx.x;
+    const i64 hi = upper_row(items, 0, m, b);
+    const i32 i = hubs[hi];
+    if (dest[i] == NO_MOVE) return;
+    const i64 s = rp[i], e = rp[i + 1];
+    const i64 j_lo = s + (b - items[hi]) * HCH;
+    const i64 j_hi = j_lo + HCH < e ? j_lo + HCH : e;
+    i32* keys = gkeys + goff[hi];
+    i64* vals = gvals + goff[hi];
+    const u32 mask = (u32)(goff[hi + 1] - goff[hi] - 1);
+    const i32 p = part[i];
+    i64 pv = 0;
+    for (i64 j = j_lo + threadIdx.x; j < j_hi; j += 256) {
+        const i32 v = col[j];
+        const i32 key = raw ? v : part[v];
+        const i64 ww = W(j);
+        if (!raw && key == p) pv += ww;
+        else table_add_global(keys, vals, mask, key, ww);
+    }
+    pv = block_sum_i64(pv);
+    if (threadIdx.x == 0 && pv) atomicAdd((unsigned long long*)&hub_pv[hi], (unsigned long long)pv);
+}
+
+__global__ void __launch_bounds__(256) k_hub_scan(const i32* __restrict__ hubs, i64 m, const i64* __restrict__ sitems,
+                                                  const i64* __restrict__ goff, const i32* __restrict__ part,
+                                                  const i32* __restrict__ dest, const i64* __restrict__ vw,
+                                                  const i64* __restrict__ vol, const i32* __restrict__ cons, float lam,
+                                                  float ratio, i32* gkeys, i64* gvals, const i64* __restrict__ hub_pv,
+                                                  u64* hub_best) {
+    const i64 b = blockIdx.x;
+    const i64 hi = upper_row(sitems, 0, m, b);
+    const i32 i = hubs[hi];
+    if (dest[i] == NO_MOVE) return;
+    const i64 s_lo = goff[hi] + (b - sitems[hi]) * HCH;
+    const i64 s_end = goff[hi + 1];
+    const i64 s_hi = s_lo + HCH < s_end ? s_lo + HCH : s_end;
+    const i32 p = part[i];
+    float multi, p_conn, bth;
+    own_terms(hub_pv[hi], vol[p], vw[i], lam, ratio, multi, p_conn, bth);
+    u64 bk = 0;
+    for (i64 q = s_lo + threadIdx.x; q < s_hi; q += 256) {
+        const i32 key = gkeys[q];
+        if (key == EMPTY_KEY) continue;
+        const i64 agg = gvals[q];
+        gkeys[q] = EMPTY_KEY;
+        gvals[q] = 0;
+        if (agg > 0 && (float)agg >= bth && (cons == nullptr || cons[key] == cons[p])) {
+            const float jc = fmaf(-(float)vol[key], multi, (float)agg);
+            if (jc >= bth) {
+                const u64 k2 = pack_cand(jc, key);
+                bk = k2 > bk ? k2 : bk;
+            }
+        }
+    }
+    bk = block_max_u64(bk);
+    if (threadIdx.x == 0 && bk) atomicMax((unsigned long long*)&hub_best[hi], (unsigned long long)bk);
+}
+
+__global__ void k_hub_decide(const i32* hubs, i64 m, const i32* part, const i64* vw, const i64* vol, float lam,
+                             float ratio, const i64* hub_pv, const u64* hub_best, i32* dest, float* gain) {
+    for (i64 hi = (i64)blockIdx.x * blockDim.x + threadIdx.x; hi < m; hi += (i64)gridDim.x * blockDim.x) {
+        const i32 i = hubs[hi];
+        if (dest[i] == NO_MOVE) continue;
+        float multi, p_conn, bth;
+        own_terms(hub_pv[hi], vol[part[i]], vw[i], lam, ratio, multi, p_conn, bth);
+        decide(i, p_conn, hub_best[hi], dest, gain);
+    }
+}
+
 static int cap_grid(i64 blocks, int cap) { return (int)std::max<i64>(1, std::min<i64>(blocks, cap)); }
 
 static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32* part, const i64* vol,
                       const i32* cons, bool raw, float lam, float ratio, i32* dest, float* gain) {
     const i64* st = lp.sc_start;
     const i32* L = lp.sc_list.p;
-    if (lp.gslots > 0) s.ensure_global(c, lp.gslots);
     PGGC_WDISPATCH(g, {
         i64 m;
         if ((m = st[1] - st[0]) > 0)
@@ -275,13 +367,21 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
                 g.rp, g.col, W, g.vw, L + st[5], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr, nullptr,
                 nullptr);
         if ((m = st[7] - st[6]) > 0)
-            k_score_table<256, 8192, false><<<cap_grid(m, 1 << 18), 256, 0, c.stream>>>(
+            k_score_table<512, 4096, false><<<cap_grid(m, 1 << 18), 512, 0, c.stream>>>(
                 g.rp, g.col, W, g.vw, L + st[6], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr, nullptr,
                 nullptr);
-        if ((m = st[8] - st[7]) > 0)
-            k_score_table<256, 1, true><<<cap_grid(m, 1 << 16), 256, 0, c.stream>>>(
-                g.rp, g.col, W, g.vw, L + st[7], m, part, vol, cons, raw, lam, ratio, dest, gain, s.gkeys, s.gvals,
-                lp.goff);
+        if ((m = st[8] - st[7]) > 0) {
+            const i32* hubs = L + st[7];
+            fill_i64(c, lp.hub_pv, m, 0);
+            fill_i64(c, (i64*)lp.hub_best.p, m, 0);
+            k_hub_insert<<<(unsigned)lp.nitems, 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.goff, part,
+                                                                   dest, raw, lp.gkeys, lp.gvals, lp.hub_pv);
+            k_hub_scan<<<(unsigned)lp.nsitems, 256, 0, c.stream>>>(hubs, m, lp.sitems, lp.goff, part, dest, g.vw, vol,
+                                                                  cons, lam, ratio, lp.gkeys, lp.gvals, lp.hub_pv,
+                                                                  lp.hub_best);
+            k_hub_decide<<<grid_for(m, 256, 1024), 256, 0, c.stream>>>(hubs, m, part, g.vw, vol, lam, ratio, lp.hub_pv,
+                                                                       lp.hub_best, dest, gain);
+        }
     });
     PGGC_LAUNCH_CHECK();
 }
@@ -402,6 +502,7 @@ __global__ void __launch_bounds__(1024) k_ab_block(const i64* __restrict__ rp, c
         float acc = 0.0f;
         for (i64 j = rp[i] + t; j < rp[i + 1]; j += 1024)
             ab_term(col, W, vw, part, dest, gain, j, p, best, multi, igain, hi, acc);
+        xlane_guard();
         sh[t] = acc;
         __syncthreads();
         for (int off = 512; off >= 64; off >>= 1) {

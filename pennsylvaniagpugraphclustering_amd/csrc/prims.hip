@@ -61,6 +61,56 @@ void Ctx::trim() {
     bytes_cached = 0;
 }
 
+hipEvent_t Ctx::kt_event() {
+    if (!kt.pool.empty()) {
+        hipEvent_t e = kt.pool.back();
+        kt.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    PGGC_HIP(hipEventCreate(&e));
+    return e;
+}
+
+int Ctx::kt_start(int slot) {
+    if (kt.pend.size() >= 4096) kt_resolve();
+    KTimer::Pending p{slot, kt_event(), kt_event(), 0.0};
+    PGGC_HIP(hipEventRecord(p.a, stream));
+    kt.pend.push_back(p);
+    return (int)kt.pend.size() - 1;
+}
+
+void Ctx::kt_stop(int tok, double bytes) {
+    KTimer::Pending& p = kt.pend[tok];
+    p.bytes = bytes;
+    PGGC_HIP(hipEventRecord(p.b, stream));
+}
+
+void Ctx::kt_resolve() {
+    if (kt.pend.empty()) return;
+    PGGC_HIP(hipStreamSynchronize(stream));
+    for (auto& p : kt.pend) {
+        float t = 0.0f;
+        PGGC_HIP(hipEventElapsedTime(&t, p.a, p.b));
+        kt.launches[p.slot]++;
+        kt.ms[p.slot] += t;
+        kt.bytes[p.slot] += p.bytes;
+        kt.pool.push_back(p.a);
+        kt.pool.push_back(p.b);
+    }
+    kt.pend.clear();
+}
+
+void Ctx::kt_free() {
+    for (auto& p : kt.pend) {
+        kt.pool.push_back(p.a);
+        kt.pool.push_back(p.b);
+    }
+    kt.pend.clear();
+    for (hipEvent_t e : kt.pool) (void)hipEventDestroy(e);
+    kt.pool.clear();
+}
+
 const i64* Ctx::read(const i64* dev, int count) {
     PGGC_HIP(hipMemcpyAsync(pinned, dev, sizeof(i64) * count, hipMemcpyDeviceToHost, stream));
     PGGC_HIP(hipStreamSynchronize(stream));
@@ -253,6 +303,7 @@ __global__ void __launch_bounds__(RT) k_radix_scatter(const u64* kin, const i32*
             peers &= ((d >> b) & 1u) ? bal : ~bal;
         }
         u32 rank = (u32)__popcll(peers & lt);
+        xlane_guard();
         if (valid && rank == 0) wc[w][d] = (u32)__popcll(peers);
         __syncthreads();
         u32 c0 = wc[0][t], c1 = wc[1][t], c2 = wc[2][t], c3 = wc[3][t];
@@ -328,6 +379,7 @@ __global__ void __launch_bounds__(256) k_sumsq(const i64* in, i64 n, U128* part)
         s = add128(s, q);
     }
     __shared__ U128 sh[256];
+    xlane_guard();
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -341,6 +393,7 @@ __global__ void __launch_bounds__(256) k_sumsq_final(const U128* part, int np, i
     U128 s{0, 0};
     for (int x = threadIdx.x; x < np; x += blockDim.x) s = add128(s, part[x]);
     __shared__ U128 sh[256];
+    xlane_guard();
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
