@@ -53,6 +53,22 @@ def cpu_baseline(scale, seed=2):
                       f"{dt:.2f} s, Q={q:.6f}"}
 
 
+def roofline(kstats):
+    """Roofline object for the dominant timed kernel group that carries an
+    algorithmic byte count (HIP events on the library stream)."""
+    best = None
+    for name, (n, ms, by) in kstats.items():
+        if n and by > 0 and (best is None or ms > kstats[best][1]):
+            best = name
+    if best is None:
+        return None
+    n, ms, by = kstats[best]
+    achieved = by / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": best, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launches": n,
+            "avg_launch_us": ms / n * 1000.0, "bytes_per_launch": by / n}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -81,12 +97,20 @@ def main():
         q, part, info = step()
         print(f"# warmup {w}: Q={q:.6f} info={info}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
+    ctx.set_timing(True)
     t0 = time.perf_counter()
     for s in range(a.steps):
         q, part, info = step()
         print(f"# step {s}: Q={q:.6f}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / max(a.steps, 1)
+    kstats = ctx.kernel_stats()
+    ctx.set_timing(False)
+    for name, (n, ms, by) in kstats.items():
+        if n:
+            print(f"# kernel group {name}: {n} launches, {ms / max(a.steps, 1):.1f} ms/step, "
+                  f"{ms / n * 1000:.1f} us avg" + (f", {by / (ms * 1e-3) / 1e9:.0f} GB/s" if by else ""),
+                  file=sys.stderr, flush=True)
 
     extra = {}
     if not a.no_leiden:
@@ -125,7 +149,8 @@ def main():
         "levels": info["levels"],
         "outer_iterations": info["outer"],
         "edges_per_s_per_sweep": m * info["sweeps"] / dt,
-        "roofline": None,
+        "roofline": roofline(kstats),
+        "kernel_groups_ms_per_step": {k: v[1] / max(a.steps, 1) for k, v in kstats.items() if v[0]},
         "cpu_baseline": cpu,
     }
     out.update(extra)

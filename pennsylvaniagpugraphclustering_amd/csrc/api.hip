@@ -29,38 +29,48 @@ struct RunStats {
 
 // ---------------------------------------------------------------- kernels
 __global__ void k_project(i64 n, const i32* coarse, i32* fine) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x)
-        fine[x] = coarse[fine[x]];
+    const i64 x = thread_item();
+    if (x >= n) return;
+    fine[x] = coarse[fine[x]];
 }
 __global__ void k_down_winner(i64 n, const i32* map, i32* win) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
-        atomicMax(&win[map[i]], (i32)i);
+    const i64 i = thread_item();
+    if (i >= n) return;
+    atomicMax(&win[map[i]], (i32)i);
 }
 __global__ void k_down_gather(i64 nc, const i32* win, const i32* in, i32* out) {
-    for (i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += (i64)gridDim.x * blockDim.x)
-        if (win[c] >= 0) out[c] = in[win[c]];
+    const i64 c = thread_item();
+    if (c >= nc) return;
+    const i32 w = win[c];
+    if (w >= 0) out[c] = in[w];
 }
 __global__ void k_downsample(i64 n, const i32* map, const i32* in, i32* out) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
-        out[map[i]] = in[i];
+    const i64 i = thread_item();
+    if (i >= n) return;
+    out[map[i]] = in[i];
 }
 template <class WT>
 __global__ void __launch_bounds__(256) k_wdeg(const i64* rp, WT W, i64 n, i64* out) {
     const int lane = threadIdx.x & 63;
-    for (i64 i = (i64)blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += (i64)gridDim.x * 4) {
-        i64 acc = 0;
-        for (i64 j = rp[i] + lane; j < rp[i + 1]; j += 64) acc += W(j);
-        acc = wave_sum_i64(acc);
-        if (lane == 0) out[i] = acc;
+    const i64 i = block_id() * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    i64 acc = 0;
+    const i64 e = rp[i + 1];
+    for (i64 base = rp[i]; base < e; base += 64) {
+        const i64 j = base + lane;
+        if (j < e) acc += W(j);
     }
+    acc = wave_sum_i64(acc);
+    if (lane == 0) out[i] = acc;
 }
 __global__ void k_deg(const i64* rp, i64 n, i64* out) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
-        out[i] = rp[i + 1] - rp[i];
+    const i64 i = thread_item();
+    if (i >= n) return;
+    out[i] = rp[i + 1] - rp[i];
 }
 
 static void project(Ctx& c, i64 n, const i32* coarse, i32* fine) {
-    k_project<<<grid_for(n, 1024, 8192), 256, 0, c.stream>>>(n, coarse, fine);
+    k_project<<<grid_items(n, 256), 256, 0, c.stream>>>(n, coarse, fine);
     PGGC_LAUNCH_CHECK();
 }
 
@@ -69,8 +79,8 @@ static void project(Ctx& c, i64 n, const i32* coarse, i32* fine) {
 static void downsample_max(Ctx& c, i64 n, const i32* map, i64 nc, const i32* in, i32* out) {
     Buf<i32> win(c, std::max<i64>(nc, 1));
     fill_i32(c, win, nc, -1);
-    k_down_winner<<<grid_for(n, 1024, 8192), 256, 0, c.stream>>>(n, map, win);
-    k_down_gather<<<grid_for(nc, 1024, 8192), 256, 0, c.stream>>>(nc, win, in, out);
+    k_down_winner<<<grid_items(n, 256), 256, 0, c.stream>>>(n, map, win);
+    k_down_gather<<<grid_items(nc, 256), 256, 0, c.stream>>>(nc, win, in, out);
     PGGC_LAUNCH_CHECK();
 }
 
@@ -194,7 +204,7 @@ static void leiden_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& 
             og->vw = Buf<i64>(c, std::max<i64>(nc, 1));
             coarsen_vertex_weights(c, g, cm, nc, og->vw);
             part = Buf<i32>(c, std::max<i64>(nc, 1));
-            k_downsample<<<grid_for(g.n, 1024, 8192), 256, 0, c.stream>>>(g.n, cm, louv, part);
+            k_downsample<<<grid_items(g.n, 256), 256, 0, c.stream>>>(g.n, cm, louv, part);
             PGGC_LAUNCH_CHECK();
             levels.push_back(owned_level(std::move(og)));
         } else {
@@ -244,7 +254,7 @@ static void state_init(Ctx& c, const DGraph& g, int objective, double mult, Refi
         i64 gd = g.nnz;
         if (g.wk != W_UNIT) {
             Buf<i64> wd(c, std::max<i64>(g.n, 1));
-            PGGC_WDISPATCH(g, (k_wdeg<<<grid_for(g.n, 4, 1 << 16), 256, 0, c.stream>>>(g.rp, W, g.n, wd)));
+            PGGC_WDISPATCH(g, (k_wdeg<<<grid_items(g.n, 4), 256, 0, c.stream>>>(g.rp, W, g.n, wd)));
             PGGC_LAUNCH_CHECK();
             sum_i64(c, wd, g.n, tmp);
             gd = c.read1(tmp);
@@ -419,9 +429,9 @@ int pggc_vertex_weights(pggc_ctx* ctx, const pggc_graph* gg, int objective, int6
         Ctx& c = ctx->c;
         if (g.n == 0) return;
         if (objective == PGGC_OBJ_MOD || (objective == PGGC_OBJ_WMOD && g.wk == W_UNIT)) {
-            k_deg<<<grid_for(g.n, 1024, 8192), 256, 0, c.stream>>>(g.rp, g.n, out);
+            k_deg<<<grid_items(g.n, 256), 256, 0, c.stream>>>(g.rp, g.n, out);
         } else if (objective == PGGC_OBJ_WMOD) {
-            PGGC_WDISPATCH(g, (k_wdeg<<<grid_for(g.n, 4, 1 << 16), 256, 0, c.stream>>>(g.rp, W, g.n, out)));
+            PGGC_WDISPATCH(g, (k_wdeg<<<grid_items(g.n, 4), 256, 0, c.stream>>>(g.rp, W, g.n, out)));
         } else {
             fill_i64(c, out, g.n, 1);
         }

@@ -138,6 +138,26 @@ __device__ __forceinline__ u32 slot_hash(i32 key, u32 mask) {
 
 inline i64 ceil_div(i64 a, i64 b) { return (a + b - 1) / b; }
 
+// Exact launch grids: one work item per thread, lane group, wave or block.
+// Blocks are laid out on a 2-D grid so counts beyond one dimension's limit
+// still launch; kernels recover the linear block id with block_id().
+//
+// Kernels whose per-item body has divergent control flow or divergent stores
+// take one item per thread (no per-lane grid-stride loop).  On gfx950 a
+// per-lane grid-stride loop around such a body gave run-to-run differences in
+// the afterburner at 160k+ items, while the same body with one item per
+// thread was exact; grid-stride loops remain only around straight-line
+// bodies (fills, copies, reductions).
+constexpr i64 GRID_X = 32768;
+inline dim3 grid_blocks(i64 blocks) {
+    if (blocks < 1) blocks = 1;
+    if (blocks <= GRID_X) return dim3((unsigned)blocks);
+    return dim3((unsigned)GRID_X, (unsigned)ceil_div(blocks, GRID_X));
+}
+inline dim3 grid_items(i64 items, int per_block) { return grid_blocks(ceil_div(items, per_block)); }
+__device__ __forceinline__ i64 block_id() { return (i64)blockIdx.y * gridDim.x + blockIdx.x; }
+__device__ __forceinline__ i64 thread_item() { return block_id() * blockDim.x + threadIdx.x; }
+
 inline int grid_for(i64 work, int per_block, int cap = 1 << 20) {
     i64 g = ceil_div(work, per_block);
     if (g < 1) g = 1;

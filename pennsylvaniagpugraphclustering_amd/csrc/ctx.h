@@ -32,8 +32,10 @@ struct KTimer {
         int slot;
         hipEvent_t a, b;
         double bytes;
+        bool stopped;
     };
     bool on = false;
+    int open = 0;  // regions started and not yet stopped
     std::vector<Pending> pend;
     std::vector<hipEvent_t> pool;
     i64 launches[KS_COUNT] = {0};
@@ -71,16 +73,11 @@ struct Ctx {
     void kt_stop(int token, double bytes);
     void kt_resolve();
     void kt_free();
-};
-
-// Brackets the launches of one timed region (no-op when timing is off).
-struct KScope {
-    Ctx& c;
-    int tok;
-    double bytes;
-    KScope(Ctx& ctx, int slot, double b) : c(ctx), tok(ctx.kt.on ? ctx.kt_start(slot) : -1), bytes(b) {}
-    ~KScope() {
-        if (tok >= 0) c.kt_stop(tok, bytes);
+    // timed region on `stream`: token (or -1 when timing is off) ...
+    int kt_begin(int slot) { return kt.on ? kt_start(slot) : -1; }
+    // ... closed with the algorithmic bytes it moved
+    void kt_end(int tok, double bytes) {
+        if (tok >= 0) kt_stop(tok, bytes);
     }
 };
 

@@ -4,23 +4,28 @@
 
 namespace pggc {
 
-// Cross-lane guard.  On gfx950 with this toolchain a ds_bpermute (the
-// instruction behind __shfl*) issued right after the EXEC restore that ends a
-// divergent loop or branch was observed to read stale lanes (run-to-run
-// different reductions); 16 wait states before the cross-lane sequence made
-// every reduction exact and deterministic.  Every cross-lane helper and every
-// kernel-level shuffle/ballot sequence starts with this guard.
-__device__ __forceinline__ void xlane_guard() { asm volatile("s_nop 7\n\ts_nop 7" ::: "memory"); }
+// Cross-lane fence.  On gfx950 a ds_bpermute (the instruction behind
+// __shfl*) issued a few instructions after the EXEC restore that ends a
+// divergent loop or branch gave run-to-run different reductions, while the
+// same reduction after a wave-uniform loop, or through LDS, was exact.  Every
+// value that enters a shuffle first passes through this fence: 16 wait
+// states in an asm statement that takes the value as an in/out operand, so
+// the compiler cannot hoist the shuffle above the wait states (a plain asm
+// with a memory clobber does not order register-only shuffles).
+template <class T>
+__device__ __forceinline__ void xlane_fence(T& v) {
+    asm volatile("s_nop 7\n\ts_nop 7" : "+v"(v));
+}
 
 __device__ __forceinline__ i64 wave_sum_i64(i64 v) {
-    xlane_guard();
+    xlane_fence(v);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
 
 __device__ __forceinline__ u64 wave_max_u64(u64 v) {
-    xlane_guard();
+    xlane_fence(v);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         u64 y = __shfl_xor(v, o);
@@ -32,14 +37,14 @@ __device__ __forceinline__ u64 wave_max_u64(u64 v) {
 // group reductions over `G` consecutive lanes (G a power of two <= 64)
 template <int G>
 __device__ __forceinline__ i64 group_sum_i64(i64 v) {
-    xlane_guard();
+    xlane_fence(v);
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
     return v;
 }
 template <int G>
 __device__ __forceinline__ u64 group_max_u64(u64 v) {
-    xlane_guard();
+    xlane_fence(v);
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) {
         u64 y = __shfl_xor(v, o, G);
@@ -49,7 +54,7 @@ __device__ __forceinline__ u64 group_max_u64(u64 v) {
 }
 template <int G>
 __device__ __forceinline__ i64 group_max_i64(i64 v) {
-    xlane_guard();
+    xlane_fence(v);
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) {
         i64 y = __shfl_xor(v, o, G);
@@ -58,29 +63,40 @@ __device__ __forceinline__ i64 group_max_i64(i64 v) {
     return v;
 }
 
-// Sum over the whole block; every thread receives the total.
+// Block reductions through an LDS halving tree (no lane shuffles, so they
+// are safe after loops whose lanes exit at different iterations).  BS is the
+// block size (a power of two); every thread must call them and receives the
+// result.
+template <int BS>
 __device__ __forceinline__ i64 block_sum_i64(i64 v) {
-    __shared__ i64 ws[16];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    v = wave_sum_i64(v);
-    if (lane == 0) ws[w] = v;
+    __shared__ i64 sh[BS];
+    const int t = threadIdx.x;
+    sh[t] = v;
     __syncthreads();
-    i64 t = 0;
-    for (int k = 0; k < nw; k++) t += ws[k];
+#pragma unroll
+    for (int o = BS / 2; o > 0; o >>= 1) {
+        if (t < o) sh[t] = sh[t] + sh[t + o];
+        __syncthreads();
+    }
+    const i64 r = sh[0];
     __syncthreads();
-    return t;
+    return r;
 }
 
+template <int BS>
 __device__ __forceinline__ u64 block_max_u64(u64 v) {
-    __shared__ u64 ws[16];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    v = wave_max_u64(v);
-    if (lane == 0) ws[w] = v;
+    __shared__ u64 sh[BS];
+    const int t = threadIdx.x;
+    sh[t] = v;
     __syncthreads();
-    u64 t = 0;
-    for (int k = 0; k < nw; k++) t = ws[k] > t ? ws[k] : t;
+#pragma unroll
+    for (int o = BS / 2; o > 0; o >>= 1) {
+        if (t < o) sh[t] = sh[t + o] > sh[t] ? sh[t + o] : sh[t];
+        __syncthreads();
+    }
+    const u64 r = sh[0];
     __syncthreads();
-    return t;
+    return r;
 }
 
 // Exclusive prefix sum over the block in thread order; *total = block sum.
@@ -88,7 +104,7 @@ __device__ __forceinline__ i64 block_excl_scan_i64(i64 v, i64* total) {
     __shared__ i64 ws[16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
     i64 x = v;
-    xlane_guard();
+    xlane_fence(x);
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         i64 y = __shfl_up(x, o);

@@ -27,7 +27,7 @@ __global__ void __launch_bounds__(PT) k_part_count(const i32* cls, i64 n, i64* c
     }
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        i64 s = block_sum_i64(c[k]);
+        i64 s = block_sum_i64<PT>(c[k]);
         if (threadIdx.x == 0) counts[(i64)k * nb + blockIdx.x] = s;
     }
 }
@@ -93,11 +93,11 @@ void partition_classes(Ctx& c, const i32* cls, i64 n, int K, i32* out, i64* star
 
 // ------------------------------------------------------------------ orderings
 __global__ void k_order_classes(const i64* rp, i64 n, i32* c1, i32* c2) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
-        i64 d = rp[i + 1] - rp[i];
-        c1[i] = d < LARGE_CUTOFF ? 0 : (d < MASSIVE_CUTOFF ? 1 : 2);
-        c2[i] = d < MID_CUTOFF ? 0 : (d < MASSIVE_CUTOFF ? 1 : 2);
-    }
+    const i64 i = thread_item();
+    if (i >= n) return;
+    i64 d = rp[i + 1] - rp[i];
+    c1[i] = d < LARGE_CUTOFF ? 0 : (d < MASSIVE_CUTOFF ? 1 : 2);
+    c2[i] = d < MID_CUTOFF ? 0 : (d < MASSIVE_CUTOFF ? 1 : 2);
 }
 
 void make_orders(Ctx& c, const DGraph& g, Orders& o) {
@@ -107,7 +107,7 @@ void make_orders(Ctx& c, const DGraph& g, Orders& o) {
         o.o2 = Buf<i32>(c, std::max<i64>(n, 1));
     }
     Buf<i32> c1(c, std::max<i64>(n, 1)), c2(c, std::max<i64>(n, 1));
-    k_order_classes<<<grid_for(n, 256 * 4, 8192), 256, 0, c.stream>>>(g.rp, n, c1, c2);
+    k_order_classes<<<grid_items(n, 256), 256, 0, c.stream>>>(g.rp, n, c1, c2);
     PGGC_LAUNCH_CHECK();
     i64 s1[4], s2[4];
     partition_k<3>(c, c1, n, o.o1, s1);
@@ -160,25 +160,31 @@ __global__ void __launch_bounds__(EB) k_eval_uncut(const i64* rp, const i32* col
             if (lab[col[j]] == lr) acc += W(j);
         }
     }
-    acc = block_sum_i64(acc);
+    acc = block_sum_i64<EB>(acc);
     if (threadIdx.x == 0 && acc) atomicAdd((unsigned long long*)uncut, (unsigned long long)acc);
 }
 
 __global__ void k_eval_vol(const i32* lab, const i64* vw, i64 n, i64* vol) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
-        atomicAdd((unsigned long long*)&vol[lab[i]], (unsigned long long)vw[i]);
+    const i64 i = thread_item();
+    if (i >= n) return;
+    atomicAdd((unsigned long long*)&vol[lab[i]], (unsigned long long)vw[i]);
 }
 
 void evaluate(Ctx& c, const DGraph& g, const i32* labels, i64 L, i64* vol, i64* dev_uncut) {
     fill_i64(c, vol, L, 0);
     fill_i64(c, dev_uncut, 1, 0);
     if (g.nnz > 0) {
+        // algorithmic bytes: row offsets, columns, labels of both ends, weights
+        const double wb = g.wk == W_UNIT ? 0.0 : (g.wk == W_I32 ? 4.0 : 8.0);
+        const double bytes = 8.0 * (g.n + 1) + 4.0 * g.n + (8.0 + wb) * g.nnz;
+        const int tk = c.kt_begin(KS_EVAL);
         PGGC_WDISPATCH(g, (k_eval_uncut<<<(int)ceil_div(g.nnz, EC), EB, 0, c.stream>>>(g.rp, g.col, W, labels, g.n,
                                                                                        g.nnz, dev_uncut)));
+        c.kt_end(tk, bytes);
         PGGC_LAUNCH_CHECK();
     }
     if (g.n > 0) {
-        k_eval_vol<<<grid_for(g.n, 256 * 4, 8192), 256, 0, c.stream>>>(labels, g.vw, g.n, vol);
+        k_eval_vol<<<grid_items(g.n, 256), 256, 0, c.stream>>>(labels, g.vw, g.n, vol);
         PGGC_LAUNCH_CHECK();
     }
 }
@@ -217,42 +223,43 @@ void own_weight(Ctx& c, const DGraph& g, const i32* labels, i64* out) {
 }
 
 __global__ void k_coarsen_vw(const i32* map, const i64* vw, i64 n, i64* out) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
-        atomicAdd((unsigned long long*)&out[map[i]], (unsigned long long)vw[i]);
+    const i64 i = thread_item();
+    if (i >= n) return;
+    atomicAdd((unsigned long long*)&out[map[i]], (unsigned long long)vw[i]);
 }
 
 void coarsen_vertex_weights(Ctx& c, const DGraph& g, const i32* map, i64 nc, i64* out) {
     fill_i64(c, out, nc, 0);
     if (g.n == 0) return;
-    k_coarsen_vw<<<grid_for(g.n, 256 * 4, 8192), 256, 0, c.stream>>>(map, g.vw, g.n, out);
+    k_coarsen_vw<<<grid_items(g.n, 256), 256, 0, c.stream>>>(map, g.vw, g.n, out);
     PGGC_LAUNCH_CHECK();
 }
 
 // ---------------------------------------------------------------- contraction
 __global__ void k_ccount(const i64* rp, const i32* vcmap, i64 n, i64* cnt, i64* bound) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
-        i32 u = vcmap[i];
-        atomicAdd((unsigned long long*)&cnt[u], 1ull);
-        atomicAdd((unsigned long long*)&bound[u], (unsigned long long)(rp[i + 1] - rp[i]));
-    }
+    const i64 i = thread_item();
+    if (i >= n) return;
+    i32 u = vcmap[i];
+    atomicAdd((unsigned long long*)&cnt[u], 1ull);
+    atomicAdd((unsigned long long*)&bound[u], (unsigned long long)(rp[i + 1] - rp[i]));
 }
 
 __global__ void k_cscatter(const i32* vcmap, i64 n, i64* pos, i32* members) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
-        i32 u = vcmap[i];
-        i64 k = (i64)atomicAdd((unsigned long long*)&pos[u], 1ull);
-        members[k] = (i32)i;
-    }
+    const i64 i = thread_item();
+    if (i >= n) return;
+    i32 u = vcmap[i];
+    i64 k = (i64)atomicAdd((unsigned long long*)&pos[u], 1ull);
+    members[k] = (i32)i;
 }
 
 constexpr i64 C_SMALL = 256;   // bound <= C_SMALL: 64-thread block, 512-slot LDS table
 constexpr i64 C_MID = 2048;    // bound <= C_MID: 256-thread block, 4096-slot LDS table
 
 __global__ void k_cclass(const i64* bound, i64 nc, i32* cls) {
-    for (i64 u = (i64)blockIdx.x * blockDim.x + threadIdx.x; u < nc; u += (i64)gridDim.x * blockDim.x) {
-        i64 b = bound[u];
-        cls[u] = b == 0 ? 3 : (b <= C_SMALL ? 0 : (b <= C_MID ? 1 : 2));
-    }
+    const i64 u = thread_item();
+    if (u >= nc) return;
+    i64 b = bound[u];
+    cls[u] = b == 0 ? 3 : (b <= C_SMALL ? 0 : (b <= C_MID ? 1 : 2));
 }
 
 __host__ __device__ __forceinline__ i64 table_slots(i64 bound, i64 nc) {
@@ -263,8 +270,9 @@ __host__ __device__ __forceinline__ i64 table_slots(i64 bound, i64 nc) {
 }
 
 __global__ void k_goff_sizes(const i32* list, i64 m, const i64* bound, i64 nc, i64* sz) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (i64)gridDim.x * blockDim.x)
-        sz[x] = table_slots(bound[list[x]], nc);
+    const i64 x = thread_item();
+    if (x >= m) return;
+    sz[x] = table_slots(bound[list[x]], nc);
 }
 
 // One block per coarse vertex: hash-merge the members' edges by coarse
@@ -281,7 +289,9 @@ __global__ void __launch_bounds__(BS) k_cagg(const i64* rp, const i32* col, WT W
     __shared__ i64 rs[BS];
     __shared__ int s_cnt;
     const int t = threadIdx.x;
-    for (i64 li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const i64 li = block_id();
+    if (li >= nlist) return;
+    {
         const i32 u = list[li];
         const i64 S = table_slots(bound[u], nc);
         const u32 mask = (u32)(S - 1);
@@ -307,7 +317,9 @@ __global__ void __launch_bounds__(BS) k_cagg(const i64* rp, const i32* col, WT W
             pre[t] = ex;
             rs[t] = st;
             __syncthreads();
-            for (i64 e = t; e < tot; e += BS) {
+            for (i64 eb = 0; eb < tot; eb += BS) {  // block-uniform loop control
+                const i64 e = eb + t;
+                if (e >= tot) continue;
                 int lo = 0, hi = BS;
                 while (hi - lo > 1) {
                     int md = (lo + hi) >> 1;
@@ -329,8 +341,11 @@ __global__ void __launch_bounds__(BS) k_cagg(const i64* rp, const i32* col, WT W
             __syncthreads();
             bitonic_sort_lds<BS>(keys, vals, (int)S);
             int c = 0;
-            for (i64 s = t; s < S; s += BS) c += keys[s] != 0x7fffffff;
-            c = (int)block_sum_i64(c);
+            for (i64 sb = 0; sb < S; sb += BS) {  // block-uniform loop control
+                const i64 s = sb + t;
+                if (s < S) c += keys[s] != 0x7fffffff;
+            }
+            c = (int)block_sum_i64<BS>(c);
             i64 base = toff[u];
             for (i64 s = t; s < c; s += BS) {
                 tcol[base + s] = keys[s];
@@ -357,22 +372,26 @@ __global__ void __launch_bounds__(BS) k_cagg(const i64* rp, const i32* col, WT W
 }
 
 // Big coarse rows (bound > C_MID): the flattened member-edge range of each
-// such row is split into work items of BCH edges, each item a block that
-// inserts into the row's HBM hash table; a second pass splits each table into
-// slot items and compacts the occupied slots.
-constexpr int BCH = 4096;  // edges (or slots) per work item
+// such row is split into work items of BCH_E edges, each item a block that
+// merges its edges in LDS and then inserts the distinct pairs into the row's
+// HBM hash table; a second pass splits each table into slot items of BCH_S
+// slots and compacts the occupied slots.
+constexpr int BCH_E = 2048;      // member edges per insert item (8 per thread)
+constexpr int BCH_S = 4096;      // table slots per extract item (16 per thread)
+constexpr int BLDS = 2 * BCH_E;  // LDS pre-aggregation slots
 
 __global__ void k_member_degs(const i32* members, const i64* rp, i64 n, i64* mdp) {
-    for (i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (i64)gridDim.x * blockDim.x) {
-        i32 v = members[k];
-        mdp[k] = rp[v + 1] - rp[v];
-    }
+    const i64 k = thread_item();
+    if (k >= n) return;
+    i32 v = members[k];
+    mdp[k] = rp[v + 1] - rp[v];
 }
 __global__ void k_items(const i32* list, i64 m, const i64* bound, const i64* goff, i64 nc, int slots, i64* items) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (i64)gridDim.x * blockDim.x) {
-        i64 len = slots ? goff[x + 1] - goff[x] : bound[list[x]];
-        items[x] = (len + BCH - 1) / BCH;
-    }
+    const i64 x = thread_item();
+    if (x >= m) return;
+    i64 len = slots ? goff[x + 1] - goff[x] : bound[list[x]];
+    const i64 per = slots ? BCH_S : BCH_E;
+    items[x] = (len + per - 1) / per;
 }
 
 template <class WT>
@@ -380,29 +399,33 @@ __global__ void __launch_bounds__(256) k_cbig_insert(const i64* rp, const i32* c
                                                       const i32* members, const i64* mstart, const i64* mdp,
                                                       const i32* big, i64 nbig, const i64* items, const i64* goff,
                                                       i32* gkeys, i64* gvals) {
-    const i64 b = blockIdx.x;
+    const i64 b = block_id();
+    if (b >= items[nbig]) return;
     const i64 ui = upper_row(items, 0, nbig, b);
     const i32 u = big[ui];
     const i64 m0 = mstart[u], m1 = mstart[u + 1];
-    const i64 e_lo = mdp[m0] + (b - items[ui]) * BCH;
+    const i64 e_lo = mdp[m0] + (b - items[ui]) * BCH_E;
     const i64 e_end = mdp[m1];
-    const i64 e_hi = e_lo + BCH < e_end ? e_lo + BCH : e_end;
-    i32* keys = gkeys + goff[ui];
-    i64* vals = gvals + goff[ui];
-    const u32 mask = (u32)(goff[ui + 1] - goff[ui] - 1);
-    const i64 e0 = e_lo + (i64)threadIdx.x * 16;
-    if (e0 >= e_hi) return;
-    i64 k = upper_row(mdp, m0, m1, e0);
-    i64 nxt = mdp[k + 1];
-    for (int q = 0; q < 16; q++) {
-        const i64 e = e0 + q;
-        if (e >= e_hi) break;
-        if (e >= nxt) k = advance_row(mdp, k, m1, e, nxt);
-        const i32 v = members[k];
-        const i64 j = rp[v] + (e - mdp[k]);
-        const i32 cv = vcmap[col[j]];
-        if (cv != u) table_add_global(keys, vals, mask, cv, W(j));
+    const i64 e_hi = e_lo + BCH_E < e_end ? e_lo + BCH_E : e_end;
+    __shared__ i32 lk[BLDS];
+    __shared__ i64 lv[BLDS];
+    lds_agg_clear<256, BLDS>(lk, lv);
+    constexpr int EPI = BCH_E / 256;
+    const i64 e0 = e_lo + (i64)threadIdx.x * EPI;
+    if (e0 < e_hi) {
+        i64 k = upper_row(mdp, m0, m1, e0);
+        i64 nxt = mdp[k + 1];
+        for (int q = 0; q < EPI; q++) {
+            const i64 e = e0 + q;
+            if (e >= e_hi) break;
+            if (e >= nxt) k = advance_row(mdp, k, m1, e, nxt);
+            const i32 v = members[k];
+            const i64 j = rp[v] + (e - mdp[k]);
+            const i32 cv = vcmap[col[j]];
+            if (cv != u) table_add_lds(lk, lv, BLDS - 1, cv, W(j));
+        }
     }
+    lds_agg_flush<256, BLDS>(lk, lv, gkeys + goff[ui], gvals + goff[ui], (u32)(goff[ui + 1] - goff[ui] - 1));
 }
 
 __global__ void __launch_bounds__(256) k_cbig_extract(const i32* big, i64 nbig, const i64* items, const i64* goff,
@@ -410,17 +433,17 @@ __global__ void __launch_bounds__(256) k_cbig_extract(const i32* big, i64 nbig, 
                                                        i32* tcol, i64* tw, i64* ccnt) {
     __shared__ int s_cnt;
     __shared__ i64 s_base;
-    const i64 b = blockIdx.x;
+    const i64 b = block_id();
+    if (b >= items[nbig]) return;
     const i64 ui = upper_row(items, 0, nbig, b);
     const i32 u = big[ui];
-    const i64 s_lo = goff[ui] + (b - items[ui]) * BCH;
+    const i64 s_lo = goff[ui] + (b - items[ui]) * BCH_S;
     const i64 s_end = goff[ui + 1];
-    const i64 s_hi = s_lo + BCH < s_end ? s_lo + BCH : s_end;
+    const i64 s_hi = s_lo + BCH_S < s_end ? s_lo + BCH_S : s_end;
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
     i32 kk[16];
     int pos[16];
-    xlane_guard();
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const i64 sl = s_lo + threadIdx.x + (i64)q * 256;
@@ -463,14 +486,16 @@ __global__ void __launch_bounds__(EB) k_crow_copy(const i64* crp, i64 nc, i64 nn
 // Sorting of rows merged in global tables: flatten, radix sort by
 // (row rank, column), scatter back.
 __global__ void k_big_count(const i32* list, i64 m, const i64* crp, i64* cnt) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (i64)gridDim.x * blockDim.x) {
-        i32 u = list[x];
-        cnt[x] = crp[u + 1] - crp[u];
-    }
+    const i64 x = thread_item();
+    if (x >= m) return;
+    i32 u = list[x];
+    cnt[x] = crp[u + 1] - crp[u];
 }
 __global__ void k_big_flatten(const i32* list, i64 m, const i64* crp, const i64* boff, const i32* ocol, u64* key,
                               i32* val, i64* target) {
-    for (i64 x = blockIdx.x; x < m; x += gridDim.x) {
+    const i64 x = block_id();
+    if (x >= m) return;
+    {
         i32 u = list[x];
         i64 s = crp[u], e = crp[u + 1], b = boff[x];
         for (i64 j = s + threadIdx.x; j < e; j += blockDim.x) {
@@ -506,39 +531,41 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
     Buf<i32> members(c, std::max<i64>(n, 1));
     fill_i64(c, cnt, nc + 1, 0);
     fill_i64(c, bound, nc + 1, 0);
-    k_ccount<<<grid_for(n, 256 * 4, 8192), 256, 0, c.stream>>>(g.rp, vcmap, n, cnt, bound);
+    k_ccount<<<grid_items(n, 256), 256, 0, c.stream>>>(g.rp, vcmap, n, cnt, bound);
     PGGC_LAUNCH_CHECK();
     scan_i64(c, cnt, mstart, nc);
     scan_i64(c, bound, toff, nc);
     PGGC_HIP(hipMemcpyAsync(pos, mstart, nc * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
-    k_cscatter<<<grid_for(n, 256 * 4, 8192), 256, 0, c.stream>>>(vcmap, n, pos, members);
+    k_cscatter<<<grid_items(n, 256), 256, 0, c.stream>>>(vcmap, n, pos, members);
     PGGC_LAUNCH_CHECK();
     const i64 tb = c.read1(toff.p + nc);
     Buf<i32> tcol(c, std::max<i64>(tb, 1));
     Buf<i64> tw(c, std::max<i64>(tb, 1));
     Buf<i64> ccnt(c, nc + 1);
     Buf<i32> cls(c, std::max<i64>(nc, 1)), list(c, std::max<i64>(nc, 1));
-    k_cclass<<<grid_for(nc, 256 * 4, 8192), 256, 0, c.stream>>>(bound, nc, cls);
+    k_cclass<<<grid_items(nc, 256), 256, 0, c.stream>>>(bound, nc, cls);
     PGGC_LAUNCH_CHECK();
     i64 st[5];
     partition_k<4>(c, cls, nc, list, st);
     fill_i64(c, ccnt, nc + 1, 0);
     const i64 n0 = st[1] - st[0], n1 = st[2] - st[1], n2 = st[3] - st[2];
+    int tk = c.kt_begin(KS_CONTRACT_AGG);
     PGGC_WDISPATCH(g, {
         if (n0 > 0)
-            k_cagg<64, 512, false><<<grid_for(n0, 1, 1 << 20), 64, 0, c.stream>>>(
+            k_cagg<64, 512, false><<<grid_blocks(n0), 64, 0, c.stream>>>(
                 g.rp, g.col, W, vcmap, nc, list.p + st[0], n0, mstart, members, bound, nullptr, nullptr, nullptr,
                 toff, tcol, tw, ccnt);
         if (n1 > 0)
-            k_cagg<256, 4096, false><<<grid_for(n1, 1, 1 << 20), 256, 0, c.stream>>>(
+            k_cagg<256, 4096, false><<<grid_blocks(n1), 256, 0, c.stream>>>(
                 g.rp, g.col, W, vcmap, nc, list.p + st[1], n1, mstart, members, bound, nullptr, nullptr, nullptr,
                 toff, tcol, tw, ccnt);
     });
+    c.kt_end(tk, 0.0);
     PGGC_LAUNCH_CHECK();
     if (n2 > 0) {
         const i32* big = list.p + st[2];
         Buf<i64> goff(c, n2 + 1), items(c, n2 + 1), mdp(c, n + 1);
-        k_goff_sizes<<<grid_for(n2, 256, 4096), 256, 0, c.stream>>>(big, n2, bound, nc, goff);
+        k_goff_sizes<<<grid_items(n2, 256), 256, 0, c.stream>>>(big, n2, bound, nc, goff);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, goff, goff, n2);
         const i64 gs = c.read1(goff.p + n2);
@@ -546,23 +573,25 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
         Buf<i64> gvals(c, gs);
         fill_i32(c, gkeys, gs, EMPTY_KEY);
         fill_i64(c, gvals, gs, 0);
-        k_member_degs<<<grid_for(n, 1024, 8192), 256, 0, c.stream>>>(members, g.rp, n, mdp);
+        k_member_degs<<<grid_items(n, 256), 256, 0, c.stream>>>(members, g.rp, n, mdp);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, mdp, mdp, n);
-        k_items<<<grid_for(n2, 256, 4096), 256, 0, c.stream>>>(big, n2, bound, goff, nc, 0, items);
+        k_items<<<grid_items(n2, 256), 256, 0, c.stream>>>(big, n2, bound, goff, nc, 0, items);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, items, items, n2);
         const i64 I = c.read1(items.p + n2);
+        tk = c.kt_begin(KS_CONTRACT_BIG);
         if (I > 0) {
-            PGGC_WDISPATCH(g, (k_cbig_insert<<<(unsigned)I, 256, 0, c.stream>>>(
+            PGGC_WDISPATCH(g, (k_cbig_insert<<<grid_blocks(I), 256, 0, c.stream>>>(
                                   g.rp, g.col, W, vcmap, members, mstart, mdp, big, n2, items, goff, gkeys, gvals)));
             PGGC_LAUNCH_CHECK();
         }
-        k_items<<<grid_for(n2, 256, 4096), 256, 0, c.stream>>>(big, n2, bound, goff, nc, 1, items);
+        k_items<<<grid_items(n2, 256), 256, 0, c.stream>>>(big, n2, bound, goff, nc, 1, items);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, items, items, n2);
         const i64 I2 = c.read1(items.p + n2);
-        k_cbig_extract<<<(unsigned)I2, 256, 0, c.stream>>>(big, n2, items, goff, gkeys, gvals, toff, tcol, tw, ccnt);
+        k_cbig_extract<<<grid_blocks(I2), 256, 0, c.stream>>>(big, n2, items, goff, gkeys, gvals, toff, tcol, tw, ccnt);
+        c.kt_end(tk, 0.0);
         PGGC_LAUNCH_CHECK();
     }
     out.n = nc;
@@ -572,14 +601,19 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
     out.col = Buf<i32>(c, std::max<i64>(out.nnz, 1));
     out.w = Buf<i64>(c, std::max<i64>(out.nnz, 1));
     if (out.nnz > 0) {
+        // algorithmic bytes: coarse offsets, row starts in the temporary rows,
+        // (column, weight) read and written once
+        const double bytes = 16.0 * (nc + 1) + 24.0 * out.nnz;
+        const int tkc = c.kt_begin(KS_CONTRACT_COPY);
         k_crow_copy<<<(int)ceil_div(out.nnz, EC), EB, 0, c.stream>>>(out.rp, nc, out.nnz, toff, tcol, tw, out.col,
                                                                    out.w);
+        c.kt_end(tkc, bytes);
         PGGC_LAUNCH_CHECK();
     }
     if (n2 > 0) {
         const i32* big = list.p + st[2];
         Buf<i64> boff(c, n2 + 1);
-        k_big_count<<<grid_for(n2, 256, 4096), 256, 0, c.stream>>>(big, n2, out.rp, boff);
+        k_big_count<<<grid_items(n2, 256), 256, 0, c.stream>>>(big, n2, out.rp, boff);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, boff, boff, n2);
         i64 E = c.read1(boff.p + n2);
@@ -587,7 +621,7 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
             Buf<u64> key(c, E);
             Buf<i32> val(c, E);
             Buf<i64> target(c, E), wtmp(c, E);
-            k_big_flatten<<<grid_for(n2, 1, 65536), 256, 0, c.stream>>>(big, n2, out.rp, boff, out.col, key, val,
+            k_big_flatten<<<grid_blocks(n2), 256, 0, c.stream>>>(big, n2, out.rp, boff, out.col, key, val,
                                                                          target);
             PGGC_LAUNCH_CHECK();
             radix_sort_pairs(c, key, val, E, 32 + bits_for(n2));

@@ -33,6 +33,29 @@ __device__ __forceinline__ void table_add_global(i32* keys, i64* vals, u32 mask,
     atomicAdd((unsigned long long*)&vals[h], (unsigned long long)w);
 }
 
+// Work-item pre-aggregation: a block merges the (key, weight) pairs of one
+// work item (at most LSLOTS / 2 pairs) in an LDS table, then adds each
+// distinct key once to the HBM table.  Hot keys (a hub whose neighbours sit in
+// a few clusters) then cost one global atomic per item instead of one per
+// edge.
+template <int BS, int LSLOTS>
+__device__ __forceinline__ void lds_agg_clear(i32* lk, i64* lv) {
+    for (int q = threadIdx.x; q < LSLOTS; q += BS) {
+        lk[q] = EMPTY_KEY;
+        lv[q] = 0;
+    }
+    __syncthreads();
+}
+template <int BS, int LSLOTS>
+__device__ __forceinline__ void lds_agg_flush(const i32* lk, const i64* lv, i32* gkeys, i64* gvals, u32 gmask) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < LSLOTS; q += BS) {
+        const i32 key = lk[q];
+        const i64 w = lv[q];
+        if (key != EMPTY_KEY) table_add_global(gkeys, gvals, gmask, key, w);
+    }
+}
+
 // Ascending bitonic sort of S (power of two) pairs held in LDS by a block of
 // BS threads.  Must be called by every thread of the block.
 template <int BS>

@@ -11,41 +11,41 @@ namespace pggc {
 // :281-287
 __global__ void k_well(const i64* vw, const i32* louv, const i64* vol, const i64* pv, i64 n, float gamma,
                        i32* well) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
-        i64 wd = vw[i];
-        well[i] = ((float)pv[i] >= gamma * (float)wd * (float)(vol[louv[i]] - wd)) ? 1 : 0;
-    }
+    const i64 i = thread_item();
+    if (i >= n) return;
+    i64 wd = vw[i];
+    well[i] = ((float)pv[i] >= gamma * (float)wd * (float)(vol[louv[i]] - wd)) ? 1 : 0;
 }
 
 // :296-393 (unit weights): first satisfactory neighbour cyclically from a
 // hashed start; 1/1000 of the vertices keep themselves.
 __global__ void k_hn_uniform(const i64* rp, const i32* col, const i64* vw, const i32* louv, const i32* well, i64 n,
                              float gamma, i32 seed, i32* hn) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
-        const i64 s = rp[i], e = rp[i + 1], width = e - s;
-        const float multi = gamma * (float)vw[i];
-        const u32 h = murmur3_i32((i32)((u32)seed + (u32)i));
-        const i32 jx32 = (i32)h;
-        if (jx32 % 1000 == 0 || well[i] == 0 || width == 0) {
-            hn[i] = (i32)i;
-            continue;
-        }
-        const i64 jx = s + (i64)(h % (u32)width);
-        const i32 ci = louv[i];
-        i32 pick = (i32)i;
-        bool found = false;
-        for (i64 j = jx; j < e && !found; j++) {
-            i32 v = col[j];
-            if (ci != louv[v] || well[v] == 0) continue;
-            if (1.0f >= multi * (float)vw[v]) { pick = v; found = true; }
-        }
-        for (i64 j = s; j < jx && !found; j++) {
-            i32 v = col[j];
-            if (ci != louv[v] || well[v] == 0) continue;
-            if (1.0f >= multi * (float)vw[v]) { pick = v; found = true; }
-        }
-        hn[i] = pick;
+    const i64 i = thread_item();
+    if (i >= n) return;
+    const i64 s = rp[i], e = rp[i + 1], width = e - s;
+    const float multi = gamma * (float)vw[i];
+    const u32 h = murmur3_i32((i32)((u32)seed + (u32)i));
+    const i32 jx32 = (i32)h;
+    if (jx32 % 1000 == 0 || well[i] == 0 || width == 0) {
+        hn[i] = (i32)i;
+        return;
     }
+    const i64 jx = s + (i64)(h % (u32)width);
+    const i32 ci = louv[i];
+    i32 pick = (i32)i;
+    bool found = false;
+    for (i64 j = jx; j < e && !found; j++) {
+        i32 v = col[j];
+        if (ci != louv[v] || well[v] == 0) continue;
+        if (1.0f >= multi * (float)vw[v]) { pick = v; found = true; }
+    }
+    for (i64 j = s; j < jx && !found; j++) {
+        i32 v = col[j];
+        if (ci != louv[v] || well[v] == 0) continue;
+        if (1.0f >= multi * (float)vw[v]) { pick = v; found = true; }
+    }
+    hn[i] = pick;
 }
 
 // :395-474 (weighted): argmax over neighbours of rand * (w - gamma w_i w_v)^4
@@ -67,26 +67,26 @@ __device__ __forceinline__ bool hn_sample(const i32* col, WT W, const i64* vw, c
 template <class WT>
 __global__ void k_hn_weighted_small(const i64* rp, const i32* col, WT W, const i64* vw, const i32* louv,
                                     const i32* well, const i32* list, i64 m, float gamma, i32 seed, i32* hn) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (i64)gridDim.x * blockDim.x) {
-        const i32 i = list[x];
-        const i64 s = rp[i], e = rp[i + 1];
-        const float multi = gamma * (float)vw[i];
-        if (murmur3_i32((i32)((u32)seed + (u32)i)) % 10000u == 0 || well[i] == 0) {
-            hn[i] = i;
-            continue;
-        }
-        const i32 ci = louv[i];
-        float m_ = 0.0f;
-        i64 am = e;
-        for (i64 j = s; j < e; j++) {
-            float smp;
-            if (hn_sample(col, W, vw, louv, well, j, ci, multi, seed, smp) && smp >= m_) {
-                m_ = smp;
-                am = j;
-            }
-        }
-        hn[i] = (am < e) ? col[am] : i;
+    const i64 x = thread_item();
+    if (x >= m) return;
+    const i32 i = list[x];
+    const i64 s = rp[i], e = rp[i + 1];
+    const float multi = gamma * (float)vw[i];
+    if (murmur3_i32((i32)((u32)seed + (u32)i)) % 10000u == 0 || well[i] == 0) {
+        hn[i] = i;
+        return;
     }
+    const i32 ci = louv[i];
+    float m_ = 0.0f;
+    i64 am = e;
+    for (i64 j = s; j < e; j++) {
+        float smp;
+        if (hn_sample(col, W, vw, louv, well, j, ci, multi, seed, smp) && smp >= m_) {
+            m_ = smp;
+            am = j;
+        }
+    }
+    hn[i] = (am < e) ? col[am] : i;
 }
 
 template <class WT>
@@ -94,147 +94,156 @@ __global__ void __launch_bounds__(256) k_hn_weighted_large(const i64* rp, const 
                                                            const i32* louv, const i32* well, const i32* list, i64 m,
                                                            float gamma, i32 seed, i32* hn) {
     const int lane = threadIdx.x & 63;
-    for (i64 x = (i64)blockIdx.x * 4 + (threadIdx.x >> 6); x < m; x += (i64)gridDim.x * 4) {
-        const i32 i = list[x];
-        const i64 s = rp[i], e = rp[i + 1];
-        const float multi = gamma * (float)vw[i];
-        if (murmur3_i32((i32)((u32)seed + (u32)i)) % 10000u == 0 || well[i] == 0) {
-            if (lane == 0) hn[i] = i;
-            continue;
-        }
-        const i32 ci = louv[i];
-        u64 best = 0;
-        for (i64 j = s + lane; j < e; j += 64) {
-            float smp;
-            if (hn_sample(col, W, vw, louv, well, j, ci, multi, seed, smp)) {
-                u64 k = ((u64)ord_f32(smp) << 32) | (u64)(0xFFFFFFFFu - (u32)(j - s));
-                best = k > best ? k : best;
-            }
-        }
-        best = wave_max_u64(best);
-        if (lane == 0) hn[i] = best ? col[s + (i64)(0xFFFFFFFFu - (u32)(best & 0xffffffffull))] : i;
+    const i64 x = block_id() * 4 + (threadIdx.x >> 6);
+    if (x >= m) return;
+    const i32 i = list[x];
+    const i64 s = rp[i], e = rp[i + 1];
+    const float multi = gamma * (float)vw[i];
+    if (murmur3_i32((i32)((u32)seed + (u32)i)) % 10000u == 0 || well[i] == 0) {
+        if (lane == 0) hn[i] = i;
+        return;
     }
+    const i32 ci = louv[i];
+    u64 best = 0;
+    for (i64 base = s; base < e; base += 64) {
+        const i64 j = base + lane;
+        if (j >= e) continue;
+        float smp;
+        if (hn_sample(col, W, vw, louv, well, j, ci, multi, seed, smp)) {
+            u64 k = ((u64)ord_f32(smp) << 32) | (u64)(0xFFFFFFFFu - (u32)(j - s));
+            best = k > best ? k : best;
+        }
+    }
+    best = wave_max_u64(best);
+    if (lane == 0) hn[i] = best ? col[s + (i64)(0xFFFFFFFFu - (u32)(best & 0xffffffffull))] : i;
 }
 
 // :482-494
 __global__ void k_order(i64 n, i32 seed, i32* order) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
-        i32 o = (i32)murmur3_i32((i32)((u32)i + (u32)seed));
-        i32 r = o % SPLIT;
-        order[i] = r < 0 ? -r : r;
-    }
+    const i64 i = thread_item();
+    if (i >= n) return;
+    i32 o = (i32)murmur3_i32((i32)((u32)i + (u32)seed));
+    i32 r = o % SPLIT;
+    order[i] = r < 0 ? -r : r;
 }
 __global__ void k_unselect(i64 n, const i32* order, i32* hn) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
-        if (order[i] == order[hn[i]]) hn[i] = (i32)i;
+    const i64 i = thread_item();
+    if (i >= n) return;
+    if (order[i] == order[hn[i]]) hn[i] = (i32)i;
 }
 
 // find_trees (:212-260)
 __global__ void k_trees1(i64 n, const i32* hn, const i32* order, i32* vcmap) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
-        i32 now = (i32)i, then = hn[i];
-        if (now == then) {
-            vcmap[i] = (i32)i;
-            continue;
-        }
-        while (order[now] > order[then]) {
-            now = then;
-            then = hn[then];
-        }
-        if (now != i) {
-            vcmap[i] = now;
-            vcmap[now] = now;
-        }
+    const i64 i = thread_item();
+    if (i >= n) return;
+    i32 now = (i32)i, then = hn[i];
+    if (now == then) {
+        vcmap[i] = (i32)i;
+        return;
+    }
+    while (order[now] > order[then]) {
+        now = then;
+        then = hn[then];
+    }
+    if (now != i) {
+        vcmap[i] = now;
+        vcmap[now] = now;
     }
 }
 __global__ void k_trees_flip(i64 n, const i32* vcmap, i32* order) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
-        if (vcmap[i] == ORD_MAX) order[i] = 2 * SPLIT - order[i];
+    const i64 i = thread_item();
+    if (i >= n) return;
+    if (vcmap[i] == ORD_MAX) order[i] = 2 * SPLIT - order[i];
 }
 __global__ void k_trees2(i64 n, const i32* hn, const i32* order, i32* vcmap) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
-        if (vcmap[i] != ORD_MAX) continue;
-        i32 now = (i32)i, then = hn[i];
-        while (__hip_atomic_load(&vcmap[now], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == ORD_MAX &&
-               order[now] > order[then]) {
-            now = then;
-            then = hn[then];
-        }
-        vcmap[i] = __hip_atomic_load(&vcmap[now], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const i64 i = thread_item();
+    if (i >= n) return;
+    if (vcmap[i] != ORD_MAX) return;
+    i32 now = (i32)i, then = hn[i];
+    while (__hip_atomic_load(&vcmap[now], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == ORD_MAX &&
+           order[now] > order[then]) {
+        now = then;
+        then = hn[then];
     }
+    vcmap[i] = __hip_atomic_load(&vcmap[now], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ensure_gamma_connectivity (:41-187)
 __global__ void k_gkeys(i64 n, const i32* vcmap, const i32* order, u64* key, i32* val) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
-        key[i] = ((u64)(u32)vcmap[i] << 31) | (u64)(u32)order[i];
-        val[i] = (i32)i;
-    }
+    const i64 i = thread_item();
+    if (i >= n) return;
+    key[i] = ((u64)(u32)vcmap[i] << 31) | (u64)(u32)order[i];
+    val[i] = (i32)i;
 }
 __global__ void k_gstarts(i64 n, const i32* ids, const i32* vcmap, i64* cbeg, const i64* vw, i64* wsorted) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x) {
-        i32 i = ids[x];
-        i32 c = vcmap[i];
-        if (x == 0 || vcmap[ids[x - 1]] != c) cbeg[c] = x;
-        wsorted[x] = vw[i];
-    }
+    const i64 x = thread_item();
+    if (x >= n) return;
+    i32 i = ids[x];
+    i32 c = vcmap[i];
+    if (x == 0 || vcmap[ids[x - 1]] != c) cbeg[c] = x;
+    wsorted[x] = vw[i];
 }
 
 template <class WT>
 __global__ void __launch_bounds__(256) k_inner(const i64* rp, const i32* col, WT W, const i32* vcmap,
                                                const i32* order, i64 n, i64* inner) {
     const int lane = threadIdx.x & 63;
-    for (i64 i = (i64)blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += (i64)gridDim.x * 4) {
-        const i32 ci = vcmap[i], oi = order[i];
-        i64 acc = 0;
-        for (i64 j = rp[i] + lane; j < rp[i + 1]; j += 64) {
-            i32 v = col[j];
-            if (vcmap[v] == ci && order[v] < oi) acc += W(j);
-        }
-        acc = wave_sum_i64(acc);
-        if (lane == 0) inner[i] = acc;
+    const i64 i = block_id() * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const i32 ci = vcmap[i], oi = order[i];
+    i64 acc = 0;
+    const i64 e = rp[i + 1];
+    for (i64 base = rp[i]; base < e; base += 64) {
+        const i64 j = base + lane;
+        if (j >= e) continue;
+        i32 v = col[j];
+        if (vcmap[v] == ci && order[v] < oi) acc += W(j);
     }
+    acc = wave_sum_i64(acc);
+    if (lane == 0) inner[i] = acc;
 }
 
 __global__ void k_outer_vals(i64 n, const i32* ids, const i64* pv, const i64* inner, i64* ov) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x) {
-        i32 i = ids[x];
-        ov[x] = pv[i] - 2 * inner[i];
-    }
+    const i64 x = thread_item();
+    if (x >= n) return;
+    i32 i = ids[x];
+    ov[x] = pv[i] - 2 * inner[i];
 }
 
 __global__ void k_breaks(i64 n, const i32* ids, const i32* vcmap, const i32* louv, const i64* cbeg, const i64* tsz,
                          const i64* osc, const i64* inner, const i64* vw, const i64* vol, float gamma,
                          long long* breakers) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x) {
-        i32 i = ids[x];
-        i32 c = vcmap[i];
-        i64 cb = cbeg[c];
-        i64 prev = tsz[x] - tsz[cb];
-        if ((float)inner[i] < gamma * (float)vw[i] * (float)prev) atomicMin(&breakers[c], (long long)x);
-        i64 cur = tsz[x] + vw[i] - tsz[cb];
-        i64 other = vol[louv[i]] - cur;
-        i64 out = osc[x + 1] - osc[cb];
-        if ((float)out < gamma * (float)cur * (float)other) atomicMin(&breakers[c], (long long)(x + 1));
-    }
+    const i64 x = thread_item();
+    if (x >= n) return;
+    i32 i = ids[x];
+    i32 c = vcmap[i];
+    i64 cb = cbeg[c];
+    i64 prev = tsz[x] - tsz[cb];
+    if ((float)inner[i] < gamma * (float)vw[i] * (float)prev) atomicMin(&breakers[c], (long long)x);
+    i64 cur = tsz[x] + vw[i] - tsz[cb];
+    i64 other = vol[louv[i]] - cur;
+    i64 out = osc[x + 1] - osc[cb];
+    if ((float)out < gamma * (float)cur * (float)other) atomicMin(&breakers[c], (long long)(x + 1));
 }
 
 __global__ void k_break_apply(i64 n, const i32* ids, const long long* breakers, i32* vcmap) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x) {
-        i32 i = ids[x];
-        i32 c = vcmap[i];
-        if (x >= breakers[c]) vcmap[i] = i;
-    }
+    const i64 x = thread_item();
+    if (x >= n) return;
+    i32 i = ids[x];
+    i32 c = vcmap[i];
+    if (x >= breakers[c]) vcmap[i] = i;
 }
 
 // contigitize_clusters (:189-210)
 __global__ void k_root_flags(i64 n, const i32* vcmap, i32* f) {
-    for (i64 u = (i64)blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (i64)gridDim.x * blockDim.x)
-        f[u] = vcmap[u] == u ? 1 : 0;
+    const i64 u = thread_item();
+    if (u >= n) return;
+    f[u] = vcmap[u] == u ? 1 : 0;
 }
 __global__ void k_root_relabel(i64 n, const i64* rank, const i32* in, i32* out) {
-    for (i64 u = (i64)blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (i64)gridDim.x * blockDim.x)
-        out[u] = (i32)rank[in[u]];
+    const i64 u = thread_item();
+    if (u >= n) return;
+    out[u] = (i32)rank[in[u]];
 }
 
 static int bitsfor(i64 v) {
@@ -247,7 +256,7 @@ i64 leiden_refine(Ctx& c, const DGraph& g, const Orders& ord, const i32* louv, c
                   const i64* pv, bool uniform, i32* vcmap_out) {
     const i64 n = g.n;
     const float gamma = (float)lambda;
-    const int G = grid_for(n, 1024, 8192);
+    const dim3 G = grid_items(n, 256);
     Buf<i32> well(c, n + 1), hn(c, n + 1), order(c, n + 1), vcmap(c, n + 1);
     k_well<<<G, 256, 0, c.stream>>>(g.vw, louv, vol, pv, n, gamma, well);
     PGGC_LAUNCH_CHECK();
@@ -260,10 +269,10 @@ i64 leiden_refine(Ctx& c, const DGraph& g, const Orders& ord, const i32* louv, c
         const i64 big = ord.off_large;  // order1: degree < 128 first
         PGGC_WDISPATCH(g, {
             if (big > 0)
-                k_hn_weighted_small<<<grid_for(big, 256, 8192), 256, 0, c.stream>>>(
+                k_hn_weighted_small<<<grid_items(big, 256), 256, 0, c.stream>>>(
                     g.rp, g.col, W, g.vw, louv, well, ord.o1, big, gamma, seed, hn);
             if (n > big)
-                k_hn_weighted_large<<<grid_for(n - big, 4, 1 << 16), 256, 0, c.stream>>>(
+                k_hn_weighted_large<<<grid_items(n - big, 4), 256, 0, c.stream>>>(
                     g.rp, g.col, W, g.vw, louv, well, ord.o1.p + big, n - big, gamma, seed, hn);
         });
         PGGC_LAUNCH_CHECK();
@@ -289,7 +298,7 @@ i64 leiden_refine(Ctx& c, const DGraph& g, const Orders& ord, const i32* louv, c
     k_gstarts<<<G, 256, 0, c.stream>>>(n, ids, vcmap, cbeg, g.vw, wsorted);
     PGGC_LAUNCH_CHECK();
     scan_i64(c, wsorted, tsz, n);
-    PGGC_WDISPATCH(g, (k_inner<<<grid_for(n, 4, 1 << 16), 256, 0, c.stream>>>(g.rp, g.col, W, vcmap, order, n,
+    PGGC_WDISPATCH(g, (k_inner<<<grid_items(n, 4), 256, 0, c.stream>>>(g.rp, g.col, W, vcmap, order, n,
                                                                                inner)));
     PGGC_LAUNCH_CHECK();
     k_outer_vals<<<G, 256, 0, c.stream>>>(n, ids, pv, inner, ov);

@@ -56,7 +56,9 @@ constexpr int NCLS = 8;
 __host__ __device__ __forceinline__ int score_class(i64 d) {
     return d <= 4 ? 0 : d <= 8 ? 1 : d <= 16 ? 2 : d <= 32 ? 3 : d <= 64 ? 4 : d <= 512 ? 5 : d <= 2048 ? 6 : 7;
 }
-constexpr int HCH = 4096;  // hub edges (or table slots) per work item
+constexpr int HCH_E = 2048;  // hub edges per insert work item
+constexpr int HCH_S = 4096;  // hub table slots per scan work item
+constexpr int HLDS = 2 * HCH_E;  // LDS pre-aggregation slots per insert item
 __host__ __device__ __forceinline__ i64 tslots(i64 d) {
     i64 s = 2;
     while (s < 2 * d) s <<= 1;
@@ -64,20 +66,22 @@ __host__ __device__ __forceinline__ i64 tslots(i64 d) {
 }
 
 __global__ void k_score_classes(const i64* rp, i64 n, i32* cls) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
-        cls[i] = score_class(rp[i + 1] - rp[i]);
+    const i64 i = thread_item();
+    if (i >= n) return;
+    cls[i] = score_class(rp[i + 1] - rp[i]);
 }
 __global__ void k_global_sizes(const i64* rp, const i32* list, i64 m, i64* sz, i64* items) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (i64)gridDim.x * blockDim.x) {
-        i32 i = list[x];
-        i64 d = rp[i + 1] - rp[i];
-        sz[x] = tslots(d);
-        items[x] = (d + HCH - 1) / HCH;
-    }
+    const i64 x = thread_item();
+    if (x >= m) return;
+    i32 i = list[x];
+    i64 d = rp[i + 1] - rp[i];
+    sz[x] = tslots(d);
+    items[x] = (d + HCH_E - 1) / HCH_E;
 }
 __global__ void k_slot_items(const i64* goff, i64 m, i64* sitems) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (i64)gridDim.x * blockDim.x)
-        sitems[x] = (goff[x + 1] - goff[x] + HCH - 1) / HCH;
+    const i64 x = thread_item();
+    if (x >= m) return;
+    sitems[x] = (goff[x + 1] - goff[x] + HCH_S - 1) / HCH_S;
 }
 
 void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
@@ -87,7 +91,7 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
     make_orders(c, g, p.ord);
     Buf<i32> cls(c, std::max<i64>(g.n, 1));
     p.sc_list = Buf<i32>(c, std::max<i64>(g.n, 1));
-    k_score_classes<<<grid_for(g.n, 1024, 8192), 256, 0, c.stream>>>(g.rp, g.n, cls);
+    k_score_classes<<<grid_items(g.n, 256), 256, 0, c.stream>>>(g.rp, g.n, cls);
     PGGC_LAUNCH_CHECK();
     partition_classes(c, cls, g.n, NCLS, p.sc_list, p.sc_start);
     i64 m7 = p.sc_start[8] - p.sc_start[7];
@@ -98,12 +102,12 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
         p.sitems = Buf<i64>(c, m7 + 1);
         p.hub_pv = Buf<i64>(c, m7);
         p.hub_best = Buf<u64>(c, m7);
-        k_global_sizes<<<grid_for(m7, 256, 4096), 256, 0, c.stream>>>(g.rp, p.sc_list.p + p.sc_start[7], m7, p.goff,
+        k_global_sizes<<<grid_items(m7, 256), 256, 0, c.stream>>>(g.rp, p.sc_list.p + p.sc_start[7], m7, p.goff,
                                                                         p.items);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, p.goff, p.goff, m7);
         scan_i64(c, p.items, p.items, m7);
-        k_slot_items<<<grid_for(m7, 256, 4096), 256, 0, c.stream>>>(p.goff, m7, p.sitems);
+        k_slot_items<<<grid_items(m7, 256), 256, 0, c.stream>>>(p.goff, m7, p.sitems);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, p.sitems, p.sitems, m7);
         p.gslots = c.read1(p.goff.p + m7);
@@ -156,9 +160,11 @@ __global__ void __launch_bounds__(256) k_score_group(const i64* __restrict__ rp,
                                                      i32* __restrict__ dest, float* __restrict__ gain) {
     constexpr int GPB = 256 / G;
     const int gl = threadIdx.x & (G - 1);
-    for (i64 gi = (i64)blockIdx.x * GPB + threadIdx.x / G; gi < m; gi += (i64)gridDim.x * GPB) {
+    const i64 gi = block_id() * GPB + threadIdx.x / G;
+    if (gi >= m) return;
+    {
         const i32 i = list[gi];
-        if (dest[i] == NO_MOVE) continue;
+        if (dest[i] == NO_MOVE) return;
         const i64 s = rp[i];
         const int d = (int)(rp[i + 1] - s);
         const i32 p = part[i];
@@ -172,7 +178,8 @@ __global__ void __launch_bounds__(256) k_score_group(const i64* __restrict__ rp,
             wt = W(s + gl);
         }
         i64 agg = 0;
-        xlane_guard();
+        xlane_fence(key);
+        xlane_fence(wt);
 #pragma unroll
         for (int k = 0; k < G; k++) {
             i32 kk = __shfl(key, k, G);
@@ -205,9 +212,11 @@ __global__ void __launch_bounds__(BS) k_score_table(const i64* __restrict__ rp, 
     __shared__ i32 skeys[GLOBAL ? 1 : SLOTS];
     __shared__ i64 svals[GLOBAL ? 1 : SLOTS];
     const int t = threadIdx.x;
-    for (i64 li = blockIdx.x; li < m; li += gridDim.x) {
+    const i64 li = block_id();
+    if (li >= m) return;
+    {
         const i32 i = list[li];
-        if (dest[i] == NO_MOVE) continue;
+        if (dest[i] == NO_MOVE) return;
         const i64 s = rp[i];
         const i64 d = rp[i + 1] - s;
         const i64 S = tslots(d);
@@ -222,7 +231,9 @@ __global__ void __launch_bounds__(BS) k_score_table(const i64* __restrict__ rp, 
         __syncthreads();
         const i32 p = part[i];
         i64 pv = 0;
-        for (i64 e = t; e < d; e += BS) {
+        for (i64 eb = 0; eb < d; eb += BS) {  // block-uniform loop control
+            const i64 e = eb + t;
+            if (e >= d) continue;
             i32 v = col[s + e];
             i32 key = raw ? v : part[v];
             i64 ww = W(s + e);
@@ -231,11 +242,13 @@ __global__ void __launch_bounds__(BS) k_score_table(const i64* __restrict__ rp, 
             else table_add_lds(keys, vals, mask, key, ww);
         }
         if (GLOBAL) __threadfence();
-        pv = block_sum_i64(pv);
+        pv = block_sum_i64<BS>(pv);
         float multi, p_conn, b;
         own_terms(pv, vol[p], vw[i], lam, ratio, multi, p_conn, b);
         u64 bk = 0;
-        for (i64 q = t; q < S; q += BS) {
+        for (i64 qb = 0; qb < S; qb += BS) {  // block-uniform loop control
+            const i64 q = qb + t;
+            if (q >= S) continue;
             i32 key;
             i64 agg;
             if (GLOBAL) {
@@ -255,14 +268,14 @@ __global__ void __launch_bounds__(BS) k_score_table(const i64* __restrict__ rp, 
                 }
             }
         }
-        bk = block_max_u64(bk);
+        bk = block_max_u64<BS>(bk);
         if (t == 0) decide(i, p_conn, bk, dest, gain);
         __syncthreads();
     }
 }
 
-// Hubs: edge work items of HCH edges insert into the hub's HBM table and add
-// the own-cluster weight; slot work items then scan (and reset) the table and
+// Hubs: edge work items of HCH_E edges pre-aggregate in LDS, then insert into
+// the hub's HBM table and add the own-cluster weight; slot work items then scan (and reset) the table and
 // fold the packed best candidate with atomicMax; a last pass decides.
 template <class WT>
 __global__ void __launch_bounds__(256) k_hub_insert(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
@@ -270,26 +283,30 @@ __global__ void __launch_bounds__(256) k_hub_insert(const i64* __restrict__ rp, 
                                                     const i64* __restrict__ goff, const i32* __restrict__ part,
                                                     const i32* __restrict__ dest, int raw, i32* gkeys, i64* gvals,
                                                     i64* hub_pv) {
-    const i64 b = blockIdx.x;
+    const i64 b = block_id();
+    if (b >= items[m]) return;
     const i64 hi = upper_row(items, 0, m, b);
     const i32 i = hubs[hi];
     if (dest[i] == NO_MOVE) return;
     const i64 s = rp[i], e = rp[i + 1];
-    const i64 j_lo = s + (b - items[hi]) * HCH;
-    const i64 j_hi = j_lo + HCH < e ? j_lo + HCH : e;
-    i32* keys = gkeys + goff[hi];
-    i64* vals = gvals + goff[hi];
-    const u32 mask = (u32)(goff[hi + 1] - goff[hi] - 1);
+    const i64 j_lo = s + (b - items[hi]) * HCH_E;
+    const i64 j_hi = j_lo + HCH_E < e ? j_lo + HCH_E : e;
+    __shared__ i32 lk[HLDS];
+    __shared__ i64 lv[HLDS];
+    lds_agg_clear<256, HLDS>(lk, lv);
     const i32 p = part[i];
     i64 pv = 0;
-    for (i64 j = j_lo + threadIdx.x; j < j_hi; j += 256) {
+    for (i64 jb = j_lo; jb < j_hi; jb += 256) {  // block-uniform loop control
+        const i64 j = jb + threadIdx.x;
+        if (j >= j_hi) continue;
         const i32 v = col[j];
         const i32 key = raw ? v : part[v];
         const i64 ww = W(j);
         if (!raw && key == p) pv += ww;
-        else table_add_global(keys, vals, mask, key, ww);
+        else table_add_lds(lk, lv, HLDS - 1, key, ww);
     }
-    pv = block_sum_i64(pv);
+    lds_agg_flush<256, HLDS>(lk, lv, gkeys + goff[hi], gvals + goff[hi], (u32)(goff[hi + 1] - goff[hi] - 1));
+    pv = block_sum_i64<256>(pv);
     if (threadIdx.x == 0 && pv) atomicAdd((unsigned long long*)&hub_pv[hi], (unsigned long long)pv);
 }
 
@@ -299,18 +316,21 @@ __global__ void __launch_bounds__(256) k_hub_scan(const i32* __restrict__ hubs, 
                                                   const i64* __restrict__ vol, const i32* __restrict__ cons, float lam,
                                                   float ratio, i32* gkeys, i64* gvals, const i64* __restrict__ hub_pv,
                                                   u64* hub_best) {
-    const i64 b = blockIdx.x;
+    const i64 b = block_id();
+    if (b >= sitems[m]) return;
     const i64 hi = upper_row(sitems, 0, m, b);
     const i32 i = hubs[hi];
     if (dest[i] == NO_MOVE) return;
-    const i64 s_lo = goff[hi] + (b - sitems[hi]) * HCH;
+    const i64 s_lo = goff[hi] + (b - sitems[hi]) * HCH_S;
     const i64 s_end = goff[hi + 1];
-    const i64 s_hi = s_lo + HCH < s_end ? s_lo + HCH : s_end;
+    const i64 s_hi = s_lo + HCH_S < s_end ? s_lo + HCH_S : s_end;
     const i32 p = part[i];
     float multi, p_conn, bth;
     own_terms(hub_pv[hi], vol[p], vw[i], lam, ratio, multi, p_conn, bth);
     u64 bk = 0;
-    for (i64 q = s_lo + threadIdx.x; q < s_hi; q += 256) {
+    for (i64 qb = s_lo; qb < s_hi; qb += 256) {  // block-uniform loop control
+        const i64 q = qb + threadIdx.x;
+        if (q >= s_hi) continue;
         const i32 key = gkeys[q];
         if (key == EMPTY_KEY) continue;
         const i64 agg = gvals[q];
@@ -324,22 +344,21 @@ __global__ void __launch_bounds__(256) k_hub_scan(const i32* __restrict__ hubs, 
             }
         }
     }
-    bk = block_max_u64(bk);
+    bk = block_max_u64<256>(bk);
     if (threadIdx.x == 0 && bk) atomicMax((unsigned long long*)&hub_best[hi], (unsigned long long)bk);
 }
 
 __global__ void k_hub_decide(const i32* hubs, i64 m, const i32* part, const i64* vw, const i64* vol, float lam,
                              float ratio, const i64* hub_pv, const u64* hub_best, i32* dest, float* gain) {
-    for (i64 hi = (i64)blockIdx.x * blockDim.x + threadIdx.x; hi < m; hi += (i64)gridDim.x * blockDim.x) {
-        const i32 i = hubs[hi];
-        if (dest[i] == NO_MOVE) continue;
-        float multi, p_conn, bth;
-        own_terms(hub_pv[hi], vol[part[i]], vw[i], lam, ratio, multi, p_conn, bth);
-        decide(i, p_conn, hub_best[hi], dest, gain);
-    }
+    const i64 hi = thread_item();
+    if (hi >= m) return;
+    const i32 i = hubs[hi];
+    if (dest[i] == NO_MOVE) return;
+    float multi, p_conn, bth;
+    own_terms(hub_pv[hi], vol[part[i]], vw[i], lam, ratio, multi, p_conn, bth);
+    decide(i, p_conn, hub_best[hi], dest, gain);
 }
 
-static int cap_grid(i64 blocks, int cap) { return (int)std::max<i64>(1, std::min<i64>(blocks, cap)); }
 
 static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32* part, const i64* vol,
                       const i32* cons, bool raw, float lam, float ratio, i32* dest, float* gain) {
@@ -347,40 +366,46 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
     const i32* L = lp.sc_list.p;
     PGGC_WDISPATCH(g, {
         i64 m;
+        int tk = c.kt_begin(KS_SCORE_GROUP);
         if ((m = st[1] - st[0]) > 0)
-            k_score_group<4><<<cap_grid(ceil_div(m, 64), 1 << 16), 256, 0, c.stream>>>(
+            k_score_group<4><<<grid_items(m, 64), 256, 0, c.stream>>>(
                 g.rp, g.col, W, g.vw, L + st[0], m, part, vol, cons, raw, lam, ratio, dest, gain);
         if ((m = st[2] - st[1]) > 0)
-            k_score_group<8><<<cap_grid(ceil_div(m, 32), 1 << 16), 256, 0, c.stream>>>(
+            k_score_group<8><<<grid_items(m, 32), 256, 0, c.stream>>>(
                 g.rp, g.col, W, g.vw, L + st[1], m, part, vol, cons, raw, lam, ratio, dest, gain);
         if ((m = st[3] - st[2]) > 0)
-            k_score_group<16><<<cap_grid(ceil_div(m, 16), 1 << 16), 256, 0, c.stream>>>(
+            k_score_group<16><<<grid_items(m, 16), 256, 0, c.stream>>>(
                 g.rp, g.col, W, g.vw, L + st[2], m, part, vol, cons, raw, lam, ratio, dest, gain);
         if ((m = st[4] - st[3]) > 0)
-            k_score_group<32><<<cap_grid(ceil_div(m, 8), 1 << 16), 256, 0, c.stream>>>(
+            k_score_group<32><<<grid_items(m, 8), 256, 0, c.stream>>>(
                 g.rp, g.col, W, g.vw, L + st[3], m, part, vol, cons, raw, lam, ratio, dest, gain);
         if ((m = st[5] - st[4]) > 0)
-            k_score_group<64><<<cap_grid(ceil_div(m, 4), 1 << 16), 256, 0, c.stream>>>(
+            k_score_group<64><<<grid_items(m, 4), 256, 0, c.stream>>>(
                 g.rp, g.col, W, g.vw, L + st[4], m, part, vol, cons, raw, lam, ratio, dest, gain);
+        c.kt_end(tk, 0.0);
+        tk = c.kt_begin(KS_SCORE_TABLE);
         if ((m = st[6] - st[5]) > 0)
-            k_score_table<64, 1024, false><<<cap_grid(m, 1 << 18), 64, 0, c.stream>>>(
+            k_score_table<64, 1024, false><<<grid_blocks(m), 64, 0, c.stream>>>(
                 g.rp, g.col, W, g.vw, L + st[5], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr, nullptr,
                 nullptr);
         if ((m = st[7] - st[6]) > 0)
-            k_score_table<512, 4096, false><<<cap_grid(m, 1 << 18), 512, 0, c.stream>>>(
+            k_score_table<512, 4096, false><<<grid_blocks(m), 512, 0, c.stream>>>(
                 g.rp, g.col, W, g.vw, L + st[6], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr, nullptr,
                 nullptr);
+        c.kt_end(tk, 0.0);
         if ((m = st[8] - st[7]) > 0) {
+            tk = c.kt_begin(KS_SCORE_HUB);
             const i32* hubs = L + st[7];
             fill_i64(c, lp.hub_pv, m, 0);
             fill_i64(c, (i64*)lp.hub_best.p, m, 0);
-            k_hub_insert<<<(unsigned)lp.nitems, 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.goff, part,
+            k_hub_insert<<<grid_blocks(lp.nitems), 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.goff, part,
                                                                    dest, raw, lp.gkeys, lp.gvals, lp.hub_pv);
-            k_hub_scan<<<(unsigned)lp.nsitems, 256, 0, c.stream>>>(hubs, m, lp.sitems, lp.goff, part, dest, g.vw, vol,
+            k_hub_scan<<<grid_blocks(lp.nsitems), 256, 0, c.stream>>>(hubs, m, lp.sitems, lp.goff, part, dest, g.vw, vol,
                                                                   cons, lam, ratio, lp.gkeys, lp.gvals, lp.hub_pv,
                                                                   lp.hub_best);
-            k_hub_decide<<<grid_for(m, 256, 1024), 256, 0, c.stream>>>(hubs, m, part, g.vw, vol, lam, ratio, lp.hub_pv,
+            k_hub_decide<<<grid_items(m, 256), 256, 0, c.stream>>>(hubs, m, part, g.vw, vol, lam, ratio, lp.hub_pv,
                                                                        lp.hub_best, dest, gain);
+            c.kt_end(tk, 0.0);
         }
     });
     PGGC_LAUNCH_CHECK();
@@ -388,8 +413,9 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
 
 // ------------------------------------------------------------- candidates
 __global__ void k_cand_flags(const i32* o1, i64 n, const i32* dest, i32* flag) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x)
-        flag[x] = dest[o1[x]] != NO_MOVE ? 1 : 0;
+    const i64 x = thread_item();
+    if (x >= n) return;
+    flag[x] = dest[o1[x]] != NO_MOVE ? 1 : 0;
 }
 // first list positions whose degree reaches LARGE_CUTOFF / MASSIVE_CUTOFF
 __global__ void k_list_bounds(const i32* list, i64 k, const i64* rp, i64* out) {
@@ -408,7 +434,7 @@ __global__ void k_list_bounds(const i32* list, i64 k, const i64* rp, i64* out) {
 // candidates in order1 order; returns count, b[0]/b[1] = class boundaries
 static i64 gather_candidates(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32* dest, i32* out,
                              i64* b) {
-    k_cand_flags<<<grid_for(g.n, 1024, 8192), 256, 0, c.stream>>>(lp.ord.o1, g.n, dest, s.flag);
+    k_cand_flags<<<grid_items(g.n, 256), 256, 0, c.stream>>>(lp.ord.o1, g.n, dest, s.flag);
     PGGC_LAUNCH_CHECK();
     i64 k = compact_flags(c, s.flag, lp.ord.o1, g.n, out, s.scal);
     if (k > 0) {
@@ -446,18 +472,18 @@ __global__ void __launch_bounds__(256) k_ab_serial(const i64* __restrict__ rp, c
                                                    const i64* __restrict__ vw, const i32* __restrict__ cand, i64 k,
                                                    const i32* __restrict__ part, const i32* __restrict__ dest,
                                                    const float* __restrict__ gain, float lam, i32* keep, i32* mflag) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < k; x += (i64)gridDim.x * blockDim.x) {
-        const i32 i = cand[x];
-        const i32 best = dest[i], p = part[i];
-        const float multi = (float)vw[i] * lam;
-        const float igain = gain[i];
-        const i32 hi = (i32)xorshift32((u32)i);
-        float acc = 0.0f;
-        for (i64 j = rp[i]; j < rp[i + 1]; j++) ab_term(col, W, vw, part, dest, gain, j, p, best, multi, igain, hi, acc);
-        const int kp = (igain + acc >= 0.0f) ? 1 : 0;
-        keep[x] = kp;
-        if (kp) mflag[i] = 1;
-    }
+    const i64 x = thread_item();
+    if (x >= k) return;
+    const i32 i = cand[x];
+    const i32 best = dest[i], p = part[i];
+    const float multi = (float)vw[i] * lam;
+    const float igain = gain[i];
+    const i32 hi = (i32)xorshift32((u32)i);
+    float acc = 0.0f;
+    for (i64 j = rp[i]; j < rp[i + 1]; j++) ab_term(col, W, vw, part, dest, gain, j, p, best, multi, igain, hi, acc);
+    const int kp = (igain + acc >= 0.0f) ? 1 : 0;
+    keep[x] = kp;
+    mflag[i] = kp;  // candidates start with mflag 0: an unconditional store
 }
 
 template <class WT>
@@ -466,16 +492,21 @@ __global__ void __launch_bounds__(256) k_ab_wave(const i64* __restrict__ rp, con
                                                  i64 k1, const i32* __restrict__ part, const i32* __restrict__ dest,
                                                  const float* __restrict__ gain, float lam, i32* keep, i32* mflag) {
     const int lane = threadIdx.x & 63;
-    for (i64 x = k0 + (i64)blockIdx.x * 4 + (threadIdx.x >> 6); x < k1; x += (i64)gridDim.x * 4) {
+    const i64 x = k0 + block_id() * 4 + (threadIdx.x >> 6);
+    if (x >= k1) return;
+    {
         const i32 i = cand[x];
         const i32 best = dest[i], p = part[i];
         const float multi = (float)vw[i] * lam;
         const float igain = gain[i];
         const i32 hi = (i32)xorshift32((u32)i);
         float acc = 0.0f;
-        for (i64 j = rp[i] + lane; j < rp[i + 1]; j += 64)
-            ab_term(col, W, vw, part, dest, gain, j, p, best, multi, igain, hi, acc);
-        xlane_guard();
+        const i64 e = rp[i + 1];
+        for (i64 base = rp[i]; base < e; base += 64) {  // wave-uniform loop control
+            const i64 j = base + lane;
+            if (j < e) ab_term(col, W, vw, part, dest, gain, j, p, best, multi, igain, hi, acc);
+        }
+        xlane_fence(acc);
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_down(acc, off);
         if (lane == 0) {
@@ -493,16 +524,20 @@ __global__ void __launch_bounds__(1024) k_ab_block(const i64* __restrict__ rp, c
                                                    const float* __restrict__ gain, float lam, i32* keep, i32* mflag) {
     __shared__ float sh[1024];
     const int t = threadIdx.x;
-    for (i64 x = k0 + blockIdx.x; x < k1; x += gridDim.x) {
+    const i64 x = k0 + block_id();
+    if (x >= k1) return;
+    {
         const i32 i = cand[x];
         const i32 best = dest[i], p = part[i];
         const float multi = (float)vw[i] * lam;
         const float igain = gain[i];
         const i32 hi = (i32)xorshift32((u32)i);
         float acc = 0.0f;
-        for (i64 j = rp[i] + t; j < rp[i + 1]; j += 1024)
-            ab_term(col, W, vw, part, dest, gain, j, p, best, multi, igain, hi, acc);
-        xlane_guard();
+        const i64 e = rp[i + 1];
+        for (i64 base = rp[i]; base < e; base += 1024) {  // block-uniform loop control
+            const i64 j = base + t;
+            if (j < e) ab_term(col, W, vw, part, dest, gain, j, p, best, multi, igain, hi, acc);
+        }
         sh[t] = acc;
         __syncthreads();
         for (int off = 512; off >= 64; off >>= 1) {
@@ -511,7 +546,7 @@ __global__ void __launch_bounds__(1024) k_ab_block(const i64* __restrict__ rp, c
         }
         if (t < 64) {
             acc = sh[t];
-            xlane_guard();
+            xlane_fence(acc);
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_down(acc, off);
             if (t == 0) {
@@ -526,57 +561,62 @@ __global__ void __launch_bounds__(1024) k_ab_block(const i64* __restrict__ rp, c
 
 static i64 afterburner(Ctx& c, MoveScratch& s, const DGraph& g, const i32* cand, i64 k, const i64* b, float lam,
                        i32* moves) {
+    const int tk = c.kt_begin(KS_AFTERBURNER);
     PGGC_WDISPATCH(g, {
         if (b[0] > 0)
-            k_ab_serial<<<cap_grid(ceil_div(b[0], 256), 1 << 16), 256, 0, c.stream>>>(
+            k_ab_serial<<<grid_items(b[0], 256), 256, 0, c.stream>>>(
                 g.rp, g.col, W, g.vw, cand, b[0], s.part, s.dest, s.gain, lam, s.flag, s.mflag);
         if (b[1] > b[0])
-            k_ab_wave<<<cap_grid(ceil_div(b[1] - b[0], 4), 1 << 16), 256, 0, c.stream>>>(
+            k_ab_wave<<<grid_items(b[1] - b[0], 4), 256, 0, c.stream>>>(
                 g.rp, g.col, W, g.vw, cand, b[0], b[1], s.part, s.dest, s.gain, lam, s.flag, s.mflag);
         if (k > b[1])
-            k_ab_block<<<cap_grid(k - b[1], 1 << 14), 1024, 0, c.stream>>>(
+            k_ab_block<<<grid_blocks(k - b[1]), 1024, 0, c.stream>>>(
                 g.rp, g.col, W, g.vw, cand, b[1], k, s.part, s.dest, s.gain, lam, s.flag, s.mflag);
     });
+    c.kt_end(tk, 0.0);
     PGGC_LAUNCH_CHECK();
     return compact_flags(c, s.flag, cand, k, moves, s.scal);
 }
 
 // ------------------------------------------------------------- new labels
 __global__ void k_new_flags(const i32* moves, i64 k, const i32* dest, i32* flag) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < k; x += (i64)gridDim.x * blockDim.x)
-        flag[x] = dest[moves[x]] == NEW_PART ? 1 : 0;
+    const i64 x = thread_item();
+    if (x >= k) return;
+    flag[x] = dest[moves[x]] == NEW_PART ? 1 : 0;
 }
 __global__ void k_mark_used(const i32* part, i64 n, i32* used) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
-        used[part[i]] = 1;
+    const i64 i = thread_item();
+    if (i >= n) return;
+    used[part[i]] = 1;
 }
 __global__ void k_unused_flags(const i32* used, i64 L, i32* flag) {
-    for (i64 l = (i64)blockIdx.x * blockDim.x + threadIdx.x; l < L; l += (i64)gridDim.x * blockDim.x)
-        flag[l] = used[l] == 0 ? 1 : 0;
+    const i64 l = thread_item();
+    if (l >= L) return;
+    flag[l] = used[l] == 0 ? 1 : 0;
 }
 __global__ void k_assign_new(const i32* newv, i64 K, const i32* avail, i64 A, i32 curr, i32* dest, i32* cons,
                              const i32* part) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < K; x += (i64)gridDim.x * blockDim.x) {
-        i32 i = newv[x];
-        i32 l = x < A ? avail[x] : (i32)(x - A) + curr;
-        dest[i] = l;
-        if (cons) cons[l] = cons[part[i]];
-    }
+    const i64 x = thread_item();
+    if (x >= K) return;
+    i32 i = newv[x];
+    i32 l = x < A ? avail[x] : (i32)(x - A) + curr;
+    dest[i] = l;
+    if (cons) cons[l] = cons[part[i]];
 }
 
 // set_new_cluster_ids (src/local_mover.cpp:215-274)
 static void set_new_cluster_ids(Ctx& c, MoveScratch& s, const DGraph& g, RefineData& rd, const i32* moves, i64 k,
                                 i32* cons) {
-    k_new_flags<<<grid_for(k, 1024, 8192), 256, 0, c.stream>>>(moves, k, s.dest, s.flag);
+    k_new_flags<<<grid_items(k, 256), 256, 0, c.stream>>>(moves, k, s.dest, s.flag);
     PGGC_LAUNCH_CHECK();
     i64 K = compact_flags(c, s.flag, moves, k, s.tmp, s.scal);
     if (K == 0) return;
     const i32 curr = rd.label_count;
     fill_i32(c, s.flag, curr, 0);
-    k_mark_used<<<grid_for(g.n, 1024, 8192), 256, 0, c.stream>>>(s.part, g.n, s.flag);
+    k_mark_used<<<grid_items(g.n, 256), 256, 0, c.stream>>>(s.part, g.n, s.flag);
     PGGC_LAUNCH_CHECK();
     Buf<i32> uf(c, std::max<i64>(curr, 1));
-    k_unused_flags<<<grid_for(curr, 1024, 8192), 256, 0, c.stream>>>(s.flag, curr, uf);
+    k_unused_flags<<<grid_items(curr, 256), 256, 0, c.stream>>>(s.flag, curr, uf);
     PGGC_LAUNCH_CHECK();
     i64 A = compact_flags(c, uf, nullptr, curr, s.tmp2, s.scal);
     if (K > A) {
@@ -585,86 +625,148 @@ static void set_new_cluster_ids(Ctx& c, MoveScratch& s, const DGraph& g, RefineD
         fill_i64(c, rd.vol + curr, need, 0);
         rd.label_count = (i32)(curr + need);
     }
-    k_assign_new<<<grid_for(K, 1024, 8192), 256, 0, c.stream>>>(s.tmp, K, s.tmp2, A, curr, s.dest, cons, s.part);
+    k_assign_new<<<grid_items(K, 256), 256, 0, c.stream>>>(s.tmp, K, s.tmp2, A, curr, s.dest, cons, s.part);
     PGGC_LAUNCH_CHECK();
 }
 
 // ------------------------------------------------------------- perform moves
+// Exact uncut delta of the moves (src/local_mover.cpp:1274-1277 computes the
+// same quantity from pvals): per mover, the sum over its edges of
+// w * ([new_i == new_v] - [old_i == old_v]), counted once for mover pairs and
+// twice for edges to vertices that stay.  Move lists are in order1 order, so
+// movers of degree >= LARGE_CUTOFF form a suffix: they take one wave each
+// (counted from the end); the others one thread each.  out[x] = delta of mover x.
 template <class WT>
-__global__ void __launch_bounds__(256) k_delta_uncut(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
-                                                     const i32* __restrict__ moves, i64 k,
-                                                     const i32* __restrict__ part, const i32* __restrict__ dest,
-                                                     const i32* __restrict__ mflag, i64* out) {
+__device__ __forceinline__ i64 delta_term(const i32* col, WT W, const i32* part, const i32* dest, const i32* mflag,
+                                          i64 j, i32 oi, i32 ni) {
+    const i32 v = col[j];
+    const i32 ov = part[v];
+    const int mv = mflag[v];
+    const i32 nv = mv ? dest[v] : ov;
+    const i64 dd = (i64)(ni == nv) - (i64)(oi == ov);
+    return dd * W(j) * (mv ? 1 : 2);
+}
+template <class WT>
+__global__ void __launch_bounds__(256) k_delta_uncut_thread(const i64* __restrict__ rp, const i32* __restrict__ col,
+                                                            WT W, const i32* __restrict__ moves, i64 k,
+                                                            const i32* __restrict__ part,
+                                                            const i32* __restrict__ dest,
+                                                            const i32* __restrict__ mflag, i64* out) {
+    const i64 x = thread_item();
+    if (x >= k) return;
+    const i32 i = moves[x];
+    const i64 s = rp[i], e = rp[i + 1];
+    if (e - s >= LARGE_CUTOFF) return;
+    const i32 oi = part[i], ni = dest[i];
+    i64 acc = 0;
+    for (i64 j = s; j < e; j++) acc += delta_term(col, W, part, dest, mflag, j, oi, ni);
+    out[x] = acc;
+}
+template <class WT>
+__global__ void __launch_bounds__(256) k_delta_uncut_wave(const i64* __restrict__ rp, const i32* __restrict__ col,
+                                                          WT W, const i32* __restrict__ moves, i64 k, i64 nw,
+                                                          const i32* __restrict__ part, const i32* __restrict__ dest,
+                                                          const i32* __restrict__ mflag, i64* out) {
     const int lane = threadIdx.x & 63;
-    i64 tot = 0;
-    for (i64 x = (i64)blockIdx.x * 4 + (threadIdx.x >> 6); x < k; x += (i64)gridDim.x * 4) {
-        const i32 i = moves[x];
-        const i32 oi = part[i], ni = dest[i];
-        i64 acc = 0;
-        for (i64 j = rp[i] + lane; j < rp[i + 1]; j += 64) {
-            const i32 v = col[j];
-            const i32 ov = part[v];
-            const int mv = mflag[v];
-            const i32 nv = mv ? dest[v] : ov;
-            const i64 dd = (i64)(ni == nv) - (i64)(oi == ov);
-            acc += dd * W(j) * (mv ? 1 : 2);
-        }
-        tot += acc;
+    const i64 w = block_id() * 4 + (threadIdx.x >> 6);
+    if (w >= nw) return;
+    const i64 x = k - 1 - w;
+    const i32 i = moves[x];
+    const i64 s = rp[i], e = rp[i + 1];
+    if (e - s < LARGE_CUTOFF) return;
+    const i32 oi = part[i], ni = dest[i];
+    i64 acc = 0;
+    for (i64 base = s; base < e; base += 64) {  // wave-uniform loop control
+        const i64 j = base + lane;
+        if (j < e) acc += delta_term(col, W, part, dest, mflag, j, oi, ni);
     }
-    tot = wave_sum_i64(tot);
-    if (lane == 0 && tot != 0) atomicAdd((unsigned long long*)out, (unsigned long long)tot);
+    acc = wave_sum_i64(acc);
+    if (lane == 0) out[x] = acc;
 }
 
 __global__ void k_apply_moves(const i32* moves, i64 k, const i64* vw, i32* part, const i32* dest, i64* vol) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < k; x += (i64)gridDim.x * blockDim.x) {
-        const i32 i = moves[x];
-        const i32 p = part[i], b = dest[i];
-        const i64 w = vw[i];
-        if (b < 0) continue;  // unresolved NEW_PART cannot reach here (p_conn is 0 for singletons)
-        atomicAdd((unsigned long long*)&vol[p], (unsigned long long)(-w));
-        atomicAdd((unsigned long long*)&vol[b], (unsigned long long)w);
-        part[i] = b;
-    }
+    const i64 x = thread_item();
+    if (x >= k) return;
+    const i32 i = moves[x];
+    const i32 p = part[i], b = dest[i];
+    const i64 w = vw[i];
+    if (b < 0) return;  // unresolved NEW_PART cannot reach here (p_conn is 0 for singletons)
+    atomicAdd((unsigned long long*)&vol[p], (unsigned long long)(-w));
+    atomicAdd((unsigned long long*)&vol[b], (unsigned long long)w);
+    part[i] = b;
 }
 
-__global__ void __launch_bounds__(256) k_reset_affected(const i64* __restrict__ rp, const i32* __restrict__ col,
-                                                        const i32* __restrict__ moves, i64 k, i32* dest,
-                                                        i32* mflag) {
+// dest := NULL_PART for the movers and their neighbours (src/local_mover.cpp:717,798),
+// same degree split as the uncut delta
+__global__ void __launch_bounds__(256) k_reset_affected_thread(const i64* __restrict__ rp,
+                                                               const i32* __restrict__ col,
+                                                               const i32* __restrict__ moves, i64 k, i32* dest,
+                                                               i32* mflag) {
+    const i64 x = thread_item();
+    if (x >= k) return;
+    const i32 i = moves[x];
+    const i64 s = rp[i], e = rp[i + 1];
+    if (e - s >= LARGE_CUTOFF) return;
+    for (i64 j = s; j < e; j++) dest[col[j]] = NULL_PART;
+    dest[i] = NULL_PART;
+    mflag[i] = 0;
+}
+__global__ void __launch_bounds__(256) k_reset_affected_wave(const i64* __restrict__ rp, const i32* __restrict__ col,
+                                                             const i32* __restrict__ moves, i64 k, i64 nw, i32* dest,
+                                                             i32* mflag) {
     const int lane = threadIdx.x & 63;
-    for (i64 x = (i64)blockIdx.x * 4 + (threadIdx.x >> 6); x < k; x += (i64)gridDim.x * 4) {
-        const i32 i = moves[x];
-        for (i64 j = rp[i] + lane; j < rp[i + 1]; j += 64) dest[col[j]] = NULL_PART;
-        if (lane == 0) {
-            dest[i] = NULL_PART;
-            mflag[i] = 0;
-        }
+    const i64 w = block_id() * 4 + (threadIdx.x >> 6);
+    if (w >= nw) return;
+    const i32 i = moves[k - 1 - w];
+    const i64 s = rp[i], e = rp[i + 1];
+    if (e - s < LARGE_CUTOFF) return;
+    for (i64 j = s + lane; j < e; j += 64) dest[col[j]] = NULL_PART;
+    if (lane == 0) {
+        dest[i] = NULL_PART;
+        mflag[i] = 0;
     }
 }
 __global__ void k_clear_mflag(const i32* moves, i64 k, i32* mflag) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < k; x += (i64)gridDim.x * blockDim.x)
-        mflag[moves[x]] = 0;
+    const i64 x = thread_item();
+    if (x >= k) return;
+    mflag[moves[x]] = 0;
 }
 __global__ void k_set_mflag(const i32* moves, i64 k, i32* mflag) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < k; x += (i64)gridDim.x * blockDim.x)
-        mflag[moves[x]] = 1;
+    const i64 x = thread_item();
+    if (x >= k) return;
+    mflag[moves[x]] = 1;
 }
 
-// perform_moves (src/local_mover.cpp:1241-1279)
+// perform_moves (src/local_mover.cpp:1241-1279).  moves is in order1 order;
+// n_large = vertices of the level with degree >= LARGE_CUTOFF.
 static void perform_moves(Ctx& c, MoveScratch& s, const DGraph& g, RefineData& curr, const i32* moves, i64 k,
-                          bool& tables) {
-    fill_i64(c, s.scal, 1, 0);
-    PGGC_WDISPATCH(g, (k_delta_uncut<<<cap_grid(ceil_div(k, 4), 1 << 16), 256, 0, c.stream>>>(
-                          g.rp, g.col, W, moves, k, s.part, s.dest, s.mflag, s.scal)));
-    PGGC_LAUNCH_CHECK();
-    k_apply_moves<<<grid_for(k, 256, 8192), 256, 0, c.stream>>>(moves, k, g.vw, s.part, s.dest, curr.vol);
+                          i64 n_large, bool& tables) {
+    const i64 nw = std::min(k, n_large);
+    const int tk = c.kt_begin(KS_MOVES);
+    {
+        Buf<i64> du(c, std::max<i64>(k, 1));
+        PGGC_WDISPATCH(g, {
+            k_delta_uncut_thread<<<grid_items(k, 256), 256, 0, c.stream>>>(g.rp, g.col, W, moves, k, s.part, s.dest,
+                                                                          s.mflag, du);
+            if (nw > 0)
+                k_delta_uncut_wave<<<grid_items(nw, 4), 256, 0, c.stream>>>(g.rp, g.col, W, moves, k, nw, s.part,
+                                                                           s.dest, s.mflag, du);
+        });
+        PGGC_LAUNCH_CHECK();
+        sum_i64(c, du, k, s.scal);
+    }
+    k_apply_moves<<<grid_items(k, 256), 256, 0, c.stream>>>(moves, k, g.vw, s.part, s.dest, curr.vol);
+    c.kt_end(tk, 0.0);
     PGGC_LAUNCH_CHECK();
     if (!tables) {
         tables = true;
         fill_i32(c, s.dest, g.n, NULL_PART);
-        k_clear_mflag<<<grid_for(k, 256, 8192), 256, 0, c.stream>>>(moves, k, s.mflag);
+        k_clear_mflag<<<grid_items(k, 256), 256, 0, c.stream>>>(moves, k, s.mflag);
     } else {
-        k_reset_affected<<<cap_grid(ceil_div(k, 4), 1 << 16), 256, 0, c.stream>>>(g.rp, g.col, moves, k, s.dest,
-                                                                                  s.mflag);
+        k_reset_affected_thread<<<grid_items(k, 256), 256, 0, c.stream>>>(g.rp, g.col, moves, k, s.dest, s.mflag);
+        if (nw > 0)
+            k_reset_affected_wave<<<grid_items(nw, 4), 256, 0, c.stream>>>(g.rp, g.col, moves, k, nw, s.dest,
+                                                                           s.mflag);
     }
     PGGC_LAUNCH_CHECK();
     sum_sq_u128(c, curr.vol, curr.label_count, s.scal.p + 1);
@@ -675,27 +777,29 @@ static void perform_moves(Ctx& c, MoveScratch& s, const DGraph& g, RefineData& c
 
 // ------------------------------------------------------------- relabel
 __global__ void k_relabel_vol(const i32* used, const i64* pos, i64 L, const i64* vol, i64* nvol) {
-    for (i64 l = (i64)blockIdx.x * blockDim.x + threadIdx.x; l < L; l += (i64)gridDim.x * blockDim.x)
-        if (used[l]) nvol[pos[l]] = vol[l];
+    const i64 l = thread_item();
+    if (l >= L) return;
+    if (used[l]) nvol[pos[l]] = vol[l];
 }
 __global__ void k_relabel_part(const i64* pos, i64 n, i32* part) {
-    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
-        part[i] = (i32)pos[part[i]];
+    const i64 i = thread_item();
+    if (i >= n) return;
+    part[i] = (i32)pos[part[i]];
 }
 
 // relabel_contiguously (src/local_mover.cpp:100-129)
 static void relabel(Ctx& c, MoveScratch& s, i64 n, i32* labels, RefineData& rd) {
     const i64 L = rd.label_count;
     fill_i32(c, s.flag, L, 0);
-    k_mark_used<<<grid_for(n, 1024, 8192), 256, 0, c.stream>>>(labels, n, s.flag);
+    k_mark_used<<<grid_items(n, 256), 256, 0, c.stream>>>(labels, n, s.flag);
     PGGC_LAUNCH_CHECK();
     Buf<i64> pos(c, L + 1), nvol(c, std::max<i64>(L, 1));
     scan_i32_to_i64(c, s.flag, pos, L);
-    k_relabel_vol<<<grid_for(L, 1024, 8192), 256, 0, c.stream>>>(s.flag, pos, L, rd.vol, nvol);
+    k_relabel_vol<<<grid_items(L, 256), 256, 0, c.stream>>>(s.flag, pos, L, rd.vol, nvol);
     PGGC_LAUNCH_CHECK();
     i64 t = c.read1(pos.p + L);
     PGGC_HIP(hipMemcpyAsync(rd.vol, nvol, t * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
-    k_relabel_part<<<grid_for(n, 1024, 8192), 256, 0, c.stream>>>(pos, n, labels);
+    k_relabel_part<<<grid_items(n, 256), 256, 0, c.stream>>>(pos, n, labels);
     PGGC_LAUNCH_CHECK();
     rd.label_count = (i32)t;
 }
@@ -756,7 +860,7 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
             i64 km = afterburner(c, s, g, s.cand, k, b, lam, s.moves);
             if (km == 0) break;
             if (iter > 1 || !is_initial) set_new_cluster_ids(c, s, g, curr, s.moves, km, constrained ? cons : nullptr);
-            perform_moves(c, s, g, curr, s.moves, km, tables);
+            perform_moves(c, s, g, curr, s.moves, km, g.n - lp.ord.off_large, tables);
             if (stats) stats->moves += km;
             if (curr.obj > best.obj) {
                 copy_state(c, best, curr);
@@ -770,8 +874,9 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
 // afterburner_filter_strict (src/local_mover.cpp:131-213): exact sequential
 // FP32 evaluation of each move given all earlier moves in list order.
 __global__ void k_prio(const i32* moves, i64 k, i64* prio) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < k; x += (i64)gridDim.x * blockDim.x)
-        prio[moves[x]] = x;
+    const i64 x = thread_item();
+    if (x >= k) return;
+    prio[moves[x]] = x;
 }
 
 template <class WT>
@@ -780,36 +885,36 @@ __global__ void __launch_bounds__(256) k_strict_gain(const i64* __restrict__ rp,
                                                      i64 k, const i64* __restrict__ prio,
                                                      const i32* __restrict__ part, const i32* __restrict__ dest,
                                                      const float* __restrict__ gain, float lam, float* sg) {
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < k; x += (i64)gridDim.x * blockDim.x) {
-        const i32 i = moves[x];
-        const i32 best = dest[i], p = part[i];
-        const float multi = (float)vw[i] * lam;
-        float oc = gain[i];
-        const i64 pi = prio[i];
-        for (i64 j = rp[i]; j < rp[i + 1]; j++) {
-            const i32 v = col[j];
-            const i64 wgt = W(j);
-            if (prio[v] < pi) {
-                i32 vp = dest[v];
-                oc = oc - (float)((vp == p) ? wgt : 0);
-                oc = oc + (float)((vp == best && best != NEW_PART) ? wgt : 0);
-                vp = part[v];
-                oc = oc + (float)((vp == p) ? wgt : 0);
-                oc = oc - (float)((vp == best) ? wgt : 0);
-            }
-        }
-        for (i64 jx = 0; jx < x; jx++) {
-            const i32 v = moves[jx];
+    const i64 x = thread_item();
+    if (x >= k) return;
+    const i32 i = moves[x];
+    const i32 best = dest[i], p = part[i];
+    const float multi = (float)vw[i] * lam;
+    float oc = gain[i];
+    const i64 pi = prio[i];
+    for (i64 j = rp[i]; j < rp[i + 1]; j++) {
+        const i32 v = col[j];
+        const i64 wgt = W(j);
+        if (prio[v] < pi) {
             i32 vp = dest[v];
-            const float wg = -multi * (float)vw[v];
-            oc = oc - ((vp == p) ? wg : 0.0f);
-            oc = oc + ((vp == best && best != NEW_PART) ? wg : 0.0f);
+            oc = oc - (float)((vp == p) ? wgt : 0);
+            oc = oc + (float)((vp == best && best != NEW_PART) ? wgt : 0);
             vp = part[v];
-            oc = oc + ((vp == p) ? wg : 0.0f);
-            oc = oc - ((vp == best) ? wg : 0.0f);
+            oc = oc + (float)((vp == p) ? wgt : 0);
+            oc = oc - (float)((vp == best) ? wgt : 0);
         }
-        sg[x] = oc;
     }
+    for (i64 jx = 0; jx < x; jx++) {
+        const i32 v = moves[jx];
+        i32 vp = dest[v];
+        const float wg = -multi * (float)vw[v];
+        oc = oc - ((vp == p) ? wg : 0.0f);
+        oc = oc + ((vp == best && best != NEW_PART) ? wg : 0.0f);
+        vp = part[v];
+        oc = oc + ((vp == p) ? wg : 0.0f);
+        oc = oc - ((vp == best) ? wg : 0.0f);
+    }
+    sg[x] = oc;
 }
 
 // sequential inclusive FP32 prefix sum and first arg-max (one thread)
@@ -853,18 +958,18 @@ void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i
         if (k == 0) break;
         if (it > 0 || !is_initial) set_new_cluster_ids(c, s, g, curr, s.moves, k, constrained ? cons : nullptr);
         fill_i64(c, s.prio, n, n + 1);
-        k_prio<<<grid_for(k, 256, 8192), 256, 0, c.stream>>>(s.moves, k, s.prio);
+        k_prio<<<grid_items(k, 256), 256, 0, c.stream>>>(s.moves, k, s.prio);
         PGGC_LAUNCH_CHECK();
-        PGGC_WDISPATCH(g, (k_strict_gain<<<grid_for(k, 256, 1 << 16), 256, 0, c.stream>>>(
+        PGGC_WDISPATCH(g, (k_strict_gain<<<grid_items(k, 256), 256, 0, c.stream>>>(
                               g.rp, g.col, W, g.vw, s.moves, k, s.prio, s.part, s.dest, s.gain, lam, sg)));
         PGGC_LAUNCH_CHECK();
         k_strict_scan<<<1, 64, 0, c.stream>>>(s.moves, k, sg, s.gain, s.scal);
         PGGC_LAUNCH_CHECK();
         i64 tr = c.read1(s.scal);
         if (tr == 0) break;
-        k_set_mflag<<<grid_for(tr, 256, 8192), 256, 0, c.stream>>>(s.moves, tr, s.mflag);
+        k_set_mflag<<<grid_items(tr, 256), 256, 0, c.stream>>>(s.moves, tr, s.mflag);
         PGGC_LAUNCH_CHECK();
-        perform_moves(c, s, g, curr, s.moves, tr, tables);
+        perform_moves(c, s, g, curr, s.moves, tr, g.n - lp.ord.off_large, tables);
         if (stats) stats->moves += tr;
         if (curr.obj > best.obj) {
             copy_state(c, best, curr);
@@ -893,7 +998,7 @@ i64 sweep_once(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32
     i64 km = 0;
     if (k > 0) {
         km = afterburner(c, s, g, s.cand, k, b, lam, moves_out);
-        k_clear_mflag<<<grid_for(std::max<i64>(km, 1), 256, 8192), 256, 0, c.stream>>>(moves_out, km, s.mflag);
+        k_clear_mflag<<<grid_items(km, 256), 256, 0, c.stream>>>(moves_out, km, s.mflag);
         PGGC_LAUNCH_CHECK();
     }
     PGGC_HIP(hipMemcpyAsync(dest, s.dest, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));

@@ -73,10 +73,11 @@ hipEvent_t Ctx::kt_event() {
 }
 
 int Ctx::kt_start(int slot) {
-    if (kt.pend.size() >= 4096) kt_resolve();
-    KTimer::Pending p{slot, kt_event(), kt_event(), 0.0};
+    if (kt.open == 0 && kt.pend.size() >= 4096) kt_resolve();
+    KTimer::Pending p{slot, kt_event(), kt_event(), 0.0, false};
     PGGC_HIP(hipEventRecord(p.a, stream));
     kt.pend.push_back(p);
+    kt.open++;
     return (int)kt.pend.size() - 1;
 }
 
@@ -84,21 +85,26 @@ void Ctx::kt_stop(int tok, double bytes) {
     KTimer::Pending& p = kt.pend[tok];
     p.bytes = bytes;
     PGGC_HIP(hipEventRecord(p.b, stream));
+    p.stopped = true;
+    kt.open--;
 }
 
 void Ctx::kt_resolve() {
     if (kt.pend.empty()) return;
     PGGC_HIP(hipStreamSynchronize(stream));
     for (auto& p : kt.pend) {
-        float t = 0.0f;
-        PGGC_HIP(hipEventElapsedTime(&t, p.a, p.b));
-        kt.launches[p.slot]++;
-        kt.ms[p.slot] += t;
-        kt.bytes[p.slot] += p.bytes;
+        if (p.stopped) {  // regions left open by an error are dropped
+            float t = 0.0f;
+            PGGC_HIP(hipEventElapsedTime(&t, p.a, p.b));
+            kt.launches[p.slot]++;
+            kt.ms[p.slot] += t;
+            kt.bytes[p.slot] += p.bytes;
+        }
         kt.pool.push_back(p.a);
         kt.pool.push_back(p.b);
     }
     kt.pend.clear();
+    kt.open = 0;
 }
 
 void Ctx::kt_free() {
@@ -155,7 +161,7 @@ __global__ void __launch_bounds__(ST) k_tile_sums(Load ld, i64 n, i64* bsum) {
         i64 x = base + (i64)k * ST + threadIdx.x;
         if (x < n) s += ld(x);
     }
-    i64 tot = block_sum_i64(s);
+    i64 tot = block_sum_i64<ST>(s);
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
@@ -295,7 +301,6 @@ __global__ void __launch_bounds__(RT) k_radix_scatter(const u64* kin, const i32*
         u32 d = (u32)(key >> shift) & 255u;
         wc[0][t] = 0; wc[1][t] = 0; wc[2][t] = 0; wc[3][t] = 0;
         __syncthreads();
-        xlane_guard();
         u64 peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
@@ -303,7 +308,6 @@ __global__ void __launch_bounds__(RT) k_radix_scatter(const u64* kin, const i32*
             peers &= ((d >> b) & 1u) ? bal : ~bal;
         }
         u32 rank = (u32)__popcll(peers & lt);
-        xlane_guard();
         if (valid && rank == 0) wc[w][d] = (u32)__popcll(peers);
         __syncthreads();
         u32 c0 = wc[0][t], c1 = wc[1][t], c2 = wc[2][t], c3 = wc[3][t];
@@ -348,8 +352,12 @@ void radix_sort_pairs(Ctx& c, u64* keys, i32* vals, i64 n, int bits) {
 // ------------------------------------------------------------ exact reductions
 __global__ void __launch_bounds__(256) k_sum_i64(const i64* in, i64 n, i64* out) {
     i64 s = 0;
-    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x) s += in[x];
-    s = block_sum_i64(s);
+    // block-uniform loop control (the reduction below is cross-lane)
+    for (i64 base = (i64)blockIdx.x * blockDim.x; base < n; base += (i64)gridDim.x * blockDim.x) {
+        const i64 x = base + threadIdx.x;
+        if (x < n) s += in[x];
+    }
+    s = block_sum_i64<256>(s);
     if (threadIdx.x == 0) atomicAdd((unsigned long long*)out, (unsigned long long)s);
 }
 
@@ -379,7 +387,6 @@ __global__ void __launch_bounds__(256) k_sumsq(const i64* in, i64 n, U128* part)
         s = add128(s, q);
     }
     __shared__ U128 sh[256];
-    xlane_guard();
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -393,7 +400,6 @@ __global__ void __launch_bounds__(256) k_sumsq_final(const U128* part, int np, i
     U128 s{0, 0};
     for (int x = threadIdx.x; x < np; x += blockDim.x) s = add128(s, part[x]);
     __shared__ U128 sh[256];
-    xlane_guard();
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
