@@ -15,9 +15,11 @@
  *    the last failure is available from pggc_last_error().
  *  - A context owns one HIP stream and a caching device allocator.  It is not
  *    thread-safe; different contexts are independent.
- *  - Graphs are symmetric CSR without self loops: row_ptr int64[n+1],
- *    col int32[nnz], at most one of w (int32[nnz]) / w64 (int64[nnz]) set
- *    (neither = unit weights), vtx_w int64[n].
+ *  - Graphs are symmetric CSR without self loops and without repeated
+ *    columns in a row (as the reference's readers produce them,
+ *    app/io/io_mtx.cpp:71-93,127-129): row_ptr int64[n+1], col int32[nnz], at
+ *    most one of w (int32[nnz]) / w64 (int64[nnz]) set (neither = unit
+ *    weights), vtx_w int64[n].
  *  - Results are deterministic for a given seed.
  */
 #ifndef PGGC_H

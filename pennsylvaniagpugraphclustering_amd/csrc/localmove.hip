@@ -177,17 +177,22 @@ __global__ void __launch_bounds__(256) k_score_group(const i64* __restrict__ rp,
             key = raw ? v : part[v];
             wt = W(s + gl);
         }
-        i64 agg = 0;
-        xlane_fence(key);
-        xlane_fence(wt);
+        i64 agg = wt;
+        bool own = false;
+        i64 pv = 0;
+        if (!raw) {  // raw rows have distinct columns and no own-cluster edge
+            agg = 0;
+            xlane_fence(key);
+            xlane_fence(wt);
 #pragma unroll
-        for (int k = 0; k < G; k++) {
-            i32 kk = __shfl(key, k, G);
-            i64 ww = __shfl(wt, k, G);
-            agg += (kk == key) ? ww : 0;
+            for (int k = 0; k < G; k++) {
+                i32 kk = __shfl(key, k, G);
+                i64 ww = __shfl(wt, k, G);
+                agg += (kk == key) ? ww : 0;
+            }
+            own = has && key == p;
+            pv = group_max_i64<G>(own ? agg : (i64)0);
         }
-        const bool own = !raw && has && key == p;
-        const i64 pv = group_max_i64<G>(own ? agg : (i64)0);
         float multi, p_conn, b;
         own_terms(pv, vol[p], wv, lam, ratio, multi, p_conn, b);
         u64 bk = 0;
@@ -272,6 +277,82 @@ __global__ void __launch_bounds__(BS) k_score_table(const i64* __restrict__ rp, 
         if (t == 0) decide(i, p_conn, bk, dest, gain);
         __syncthreads();
     }
+}
+
+// Raw sweep (first sweep of an initial local_move, src/local_mover.cpp:1391-1395):
+// every neighbour is its own cluster with weight w_ij (rows carry no duplicate
+// columns and no self loop), so the own-cluster weight is 0 and the best
+// candidate is a streaming arg-max over the adjacency, with the same packed
+// (gain, smallest label) order as the table path.
+template <class WT>
+__device__ __forceinline__ u64 raw_cand(const i32* __restrict__ col, WT W, const i64* __restrict__ vol,
+                                        const i32* __restrict__ cons, i32 cp, i64 j, float multi, float b) {
+    const i32 v = col[j];
+    const i64 agg = W(j);
+    if (agg > 0 && (float)agg >= b && (cons == nullptr || cons[v] == cp)) {
+        const float jc = fmaf(-(float)vol[v], multi, (float)agg);
+        if (jc >= b) return pack_cand(jc, v);
+    }
+    return 0;
+}
+
+// one block per vertex (64 < degree <= 2048)
+template <int BS, class WT>
+__global__ void __launch_bounds__(BS) k_score_raw(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
+                                                  const i64* __restrict__ vw, const i32* __restrict__ list, i64 m,
+                                                  const i32* __restrict__ part, const i64* __restrict__ vol,
+                                                  const i32* __restrict__ cons, float lam, float ratio,
+                                                  i32* __restrict__ dest, float* __restrict__ gain) {
+    const i64 li = block_id();
+    if (li >= m) return;
+    const i32 i = list[li];
+    if (dest[i] == NO_MOVE) return;
+    const i64 s = rp[i], e = rp[i + 1];
+    const i32 p = part[i];
+    const i32 cp = cons ? cons[p] : 0;
+    float multi, p_conn, b;
+    own_terms(0, vol[p], vw[i], lam, ratio, multi, p_conn, b);
+    u64 bk = 0;
+    for (i64 jb = s; jb < e; jb += BS) {  // block-uniform loop control
+        const i64 j = jb + threadIdx.x;
+        if (j < e) {
+            const u64 k2 = raw_cand(col, W, vol, cons, cp, j, multi, b);
+            bk = k2 > bk ? k2 : bk;
+        }
+    }
+    bk = block_max_u64<BS>(bk);
+    if (threadIdx.x == 0) decide(i, p_conn, bk, dest, gain);
+}
+
+// hubs: one block per edge item, folded into hub_best with atomicMax
+template <class WT>
+__global__ void __launch_bounds__(256) k_hub_raw(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
+                                                 const i64* __restrict__ vw, const i32* __restrict__ hubs, i64 m,
+                                                 const i64* __restrict__ items, const i32* __restrict__ part,
+                                                 const i64* __restrict__ vol, const i32* __restrict__ cons, float lam,
+                                                 float ratio, const i32* __restrict__ dest, u64* hub_best) {
+    const i64 b = block_id();
+    if (b >= items[m]) return;
+    const i64 hi = upper_row(items, 0, m, b);
+    const i32 i = hubs[hi];
+    if (dest[i] == NO_MOVE) return;
+    const i64 s = rp[i], e = rp[i + 1];
+    const i64 j_lo = s + (b - items[hi]) * HCH_E;
+    const i64 j_hi = j_lo + HCH_E < e ? j_lo + HCH_E : e;
+    const i32 p = part[i];
+    const i32 cp = cons ? cons[p] : 0;
+    float multi, p_conn, bth;
+    own_terms(0, vol[p], vw[i], lam, ratio, multi, p_conn, bth);
+    u64 bk = 0;
+    for (i64 jb = j_lo; jb < j_hi; jb += 256) {  // block-uniform loop control
+        const i64 j = jb + threadIdx.x;
+        if (j < j_hi) {
+            const u64 k2 = raw_cand(col, W, vol, cons, cp, j, multi, bth);
+            bk = k2 > bk ? k2 : bk;
+        }
+    }
+    bk = block_max_u64<256>(bk);
+    if (threadIdx.x == 0 && bk) atomicMax((unsigned long long*)&hub_best[hi], (unsigned long long)bk);
 }
 
 // Hubs: edge work items of HCH_E edges pre-aggregate in LDS, then insert into
@@ -384,25 +465,40 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
                 g.rp, g.col, W, g.vw, L + st[4], m, part, vol, cons, raw, lam, ratio, dest, gain);
         c.kt_end(tk, 0.0);
         tk = c.kt_begin(KS_SCORE_TABLE);
-        if ((m = st[6] - st[5]) > 0)
-            k_score_table<64, 1024, false><<<grid_blocks(m), 64, 0, c.stream>>>(
-                g.rp, g.col, W, g.vw, L + st[5], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr, nullptr,
-                nullptr);
-        if ((m = st[7] - st[6]) > 0)
-            k_score_table<512, 4096, false><<<grid_blocks(m), 512, 0, c.stream>>>(
-                g.rp, g.col, W, g.vw, L + st[6], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr, nullptr,
-                nullptr);
+        if (raw) {
+            if ((m = st[6] - st[5]) > 0)
+                k_score_raw<64><<<grid_blocks(m), 64, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[5], m, part, vol,
+                                                                     cons, lam, ratio, dest, gain);
+            if ((m = st[7] - st[6]) > 0)
+                k_score_raw<256><<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[6], m, part, vol,
+                                                                       cons, lam, ratio, dest, gain);
+        } else {
+            if ((m = st[6] - st[5]) > 0)
+                k_score_table<64, 1024, false><<<grid_blocks(m), 64, 0, c.stream>>>(
+                    g.rp, g.col, W, g.vw, L + st[5], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr,
+                    nullptr, nullptr);
+            if ((m = st[7] - st[6]) > 0)
+                k_score_table<512, 4096, false><<<grid_blocks(m), 512, 0, c.stream>>>(
+                    g.rp, g.col, W, g.vw, L + st[6], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr,
+                    nullptr, nullptr);
+        }
         c.kt_end(tk, 0.0);
         if ((m = st[8] - st[7]) > 0) {
             tk = c.kt_begin(KS_SCORE_HUB);
             const i32* hubs = L + st[7];
             fill_i64(c, lp.hub_pv, m, 0);
             fill_i64(c, (i64*)lp.hub_best.p, m, 0);
-            k_hub_insert<<<grid_blocks(lp.nitems), 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.goff, part,
-                                                                   dest, raw, lp.gkeys, lp.gvals, lp.hub_pv);
-            k_hub_scan<<<grid_blocks(lp.nsitems), 256, 0, c.stream>>>(hubs, m, lp.sitems, lp.goff, part, dest, g.vw, vol,
-                                                                  cons, lam, ratio, lp.gkeys, lp.gvals, lp.hub_pv,
-                                                                  lp.hub_best);
+            if (raw) {
+                k_hub_raw<<<grid_blocks(lp.nitems), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, hubs, m, lp.items, part,
+                                                                       vol, cons, lam, ratio, dest, lp.hub_best);
+            } else {
+                k_hub_insert<<<grid_blocks(lp.nitems), 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.goff,
+                                                                          part, dest, raw, lp.gkeys, lp.gvals,
+                                                                          lp.hub_pv);
+                k_hub_scan<<<grid_blocks(lp.nsitems), 256, 0, c.stream>>>(hubs, m, lp.sitems, lp.goff, part, dest,
+                                                                         g.vw, vol, cons, lam, ratio, lp.gkeys,
+                                                                         lp.gvals, lp.hub_pv, lp.hub_best);
+            }
             k_hub_decide<<<grid_items(m, 256), 256, 0, c.stream>>>(hubs, m, part, g.vw, vol, lam, ratio, lp.hub_pv,
                                                                        lp.hub_best, dest, gain);
             c.kt_end(tk, 0.0);
