@@ -99,7 +99,8 @@ int pggc_ctx_set_verbose(pggc_ctx* ctx, int level);
  * launches, summed milliseconds and summed algorithmic bytes of one slot. */
 #define PGGC_KS_SCORE_GROUP 0    /* neighbour-cluster scoring, d <= 64 */
 #define PGGC_KS_SCORE_TABLE 1    /* scoring with LDS tables, 64 < d <= 2048 */
-#define PGGC_KS_SCORE_HUB 2      /* scoring of hubs, HBM tables */
+#define PGGC_KS_SCORE_HUB 2      /* hub edge aggregation (k_hub_insert), unit-weight levels;
+                                    bytes counted on the device */
 #define PGGC_KS_AFTERBURNER 3    /* conflict filter */
 #define PGGC_KS_MOVES 4          /* exact uncut delta + volume moves */
 #define PGGC_KS_CONTRACT_AGG 5   /* coarse-row merge in LDS */

@@ -46,13 +46,16 @@ struct Error : std::runtime_error {
 // Edge-weight accessors: unit (uniform graphs), int32 (input graphs) and
 // int64 (coarse graphs, whose summed weights can exceed int32).
 struct WUnit {
+    static constexpr int bytes = 0;  // weight bytes read per edge
     __device__ __forceinline__ i64 operator()(i64) const { return 1; }
 };
 struct W32 {
+    static constexpr int bytes = 4;
     const i32* p;
     __device__ __forceinline__ i64 operator()(i64 j) const { return p[j]; }
 };
 struct W64 {
+    static constexpr int bytes = 8;
     const i64* p;
     __device__ __forceinline__ i64 operator()(i64 j) const { return p[j]; }
 };

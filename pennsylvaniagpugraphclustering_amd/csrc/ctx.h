@@ -17,7 +17,7 @@ namespace pggc {
 enum KSlot {
     KS_SCORE_GROUP = 0,  // k_score_group<G>
     KS_SCORE_TABLE,      // k_score_table (LDS tables)
-    KS_SCORE_HUB,        // k_hub_insert + k_hub_scan + k_hub_decide
+    KS_SCORE_HUB,        // k_hub_insert on unit-weight levels (bytes counted on the device)
     KS_AFTERBURNER,      // k_ab_serial / k_ab_wave / k_ab_block
     KS_MOVES,            // k_delta_uncut + k_apply_moves
     KS_CONTRACT_AGG,     // k_cagg (LDS merge of coarse rows)
@@ -37,6 +37,7 @@ struct KTimer {
     bool on = false;
     int open = 0;  // regions started and not yet stopped
     std::vector<Pending> pend;
+    u64* dcnt = nullptr;  // per-region device byte counters (kernels add to dcnt[token])
     std::vector<hipEvent_t> pool;
     i64 launches[KS_COUNT] = {0};
     double ms[KS_COUNT] = {0};
@@ -75,10 +76,12 @@ struct Ctx {
     void kt_free();
     // timed region on `stream`: token (or -1 when timing is off) ...
     int kt_begin(int slot) { return kt.on ? kt_start(slot) : -1; }
-    // ... closed with the algorithmic bytes it moved
+    // ... closed with the algorithmic bytes it moved (added to what the
+    // kernels counted in kt_counter(tok))
     void kt_end(int tok, double bytes) {
         if (tok >= 0) kt_stop(tok, bytes);
     }
+    u64* kt_counter(int tok) { return tok >= 0 ? kt.dcnt + tok : nullptr; }
 };
 
 // RAII device buffer from the context's caching allocator.

@@ -33,6 +33,29 @@ __device__ __forceinline__ void table_add_global(i32* keys, i64* vals, u32 mask,
     atomicAdd((unsigned long long*)&vals[h], (unsigned long long)w);
 }
 
+// Packed HBM table: one u64 per slot, key in the high 32 bits and the summed
+// weight in the low 32 bits (callers guarantee every sum fits 32 bits), so an
+// insert touches one cache line and a new key costs a single CAS.
+constexpr u64 PEMPTY = 0xFFFFFFFF00000000ull;  // key -1, weight 0
+
+__device__ __forceinline__ void ptable_add_from(u64* tab, u32 mask, i32 key, u64 w, u32 h) {
+    while (true) {
+        const u64 cur = __hip_atomic_load(&tab[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((u32)(cur >> 32) == (u32)key) break;
+        if (cur == PEMPTY) {
+            const u64 old = atomicCAS((unsigned long long*)&tab[h], (unsigned long long)PEMPTY,
+                                      (unsigned long long)(((u64)(u32)key << 32) | w));
+            if (old == PEMPTY) return;
+            if ((u32)(old >> 32) == (u32)key) break;
+        }
+        h = (h + 1) & mask;
+    }
+    atomicAdd((unsigned long long*)&tab[h], (unsigned long long)w);
+}
+__device__ __forceinline__ void ptable_add(u64* tab, u32 mask, i32 key, u64 w) {
+    ptable_add_from(tab, mask, key, w, slot_hash(key, mask));
+}
+
 // Work-item pre-aggregation: a block merges the (key, weight) pairs of one
 // work item (at most LSLOTS / 2 pairs) in an LDS table, then adds each
 // distinct key once to the HBM table.  Hot keys (a hub whose neighbours sit in
@@ -53,6 +76,46 @@ __device__ __forceinline__ void lds_agg_flush(const i32* lk, const i64* lv, i32*
         const i32 key = lk[q];
         const i64 w = lv[q];
         if (key != EMPTY_KEY) table_add_global(gkeys, gvals, gmask, key, w);
+    }
+}
+// Batched flush into a packed table: each thread first loads the home slots
+// of all its keys (independent loads in flight together), then adds to the
+// slots that already hold the key (no-return atomics), then claims empty
+// slots with CAS; only keys that meet another key fall back to probing.
+template <int BS, int LSLOTS>
+__device__ __forceinline__ void lds_agg_flush_packed(const i32* lk, const i64* lv, u64* tab, u32 gmask) {
+    __syncthreads();
+    constexpr int R = LSLOTS / BS;
+    i32 key[R];
+    u64 w[R], cur[R];
+    u32 h[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        key[r] = lk[threadIdx.x + r * BS];
+        w[r] = (u64)lv[threadIdx.x + r * BS];
+        h[r] = slot_hash(key[r], gmask);
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+        cur[r] = key[r] != EMPTY_KEY ? __hip_atomic_load(&tab[h[r]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                     : PEMPTY;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (key[r] != EMPTY_KEY && (u32)(cur[r] >> 32) == (u32)key[r]) {
+            atomicAdd((unsigned long long*)&tab[h[r]], (unsigned long long)w[r]);
+            key[r] = EMPTY_KEY;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+        if (key[r] != EMPTY_KEY && cur[r] == PEMPTY)
+            cur[r] = atomicCAS((unsigned long long*)&tab[h[r]], (unsigned long long)PEMPTY,
+                               (unsigned long long)(((u64)(u32)key[r] << 32) | w[r]));
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (key[r] == EMPTY_KEY || cur[r] == PEMPTY) continue;  // done, or the CAS claimed the slot
+        if ((u32)(cur[r] >> 32) == (u32)key[r]) atomicAdd((unsigned long long*)&tab[h[r]], (unsigned long long)w[r]);
+        else ptable_add_from(tab, gmask, key[r], w[r], (h[r] + 1) & gmask);
     }
 }
 

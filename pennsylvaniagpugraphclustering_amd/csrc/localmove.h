@@ -36,12 +36,20 @@ struct LevelPlan {
     Buf<i32> sc_list;           // vertices grouped by scoring class
     i64 sc_start[9] = {0};
     // hubs (largest degree class): HBM hash tables, self-cleaning (every
-    // scan resets the slots it reads), edge and slot work items per hub
+    // scan resets the slots it reads), edge and slot work items per hub.
+    // packed: one u64 per slot (key << 32 | weight), used when every hub's
+    // row weight fits 32 bits; otherwise separate key / int64 weight arrays.
+    // imap / smap: block -> work item, so that all items of one hub run on
+    // one XCD (blocks are dispatched round-robin over the 8 XCDs) and its
+    // table stays in that XCD's L2; -1 = idle block.
     Buf<i64> goff, items, sitems, hub_pv;
     Buf<u64> hub_best;
     Buf<i32> gkeys;
     Buf<i64> gvals;
-    i64 gslots = 0, nitems = 0, nsitems = 0;
+    Buf<u64> gtab;
+    Buf<i32> imap, smap;
+    bool packed = false;
+    i64 gslots = 0, nitems = 0, nsitems = 0, igrid = 0, sgrid = 0;
 };
 
 // Device scratch sized for the top level, shared by every level.

@@ -84,6 +84,71 @@ __global__ void k_slot_items(const i64* goff, i64 m, i64* sitems) {
     sitems[x] = (goff[x + 1] - goff[x] + HCH_S - 1) / HCH_S;
 }
 
+// row weight of each hub (one block per edge item)
+template <class WT>
+__global__ void __launch_bounds__(256) k_hub_wsum(const i64* __restrict__ rp, WT W, const i32* __restrict__ hubs,
+                                                  i64 m, const i64* __restrict__ items, i64* hsum) {
+    const i64 b = block_id();
+    if (b >= items[m]) return;
+    const i64 hi = upper_row(items, 0, m, b);
+    const i32 i = hubs[hi];
+    const i64 e = rp[i + 1];
+    const i64 j_lo = rp[i] + (b - items[hi]) * HCH_E;
+    const i64 j_hi = j_lo + HCH_E < e ? j_lo + HCH_E : e;
+    i64 acc = 0;
+    for (i64 jb = j_lo; jb < j_hi; jb += 256) {
+        const i64 j = jb + threadIdx.x;
+        if (j < j_hi) acc += W(j);
+    }
+    acc = block_sum_i64<256>(acc);
+    if (threadIdx.x == 0) atomicAdd((unsigned long long*)&hsum[hi], (unsigned long long)acc);
+}
+
+// Block -> work-item maps that keep each hub on one XCD: hubs are assigned
+// longest-first to the least loaded of the 8 XCDs (load = insert items + scan
+// items); block 8k + x runs the k-th item of XCD x.
+constexpr int NXCD = 8;
+static void build_map(const std::vector<i64>& pre, const std::vector<int>& xcd_of, i64 m, std::vector<i32>& map,
+                      i64& grid) {
+    std::vector<std::vector<i32>> per(NXCD);
+    for (i64 h = 0; h < m; h++)
+        for (i64 it = pre[h]; it < pre[h + 1]; it++) per[xcd_of[h]].push_back((i32)it);
+    size_t mx = 0;
+    for (auto& v : per) mx = std::max(mx, v.size());
+    grid = (i64)mx * NXCD;
+    map.assign(std::max<i64>(grid, 1), -1);
+    for (int x = 0; x < NXCD; x++)
+        for (size_t k = 0; k < per[x].size(); k++) map[k * NXCD + x] = per[x][k];
+}
+
+static void xcd_maps(Ctx& c, LevelPlan& p, i64 m) {
+    std::vector<i64> it(m + 1), st(m + 1);
+    PGGC_HIP(hipMemcpyAsync(it.data(), p.items, (m + 1) * sizeof(i64), hipMemcpyDeviceToHost, c.stream));
+    PGGC_HIP(hipMemcpyAsync(st.data(), p.sitems, (m + 1) * sizeof(i64), hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    std::vector<i64> order(m);
+    for (i64 h = 0; h < m; h++) order[h] = h;
+    auto load = [&](i64 h) { return (it[h + 1] - it[h]) + (st[h + 1] - st[h]); };
+    std::stable_sort(order.begin(), order.end(), [&](i64 a, i64 b) { return load(a) > load(b); });
+    std::vector<i64> tot(NXCD, 0);
+    std::vector<int> xcd_of(m, 0);
+    for (i64 h : order) {
+        int best = 0;
+        for (int x = 1; x < NXCD; x++)
+            if (tot[x] < tot[best]) best = x;
+        xcd_of[h] = best;
+        tot[best] += load(h);
+    }
+    std::vector<i32> map;
+    build_map(it, xcd_of, m, map, p.igrid);
+    p.imap = Buf<i32>(c, map.size());
+    PGGC_HIP(hipMemcpyAsync(p.imap, map.data(), map.size() * sizeof(i32), hipMemcpyHostToDevice, c.stream));
+    build_map(st, xcd_of, m, map, p.sgrid);
+    p.smap = Buf<i32>(c, map.size());
+    PGGC_HIP(hipMemcpyAsync(p.smap, map.data(), map.size() * sizeof(i32), hipMemcpyHostToDevice, c.stream));
+    c.sync();  // the host vectors die here
+}
+
 void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
     if (p.key == (const void*)g.rp && p.n == g.n) return;
     p.key = g.rp;
@@ -113,10 +178,31 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
         p.gslots = c.read1(p.goff.p + m7);
         p.nitems = c.read1(p.items.p + m7);
         p.nsitems = c.read1(p.sitems.p + m7);
-        p.gkeys = Buf<i32>(c, p.gslots);
-        p.gvals = Buf<i64>(c, p.gslots);
-        fill_i32(c, p.gkeys, p.gslots, EMPTY_KEY);
-        fill_i64(c, p.gvals, p.gslots, 0);
+        const i32* hubs = p.sc_list.p + p.sc_start[7];
+        // packed slots need every hub row weight below 2^32
+        p.packed = true;
+        if (g.wk != W_UNIT) {
+            Buf<i64> hsum(c, m7);
+            fill_i64(c, hsum, m7, 0);
+            PGGC_WDISPATCH(g, (k_hub_wsum<<<grid_blocks(p.nitems), 256, 0, c.stream>>>(g.rp, W, hubs, m7, p.items,
+                                                                                    hsum)));
+            PGGC_LAUNCH_CHECK();
+            std::vector<i64> hs(m7);
+            PGGC_HIP(hipMemcpyAsync(hs.data(), hsum, m7 * sizeof(i64), hipMemcpyDeviceToHost, c.stream));
+            c.sync();
+            for (i64 x = 0; x < m7; x++)
+                if (hs[x] >= ((i64)1 << 32) - 1) p.packed = false;
+        }
+        if (p.packed) {
+            p.gtab = Buf<u64>(c, p.gslots);
+            fill_i64(c, (i64*)p.gtab.p, p.gslots, (i64)PEMPTY);
+        } else {
+            p.gkeys = Buf<i32>(c, p.gslots);
+            p.gvals = Buf<i64>(c, p.gslots);
+            fill_i32(c, p.gkeys, p.gslots, EMPTY_KEY);
+            fill_i64(c, p.gvals, p.gslots, 0);
+        }
+        xcd_maps(c, p, m7);
     }
 }
 
@@ -358,14 +444,17 @@ __global__ void __launch_bounds__(256) k_hub_raw(const i64* __restrict__ rp, con
 // Hubs: edge work items of HCH_E edges pre-aggregate in LDS, then insert into
 // the hub's HBM table and add the own-cluster weight; slot work items then scan (and reset) the table and
 // fold the packed best candidate with atomicMax; a last pass decides.
-template <class WT>
+template <bool PACKED, class WT>
 __global__ void __launch_bounds__(256) k_hub_insert(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
                                                     const i32* __restrict__ hubs, i64 m, const i64* __restrict__ items,
+                                                    const i32* __restrict__ imap, i64 nmap,
                                                     const i64* __restrict__ goff, const i32* __restrict__ part,
-                                                    const i32* __restrict__ dest, int raw, i32* gkeys, i64* gvals,
-                                                    i64* hub_pv) {
-    const i64 b = block_id();
-    if (b >= items[m]) return;
+                                                    const i32* __restrict__ dest, i32* gkeys, i64* gvals, u64* gtab,
+                                                    i64* hub_pv, u64* acct) {
+    const i64 bb = block_id();
+    if (bb >= nmap) return;
+    const i64 b = imap[bb];
+    if (b < 0) return;
     const i64 hi = upper_row(items, 0, m, b);
     const i32 i = hubs[hi];
     if (dest[i] == NO_MOVE) return;
@@ -381,24 +470,34 @@ __global__ void __launch_bounds__(256) k_hub_insert(const i64* __restrict__ rp, 
         const i64 j = jb + threadIdx.x;
         if (j >= j_hi) continue;
         const i32 v = col[j];
-        const i32 key = raw ? v : part[v];
+        const i32 key = part[v];
         const i64 ww = W(j);
-        if (!raw && key == p) pv += ww;
+        if (key == p) pv += ww;
         else table_add_lds(lk, lv, HLDS - 1, key, ww);
     }
-    lds_agg_flush<256, HLDS>(lk, lv, gkeys + goff[hi], gvals + goff[hi], (u32)(goff[hi + 1] - goff[hi] - 1));
+    const u32 gmask = (u32)(goff[hi + 1] - goff[hi] - 1);
+    if (PACKED) lds_agg_flush_packed<256, HLDS>(lk, lv, gtab + goff[hi], gmask);
+    else lds_agg_flush<256, HLDS>(lk, lv, gkeys + goff[hi], gvals + goff[hi], gmask);
     pv = block_sum_i64<256>(pv);
     if (threadIdx.x == 0 && pv) atomicAdd((unsigned long long*)&hub_pv[hi], (unsigned long long)pv);
+    // algorithmic bytes of the item: column, neighbour cluster id and weight
+    // per edge, plus the hub's row bounds, cluster and cache flag
+    if (acct && threadIdx.x == 0)
+        atomicAdd((unsigned long long*)acct, (unsigned long long)((j_hi - j_lo) * (8 + WT::bytes) + 24));
 }
 
+template <bool PACKED>
 __global__ void __launch_bounds__(256) k_hub_scan(const i32* __restrict__ hubs, i64 m, const i64* __restrict__ sitems,
+                                                  const i32* __restrict__ smap, i64 nmap,
                                                   const i64* __restrict__ goff, const i32* __restrict__ part,
                                                   const i32* __restrict__ dest, const i64* __restrict__ vw,
                                                   const i64* __restrict__ vol, const i32* __restrict__ cons, float lam,
-                                                  float ratio, i32* gkeys, i64* gvals, const i64* __restrict__ hub_pv,
-                                                  u64* hub_best) {
-    const i64 b = block_id();
-    if (b >= sitems[m]) return;
+                                                  float ratio, i32* gkeys, i64* gvals, u64* gtab,
+                                                  const i64* __restrict__ hub_pv, u64* hub_best) {
+    const i64 bb = block_id();
+    if (bb >= nmap) return;
+    const i64 b = smap[bb];
+    if (b < 0) return;
     const i64 hi = upper_row(sitems, 0, m, b);
     const i32 i = hubs[hi];
     if (dest[i] == NO_MOVE) return;
@@ -412,11 +511,21 @@ __global__ void __launch_bounds__(256) k_hub_scan(const i32* __restrict__ hubs, 
     for (i64 qb = s_lo; qb < s_hi; qb += 256) {  // block-uniform loop control
         const i64 q = qb + threadIdx.x;
         if (q >= s_hi) continue;
-        const i32 key = gkeys[q];
-        if (key == EMPTY_KEY) continue;
-        const i64 agg = gvals[q];
-        gkeys[q] = EMPTY_KEY;
-        gvals[q] = 0;
+        i32 key;
+        i64 agg;
+        if (PACKED) {
+            const u64 e = gtab[q];
+            if (e == PEMPTY) continue;
+            gtab[q] = PEMPTY;
+            key = (i32)(e >> 32);
+            agg = (i64)(e & 0xffffffffull);
+        } else {
+            key = gkeys[q];
+            if (key == EMPTY_KEY) continue;
+            agg = gvals[q];
+            gkeys[q] = EMPTY_KEY;
+            gvals[q] = 0;
+        }
         if (agg > 0 && (float)agg >= bth && (cons == nullptr || cons[key] == cons[p])) {
             const float jc = fmaf(-(float)vol[key], multi, (float)agg);
             if (jc >= bth) {
@@ -484,7 +593,6 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
         }
         c.kt_end(tk, 0.0);
         if ((m = st[8] - st[7]) > 0) {
-            tk = c.kt_begin(KS_SCORE_HUB);
             const i32* hubs = L + st[7];
             fill_i64(c, lp.hub_pv, m, 0);
             fill_i64(c, (i64*)lp.hub_best.p, m, 0);
@@ -492,16 +600,27 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
                 k_hub_raw<<<grid_blocks(lp.nitems), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, hubs, m, lp.items, part,
                                                                        vol, cons, lam, ratio, dest, lp.hub_best);
             } else {
-                k_hub_insert<<<grid_blocks(lp.nitems), 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.goff,
-                                                                          part, dest, raw, lp.gkeys, lp.gvals,
-                                                                          lp.hub_pv);
-                k_hub_scan<<<grid_blocks(lp.nsitems), 256, 0, c.stream>>>(hubs, m, lp.sitems, lp.goff, part, dest,
-                                                                         g.vw, vol, cons, lam, ratio, lp.gkeys,
-                                                                         lp.gvals, lp.hub_pv, lp.hub_best);
+                tk = g.wk == W_UNIT ? c.kt_begin(KS_SCORE_HUB) : -1;
+                if (lp.packed) {
+                    k_hub_insert<true><<<grid_blocks(lp.igrid), 256, 0, c.stream>>>(
+                        g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid, lp.goff, part, dest, nullptr, nullptr,
+                        lp.gtab, lp.hub_pv, c.kt_counter(tk));
+                    c.kt_end(tk, 0.0);
+                    k_hub_scan<true><<<grid_blocks(lp.sgrid), 256, 0, c.stream>>>(
+                        hubs, m, lp.sitems, lp.smap, lp.sgrid, lp.goff, part, dest, g.vw, vol, cons, lam, ratio,
+                        nullptr, nullptr, lp.gtab, lp.hub_pv, lp.hub_best);
+                } else {
+                    k_hub_insert<false><<<grid_blocks(lp.igrid), 256, 0, c.stream>>>(
+                        g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid, lp.goff, part, dest, lp.gkeys,
+                        lp.gvals, nullptr, lp.hub_pv, c.kt_counter(tk));
+                    c.kt_end(tk, 0.0);
+                    k_hub_scan<false><<<grid_blocks(lp.sgrid), 256, 0, c.stream>>>(
+                        hubs, m, lp.sitems, lp.smap, lp.sgrid, lp.goff, part, dest, g.vw, vol, cons, lam, ratio,
+                        lp.gkeys, lp.gvals, nullptr, lp.hub_pv, lp.hub_best);
+                }
             }
             k_hub_decide<<<grid_items(m, 256), 256, 0, c.stream>>>(hubs, m, part, g.vw, vol, lam, ratio, lp.hub_pv,
                                                                        lp.hub_best, dest, gain);
-            c.kt_end(tk, 0.0);
         }
     });
     PGGC_LAUNCH_CHECK();

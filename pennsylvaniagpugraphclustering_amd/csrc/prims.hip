@@ -72,8 +72,14 @@ hipEvent_t Ctx::kt_event() {
     return e;
 }
 
+constexpr int KT_MAX = 4096;  // pending regions before a resolve
+
 int Ctx::kt_start(int slot) {
-    if (kt.open == 0 && kt.pend.size() >= 4096) kt_resolve();
+    if (kt.open == 0 && kt.pend.size() >= KT_MAX) kt_resolve();
+    if (!kt.dcnt) {
+        kt.dcnt = (u64*)alloc(KT_MAX * sizeof(u64));
+        fill_i64(*this, (i64*)kt.dcnt, KT_MAX, 0);
+    }
     KTimer::Pending p{slot, kt_event(), kt_event(), 0.0, false};
     PGGC_HIP(hipEventRecord(p.a, stream));
     kt.pend.push_back(p);
@@ -91,14 +97,18 @@ void Ctx::kt_stop(int tok, double bytes) {
 
 void Ctx::kt_resolve() {
     if (kt.pend.empty()) return;
+    std::vector<u64> cnt(kt.pend.size(), 0);
+    PGGC_HIP(hipMemcpyAsync(cnt.data(), kt.dcnt, cnt.size() * sizeof(u64), hipMemcpyDeviceToHost, stream));
     PGGC_HIP(hipStreamSynchronize(stream));
-    for (auto& p : kt.pend) {
+    fill_i64(*this, (i64*)kt.dcnt, (i64)cnt.size(), 0);
+    for (size_t x = 0; x < kt.pend.size(); x++) {
+        const auto& p = kt.pend[x];
         if (p.stopped) {  // regions left open by an error are dropped
             float t = 0.0f;
             PGGC_HIP(hipEventElapsedTime(&t, p.a, p.b));
             kt.launches[p.slot]++;
             kt.ms[p.slot] += t;
-            kt.bytes[p.slot] += p.bytes;
+            kt.bytes[p.slot] += p.bytes + (double)cnt[x];
         }
         kt.pool.push_back(p.a);
         kt.pool.push_back(p.b);
