@@ -111,6 +111,20 @@ int pggc_ctx_set_verbose(pggc_ctx* ctx, int level);
 int pggc_ctx_set_timing(pggc_ctx* ctx, int on);
 int pggc_ctx_kernel_stats(pggc_ctx* ctx, int slot, int64_t* launches, double* ms, double* bytes);
 
+/* Per-outer-iteration callback of pggc_cluster, called on the host after
+ * the first clustering and after every ex_iters iteration with the wall
+ * seconds of that iteration, the edge cut, get_objective() and the label
+ * count -- what the reference CLI prints and logs per iteration
+ * (app/cluster_main.cpp:137,156; interface/ExperimentLoggerUtil.hpp:135-141).
+ * NULL disables it. */
+typedef void (*pggc_progress_fn)(void* user, int64_t iteration, double seconds, int64_t cut, double objective,
+                                 int32_t labels);
+int pggc_ctx_set_progress(pggc_ctx* ctx, pggc_progress_fn fn, void* user);
+/* wall seconds spent in contraction and in Leiden refinement during the last
+ * pggc_cluster call (the reference logs them as the coarsen-contract and
+ * leiden-refine measurements, src/clustering_methods.cpp:89,117,220) */
+int pggc_ctx_last_times(pggc_ctx* ctx, double* contract_s, double* refine_s);
+
 /* bytes held by the context allocator (host out) */
 int pggc_ctx_memory(pggc_ctx* ctx, int64_t* live, int64_t* cached);
 
@@ -176,7 +190,8 @@ int pggc_leiden(pggc_ctx* ctx, const pggc_graph* g, pggc_state* st, int plus, in
 /* run_clustering (app/cluster_main.cpp:116-165) for one trial: the first
  * clustering plus the ex_iters loop (-1 = until the objective stops rising).
  * objective_out (host) = get_objective(); info (host, 8 entries) =
- * {label_count, uncut, levels, sweeps, moves, outer iterations, candidates, 0}. */
+ * {label_count, uncut, levels, sweeps, moves, outer iterations, candidates, total nnz of the
+ * last call's level graphs}. */
 int pggc_cluster(pggc_ctx* ctx, const pggc_graph* g, int algo, int objective, double lambda_mult, int ex_iters,
                  uint64_t seed, int32_t* part_out, double* objective_out, int64_t* info);
 

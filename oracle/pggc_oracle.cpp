@@ -812,6 +812,7 @@ static std::vector<i32> leiden_part(const Graph& top, State& rfd, bool plus, boo
     std::vector<i32> part(top.n);
     if (!improve) std::iota(part.begin(), part.end(), 0);
     else part = *input;
+    int stalled = 0;  // consecutive refinements that contracted nothing
     while (true) {
         const Graph& c = levels.back();
         double old_obj = rfd.obj;
@@ -832,7 +833,16 @@ static std::vector<i32> leiden_part(const Graph& top, State& rfd, bool plus, boo
             part.assign(nc, 0);
             downsample(louv, part, cmap);
             levels.push_back(std::move(next));
+            stalled = 0;
         } else {
+            // the reference retries forever when no neighbour passes the
+            // refinement tests (:105-110); stop after 8 tries at the current
+            // clustering (the map just pushed is the identity)
+            if (++stalled >= 8) {
+                parts.pop_back();
+                parts.push_back(part);
+                break;
+            }
             Graph same = c;
             levels.push_back(std::move(same));
         }

@@ -65,9 +65,18 @@ SIGNATURES = {
     "pggc_ctx_get_draws": (c.c_int, [P, c.POINTER(c.c_uint64)]),
     "pggc_ctx_set_verbose": (c.c_int, [P, c.c_int]),
     "pggc_ctx_memory": (c.c_int, [P, I64P, I64P]),
+    "pggc_ctx_set_progress": (c.c_int, [P, P, P]),
+    "pggc_ctx_last_times": (c.c_int, [P, c.POINTER(c.c_double), c.POINTER(c.c_double)]),
     "pggc_ctx_set_timing": (c.c_int, [P, c.c_int]),
     "pggc_ctx_kernel_stats": (c.c_int, [P, c.c_int, I64P, c.POINTER(c.c_double), c.POINTER(c.c_double)]),
     "pggc_vertex_weights": (c.c_int, [P, c.POINTER(Graph), c.c_int, P]),
+    # pggc_io.h (host only)
+    "pggc_read_graph": (c.c_int, [c.c_char_p, c.c_int, c.POINTER(c.c_void_p), c.POINTER(c.c_int)]),
+    "pggc_host_graph_info": (c.c_int, [P, I64P, I64P]),
+    "pggc_host_graph_copy": (c.c_int, [P, P, P, P]),
+    "pggc_host_graph_free": (c.c_int, [P]),
+    "pggc_write_part": (c.c_int, [c.c_char_p, P, c.c_int64]),
+    "pggc_read_ints": (c.c_int, [c.c_char_p, c.c_int64, P]),
     "pggc_state_init": (c.c_int, [P, c.POINTER(Graph), c.c_int, c.c_double, c.POINTER(State)]),
     "pggc_orderings": (c.c_int, [P, c.POINTER(Graph), P, P, I64P]),
     "pggc_evaluate": (c.c_int, [P, c.POINTER(Graph), P, c.c_int64, P, I64P, c.POINTER(c.c_uint64)]),

@@ -25,7 +25,13 @@ struct RunStats {
     MoveStats mv;
     i64 levels = 0;
     i64 total_nnz = 0;
+    double contract_s = 0.0;  // wall time in contraction (clustering_methods.cpp:117,220)
+    double refine_s = 0.0;    // wall time in Leiden refinement (:89)
 };
+
+static double since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
 
 // ---------------------------------------------------------------- kernels
 __global__ void k_project(i64 n, const i32* coarse, i32* fine) {
@@ -105,6 +111,9 @@ static Level owned_level(std::unique_ptr<OwnedGraph> og) {
     return l;
 }
 
+// consecutive non-contracting refinements tolerated at one level
+constexpr int LEIDEN_MAX_STALLS = 8;
+
 // louvain_part<constrained> (src/clustering_methods.cpp:139-222)
 static void louvain_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& rfd, bool constrained,
                          i32* constraint, i32* part_out, RunStats& rs, LevelPlan* top_plan) {
@@ -134,7 +143,10 @@ static void louvain_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData&
         if (rfd.label_count < g.n) {
             const i64 nc = rfd.label_count;
             std::unique_ptr<OwnedGraph> og(new OwnedGraph());
+            auto t0 = std::chrono::steady_clock::now();
             contract(c, g, pp, nc, *og);
+            c.sync();
+            rs.contract_s += since(t0);
             og->vw = Buf<i64>(c, std::max<i64>(nc, 1));
             PGGC_HIP(hipMemcpyAsync(og->vw, rfd.vol, nc * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
             if (constrained) {
@@ -152,7 +164,8 @@ static void louvain_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData&
             break;
         }
     }
-    rs.levels = (i64)levels.size();
+    rs.levels = (i64)levels.size();  // setLevelCount / setTotalNnz of the last call
+    rs.total_nnz = 0;
     for (auto& l : levels) rs.total_nnz += l.g.nnz;
     if (levels.size() > 1) {
         levels.pop_back();
@@ -177,6 +190,7 @@ static void leiden_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& 
     Buf<i32> part(c, std::max<i64>(top.n, 1));
     if (!improve) iota_i32(c, part, top.n);
     else PGGC_HIP(hipMemcpyAsync(part, input, top.n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+    int stalled = 0;  // consecutive refinements that contracted nothing
     while (true) {
         const size_t li = levels.size() - 1;
         const DGraph g = levels[li].g;
@@ -195,27 +209,44 @@ static void leiden_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& 
         own_weight(c, g, louv, pv);
         const bool uni = levels.size() == 1 && g.wk == W_UNIT;
         Buf<i32> cmap(c, std::max<i64>(g.n, 1));
+        auto t0 = std::chrono::steady_clock::now();
         const i64 nc = leiden_refine(c, g, lp.ord, louv, rfd.vol, rfd.lambda, pv, uni, cmap);
+        c.sync();
+        rs.refine_s += since(t0);
         const i32* cm = cmap.p;
         parts.push_back(std::move(cmap));
         if (nc < g.n) {
             std::unique_ptr<OwnedGraph> og(new OwnedGraph());
+            auto t1 = std::chrono::steady_clock::now();
             contract(c, g, cm, nc, *og);
+            c.sync();
+            rs.contract_s += since(t1);
             og->vw = Buf<i64>(c, std::max<i64>(nc, 1));
             coarsen_vertex_weights(c, g, cm, nc, og->vw);
             part = Buf<i32>(c, std::max<i64>(nc, 1));
             k_downsample<<<grid_items(g.n, 256), 256, 0, c.stream>>>(g.n, cm, louv, part);
             PGGC_LAUNCH_CHECK();
             levels.push_back(owned_level(std::move(og)));
+            stalled = 0;
         } else {
             part = std::move(louv);
+            // The reference retries the same level with fresh seeds until the
+            // refinement contracts (:105-110), which never happens when no
+            // neighbour passes the refinement tests; stop after LEIDEN_MAX_STALLS
+            // tries at the current clustering (the map just pushed is the identity).
+            if (++stalled >= LEIDEN_MAX_STALLS) {
+                parts.pop_back();
+                parts.push_back(std::move(part));
+                break;
+            }
             Level same;
             same.g = g;
             same.plan.reset(new LevelPlan());
             levels.push_back(std::move(same));
         }
     }
-    rs.levels = (i64)levels.size();
+    rs.levels = (i64)levels.size();  // setLevelCount / setTotalNnz of the last call
+    rs.total_nnz = 0;
     for (auto& l : levels) rs.total_nnz += l.g.nnz;
     for (int i = (int)levels.size() - 2; i >= 0; i--) {
         const DGraph g = levels[i].g;
@@ -276,6 +307,9 @@ static void state_init(Ctx& c, const DGraph& g, int objective, double mult, Refi
 
 // =================================================================== C ABI
 static thread_local std::string g_err;
+
+// shared with the host-only file readers (io.cpp)
+void set_last_error(const std::string& msg) { g_err = msg; }
 
 template <class F>
 static int guard(F f) {
@@ -382,6 +416,22 @@ int pggc_ctx_set_verbose(pggc_ctx* ctx, int level) {
         ctx->c.verbose = level;
     });
 }
+int pggc_ctx_set_progress(pggc_ctx* ctx, pggc_progress_fn fn, void* user) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        ctx->c.progress = fn;
+        ctx->c.progress_user = user;
+    });
+}
+
+int pggc_ctx_last_times(pggc_ctx* ctx, double* contract_s, double* refine_s) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (contract_s) *contract_s = ctx->c.last_contract_s;
+        if (refine_s) *refine_s = ctx->c.last_refine_s;
+    });
+}
+
 int pggc_ctx_set_timing(pggc_ctx* ctx, int on) {
     return guard([&] {
         CTX_CHECK(ctx);
@@ -615,13 +665,18 @@ int pggc_cluster(pggc_ctx* ctx, const pggc_graph* gg, int algo, int objective, d
         // per-iteration line as printed by the reference (app/cluster_main.cpp:137,156,
         // interface/cluster_data.hpp:98-103)
         auto t_iter = std::chrono::steady_clock::now();
+        i64 iter_no = 0;
         auto report = [&]() {
-            if (!c.verbose) return;
+            c.sync();
             auto now = std::chrono::steady_clock::now();
             double secs = std::chrono::duration<double>(now - t_iter).count();
             t_iter = now;
+            const i64 cut = (rd.top_nnz - rd.uncut) / 2;
+            if (c.progress) c.progress(c.progress_user, iter_no, secs, cut, rd.objective(), rd.label_count);
+            iter_no++;
+            if (!c.verbose) return;
             fprintf(stderr, "Cluster time: %g; Cut: %lld; Objective: %.9g; Labels: %d; sweeps so far: %lld\n", secs,
-                    (long long)((rd.top_nnz - rd.uncut) / 2), rd.objective(), rd.label_count, (long long)rs.mv.sweeps);
+                    (long long)cut, rd.objective(), rd.label_count, (long long)rs.mv.sweeps);
             fflush(stderr);
         };
         if (algo == PGGC_LOUVAIN) louvain_part(c, ctx->ms, g, rd, false, nullptr, part_out, rs, &top);
@@ -647,6 +702,8 @@ int pggc_cluster(pggc_ctx* ctx, const pggc_graph* gg, int algo, int objective, d
             report();
         }
         c.sync();
+        c.last_contract_s = rs.contract_s;
+        c.last_refine_s = rs.refine_s;
         *objective_out = rd.objective();
         if (info) {
             info[0] = rd.label_count;

@@ -56,6 +56,10 @@ struct Ctx {
     u64 seed = 0;           // leidenR seed stream
     u64 draws = 0;
     int verbose = 0;        // >0: per-iteration progress lines on stderr
+    // per-outer-iteration callback (iteration, seconds, cut, objective, labels)
+    void (*progress)(void*, int64_t, double, int64_t, double, int32_t) = nullptr;
+    void* progress_user = nullptr;
+    double last_contract_s = 0.0, last_refine_s = 0.0;  // of the last clustering call
     KTimer kt;
 
     void* alloc(size_t bytes);

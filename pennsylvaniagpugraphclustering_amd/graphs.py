@@ -230,3 +230,59 @@ def random_graph(n, avg_deg, seed, weights=None):
     v = rng.integers(0, n, m)
     w = None if weights is None else rng.integers(weights[0], weights[1] + 1, m)
     return from_edges(n, u, v, w)
+
+
+def read_graph(path, objective="Mod"):
+    """load_graph (app/io/io_mtx.cpp:337-346) through the library's reader:
+    ``.mtx`` files are Matrix Market, anything else METIS; edge weights are
+    dropped for the objectives that treat edges as unit (Mod, CPM).  Host
+    only; returns a CSR."""
+    import ctypes
+    from . import _lib
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    weighted = ctypes.c_int()
+    unit = 1 if objective in ("Mod", "CPM") else 0
+    _lib.check(lib.pggc_read_graph(str(path).encode(), unit, ctypes.byref(h), ctypes.byref(weighted)))
+    try:
+        n, nnz = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(lib.pggc_host_graph_info(h, ctypes.byref(n), ctypes.byref(nnz)))
+        rp = np.empty(n.value + 1, np.int64)
+        col = np.empty(nnz.value, np.int32)
+        w = np.empty(nnz.value, np.int32) if weighted.value else None
+        _lib.check(lib.pggc_host_graph_copy(h, rp.ctypes.data, col.ctypes.data, None if w is None else w.ctypes.data))
+    finally:
+        lib.pggc_host_graph_free(h)
+    return CSR(n.value, rp, col, w)
+
+
+def write_metis(g, path):
+    """METIS text: header 'n m [1]', one line per vertex with 1-based
+    neighbours (and weights when the graph has them)."""
+    with open(path, "w") as f:
+        f.write(f"{g.n} {g.nnz // 2}" + (" 1" if g.w is not None else "") + "\n")
+        for i in range(g.n):
+            a, b = g.rp[i], g.rp[i + 1]
+            if g.w is None:
+                f.write(" ".join(str(v + 1) for v in g.col[a:b]) + "\n")
+            else:
+                f.write(" ".join(f"{v + 1} {x}" for v, x in zip(g.col[a:b], g.w[a:b])) + "\n")
+
+
+def write_mtx(g, path, symmetric=True):
+    """Matrix Market coordinate file: the lower triangle when symmetric,
+    every directed entry otherwise (pattern when unweighted)."""
+    field = "pattern" if g.w is None or not symmetric else "integer"
+    rows = np.repeat(np.arange(g.n), np.diff(g.rp))
+    keep = (rows > g.col) if symmetric else np.ones(len(rows), bool)
+    r, c = rows[keep], g.col[keep]
+    with open(path, "w") as f:
+        f.write(f"%%MatrixMarket matrix coordinate {field} {'symmetric' if symmetric else 'general'}\n")
+        f.write("% written by graphs.write_mtx\n")
+        f.write(f"{g.n} {g.n} {len(r)}\n")
+        if field == "integer":
+            for u, v, x in zip(r, c, g.w[keep]):
+                f.write(f"{u + 1} {v + 1} {x}\n")
+        else:
+            for u, v in zip(r, c):
+                f.write(f"{u + 1} {v + 1}\n")

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared():
-    src = open(os.path.join(ROOT, "include", "pggc.h")).read()
+    src = "".join(open(os.path.join(ROOT, "include", h)).read() for h in ("pggc.h", "pggc_io.h"))
     return sorted(set(re.findall(r"\b(pggc_\w+)\s*\(", src)))
 
 

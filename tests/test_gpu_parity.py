@@ -237,3 +237,19 @@ def test_determinism_rmat20(gpu_ctx):
     q3 = C().modularity(dg, p1, "Mod")
     assert abs(q3 - q1) <= 1e-12
     assert i1["labels"] == int(p1.max().item()) + 1
+
+
+@pytest.mark.parametrize("name", ["rmat12w", "er3k_w"])
+@pytest.mark.parametrize("objective", ["CPM", "NLCC", "LCC"])
+@pytest.mark.parametrize("algo", ["louvain", "leiden+"])
+def test_lambdacc_objectives(gpu_ctx, name, objective, algo):
+    """CPM, NLCC and LCC (interface/cluster_data.hpp:135-182): whole runs
+    bit-identical to the oracle, lambda multiplier 0.5."""
+    g, _ = CASES[name]
+    vw = g.vertex_weights(objective)
+    w = None if objective == "CPM" else g.w
+    print(f"cluster {name} {objective} {algo}", flush=True)
+    q, part, info = C().cluster(dgraph(g, objective), algo, objective, 0.5, 0, seed=5)
+    rq, rpart, rinfo = O.cluster(g.rp, g.col, w, vw, objective, 0.5, algo, 0, seed=5)
+    assert np.array_equal(npy(part), rpart)
+    assert q == rq

@@ -1,0 +1,154 @@
+"""Graph and vector files of the command-line tools (host only, no GPU):
+the METIS and Matrix Market readers behind pggc_read_graph against
+graphs.from_edges on the same edges, their error paths, and the argument
+handling of the plouvain / verify_obj executables
+(app/io/io_metis.cpp, app/io/io_mtx.cpp, app/io/parse_args.cpp)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from pennsylvaniagpugraphclustering_amd import _lib, graphs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "pennsylvaniagpugraphclustering_amd", "bin")
+
+
+def same(a, b, weights=True):
+    assert a.n == b.n
+    assert np.array_equal(a.rp, b.rp)
+    assert np.array_equal(a.col, b.col)
+    if weights:
+        assert (a.w is None) == (b.w is None)
+        if a.w is not None:
+            assert np.array_equal(a.w, b.w)
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_metis_round_trip(tmp_path, weighted):
+    g = graphs.random_graph(500, 8, seed=3, weights=(1, 9) if weighted else None)
+    p = tmp_path / "g.graph"
+    graphs.write_metis(g, p)
+    same(graphs.read_graph(p, "WMod"), g)
+    # Mod treats edges as unit: the weights are dropped
+    h = graphs.read_graph(p, "Mod")
+    assert h.w is None
+    same(h, g, weights=False)
+
+
+def test_metis_comments_isolated_vertices_and_self_loops(tmp_path):
+    p = tmp_path / "s.graph"
+    # 5 vertices, 3 undirected edges (+ a self loop listed twice), vertex 4 isolated
+    p.write_text("% a comment\n5 4\n2 1 3\n1\n1 5\n\n3 1\n")
+    g = graphs.read_graph(p, "Mod")
+    assert g.n == 5
+    assert list(g.rp) == [0, 2, 3, 5, 5, 7]
+    assert list(g.col) == [1, 2, 0, 0, 4, 2, 0]  # row order as in the file, self loop dropped
+
+
+def test_metis_count_mismatch_is_an_error(tmp_path):
+    p = tmp_path / "bad.graph"
+    p.write_text("3 2\n2\n1 3\n")
+    with pytest.raises(_lib.PggcError) as e:
+        graphs.read_graph(p)
+    assert "Mismatch" in str(e.value)
+
+
+def test_metis_vertex_weights_rejected(tmp_path):
+    p = tmp_path / "vw.graph"
+    p.write_text("2 1 10 1\n1 2\n1 1\n")
+    with pytest.raises(_lib.PggcError) as e:
+        graphs.read_graph(p)
+    assert "Unsupported format flags" in str(e.value)
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_mtx_matches_from_edges(tmp_path, symmetric, weighted):
+    g = graphs.random_graph(400, 6, seed=5, weights=(1, 50) if weighted else None)
+    p = tmp_path / "g.mtx"
+    graphs.write_mtx(g, p, symmetric=symmetric)
+    h = graphs.read_graph(p, "WMod")
+    # non-symmetric files are symmetrised and lose their weights (io_mtx.cpp:254-260)
+    same(h, g, weights=symmetric)
+    if not symmetric:
+        assert h.w is None
+
+
+def test_mtx_duplicates_self_loops_and_zero_weights(tmp_path):
+    p = tmp_path / "d.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate integer symmetric\n% c\n4 4 6\n2 1 5\n3 3 1\n2 1 7\n4 2 0\n"
+                 "4 3 2\n3 1 4\n")
+    g = graphs.read_graph(p, "WMod")
+    ref = graphs.from_edges(4, [1, 3, 2], [0, 2, 0], w=[5, 2, 4])
+    same(g, ref)
+
+
+def test_mtx_rejects_bipartite_and_array(tmp_path):
+    p = tmp_path / "b.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate pattern general\n3 4 1\n1 2\n")
+    with pytest.raises(_lib.PggcError) as e:
+        graphs.read_graph(p)
+    assert "Bipartite" in str(e.value)
+    p.write_text("%%MatrixMarket matrix array real general\n2 2\n1\n2\n3\n4\n")
+    with pytest.raises(_lib.PggcError) as e:
+        graphs.read_graph(p)
+    assert "non-coordinate" in str(e.value)
+
+
+def test_partition_file_round_trip(tmp_path):
+    lib = _lib.load()
+    part = np.random.default_rng(0).integers(0, 1000, 777).astype(np.int32)
+    p = str(tmp_path / "part.txt").encode()
+    _lib.check(lib.pggc_write_part(p, part.ctypes.data, len(part)))
+    assert open(p).read().split() == [str(x) for x in part]
+    out = np.empty(len(part), np.int64)
+    _lib.check(lib.pggc_read_ints(p, len(part), out.ctypes.data))
+    assert np.array_equal(out, part)
+    with pytest.raises(_lib.PggcError):
+        _lib.check(lib.pggc_read_ints(p, len(part) + 1, np.empty(len(part) + 1, np.int64).ctypes.data))
+
+
+def run_tool(name, *args):
+    return subprocess.run([os.path.join(BIN, name), *args], capture_output=True, text=True, timeout=60)
+
+
+def test_tools_are_built():
+    for t in ("plouvain", "pleiden", "pleiden+", "verify_obj"):
+        assert os.access(os.path.join(BIN, t), os.X_OK), t
+
+
+def test_cli_argument_errors_without_gpu():
+    """Argument errors are reported before any GPU call (parse_args.cpp)."""
+    r = run_tool("plouvain")
+    assert r.returncode != 0
+    assert "Required graph input file not given" in r.stderr
+    r = run_tool("plouvain", "-i", "x.graph", "-bogus", "1")
+    assert "Invalid option name: -bogus" in r.stderr
+    r = run_tool("plouvain", "-i", "x.graph", "-objective", "Foo")
+    assert "Invalid objective name 'Foo'" in r.stderr
+    r = run_tool("pleiden+", "-i", "x.graph", "-lambda")
+    assert "Value not provided for option '-lambda'" in r.stderr
+    r = run_tool("plouvain", "-i", "x.graph", "-vtx_weights", "w.txt")
+    assert "Vertex weights file is only supported" in r.stderr
+    r = run_tool("verify_obj", "-i", "x.graph")
+    assert "Required clusters file not given" in r.stderr
+
+
+def test_cli_configuration_printout_without_gpu():
+    r = run_tool("plouvain", "-i", "g.mtx", "-objective", "NLCC", "-lambda", "0.5", "-trials", "0", "-ex_iters",
+                 "-7", "-vtx_weights", "w.txt", "-o", "out.txt", "-metrics", "m.json", "-bogus")
+    assert "Invalid option name: -bogus" in r.stderr
+    # without the bad option the configuration is printed before the (failing) GPU context
+    r = run_tool("plouvain", "-i", "/nonexistent.graph", "-objective", "NLCC", "-lambda", "0.5", "-trials", "0",
+                 "-ex_iters", "-7", "-o", "out.txt")
+    out = r.stdout
+    assert "-- Input Graph File: /nonexistent.graph" in out
+    assert "-- Vertex Weighting Method: Unit Uniform" in out
+    assert "-- Objective Type: Normalized LambdaCC" in out
+    assert "-- Lambda Multiplier: 0.5" in out
+    assert "-- Output Clusters File: out.txt" in out
+    assert "-- Successive Iteration Count: -1" in out
+    assert "-- Total trial count: 1" in out
+    assert r.returncode != 0

@@ -281,3 +281,51 @@ def test_strict_reaches_node_optimality():
         for c in set(part[g.col[g.rp[i]:g.rp[i + 1]]].tolist()) - {part[i]}:
             lab = part.copy(); lab[i] = c
             assert exact(lab) - base <= 1e-6
+
+
+@pytest.mark.parametrize("objective", ["CPM", "NLCC", "LCC"])
+@pytest.mark.parametrize("mult", [1.0, 0.3])
+def test_lambdacc_objectives_match_pairwise_definition(objective, mult):
+    """Objectives.md: lambdaCC(C) = sum_i sum_{u,v in C_i} (w(u,v) - lambda w(u) w(v)),
+    with lambda = mult/2 and unit edges for CPM (reported without the u = v
+    terms, interface/cluster_data.hpp:135-153), lambda = mult w(E)/w(V)^2 and
+    the value divided by w(E) for NLCC (:155-177), lambda = mult for LCC.
+    Evaluated here by a double loop over vertex pairs on the oracle's own
+    clustering."""
+    g = graphs.random_graph(120, 6, seed=11, weights=(1, 10))
+    vw = g.vertex_weights(objective)
+    w = None if objective == "CPM" else g.w
+    q, part, info = O.cluster(g.rp, g.col, w, vw, objective, mult, "louvain", 0)
+    n = g.n
+    W = np.zeros((n, n))
+    for u in range(n):
+        for j in range(g.rp[u], g.rp[u + 1]):
+            W[u, g.col[j]] = 1.0 if w is None else float(w[j])
+    wE = W.sum()
+    lam = {"CPM": 0.5 * mult, "NLCC": mult * wE / float(vw.sum()) ** 2, "LCC": mult}[objective]
+    total = 0.0
+    for u in range(n):
+        for v in range(n):
+            if part[u] == part[v] and (objective != "CPM" or u != v):
+                total += W[u, v] - lam * float(vw[u]) * float(vw[v])
+    if objective == "NLCC":
+        total /= wE
+    assert abs(q - total) <= 1e-9 * max(1.0, abs(total))
+
+
+def test_leiden_stops_when_refinement_cannot_contract():
+    """CPM with lambda 0.25 on a weighted R-MAT graph reaches a level where
+    Louvain still merges but no neighbour passes the weighted refinement test
+    (w_ij - gamma w_i w_v >= 0, src/leidenR.cpp:418-420), so leidenR contracts
+    nothing; the reference would retry that level forever
+    (src/clustering_methods.cpp:105-110).  The run stops after 8 such tries
+    with the clustering reached there."""
+    g = graphs.rmat(12, 16, seed=3, weights=(1, 100))
+    vw = g.vertex_weights("CPM")
+    q, part, info = O.cluster(g.rp, g.col, None, vw, "CPM", 0.5, "leiden+", 0, seed=5)
+    assert info["levels"] < 40
+    assert part.max() + 1 == info["labels"]
+    # the result is a valid clustering whose objective the evaluator reproduces
+    vol, uncut, sq = O.evaluate(g.rp, g.col, None, vw, part, info["labels"])
+    assert uncut == info["uncut"]
+    assert abs((uncut - 0.25 * sq) + 0.25 * g.n - q) < 1e-6 * abs(q)
