@@ -195,6 +195,19 @@ int pggc_leiden(pggc_ctx* ctx, const pggc_graph* g, pggc_state* st, int plus, in
 int pggc_cluster(pggc_ctx* ctx, const pggc_graph* g, int algo, int objective, double lambda_mult, int ex_iters,
                  uint64_t seed, int32_t* part_out, double* objective_out, int64_t* info);
 
+/* Memetic loop building blocks (app/memetic_clustering.cpp).
+ * gen_signature (:71-106): bit j of sig_out[(nnz+31)/32] (device) is set when
+ * CSR edge j joins two clusters of part. */
+int pggc_cut_signature(pggc_ctx* ctx, const pggc_graph* g, const int32_t* part, uint32_t* sig_out);
+/* get_cut_diff (:108-116): number of differing bits of two signatures of
+ * `words` 32-bit words (device); *diff_out on the host */
+int pggc_signature_diff(pggc_ctx* ctx, const uint32_t* a, const uint32_t* b, int64_t words, int64_t* diff_out);
+/* intersection_cluster (:27-62): out[i] (device) are equal iff both c1 and c2
+ * (c2 with l2 labels) agree on i; labels numbered in increasing (c1, c2)
+ * order, *labels_out (host) = their count */
+int pggc_intersect(pggc_ctx* ctx, int64_t n, const int32_t* c1, const int32_t* c2, int64_t l2, int32_t* out,
+                   int64_t* labels_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,6 +70,9 @@ SIGNATURES = {
     "pggc_ctx_set_timing": (c.c_int, [P, c.c_int]),
     "pggc_ctx_kernel_stats": (c.c_int, [P, c.c_int, I64P, c.POINTER(c.c_double), c.POINTER(c.c_double)]),
     "pggc_vertex_weights": (c.c_int, [P, c.POINTER(Graph), c.c_int, P]),
+    "pggc_cut_signature": (c.c_int, [P, c.POINTER(Graph), P, P]),
+    "pggc_signature_diff": (c.c_int, [P, P, P, c.c_int64, I64P]),
+    "pggc_intersect": (c.c_int, [P, c.c_int64, P, P, c.c_int64, P, I64P]),
     # pggc_io.h (host only)
     "pggc_read_graph": (c.c_int, [c.c_char_p, c.c_int, c.POINTER(c.c_void_p), c.POINTER(c.c_int)]),
     "pggc_host_graph_info": (c.c_int, [P, I64P, I64P]),

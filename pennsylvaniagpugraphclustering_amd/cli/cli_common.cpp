@@ -145,6 +145,30 @@ ClusterArgs parse_cluster_args(int argc, char** argv) {
     return a;
 }
 
+// parse_meme_args (app/io/parse_args.cpp:174-197)
+MemeArgs parse_meme_args(int argc, char** argv) {
+    MemeArgs a;
+    auto given = parse_base(argc, argv, {"-pop_size", "-time_limit"}, a);
+    if (!a.valid) return a;
+    if (given.count("-pop_size")) a.pop_size = std::stoi(given["-pop_size"]);
+    if (a.pop_size < 10) {
+        std::cout << "WARNING: Population size given as " << a.pop_size
+                  << " < 10. Setting population size to 10." << std::endl;
+        a.pop_size = 10;
+    }
+    if (given.count("-time_limit")) a.time_limit = std::stoi(given["-time_limit"]);
+    if (a.time_limit < 10) {
+        std::cout << "WARNING: Time limit given as " << a.time_limit << "s < 10s. Setting time limit to 10s."
+                  << std::endl;
+        a.time_limit = 10;
+    }
+    print_base(a);
+    std::cout << "-- Population Size: " << a.pop_size << std::endl;
+    std::cout << "-- Time Limit Seconds: " << a.time_limit << std::endl;
+    verify_config(a);
+    return a;
+}
+
 VerifyArgs parse_verify_args(int argc, char** argv) {
     VerifyArgs a;
     auto given = parse_base(argc, argv, {"-clusters"}, a);

@@ -29,9 +29,14 @@ struct ClusterArgs : BaseArgs {
 struct VerifyArgs : BaseArgs {
     std::string cluster_file;
 };
+struct MemeArgs : BaseArgs {
+    int pop_size = 10;
+    int time_limit = 10;
+};
 
 ClusterArgs parse_cluster_args(int argc, char** argv);
 VerifyArgs parse_verify_args(int argc, char** argv);
+MemeArgs parse_meme_args(int argc, char** argv);
 
 int objective_code(Objective o);      // PGGC_OBJ_*
 bool uses_unit_edges(Objective o);    // app/objective_helpers.hpp:24-29

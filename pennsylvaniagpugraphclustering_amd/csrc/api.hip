@@ -10,6 +10,7 @@
 #include "device.h"
 #include "leiden.h"
 #include "localmove.h"
+#include "meme.h"
 #include "pggc.h"
 
 using namespace pggc;
@@ -715,6 +716,34 @@ int pggc_cluster(pggc_ctx* ctx, const pggc_graph* gg, int algo, int objective, d
             info[6] = rs.mv.cands;
             info[7] = rs.total_nnz;
         }
+    });
+}
+
+int pggc_cut_signature(pggc_ctx* ctx, const pggc_graph* gg, const int32_t* part, uint32_t* sig_out) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        DGraph g = view_of(gg);
+        if (!part || (!sig_out && g.nnz > 0)) throw Error(PGGC_EINVAL, "null array");
+        cut_signature(ctx->c, g, part, sig_out);
+        ctx->c.sync();
+    });
+}
+
+int pggc_signature_diff(pggc_ctx* ctx, const uint32_t* a, const uint32_t* b, int64_t words, int64_t* diff_out) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (words < 0 || !diff_out || (words > 0 && (!a || !b))) throw Error(PGGC_EINVAL, "bad signature arguments");
+        *diff_out = signature_diff(ctx->c, a, b, words);
+    });
+}
+
+int pggc_intersect(pggc_ctx* ctx, int64_t n, const int32_t* c1, const int32_t* c2, int64_t l2, int32_t* out,
+                   int64_t* labels_out) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (n < 0 || l2 < 1 || (n > 0 && (!c1 || !c2 || !out))) throw Error(PGGC_EINVAL, "bad intersection arguments");
+        const i64 L = intersect_clusterings(ctx->c, n, c1, c2, l2, out);
+        if (labels_out) *labels_out = L;
     });
 }
 

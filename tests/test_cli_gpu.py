@@ -103,3 +103,62 @@ def test_custom_vertex_weights(tmp_path):
     q, part, _ = O.cluster(g.rp, g.col, None, vw, "NLCC", 1.0, "louvain", 0, seed=0)
     assert np.array_equal(read_part(out_p), part)
     assert f"{best_objective(out):.9g}" == f"{q:.9g}"
+
+
+def test_meme_primitives_match_numpy(gpu_ctx):
+    """gen_signature, get_cut_diff and intersection_cluster
+    (app/memetic_clustering.cpp:27-116) against numpy on the same inputs."""
+    import ctypes
+    import torch
+    from pennsylvaniagpugraphclustering_amd import _lib
+    from pennsylvaniagpugraphclustering_amd import clustering as C
+    g = graphs.random_graph(3000, 9, seed=21)
+    dg = C.DeviceGraph.from_csr(g, "Mod")
+    rng = np.random.default_rng(4)
+    a = rng.integers(0, 50, g.n).astype(np.int32)
+    b = rng.integers(0, 7, g.n).astype(np.int32)
+    lib = _lib.load()
+    words = (g.nnz + 31) // 32
+    ta, tb = torch.tensor(a, device="cuda"), torch.tensor(b, device="cuda")
+    sa = torch.zeros(words, dtype=torch.int32, device="cuda")
+    sb = torch.zeros(words, dtype=torch.int32, device="cuda")
+    _lib.check(lib.pggc_cut_signature(gpu_ctx.h, ctypes.byref(dg.c()), ta.data_ptr(), sa.data_ptr()))
+    _lib.check(lib.pggc_cut_signature(gpu_ctx.h, ctypes.byref(dg.c()), tb.data_ptr(), sb.data_ptr()))
+    rows = np.repeat(np.arange(g.n), np.diff(g.rp))
+    cut_a = a[rows] != a[g.col]
+    cut_b = b[rows] != b[g.col]
+    bits = np.zeros(words * 32, bool)
+    bits[: g.nnz] = cut_a
+    ref = np.packbits(bits.reshape(-1, 32)[:, ::-1], axis=1).view(">u4").ravel().astype(np.uint32)
+    assert np.array_equal(sa.cpu().numpy().view(np.uint32), ref)
+    diff = ctypes.c_int64()
+    _lib.check(lib.pggc_signature_diff(gpu_ctx.h, sa.data_ptr(), sb.data_ptr(), words, ctypes.byref(diff)))
+    assert diff.value == int((cut_a != cut_b).sum())
+    out = torch.empty(g.n, dtype=torch.int32, device="cuda")
+    L = ctypes.c_int64()
+    _lib.check(lib.pggc_intersect(gpu_ctx.h, g.n, ta.data_ptr(), tb.data_ptr(), 7, out.data_ptr(), ctypes.byref(L)))
+    pairs = a.astype(np.int64) * 7 + b
+    uniq, inv = np.unique(pairs, return_inverse=True)
+    assert L.value == len(uniq)
+    assert np.array_equal(out.cpu().numpy(), inv.astype(np.int32))
+
+
+def test_meme_tool(tmp_path):
+    """Ten pLeiden+ members evolved for the 10 s minimum: the final objective
+    is the best member, at least the best initial member, and equals the
+    oracle's evaluation of the written clustering."""
+    g = graphs.sbm(2000, 25, 10, 4, seed=6, permute_seed=6)
+    gp, out_p = tmp_path / "g.graph", tmp_path / "best.txt"
+    graphs.write_metis(g, gp)
+    out = run("meme", "-i", str(gp), "-pop_size", "4", "-time_limit", "1", "-o", str(out_p))
+    assert "Setting population size to 10" in out and "Setting time limit to 10s" in out
+    init = [float(x) for x in re.findall(r"Adding clustering with obj: (\S+)", out)]
+    assert len(init) == 10
+    assert re.search(r"Parent 1 obj: \S+; Parent 2 obj: \S+; Offspring obj: \S+; Post leiden obj: \S+; Min diff: \d+",
+                     out)
+    final = float(re.search(r"Final objective: (\S+)", out).group(1))
+    assert final >= max(init) - 1e-9
+    part = read_part(out_p)
+    vol, uncut, sq = O.evaluate(g.rp, g.col, None, g.vertex_weights("Mod"), part, int(part.max()) + 1)
+    q = (uncut - sq / g.nnz) / g.nnz
+    assert abs(q - final) < 1e-8

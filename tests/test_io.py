@@ -115,7 +115,7 @@ def run_tool(name, *args):
 
 
 def test_tools_are_built():
-    for t in ("plouvain", "pleiden", "pleiden+", "verify_obj"):
+    for t in ("plouvain", "pleiden", "pleiden+", "verify_obj", "meme"):
         assert os.access(os.path.join(BIN, t), os.X_OK), t
 
 
