@@ -120,6 +120,62 @@ int pggc_ctx_kernel_stats(pggc_ctx* ctx, int slot, int64_t* launches, double* ms
 typedef void (*pggc_progress_fn)(void* user, int64_t iteration, double seconds, int64_t cut, double objective,
                                  int32_t labels);
 int pggc_ctx_set_progress(pggc_ctx* ctx, pggc_progress_fn fn, void* user);
+/* Per-sweep trace of the local mover.  No counterpart in the reference,
+ * whose logger keeps per-level totals only (interface/ExperimentLoggerUtil.hpp:76-98):
+ * called on the host after every sweep with rec[0..7] = level vertex
+ * count, level nnz, phase (0 local move from singletons, 1 local move from
+ * a clustering, 2 strict local move), sweep index within the call, active
+ * vertices (scored this sweep), candidates (positive gain), accepted moves
+ * and the sweep's microseconds.  It synchronises the stream every sweep;
+ * NULL disables it. */
+typedef void (*pggc_sweep_fn)(void* user, const int64_t* rec, int nrec);
+int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user);
+
+/* ---------------------------------------------------------------------
+ * Active-set predictor.  No counterpart in the reference: a two-layer graph
+ * network (hidden width 32, bf16 on MFMA with fp32 accumulation) that, before
+ * a sweep of local_move (src/local_mover.cpp:1360-1417), predicts which
+ * active vertices will move and caches NO_MOVE for the others -- the same
+ * dest-cache the reference uses to skip vertices until a neighbour moves
+ * (:477-479,717,798).  A sweep that finds nothing to move among the vertices
+ * let through is repeated with the pruned ones restored, so a call ends
+ * under the plain method's condition.  Applies to pggc_cluster,
+ * pggc_louvain and pggc_leiden on levels with at least min_nnz nonzeros.
+ *
+ *   PGGC_GNN_TRAIN: sweeps run plain; after each, one Adam step (rate lr) on
+ *                   the weighted cross-entropy of "moved in this sweep" over
+ *                   the sweep's active vertices (moved class weighted by
+ *                   pos_weight).  Starts from fixed initial parameters if none
+ *                   were set.
+ *   PGGC_GNN_PRUNE: active vertices with logit < tau are pruned; with
+ *                   full_every = k > 0 every k-th sweep of a call restores
+ *                   and scores all of them.  Needs parameters.
+ * Parameters are PGGC_GNN_PARAMS floats: W1[16][32] (input feature major),
+ * b1[32], w2s[32], w2n[32], b2. */
+#define PGGC_GNN_OFF 0
+#define PGGC_GNN_TRAIN 1
+#define PGGC_GNN_PRUNE 2
+#define PGGC_GNN_PARAMS 609
+int pggc_gnn_configure(pggc_ctx* ctx, int mode, double tau, int64_t min_nnz, int full_every, double lr,
+                       double pos_weight);
+int pggc_gnn_set_params(pggc_ctx* ctx, const float* params, int count);  /* host array; resets Adam */
+int pggc_gnn_get_params(pggc_ctx* ctx, float* params, int count);        /* host array */
+/* counts[9] = predictor sweeps, active vertices seen, prune decisions,
+ * restore sweeps, training steps, and over training sweeps the confusion
+ * counts of (logit >= tau) against moved: tp, fp, fn, tn; *loss_sum = summed
+ * per-sweep mean loss of the training steps */
+int pggc_gnn_stats(pggc_ctx* ctx, int64_t* counts, double* loss_sum);
+int pggc_gnn_reset_stats(pggc_ctx* ctx);
+/* One predictor pass from a given state, for tests and offline evaluation.
+ * Device inputs: part[n], dest[n] (NO_MOVE = -3 marks inactive vertices),
+ * gain[n], stamp[n] (stamp == cur-1 marks last sweep's movers), vol[labels];
+ * first != 0 zeroes the gain feature.  Device outputs (nullable): rows_out
+ * [n*16] bf16 input rows, logit_out[n] (0 for inactive vertices).  With
+ * labels[n] (device, nonzero = moved): *loss_out (host) and grad_out[609]
+ * (host) of the mean loss, without a parameter update. */
+int pggc_gnn_eval(pggc_ctx* ctx, const pggc_graph* g, const int32_t* part, const int32_t* dest, const float* gain,
+                  const int32_t* stamp, int32_t cur, int first, const int64_t* vol, const int32_t* labels,
+                  uint16_t* rows_out, float* logit_out, double* loss_out, float* grad_out);
 /* wall seconds spent in contraction and in Leiden refinement during the last
  * pggc_cluster call (the reference logs them as the coarsen-contract and
  * leiden-refine measurements, src/clustering_methods.cpp:89,117,220) */

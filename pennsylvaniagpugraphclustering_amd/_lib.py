@@ -66,6 +66,13 @@ SIGNATURES = {
     "pggc_ctx_set_verbose": (c.c_int, [P, c.c_int]),
     "pggc_ctx_memory": (c.c_int, [P, I64P, I64P]),
     "pggc_ctx_set_progress": (c.c_int, [P, P, P]),
+    "pggc_ctx_set_sweep_trace": (c.c_int, [P, P, P]),
+    "pggc_gnn_configure": (c.c_int, [P, c.c_int, c.c_double, c.c_int64, c.c_int, c.c_double, c.c_double]),
+    "pggc_gnn_set_params": (c.c_int, [P, P, c.c_int]),
+    "pggc_gnn_get_params": (c.c_int, [P, P, c.c_int]),
+    "pggc_gnn_stats": (c.c_int, [P, P, P]),
+    "pggc_gnn_reset_stats": (c.c_int, [P]),
+    "pggc_gnn_eval": (c.c_int, [P, P, P, P, P, P, c.c_int32, c.c_int, P, P, P, P, P, P]),
     "pggc_ctx_last_times": (c.c_int, [P, c.POINTER(c.c_double), c.POINTER(c.c_double)]),
     "pggc_ctx_set_timing": (c.c_int, [P, c.c_int]),
     "pggc_ctx_kernel_stats": (c.c_int, [P, c.c_int, I64P, c.POINTER(c.c_double), c.POINTER(c.c_double)]),

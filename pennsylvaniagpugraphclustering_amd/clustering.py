@@ -84,6 +84,23 @@ class Context:
             out[name] = (n.value, ms.value, by.value)
         return out
 
+    SWEEP_FIELDS = ("n", "nnz", "phase", "sweep", "active", "cands", "moves", "us")
+    _SweepFn = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.c_int)
+
+    def set_sweep_trace(self, records):
+        """Append one dict per local-move sweep (SWEEP_FIELDS) to the list
+        `records`; None turns the trace off.  Synchronises every sweep."""
+        if records is None:
+            self._sweep_cb = None
+            check(self.lib.pggc_ctx_set_sweep_trace(self.h, None, None))
+            return
+
+        def cb(_user, rec, nrec):
+            records.append({k: int(rec[i]) for i, k in enumerate(self.SWEEP_FIELDS[:nrec])})
+
+        self._sweep_cb = self._SweepFn(cb)
+        check(self.lib.pggc_ctx_set_sweep_trace(self.h, ctypes.cast(self._sweep_cb, ctypes.c_void_p), None))
+
 
 _default = None
 
@@ -287,3 +304,72 @@ def cluster(g, algo="louvain", objective="Mod", mult=1.0, ex_iters=0, seed=0, ct
     check(ctx.lib.pggc_cluster(ctx.h, ctypes.byref(g.c()), ALGOS[algo], OBJECTIVES[objective], mult, ex_iters, seed,
                                _p(out), ctypes.byref(q), info))
     return q.value, out[: g.n], dict(zip(INFO_KEYS, (int(x) for x in info)))
+
+
+# ------------------------------------------------------ active-set predictor
+GNN_PARAMS = 609
+GNN_MODES = {"off": 0, "train": 1, "prune": 2}
+GNN_STAT_KEYS = ("sweeps", "active", "pruned", "restores", "train_steps", "tp", "fp", "fn", "tn")
+
+
+def gnn_configure(mode, tau=0.0, min_nnz=1 << 20, full_every=0, lr=1e-3, pos_weight=4.0, ctx=None):
+    """Predictor mode for the local mover of every later clustering call on
+    the context: "off", "train" (plain sweeps, one Adam step after each) or
+    "prune" (active vertices with logit < tau are skipped)."""
+    ctx = _ctx(ctx)
+    check(ctx.lib.pggc_gnn_configure(ctx.h, GNN_MODES[mode], float(tau), int(min_nnz), int(full_every), float(lr),
+                                     float(pos_weight)))
+
+
+def gnn_set_params(params, ctx=None):
+    ctx = _ctx(ctx)
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    check(ctx.lib.pggc_gnn_set_params(ctx.h, p.ctypes.data, int(p.size)))
+
+
+def gnn_get_params(ctx=None):
+    ctx = _ctx(ctx)
+    p = np.empty(GNN_PARAMS, np.float32)
+    check(ctx.lib.pggc_gnn_get_params(ctx.h, p.ctypes.data, GNN_PARAMS))
+    return p
+
+
+def gnn_stats(ctx=None):
+    """Counters since the last reset (GNN_STAT_KEYS) plus the summed loss."""
+    ctx = _ctx(ctx)
+    c = (ctypes.c_int64 * 9)()
+    loss = ctypes.c_double()
+    check(ctx.lib.pggc_gnn_stats(ctx.h, c, ctypes.byref(loss)))
+    out = dict(zip(GNN_STAT_KEYS, (int(x) for x in c)))
+    out["loss_sum"] = loss.value
+    return out
+
+
+def gnn_reset_stats(ctx=None):
+    ctx = _ctx(ctx)
+    check(ctx.lib.pggc_gnn_reset_stats(ctx.h))
+
+
+def gnn_eval(g, part, dest, gain, stamp, cur, first, vol, labels=None, ctx=None):
+    """One predictor pass from a given state.  Returns (input rows (n, 16)
+    as uint16 bf16 bits, logits (0 for inactive vertices), loss, gradient);
+    loss and gradient are None without labels."""
+    ctx = _ctx(ctx)
+    n = g.n
+    part_t, dest_t = _t(part, torch.int32), _t(dest, torch.int32)
+    gain_t, stamp_t = _t(gain, torch.float32), _t(stamp, torch.int32)
+    vol_t = _t(vol, torch.int64)
+    lab_t = _t(labels, torch.int32) if labels is not None else None
+    rows = torch.empty(max(n, 1) * 16, dtype=torch.int16, device=_dev())
+    logit = torch.empty(max(n, 1), dtype=torch.float32, device=_dev())
+    loss = ctypes.c_double()
+    grad = np.zeros(GNN_PARAMS, np.float32)
+    check(ctx.lib.pggc_gnn_eval(ctx.h, ctypes.byref(g.c()), _p(part_t), _p(dest_t), _p(gain_t), _p(stamp_t), int(cur),
+                                1 if first else 0, _p(vol_t), _p(lab_t) if lab_t is not None else None, _p(rows),
+                                _p(logit), ctypes.byref(loss) if lab_t is not None else None,
+                                grad.ctypes.data if lab_t is not None else None))
+    rows_np = rows[: n * 16].cpu().numpy().view(np.uint16).reshape(n, 16)
+    lg = logit[:n].cpu().numpy()
+    if lab_t is None:
+        return rows_np, lg, None, None
+    return rows_np, lg, loss.value, grad

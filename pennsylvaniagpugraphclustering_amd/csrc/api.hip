@@ -1,6 +1,7 @@
 // Multilevel drivers (src/clustering_methods.cpp), the run_clustering outer
 // loop (app/cluster_main.cpp:101-165) and the C ABI declared in pggc.h.
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <limits>
@@ -8,6 +9,7 @@
 #include <vector>
 
 #include "device.h"
+#include "gnn.h"
 #include "leiden.h"
 #include "localmove.h"
 #include "meme.h"
@@ -18,6 +20,7 @@ using namespace pggc;
 struct pggc_ctx {
     Ctx c;
     MoveScratch ms;
+    Gnn gnn;
 };
 
 namespace pggc {
@@ -386,6 +389,7 @@ int pggc_ctx_create(pggc_ctx** out, void* stream) {
         x->c.stream = (hipStream_t)stream;
         PGGC_HIP(hipGetDevice(&x->c.device));
         PGGC_HIP(hipHostMalloc((void**)&x->c.pinned, 64 * sizeof(i64), hipHostMallocDefault));
+        x->ms.gnn = &x->gnn;
         *out = x.release();
     });
 }
@@ -394,6 +398,7 @@ int pggc_ctx_destroy(pggc_ctx* ctx) {
     return guard([&] {
         if (!ctx) return;
         (void)hipStreamSynchronize(ctx->c.stream);
+        ctx->gnn = Gnn();
         ctx->ms = MoveScratch();
         for (auto& kv : ctx->c.live) (void)hipFree(kv.first);
         ctx->c.live.clear();
@@ -422,6 +427,153 @@ int pggc_ctx_set_progress(pggc_ctx* ctx, pggc_progress_fn fn, void* user) {
         CTX_CHECK(ctx);
         ctx->c.progress = fn;
         ctx->c.progress_user = user;
+    });
+}
+
+// ------------------------------------------------------ active-set predictor
+// Deterministic initial parameters: uniform(-a, a) with a = sqrt(6 / (fan in
+// + fan out)) from splitmix64, zero hidden biases, output bias at the log-odds
+// of a 10% move rate.
+static void gnn_initial_params(float* p) {
+    u64 st = 0x5eed5eedULL;
+    auto uni = [&]() {
+        st += 0x9E3779B97F4A7C15ULL;
+        u64 z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    };
+    const double a1 = std::sqrt(6.0 / (GNN_IN + GNN_HID)), a2 = std::sqrt(6.0 / (GNN_HID + 1));
+    for (int k = 0; k < GNN_IN * GNN_HID; k++) p[GP_W1 + k] = (float)(a1 * uni());
+    for (int k = 0; k < GNN_HID; k++) p[GP_B1 + k] = 0.0f;
+    for (int k = 0; k < GNN_HID; k++) p[GP_W2S + k] = (float)(a2 * uni());
+    for (int k = 0; k < GNN_HID; k++) p[GP_W2N + k] = (float)(a2 * uni());
+    p[GP_B2] = (float)std::log(0.1 / 0.9);
+}
+
+int pggc_gnn_configure(pggc_ctx* ctx, int mode, double tau, int64_t min_nnz, int full_every, double lr,
+                       double pos_weight) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (mode < PGGC_GNN_OFF || mode > PGGC_GNN_PRUNE)
+            throw Error(PGGC_EINVAL, "predictor mode must be PGGC_GNN_OFF, PGGC_GNN_TRAIN or PGGC_GNN_PRUNE");
+        if (min_nnz < 0 || full_every < 0 || !(lr > 0.0) || !(pos_weight > 0.0) || !std::isfinite(tau))
+            throw Error(PGGC_EINVAL, "invalid predictor configuration");
+        Gnn& G = ctx->gnn;
+        G.init(ctx->c);
+        if (mode == PGGC_GNN_PRUNE && !G.have_params)
+            throw Error(PGGC_EINVAL, "pruning needs trained predictor parameters (pggc_gnn_set_params)");
+        if (mode == PGGC_GNN_TRAIN && !G.have_params) {
+            std::vector<float> p(GNN_P);
+            gnn_initial_params(p.data());
+            G.set_params(ctx->c, p.data());
+        }
+        G.cfg.mode = mode;
+        G.cfg.tau = (float)tau;
+        G.cfg.min_nnz = min_nnz;
+        G.cfg.full_every = full_every;
+        G.cfg.lr = (float)lr;
+        G.cfg.pos_weight = (float)pos_weight;
+    });
+}
+
+int pggc_gnn_set_params(pggc_ctx* ctx, const float* params, int count) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (!params || count != GNN_P) throw Error(PGGC_EINVAL, "expected PGGC_GNN_PARAMS predictor parameters");
+        for (int k = 0; k < count; k++)
+            if (!std::isfinite(params[k])) throw Error(PGGC_EINVAL, "predictor parameters must be finite");
+        ctx->gnn.set_params(ctx->c, params);
+    });
+}
+
+int pggc_gnn_get_params(pggc_ctx* ctx, float* params, int count) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (!params || count != GNN_P) throw Error(PGGC_EINVAL, "expected PGGC_GNN_PARAMS predictor parameters");
+        if (!ctx->gnn.have_params) throw Error(PGGC_EINVAL, "no predictor parameters set");
+        PGGC_HIP(hipMemcpyAsync(params, ctx->gnn.prm.p, GNN_P * sizeof(float), hipMemcpyDeviceToHost, ctx->c.stream));
+        ctx->c.sync();
+    });
+}
+
+int pggc_gnn_stats(pggc_ctx* ctx, int64_t* counts, double* loss_sum) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (!counts) throw Error(PGGC_EINVAL, "null output");
+        Gnn& G = ctx->gnn;
+        G.init(ctx->c);
+        i64 d[GC_COUNT];
+        double l = 0.0;
+        PGGC_HIP(hipMemcpyAsync(d, G.dcnt.p, sizeof(d), hipMemcpyDeviceToHost, ctx->c.stream));
+        PGGC_HIP(hipMemcpyAsync(&l, G.dloss.p, sizeof(double), hipMemcpyDeviceToHost, ctx->c.stream));
+        ctx->c.sync();
+        const int64_t out[9] = {G.sweeps, d[GC_ACTIVE], d[GC_PRUNED], G.fallbacks, G.train_steps,
+                                d[GC_TP],  d[GC_FP],     d[GC_FN],     d[GC_TN]};
+        std::memcpy(counts, out, sizeof(out));
+        if (loss_sum) *loss_sum = l;
+    });
+}
+
+int pggc_gnn_reset_stats(pggc_ctx* ctx) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        ctx->gnn.init(ctx->c);
+        ctx->gnn.reset_counters(ctx->c);
+        ctx->c.sync();
+    });
+}
+
+int pggc_gnn_eval(pggc_ctx* ctx, const pggc_graph* gg, const int32_t* part, const int32_t* dest, const float* gain,
+                  const int32_t* stamp, int32_t cur, int first, const int64_t* vol, const int32_t* labels,
+                  uint16_t* rows_out, float* logit_out, double* loss_out, float* grad_out) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        DGraph g = view_of(gg);
+        if (!g.vw) throw Error(PGGC_EINVAL, "vertex weights required");
+        if (g.n > 0 && (!part || !dest || !gain || !stamp || !vol))
+            throw Error(PGGC_EINVAL, "part, dest, gain, stamp and vol are required");
+        if ((loss_out || grad_out) && !labels) throw Error(PGGC_EINVAL, "loss and gradient need labels");
+        Ctx& c = ctx->c;
+        Gnn& G = ctx->gnn;
+        if (!G.have_params) throw Error(PGGC_EINVAL, "no predictor parameters set");
+        const i64 n = g.n;
+        LevelPlan lp;
+        plan_level(c, g, lp);
+        G.ensure(c, n, true);
+        Buf<i32> d2(c, std::max<i64>(n, 1));
+        if (n > 0) {
+            PGGC_HIP(hipMemcpyAsync(d2.p, dest, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+            PGGC_HIP(hipMemcpyAsync(G.stamp.p, stamp, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+            fill_i32(c, (i32*)G.logit.p, n, 0);
+        }
+        const i64 sw = G.sweeps;
+        gnn_predict(c, G, lp, g, part, d2, gain, vol, cur, first != 0, false, true);
+        G.sweeps = sw;
+        if (rows_out && n > 0)
+            PGGC_HIP(hipMemcpyAsync(rows_out, G.in.p, n * GNN_IN * sizeof(u16), hipMemcpyDeviceToDevice, c.stream));
+        if (logit_out && n > 0)
+            PGGC_HIP(hipMemcpyAsync(logit_out, G.logit.p, n * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+        if (labels) {
+            double l0 = 0.0, l1 = 0.0;
+            PGGC_HIP(hipMemcpyAsync(&l0, G.dloss.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            c.sync();
+            gnn_backward(c, G, lp, g, labels, cur, false);
+            PGGC_HIP(hipMemcpyAsync(&l1, G.dloss.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            if (grad_out) PGGC_HIP(hipMemcpyAsync(grad_out, G.grad.p, GNN_P * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+            c.sync();
+            if (loss_out) *loss_out = l1 - l0;
+        }
+        c.sync();
+    });
+}
+
+int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        ctx->c.sweep_trace = fn;
+        ctx->c.sweep_user = user;
     });
 }
 

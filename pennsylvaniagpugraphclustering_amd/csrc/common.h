@@ -10,6 +10,7 @@ namespace pggc {
 
 using i32 = int32_t;
 using i64 = int64_t;
+using u16 = uint16_t;
 using u32 = uint32_t;
 using u64 = uint64_t;
 

@@ -59,6 +59,10 @@ struct Ctx {
     // per-outer-iteration callback (iteration, seconds, cut, objective, labels)
     void (*progress)(void*, int64_t, double, int64_t, double, int32_t) = nullptr;
     void* progress_user = nullptr;
+    // per-sweep trace of the local mover (level n, nnz, phase, sweep, active,
+    // candidates, moves, microseconds); synchronises per sweep when set
+    void (*sweep_trace)(void*, const int64_t*, int) = nullptr;
+    void* sweep_user = nullptr;
     double last_contract_s = 0.0, last_refine_s = 0.0;  // of the last clustering call
     KTimer kt;
 

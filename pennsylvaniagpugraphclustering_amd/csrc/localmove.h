@@ -53,8 +53,11 @@ struct LevelPlan {
 };
 
 // Device scratch sized for the top level, shared by every level.
+struct Gnn;
+
 struct MoveScratch {
     i64 cap = 0;
+    Gnn* gnn = nullptr;  // active-set predictor (null or mode off: plain sweeps)
     Buf<i32> part, dest, mflag, cand, moves, flag, tmp, tmp2;
     Buf<float> gain;
     Buf<i64> prio, scal;

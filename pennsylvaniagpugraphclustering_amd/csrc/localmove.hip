@@ -11,8 +11,10 @@
 //  * uncut is updated by the exact integer delta of the moved vertices
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 
 #include "device.h"
+#include "gnn.h"
 #include "hash.h"
 #include "localmove.h"
 
@@ -1035,6 +1037,48 @@ struct CurrVol {
     Buf<i64> v;
 };
 
+// ------------------------------------------------------------ sweep trace
+__global__ void __launch_bounds__(256) k_count_active(const i32* dest, i64 n, i64* out) {
+    const i64 i = thread_item();
+    const i64 a = (i < n && dest[i] != NO_MOVE) ? 1 : 0;
+    const i64 s = block_sum_i64<256>(a);
+    if (threadIdx.x == 0 && s) atomicAdd((unsigned long long*)out, (unsigned long long)s);
+}
+
+// Host side of the per-sweep trace: begin() counts the vertices the sweep
+// will score and starts the clock, end() reports the record.  Both are
+// no-ops unless a trace callback is set.
+struct SweepTrace {
+    Ctx& c;
+    const DGraph& g;
+    i64 phase, sweep = 0, active = 0;
+    std::chrono::steady_clock::time_point t0;
+    SweepTrace(Ctx& cc, const DGraph& gg, i64 ph) : c(cc), g(gg), phase(ph) {}
+    void begin(const i32* dest, bool all) {
+        if (!c.sweep_trace) return;
+        if (all) {
+            active = g.n;
+        } else {
+            Buf<i64> cnt(c, 1);
+            fill_i64(c, cnt, 1, 0);
+            if (g.n > 0) {
+                k_count_active<<<grid_items(g.n, 256), 256, 0, c.stream>>>(dest, g.n, cnt);
+                PGGC_LAUNCH_CHECK();
+            }
+            active = c.read1(cnt);
+        }
+        c.sync();
+        t0 = std::chrono::steady_clock::now();
+    }
+    void end(i64 cands, i64 moves) {
+        if (!c.sweep_trace) return;
+        c.sync();
+        const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+        const int64_t rec[8] = {g.n, g.nnz, phase, sweep++, active, cands, moves, (int64_t)us};
+        c.sweep_trace(c.sweep_user, rec, 8);
+    }
+};
+
 static RefineData begin_curr(Ctx& c, const RefineData& best, Buf<i64>& cv, i64 n) {
     i64 cap = std::max<i64>(best.cap, n);
     cv = Buf<i64>(c, cap + 1);
@@ -1059,21 +1103,50 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
     const int limits[2] = {4, 2};
     int iter = 0;
     const i32* ccons = constrained ? cons : nullptr;
+    SweepTrace tr(c, g, is_initial ? 0 : 1);
+    // active-set predictor: on levels of at least min_nnz nonzeros, from the
+    // second sweep of a call that starts at singletons (whose first sweep
+    // moves most vertices), from the first sweep otherwise
+    Gnn* G = (s.gnn && s.gnn->cfg.mode != GNN_OFF && g.nnz >= s.gnn->cfg.min_nnz) ? s.gnn : nullptr;
+    const bool train = G && G->cfg.mode == GNN_TRAIN;
+    if (G) gnn_begin(c, *G, n, train);
     for (int x = 0; x < 2; x++) {
         const float fr = SA ? ratios[x] : 0.0f;
         for (int cnt = 0; cnt < limits[x]; cnt++) {
             iter++;
             const float lam = (float)curr.lambda;
+            const bool use = G && !(is_initial && iter == 1);
+            const bool full = use && G->cfg.full_every > 0 && iter % G->cfg.full_every == 0;
+            if (full) gnn_restore(c, *G, n, s.dest);
+            if (use && !full)
+                gnn_predict(c, *G, lp, g, s.part, s.dest, s.gain, curr.vol, iter, iter == 1,
+                            G->cfg.mode == GNN_PRUNE, train);
+            tr.begin(s.dest, iter == 1 && !use);
             score_all(c, s, lp, g, s.part, curr.vol, ccons, !tables, lam, fr, s.dest, s.gain);
             i64 b[2];
             i64 k = gather_candidates(c, s, lp, g, s.dest, s.cand, b);
+            i64 km = k > 0 ? afterburner(c, s, g, s.cand, k, b, lam, s.moves) : 0;
+            if (km == 0 && G && gnn_restore(c, *G, n, s.dest)) {
+                // nothing to move among the vertices let through: score the
+                // pruned ones too, so the call ends as the plain method does
+                score_all(c, s, lp, g, s.part, curr.vol, ccons, !tables, lam, fr, s.dest, s.gain);
+                k = gather_candidates(c, s, lp, g, s.dest, s.cand, b);
+                km = k > 0 ? afterburner(c, s, g, s.cand, k, b, lam, s.moves) : 0;
+            }
             if (stats) {
                 stats->sweeps++;
                 stats->cands += k;
             }
-            if (k == 0) break;
-            i64 km = afterburner(c, s, g, s.cand, k, b, lam, s.moves);
-            if (km == 0) break;
+            if (G) gnn_stamp(c, *G, s.moves, km, iter);
+            if (train && use && !full) gnn_backward(c, *G, lp, g, nullptr, iter, true);
+            if (k == 0) {
+                tr.end(0, 0);
+                break;
+            }
+            if (km == 0) {
+                tr.end(k, 0);
+                break;
+            }
             if (iter > 1 || !is_initial) set_new_cluster_ids(c, s, g, curr, s.moves, km, constrained ? cons : nullptr);
             perform_moves(c, s, g, curr, s.moves, km, g.n - lp.ord.off_large, tables);
             if (stats) stats->moves += km;
@@ -1081,6 +1154,7 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
                 copy_state(c, best, curr);
                 PGGC_HIP(hipMemcpyAsync(best_part, s.part, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
             }
+            tr.end(k, km);
         }
     }
     relabel(c, s, n, best_part, best);
@@ -1161,8 +1235,10 @@ void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i
     bool tables = !is_initial;
     const i32* ccons = constrained ? cons : nullptr;
     Buf<float> sg(c, std::max<i64>(n, 1));
+    SweepTrace trc(c, g, 2);
     for (int it = 0; it < 6; it++) {
         const float lam = (float)curr.lambda;
+        trc.begin(s.dest, true);
         score_all(c, s, lp, g, s.part, curr.vol, ccons, !tables, lam, 0.0f, s.dest, s.gain);
         i64 b[2];
         i64 k = gather_candidates(c, s, lp, g, s.dest, s.moves, b);
@@ -1170,7 +1246,10 @@ void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i
             stats->sweeps++;
             stats->cands += k;
         }
-        if (k == 0) break;
+        if (k == 0) {
+            trc.end(0, 0);
+            break;
+        }
         if (it > 0 || !is_initial) set_new_cluster_ids(c, s, g, curr, s.moves, k, constrained ? cons : nullptr);
         fill_i64(c, s.prio, n, n + 1);
         k_prio<<<grid_items(k, 256), 256, 0, c.stream>>>(s.moves, k, s.prio);
@@ -1181,7 +1260,10 @@ void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i
         k_strict_scan<<<1, 64, 0, c.stream>>>(s.moves, k, sg, s.gain, s.scal);
         PGGC_LAUNCH_CHECK();
         i64 tr = c.read1(s.scal);
-        if (tr == 0) break;
+        if (tr == 0) {
+            trc.end(k, 0);
+            break;
+        }
         k_set_mflag<<<grid_items(tr, 256), 256, 0, c.stream>>>(s.moves, tr, s.mflag);
         PGGC_LAUNCH_CHECK();
         perform_moves(c, s, g, curr, s.moves, tr, g.n - lp.ord.off_large, tables);
@@ -1191,6 +1273,7 @@ void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i
             PGGC_HIP(hipMemcpyAsync(best_part, s.part, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
         }
         fill_i32(c, s.dest, n, NULL_PART);
+        trc.end(k, tr);
     }
     relabel(c, s, n, best_part, best);
 }
