@@ -131,6 +131,16 @@ int pggc_ctx_set_progress(pggc_ctx* ctx, pggc_progress_fn fn, void* user);
 typedef void (*pggc_sweep_fn)(void* user, const int64_t* rec, int nrec);
 int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user);
 
+/* Tuning knobs of the HIP path (no counterpart in the reference, whose
+ * equivalents are compile-time constants).  Results do not depend on them.
+ *   PGGC_KNOB_HUB_UPDATE_DIV: the hash tables of the highest-degree vertices
+ *     are kept current from the movers while a sweep moves at most
+ *     n / value vertices and rebuilt otherwise; 0 rebuilds every sweep.
+ *     Default 32. */
+#define PGGC_KNOB_HUB_UPDATE_DIV 0
+int pggc_ctx_set_knob(pggc_ctx* ctx, int knob, int64_t value);
+int pggc_ctx_get_knob(pggc_ctx* ctx, int knob, int64_t* value);
+
 /* ---------------------------------------------------------------------
  * Active-set predictor.  No counterpart in the reference: a two-layer graph
  * network (hidden width 32, bf16 on MFMA with fp32 accumulation) that, before
@@ -138,9 +148,13 @@ int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user);
  * active vertices will move and caches NO_MOVE for the others -- the same
  * dest-cache the reference uses to skip vertices until a neighbour moves
  * (:477-479,717,798).  A sweep that finds nothing to move among the vertices
- * let through is repeated with the pruned ones restored, so a call ends
- * under the plain method's condition.  Applies to pggc_cluster,
- * pggc_louvain and pggc_leiden on levels with at least min_nnz nonzeros.
+ * let through is repeated with the pruned ones restored, so a call that
+ * stops for lack of moves stops under the plain method's condition; a call
+ * that stops at its sweep limit leaves pruned vertices for the next call,
+ * which starts with every vertex active.  The predictor runs on a sweep only
+ * while the previous sweep left at least n/16 candidates.  Applies to
+ * pggc_cluster, pggc_louvain and pggc_leiden on levels with at least
+ * min_nnz nonzeros.
  *
  *   PGGC_GNN_TRAIN: sweeps run plain; after each, one Adam step (rate lr) on
  *                   the weighted cross-entropy of "moved in this sweep" over

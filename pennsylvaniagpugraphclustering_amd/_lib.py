@@ -67,6 +67,8 @@ SIGNATURES = {
     "pggc_ctx_memory": (c.c_int, [P, I64P, I64P]),
     "pggc_ctx_set_progress": (c.c_int, [P, P, P]),
     "pggc_ctx_set_sweep_trace": (c.c_int, [P, P, P]),
+    "pggc_ctx_set_knob": (c.c_int, [P, c.c_int, c.c_int64]),
+    "pggc_ctx_get_knob": (c.c_int, [P, c.c_int, P]),
     "pggc_gnn_configure": (c.c_int, [P, c.c_int, c.c_double, c.c_int64, c.c_int, c.c_double, c.c_double]),
     "pggc_gnn_set_params": (c.c_int, [P, P, c.c_int]),
     "pggc_gnn_get_params": (c.c_int, [P, P, c.c_int]),

@@ -84,6 +84,17 @@ class Context:
             out[name] = (n.value, ms.value, by.value)
         return out
 
+    KNOBS = {"hub_update_div": 0}
+
+    def set_knob(self, name, value):
+        """Tuning knob of the HIP path (results do not depend on it)."""
+        check(self.lib.pggc_ctx_set_knob(self.h, self.KNOBS[name], int(value)))
+
+    def get_knob(self, name):
+        v = ctypes.c_int64()
+        check(self.lib.pggc_ctx_get_knob(self.h, self.KNOBS[name], ctypes.byref(v)))
+        return v.value
+
     SWEEP_FIELDS = ("n", "nnz", "phase", "sweep", "active", "cands", "moves", "us")
     _SweepFn = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.c_int)
 

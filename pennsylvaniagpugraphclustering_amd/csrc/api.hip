@@ -569,6 +569,24 @@ int pggc_gnn_eval(pggc_ctx* ctx, const pggc_graph* gg, const int32_t* part, cons
     });
 }
 
+int pggc_ctx_set_knob(pggc_ctx* ctx, int knob, int64_t value) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (knob != PGGC_KNOB_HUB_UPDATE_DIV) throw Error(PGGC_EINVAL, "unknown knob");
+        if (value < 0) throw Error(PGGC_EINVAL, "knob value must be >= 0");
+        ctx->c.hub_update_div = value;
+    });
+}
+
+int pggc_ctx_get_knob(pggc_ctx* ctx, int knob, int64_t* value) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (!value) throw Error(PGGC_EINVAL, "null output");
+        if (knob != PGGC_KNOB_HUB_UPDATE_DIV) throw Error(PGGC_EINVAL, "unknown knob");
+        *value = ctx->c.hub_update_div;
+    });
+}
+
 int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user) {
     return guard([&] {
         CTX_CHECK(ctx);

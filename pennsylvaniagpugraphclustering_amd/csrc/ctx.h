@@ -64,6 +64,10 @@ struct Ctx {
     void (*sweep_trace)(void*, const int64_t*, int) = nullptr;
     void* sweep_user = nullptr;
     double last_contract_s = 0.0, last_refine_s = 0.0;  // of the last clustering call
+    // tuning knobs (pggc_ctx_set_knob): hub tables are updated from the
+    // movers while a sweep moves at most n / hub_update_div vertices and
+    // rebuilt otherwise (0: always rebuilt)
+    i64 hub_update_div = 32;
     KTimer kt;
 
     void* alloc(size_t bytes);

@@ -6,7 +6,9 @@
 // for active vertices the network predicts to stay.  Every vertex it lets
 // through is still scored by the exact gain, and a sweep that finds nothing
 // to move is repeated with every pruned vertex restored, so a local-move
-// call ends under the plain method's condition.
+// call that stops for lack of moves stops under the plain method's
+// condition (a call that stops at its sweep limit leaves pruned vertices to
+// the next call, which starts with every vertex active).
 //
 // Model (hidden width 32):
 //   per-vertex pack  pk_i = part_i | ldeg_q | margin_q | moved | active
@@ -35,12 +37,13 @@ enum GnnMode { GNN_OFF = 0, GNN_TRAIN = 1, GNN_PRUNE = 2 };
 
 // device counter slots
 enum GnnCount {
-    GC_ACTIVE = 0,   // active vertices seen by the predictor (all sweeps)
-    GC_PRUNED = 1,   // prune decisions (all sweeps)
-    GC_PENDING = 2,  // 1 while pruned vertices wait for a restore
+    GC_ACTIVE = 0,    // active vertices seen by the predictor (all sweeps)
+    GC_PRUNED = 1,    // prune decisions (all sweeps)
+    GC_RESTORED = 2,  // vertices put back by the last restore
     GC_TP = 3, GC_FP = 4, GC_FN = 5, GC_TN = 6,  // training sweeps: prediction (logit >= tau) vs moved
     GC_SWEEP_ACTIVE = 7,  // active vertices of the current sweep
-    GC_COUNT = 8
+    GC_TMP = 8,
+    GC_COUNT = 9
 };
 
 struct GnnConfig {
@@ -68,7 +71,8 @@ struct Gnn {
     Buf<u64> pk;
     Buf<u16> in, h;
     Buf<float> s, t, logit, r, ds;
-    Buf<i32> stamp, pruned;
+    Buf<i32> stamp, pruned;  // sweep of the last move / of the prune decision (0: none)
+    Buf<i64> blk;            // per-block partial counts
 
     void init(Ctx& c);
     void ensure(Ctx& c, i64 n, bool train);
@@ -89,6 +93,7 @@ void gnn_begin(Ctx& c, Gnn& G, i64 n, bool train);
 // marks the accepted moves of sweep `cur`
 void gnn_stamp(Ctx& c, Gnn& G, const i32* moves, i64 k, i32 cur);
 // restores pruned vertices to the active set; returns false if none waited
+// (reads one count back from the device)
 bool gnn_restore(Ctx& c, Gnn& G, i64 n, i32* dest);
 // loss and gradient of the last gnn_predict (run with keep_hidden) against
 // labels (nullable: stamps equal to cur), then an Adam step when `update`

@@ -65,11 +65,12 @@ __device__ __forceinline__ T block_sum_lds(T v) {
 }
 
 // ------------------------------------------------------------------ pack
-// pk = part | ldeg_q << 32 | margin_q << 40 | moved << 48 | active << 49
+// pk = part | ldeg_q << 32 | margin_q << 40 | moved << 48 | active << 49;
+// blk[block] = active vertices of the block
 __global__ void __launch_bounds__(256) k_gnn_pack(i64 n, const i64* __restrict__ rp, const i32* __restrict__ part,
                                                   const i32* __restrict__ dest, const float* __restrict__ gain,
                                                   const i32* __restrict__ stamp, i32 cur, int first,
-                                                  u64* __restrict__ pk, i64* cnt) {
+                                                  u64* __restrict__ pk, i64* __restrict__ blk) {
     const i64 i = thread_item();
     i64 a = 0;
     if (i < n) {
@@ -83,16 +84,64 @@ __global__ void __launch_bounds__(256) k_gnn_pack(i64 n, const i64* __restrict__
         a = (i64)act;
     }
     const i64 s = block_sum_i64<256>(a);
-    if (threadIdx.x == 0 && s) {
-        atomicAdd((unsigned long long*)(cnt + GC_ACTIVE), (unsigned long long)s);
-        atomicAdd((unsigned long long*)(cnt + GC_SWEEP_ACTIVE), (unsigned long long)s);
-    }
+    if (threadIdx.x == 0) blk[block_id()] = s;
+}
+
+// blk[block] = vertices with flag[i] == v (v != 0) or, for v == 0, flag[i] != 0
+__global__ void __launch_bounds__(256) k_gnn_count(i64 n, const i32* __restrict__ flag, i32 v, i64* __restrict__ blk) {
+    const i64 i = thread_item();
+    const i64 a = (i < n && (v ? flag[i] == v : flag[i] != 0)) ? 1 : 0;
+    const i64 s = block_sum_i64<256>(a);
+    if (threadIdx.x == 0) blk[block_id()] = s;
+}
+
+// cnt[to] += cnt[from]
+__global__ void k_gnn_acc(i64* cnt, int to, int from) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) cnt[to] += cnt[from];
 }
 
 // ------------------------------------------------------------- features
-// neighbour sums: 0 ldeg_q, 1 margin_q, 2 moved, 3 active, 4 same cluster,
-// 5 margin above own, 6 same & active, 7 same & moved, 8 weight, 9 own weight
+// Neighbour sums: 0 ldeg_q, 1 margin_q, 2 moved, 3 active, 4 same cluster,
+// 5 margin above own, 6 same & active, 7 same & moved, 8 weight, 9 own
+// weight.  Rows of degree <= 2048 carry them packed in three words
+// (p0 = S0 | (S1 + 128 per edge) << 32, p1 = S2 | S3 << 21 | S4 << 42,
+// p2 = S5 | S6 << 21 | S7 << 42; no field can carry into the next), and the
+// weights only when the graph has them (unit weights: S8 = degree, S9 = S4).
 constexpr int NAGG = 10;
+
+struct AggPacked {
+    u64 p0 = 0, p1 = 0, p2 = 0;
+    i64 wa = 0, wo = 0;
+};
+
+template <class WT>
+__device__ __forceinline__ void agg_edge_packed(u64 q, i64 w, i32 p, int mq_i, AggPacked& a) {
+    const u64 same = (i32)(u32)q == p ? 1 : 0;
+    const int mq = pk_margin(q);
+    const u64 mv = (q >> 48) & 1u, ac = (q >> 49) & 1u;
+    a.p0 += ((q >> 32) & 255u) | ((u64)(mq + 128) << 32);
+    a.p1 += mv | (ac << 21) | (same << 42);
+    a.p2 += (u64)(mq > mq_i ? 1 : 0) | ((same & ac) << 21) | ((same & mv) << 42);
+    if (WT::bytes > 0) {
+        a.wa += w;
+        a.wo += same ? w : 0;
+    }
+}
+
+template <class WT>
+__device__ __forceinline__ void agg_unpack(const AggPacked& a, i64 d, i64* S) {
+    constexpr u64 M21 = (1ull << 21) - 1;
+    S[0] = (i64)(a.p0 & 0xFFFFFFFFull);
+    S[1] = (i64)(a.p0 >> 32) - 128 * d;
+    S[2] = (i64)(a.p1 & M21);
+    S[3] = (i64)((a.p1 >> 21) & M21);
+    S[4] = (i64)(a.p1 >> 42);
+    S[5] = (i64)(a.p2 & M21);
+    S[6] = (i64)((a.p2 >> 21) & M21);
+    S[7] = (i64)(a.p2 >> 42);
+    S[8] = WT::bytes > 0 ? a.wa : d;
+    S[9] = WT::bytes > 0 ? a.wo : S[4];
+}
 
 __device__ __forceinline__ void agg_edge(u64 q, i64 w, i32 p, int mq_i, i64* a) {
     const i64 same = (i32)(u32)q == p ? 1 : 0;
@@ -148,16 +197,56 @@ __global__ void __launch_bounds__(256) k_gnn_agg_group(const i64* __restrict__ r
     const i64 d = rp[i + 1] - s0;
     const u64 me = pk[i];
     const i32 p = (i32)(u32)me;
-    i64 a[NAGG];
-#pragma unroll
-    for (int k = 0; k < NAGG; k++) a[k] = 0;
-    if (gl < d) agg_edge(pk[col[s0 + gl]], W(s0 + gl), p, pk_margin(me), a);
-#pragma unroll
-    for (int k = 0; k < NAGG; k++) a[k] = group_sum_i64<G>(a[k]);
-    if (gl == 0) gnn_write_row(me, vw[i], vol[p], d, a, in + (i64)i * GNN_IN);
+    AggPacked a;
+    if (gl < d) agg_edge_packed<WT>(pk[col[s0 + gl]], W(s0 + gl), p, pk_margin(me), a);
+    a.p0 = (u64)group_sum_i64<G>((i64)a.p0);
+    a.p1 = (u64)group_sum_i64<G>((i64)a.p1);
+    a.p2 = (u64)group_sum_i64<G>((i64)a.p2);
+    if (WT::bytes > 0) {
+        a.wa = group_sum_i64<G>(a.wa);
+        a.wo = group_sum_i64<G>(a.wo);
+    }
+    if (gl == 0) {
+        i64 S[NAGG];
+        agg_unpack<WT>(a, d, S);
+        gnn_write_row(me, vw[i], vol[p], d, S, in + (i64)i * GNN_IN);
+    }
 }
 
-// one block per vertex of larger degree (block-uniform loop, LDS tree)
+// one wave per vertex of degree 65..2048 (wave-uniform loop)
+template <class WT>
+__global__ void __launch_bounds__(256) k_gnn_agg_wave(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
+                                                      const i64* __restrict__ vw, const i32* __restrict__ list, i64 m,
+                                                      const u64* __restrict__ pk, const i64* __restrict__ vol,
+                                                      u16* __restrict__ in) {
+    const int lane = threadIdx.x & 63;
+    const i64 gi = block_id() * 4 + (threadIdx.x >> 6);
+    if (gi >= m) return;
+    const i32 i = list[gi];
+    const i64 s0 = rp[i], e = rp[i + 1];
+    const u64 me = pk[i];
+    const i32 p = (i32)(u32)me;
+    const int mqi = pk_margin(me);
+    AggPacked a;
+    for (i64 base = s0; base < e; base += 64) {
+        const i64 j = base + lane;
+        if (j < e) agg_edge_packed<WT>(pk[col[j]], W(j), p, mqi, a);
+    }
+    a.p0 = (u64)wave_sum_i64((i64)a.p0);
+    a.p1 = (u64)wave_sum_i64((i64)a.p1);
+    a.p2 = (u64)wave_sum_i64((i64)a.p2);
+    if (WT::bytes > 0) {
+        a.wa = wave_sum_i64(a.wa);
+        a.wo = wave_sum_i64(a.wo);
+    }
+    if (lane == 0) {
+        i64 S[NAGG];
+        agg_unpack<WT>(a, e - s0, S);
+        gnn_write_row(me, vw[i], vol[p], e - s0, S, in + (i64)i * GNN_IN);
+    }
+}
+
+// one block per vertex of degree > 2048 (block-uniform loop, LDS tree)
 template <class WT>
 __global__ void __launch_bounds__(256) k_gnn_agg_block(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
                                                        const i64* __restrict__ vw, const i32* __restrict__ list, i64 m,
@@ -268,33 +357,28 @@ __global__ void __launch_bounds__(256) k_gnn_dense(const u16* __restrict__ in, i
 
 // --------------------------------------------------------------- layer 2
 // logit_i = t_i + mean_{j in N(i)} s_j for active vertices; with `prune`,
-// those below tau get a cached NO_MOVE.
-__device__ __forceinline__ void logit_out(i32 i, float L, float tau, int prune, float* logit, i32* dest, i32* pruned,
-                                          bool lead, i64* cnt) {
-    bool pr = false;
-    if (lead) {
-        if (logit) logit[i] = L;
-        if (prune && L < tau) {
-            dest[i] = NO_MOVE;
-            pruned[i] = 1;
-            pr = true;
-        }
+// those below tau get a cached NO_MOVE and pruned[i] = cur.
+__device__ __forceinline__ void logit_out(i32 i, float L, float tau, int prune, i32 cur, float* logit, i32* dest,
+                                          i32* pruned) {
+    if (logit) logit[i] = L;
+    if (prune && L < tau) {
+        dest[i] = NO_MOVE;
+        pruned[i] = cur;
     }
-    if (prune) {
-        const u64 bal = __ballot(pr);
-        const u64 live = __ballot(1);
-        if ((int)(threadIdx.x & 63) == __ffsll((long long)live) - 1 && bal) {
-            atomicAdd((unsigned long long*)(cnt + GC_PRUNED), (unsigned long long)__popcll(bal));
-            cnt[GC_PENDING] = 1;
-        }
-    }
+}
+
+__device__ __forceinline__ float wave_sum_f32(float v) {
+    xlane_fence(v);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o);
+    return v;
 }
 
 template <int G>
 __global__ void __launch_bounds__(256) k_gnn_logit_group(const i64* __restrict__ rp, const i32* __restrict__ col,
                                                          const i32* __restrict__ list, i64 m, i32* dest,
                                                          const float* __restrict__ s, const float* __restrict__ t,
-                                                         float tau, int prune, float* logit, i32* pruned, i64* cnt) {
+                                                         float tau, int prune, i32 cur, float* logit, i32* pruned) {
     constexpr int GPB = 256 / G;
     const int gl = threadIdx.x & (G - 1);
     const i64 gi = block_id() * GPB + threadIdx.x / G;
@@ -306,8 +390,26 @@ __global__ void __launch_bounds__(256) k_gnn_logit_group(const i64* __restrict__
     xlane_fence(v);
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) v = v + __shfl_xor(v, o, G);
-    const float L = t[i] + v / (float)(d > 0 ? d : 1);
-    logit_out(i, L, tau, prune, logit, dest, pruned, gl == 0, cnt);
+    if (gl == 0) logit_out(i, t[i] + v / (float)(d > 0 ? d : 1), tau, prune, cur, logit, dest, pruned);
+}
+
+__global__ void __launch_bounds__(256) k_gnn_logit_wave(const i64* __restrict__ rp, const i32* __restrict__ col,
+                                                        const i32* __restrict__ list, i64 m, i32* dest,
+                                                        const float* __restrict__ s, const float* __restrict__ t,
+                                                        float tau, int prune, i32 cur, float* logit, i32* pruned) {
+    const int lane = threadIdx.x & 63;
+    const i64 gi = block_id() * 4 + (threadIdx.x >> 6);
+    if (gi >= m) return;
+    const i32 i = list[gi];
+    if (dest[i] == NO_MOVE) return;  // wave-uniform
+    const i64 s0 = rp[i], e = rp[i + 1];
+    float v = 0.0f;
+    for (i64 base = s0; base < e; base += 64) {
+        const i64 j = base + lane;
+        if (j < e) v = v + s[col[j]];
+    }
+    v = wave_sum_f32(v);
+    if (lane == 0) logit_out(i, t[i] + v / (float)(e > s0 ? e - s0 : 1), tau, prune, cur, logit, dest, pruned);
 }
 
 template <class T>
@@ -325,7 +427,7 @@ __device__ __forceinline__ T block_tree_256(T v, T* sh) {
 __global__ void __launch_bounds__(256) k_gnn_logit_block(const i64* __restrict__ rp, const i32* __restrict__ col,
                                                          const i32* __restrict__ list, i64 m, i32* dest,
                                                          const float* __restrict__ s, const float* __restrict__ t,
-                                                         float tau, int prune, float* logit, i32* pruned, i64* cnt) {
+                                                         float tau, int prune, i32 cur, float* logit, i32* pruned) {
     __shared__ float sh[256];
     const i64 gi = block_id();
     if (gi >= m) return;
@@ -338,8 +440,7 @@ __global__ void __launch_bounds__(256) k_gnn_logit_block(const i64* __restrict__
         if (j < e) v = v + s[col[j]];
     }
     v = block_tree_256(v, sh);
-    const float L = t[i] + v / (float)(e > s0 ? e - s0 : 1);
-    if (threadIdx.x < 64) logit_out(i, L, tau, prune, logit, dest, pruned, threadIdx.x == 0, cnt);
+    if (threadIdx.x == 0) logit_out(i, t[i] + v / (float)(e > s0 ? e - s0 : 1), tau, prune, cur, logit, dest, pruned);
 }
 
 // ------------------------------------------------------------- backward
@@ -398,6 +499,23 @@ __global__ void __launch_bounds__(256) k_gnn_ds_group(const i64* __restrict__ rp
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) v = v + __shfl_xor(v, o, G);
     if (gl == 0) ds[j] = v;
+}
+
+__global__ void __launch_bounds__(256) k_gnn_ds_wave(const i64* __restrict__ rp, const i32* __restrict__ col,
+                                                     const i32* __restrict__ list, i64 m, const float* __restrict__ r,
+                                                     float* __restrict__ ds) {
+    const int lane = threadIdx.x & 63;
+    const i64 gi = block_id() * 4 + (threadIdx.x >> 6);
+    if (gi >= m) return;
+    const i32 j = list[gi];
+    const i64 s0 = rp[j], e = rp[j + 1];
+    float v = 0.0f;
+    for (i64 base = s0; base < e; base += 64) {
+        const i64 x = base + lane;
+        if (x < e) v = v + r[col[x]];
+    }
+    v = wave_sum_f32(v);
+    if (lane == 0) ds[j] = v;
 }
 
 __global__ void __launch_bounds__(256) k_gnn_ds_block(const i64* __restrict__ rp, const i32* __restrict__ col,
@@ -508,13 +626,16 @@ __global__ void k_gnn_stamp(const i32* moves, i64 k, i32* stamp, i32 cur) {
     stamp[moves[x]] = cur;
 }
 
-__global__ void k_gnn_restore(i64 n, i32* pruned, i32* dest) {
+__global__ void __launch_bounds__(256) k_gnn_restore(i64 n, i32* pruned, i32* dest, i64* blk) {
     const i64 i = thread_item();
-    if (i >= n) return;
-    if (pruned[i]) {
+    i64 a = 0;
+    if (i < n && pruned[i]) {
         pruned[i] = 0;
         dest[i] = NULL_PART;
+        a = 1;
     }
+    const i64 s = block_sum_i64<256>(a);
+    if (threadIdx.x == 0) blk[block_id()] = s;
 }
 
 // ------------------------------------------------------------------ host
@@ -548,6 +669,7 @@ void Gnn::ensure(Ctx& c, i64 n, bool train) {
         logit = Buf<float>(c, cap);
         stamp = Buf<i32>(c, cap);
         pruned = Buf<i32>(c, cap);
+        blk = Buf<i64>(c, ceil_div(cap, 256) + 1);
         fill_i32(c, stamp, cap, -100);
         fill_i32(c, pruned, cap, 0);
         hcap = false;
@@ -576,15 +698,23 @@ void Gnn::set_params(Ctx& c, const float* host) {
     have_params = true;
 }
 
+// blk[0 .. blocks of n) -> cnt[slot] (device-side, no host read)
+static void gnn_sum_blocks(Ctx& c, Gnn& G, i64 n, int slot) { sum_i64(c, G.blk, ceil_div(n, 256), G.dcnt.p + slot); }
+
 void gnn_predict(Ctx& c, Gnn& G, LevelPlan& lp, const DGraph& g, const i32* part, i32* dest, const float* gain,
                  const i64* vol, i32 cur, bool first, bool prune, bool keep_hidden) {
     const i64 n = g.n;
     G.ensure(c, n, keep_hidden);
     G.sweeps++;
-    fill_i64(c, G.dcnt.p + GC_SWEEP_ACTIVE, 1, 0);
-    if (n == 0) return;
+    if (n == 0) {
+        fill_i64(c, G.dcnt.p + GC_SWEEP_ACTIVE, 1, 0);
+        return;
+    }
     k_gnn_pack<<<grid_items(n, 256), 256, 0, c.stream>>>(n, g.rp, part, dest, gain, G.stamp, cur, first ? 1 : 0, G.pk,
-                                                         G.dcnt);
+                                                         G.blk);
+    PGGC_LAUNCH_CHECK();
+    gnn_sum_blocks(c, G, n, GC_SWEEP_ACTIVE);
+    k_gnn_acc<<<1, 64, 0, c.stream>>>(G.dcnt, GC_ACTIVE, GC_SWEEP_ACTIVE);
     PGGC_LAUNCH_CHECK();
     const i64* st = lp.sc_start;
     const i32* L = lp.sc_list.p;
@@ -605,8 +735,11 @@ void gnn_predict(Ctx& c, Gnn& G, LevelPlan& lp, const DGraph& g, const i32* part
         if ((m = st[5] - st[4]) > 0)
             k_gnn_agg_group<64><<<grid_items(m, 4), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[4], m, G.pk,
                                                                         vol, G.in);
-        if ((m = st[8] - st[5]) > 0)
-            k_gnn_agg_block<<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[5], m, G.pk, vol,
+        if ((m = st[7] - st[5]) > 0)
+            k_gnn_agg_wave<<<grid_items(m, 4), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[5], m, G.pk, vol,
+                                                                   G.in);
+        if ((m = st[8] - st[7]) > 0)
+            k_gnn_agg_block<<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[7], m, G.pk, vol,
                                                                   G.in);
     });
     PGGC_LAUNCH_CHECK();
@@ -618,30 +751,39 @@ void gnn_predict(Ctx& c, Gnn& G, LevelPlan& lp, const DGraph& g, const i32* part
     const int pr = prune ? 1 : 0;
     if ((m = st[1] - st[0]) > 0)
         k_gnn_logit_group<4><<<grid_items(m, 64), 256, 0, c.stream>>>(g.rp, g.col, L + st[0], m, dest, G.s, G.t, tau,
-                                                                      pr, G.logit, G.pruned, G.dcnt);
+                                                                      pr, cur, G.logit, G.pruned);
     if ((m = st[2] - st[1]) > 0)
         k_gnn_logit_group<8><<<grid_items(m, 32), 256, 0, c.stream>>>(g.rp, g.col, L + st[1], m, dest, G.s, G.t, tau,
-                                                                      pr, G.logit, G.pruned, G.dcnt);
+                                                                      pr, cur, G.logit, G.pruned);
     if ((m = st[3] - st[2]) > 0)
         k_gnn_logit_group<16><<<grid_items(m, 16), 256, 0, c.stream>>>(g.rp, g.col, L + st[2], m, dest, G.s, G.t,
-                                                                       tau, pr, G.logit, G.pruned, G.dcnt);
+                                                                       tau, pr, cur, G.logit, G.pruned);
     if ((m = st[4] - st[3]) > 0)
         k_gnn_logit_group<32><<<grid_items(m, 8), 256, 0, c.stream>>>(g.rp, g.col, L + st[3], m, dest, G.s, G.t, tau,
-                                                                      pr, G.logit, G.pruned, G.dcnt);
+                                                                      pr, cur, G.logit, G.pruned);
     if ((m = st[5] - st[4]) > 0)
         k_gnn_logit_group<64><<<grid_items(m, 4), 256, 0, c.stream>>>(g.rp, g.col, L + st[4], m, dest, G.s, G.t, tau,
-                                                                      pr, G.logit, G.pruned, G.dcnt);
-    if ((m = st[8] - st[5]) > 0)
-        k_gnn_logit_block<<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, L + st[5], m, dest, G.s, G.t, tau, pr,
-                                                                G.logit, G.pruned, G.dcnt);
+                                                                      pr, cur, G.logit, G.pruned);
+    if ((m = st[7] - st[5]) > 0)
+        k_gnn_logit_wave<<<grid_items(m, 4), 256, 0, c.stream>>>(g.rp, g.col, L + st[5], m, dest, G.s, G.t, tau, pr,
+                                                                 cur, G.logit, G.pruned);
+    if ((m = st[8] - st[7]) > 0)
+        k_gnn_logit_block<<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, L + st[7], m, dest, G.s, G.t, tau, pr,
+                                                                cur, G.logit, G.pruned);
     PGGC_LAUNCH_CHECK();
+    if (prune) {
+        k_gnn_count<<<grid_items(n, 256), 256, 0, c.stream>>>(n, G.pruned, cur, G.blk);
+        PGGC_LAUNCH_CHECK();
+        gnn_sum_blocks(c, G, n, GC_TMP);
+        k_gnn_acc<<<1, 64, 0, c.stream>>>(G.dcnt, GC_PRUNED, GC_TMP);
+        PGGC_LAUNCH_CHECK();
+    }
 }
 
 void gnn_begin(Ctx& c, Gnn& G, i64 n, bool train) {
     G.ensure(c, n, train);
     fill_i32(c, G.stamp, n, -100);
     fill_i32(c, G.pruned, n, 0);
-    fill_i64(c, G.dcnt.p + GC_PENDING, 1, 0);
 }
 
 void gnn_stamp(Ctx& c, Gnn& G, const i32* moves, i64 k, i32 cur) {
@@ -651,10 +793,11 @@ void gnn_stamp(Ctx& c, Gnn& G, const i32* moves, i64 k, i32 cur) {
 }
 
 bool gnn_restore(Ctx& c, Gnn& G, i64 n, i32* dest) {
-    if (!G.dcnt.p || c.read1(G.dcnt.p + GC_PENDING) == 0) return false;
-    k_gnn_restore<<<grid_items(n, 256), 256, 0, c.stream>>>(n, G.pruned, dest);
+    if (!G.pruned.p || n == 0) return false;
+    k_gnn_restore<<<grid_items(n, 256), 256, 0, c.stream>>>(n, G.pruned, dest, G.blk);
     PGGC_LAUNCH_CHECK();
-    fill_i64(c, G.dcnt.p + GC_PENDING, 1, 0);
+    gnn_sum_blocks(c, G, n, GC_RESTORED);
+    if (c.read1(G.dcnt.p + GC_RESTORED) == 0) return false;
     G.fallbacks++;
     return true;
 }
@@ -679,8 +822,10 @@ void gnn_backward(Ctx& c, Gnn& G, LevelPlan& lp, const DGraph& g, const i32* lab
         k_gnn_ds_group<32><<<grid_items(m, 8), 256, 0, c.stream>>>(g.rp, g.col, L + st[3], m, G.r, G.ds);
     if ((m = st[5] - st[4]) > 0)
         k_gnn_ds_group<64><<<grid_items(m, 4), 256, 0, c.stream>>>(g.rp, g.col, L + st[4], m, G.r, G.ds);
-    if ((m = st[8] - st[5]) > 0)
-        k_gnn_ds_block<<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, L + st[5], m, G.r, G.ds);
+    if ((m = st[7] - st[5]) > 0)
+        k_gnn_ds_wave<<<grid_items(m, 4), 256, 0, c.stream>>>(g.rp, g.col, L + st[5], m, G.r, G.ds);
+    if ((m = st[8] - st[7]) > 0)
+        k_gnn_ds_block<<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, L + st[7], m, G.r, G.ds);
     PGGC_LAUNCH_CHECK();
     const i64 ntiles = (n + 15) / 16;
     const int nb = (int)std::min<i64>(GNN_GRAD_BLOCKS, std::max<i64>(1, (ntiles + 3) / 4));

@@ -56,6 +56,73 @@ __device__ __forceinline__ void ptable_add(u64* tab, u32 mask, i32 key, u64 w) {
     ptable_add_from(tab, mask, key, w, slot_hash(key, mask));
 }
 
+// Signed update of a table that is kept current between sweeps.  At most
+// mask+1 probes; returns false when the key is neither found nor insertable
+// (table full, or a decrement of a key the table does not hold), which
+// leaves the table stale.  A decrement never borrows across the packed key:
+// the slot holds at least the weight being moved out of it.
+__device__ __forceinline__ bool ptable_update(u64* tab, u32 mask, i32 key, i64 w) {
+    u32 h = slot_hash(key, mask);
+    for (u32 probe = 0; probe <= mask; probe++) {
+        const u64 cur = __hip_atomic_load(&tab[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((u32)(cur >> 32) == (u32)key) {
+            atomicAdd((unsigned long long*)&tab[h], (unsigned long long)w);
+            return true;
+        }
+        if (cur == PEMPTY) {
+            if (w < 0) return false;
+            const u64 old = atomicCAS((unsigned long long*)&tab[h], (unsigned long long)PEMPTY,
+                                      (unsigned long long)(((u64)(u32)key << 32) | (u64)w));
+            if (old == PEMPTY) return true;
+            if ((u32)(old >> 32) == (u32)key) {
+                atomicAdd((unsigned long long*)&tab[h], (unsigned long long)w);
+                return true;
+            }
+        }
+        h = (h + 1) & mask;
+    }
+    return false;
+}
+__device__ __forceinline__ bool table_update(i32* keys, i64* vals, u32 mask, i32 key, i64 w) {
+    u32 h = slot_hash(key, mask);
+    for (u32 probe = 0; probe <= mask; probe++) {
+        const i32 cur = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bool hit = cur == key;
+        if (!hit && cur == EMPTY_KEY) {
+            if (w < 0) return false;
+            const i32 old = atomicCAS(&keys[h], EMPTY_KEY, key);
+            hit = old == EMPTY_KEY || old == key;
+        }
+        if (hit) {
+            atomicAdd((unsigned long long*)&vals[h], (unsigned long long)w);
+            return true;
+        }
+        h = (h + 1) & mask;
+    }
+    return false;
+}
+// weight of `key` in a table (0 if absent), at most mask+1 probes
+__device__ __forceinline__ i64 ptable_get(const u64* tab, u32 mask, i32 key) {
+    u32 h = slot_hash(key, mask);
+    for (u32 probe = 0; probe <= mask; probe++) {
+        const u64 cur = tab[h];
+        if ((u32)(cur >> 32) == (u32)key) return (i64)(cur & 0xffffffffull);
+        if (cur == PEMPTY) return 0;
+        h = (h + 1) & mask;
+    }
+    return 0;
+}
+__device__ __forceinline__ i64 table_get(const i32* keys, const i64* vals, u32 mask, i32 key) {
+    u32 h = slot_hash(key, mask);
+    for (u32 probe = 0; probe <= mask; probe++) {
+        const i32 cur = keys[h];
+        if (cur == key) return vals[h];
+        if (cur == EMPTY_KEY) return 0;
+        h = (h + 1) & mask;
+    }
+    return 0;
+}
+
 // Work-item pre-aggregation: a block merges the (key, weight) pairs of one
 // work item (at most LSLOTS / 2 pairs) in an LDS table, then adds each
 // distinct key once to the HBM table.  Hot keys (a hub whose neighbours sit in

@@ -35,20 +35,26 @@ struct LevelPlan {
     Orders ord;
     Buf<i32> sc_list;           // vertices grouped by scoring class
     i64 sc_start[9] = {0};
-    // hubs (largest degree class): HBM hash tables, self-cleaning (every
-    // scan resets the slots it reads), edge and slot work items per hub.
+    // hubs (largest degree class): HBM hash tables of neighbour-cluster
+    // weights (own cluster included), edge and slot work items per hub.
+    // A table is built at the first table sweep of a local-move call and
+    // then kept current by the movers of each sweep (weight moved from the
+    // old to the new cluster key), so later sweeps only scan it; a table that
+    // runs out of slots marks the set stale and the next sweep rebuilds it.
     // packed: one u64 per slot (key << 32 | weight), used when every hub's
     // row weight fits 32 bits; otherwise separate key / int64 weight arrays.
     // imap / smap: block -> work item, so that all items of one hub run on
     // one XCD (blocks are dispatched round-robin over the 8 XCDs) and its
     // table stays in that XCD's L2; -1 = idle block.
+    // hidx: vertex -> hub index, -1 for other vertices.
     Buf<i64> goff, items, sitems, hub_pv;
     Buf<u64> hub_best;
     Buf<i32> gkeys;
     Buf<i64> gvals;
     Buf<u64> gtab;
-    Buf<i32> imap, smap;
+    Buf<i32> imap, smap, hidx;
     bool packed = false;
+    bool hub_live = false;  // tables match the current clustering
     i64 gslots = 0, nitems = 0, nsitems = 0, igrid = 0, sgrid = 0;
 };
 

@@ -151,6 +151,12 @@ static void xcd_maps(Ctx& c, LevelPlan& p, i64 m) {
     c.sync();  // the host vectors die here
 }
 
+__global__ void k_hub_index(const i32* hubs, i64 m, i32* hidx) {
+    const i64 x = thread_item();
+    if (x >= m) return;
+    hidx[hubs[x]] = (i32)x;
+}
+
 void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
     if (p.key == (const void*)g.rp && p.n == g.n) return;
     p.key = g.rp;
@@ -205,7 +211,12 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
             fill_i64(c, p.gvals, p.gslots, 0);
         }
         xcd_maps(c, p, m7);
+        p.hidx = Buf<i32>(c, std::max<i64>(g.n, 1));
+        fill_i32(c, p.hidx, g.n, -1);
+        k_hub_index<<<grid_items(m7, 256), 256, 0, c.stream>>>(hubs, m7, p.hidx);
+        PGGC_LAUNCH_CHECK();
     }
+    p.hub_live = false;
 }
 
 // ------------------------------------------------------------- scoring
@@ -444,48 +455,110 @@ __global__ void __launch_bounds__(256) k_hub_raw(const i64* __restrict__ rp, con
 }
 
 // Hubs: edge work items of HCH_E edges pre-aggregate in LDS, then insert into
-// the hub's HBM table and add the own-cluster weight; slot work items then scan (and reset) the table and
-// fold the packed best candidate with atomicMax; a last pass decides.
+// the hub's HBM table (built for every hub at the first table sweep of a
+// call, own cluster included); later sweeps keep it current from the movers
+// (k_hub_update_*).  Per sweep, k_hub_pv reads the own-cluster weight, slot
+// work items scan the table and fold the packed best candidate with
+// atomicMax, and a last pass decides.
 template <bool PACKED, class WT>
 __global__ void __launch_bounds__(256) k_hub_insert(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
                                                     const i32* __restrict__ hubs, i64 m, const i64* __restrict__ items,
                                                     const i32* __restrict__ imap, i64 nmap,
                                                     const i64* __restrict__ goff, const i32* __restrict__ part,
-                                                    const i32* __restrict__ dest, i32* gkeys, i64* gvals, u64* gtab,
-                                                    i64* hub_pv, u64* acct) {
+                                                    i32* gkeys, i64* gvals, u64* gtab, u64* acct) {
     const i64 bb = block_id();
     if (bb >= nmap) return;
     const i64 b = imap[bb];
     if (b < 0) return;
     const i64 hi = upper_row(items, 0, m, b);
     const i32 i = hubs[hi];
-    if (dest[i] == NO_MOVE) return;
     const i64 s = rp[i], e = rp[i + 1];
     const i64 j_lo = s + (b - items[hi]) * HCH_E;
     const i64 j_hi = j_lo + HCH_E < e ? j_lo + HCH_E : e;
     __shared__ i32 lk[HLDS];
     __shared__ i64 lv[HLDS];
     lds_agg_clear<256, HLDS>(lk, lv);
-    const i32 p = part[i];
-    i64 pv = 0;
     for (i64 jb = j_lo; jb < j_hi; jb += 256) {  // block-uniform loop control
         const i64 j = jb + threadIdx.x;
         if (j >= j_hi) continue;
-        const i32 v = col[j];
-        const i32 key = part[v];
-        const i64 ww = W(j);
-        if (key == p) pv += ww;
-        else table_add_lds(lk, lv, HLDS - 1, key, ww);
+        table_add_lds(lk, lv, HLDS - 1, part[col[j]], W(j));
     }
     const u32 gmask = (u32)(goff[hi + 1] - goff[hi] - 1);
     if (PACKED) lds_agg_flush_packed<256, HLDS>(lk, lv, gtab + goff[hi], gmask);
     else lds_agg_flush<256, HLDS>(lk, lv, gkeys + goff[hi], gvals + goff[hi], gmask);
-    pv = block_sum_i64<256>(pv);
-    if (threadIdx.x == 0 && pv) atomicAdd((unsigned long long*)&hub_pv[hi], (unsigned long long)pv);
     // algorithmic bytes of the item: column, neighbour cluster id and weight
-    // per edge, plus the hub's row bounds, cluster and cache flag
+    // per edge, plus the hub's row bounds and cluster
     if (acct && threadIdx.x == 0)
-        atomicAdd((unsigned long long*)acct, (unsigned long long)((j_hi - j_lo) * (8 + WT::bytes) + 24));
+        atomicAdd((unsigned long long*)acct, (unsigned long long)((j_hi - j_lo) * (8 + WT::bytes) + 20));
+}
+
+// own-cluster weight of every active hub, read from its table
+template <bool PACKED>
+__global__ void k_hub_pv(const i32* __restrict__ hubs, i64 m, const i64* __restrict__ goff,
+                         const i32* __restrict__ part, const i32* __restrict__ dest, const i32* gkeys,
+                         const i64* gvals, const u64* gtab, i64* hub_pv) {
+    const i64 hi = thread_item();
+    if (hi >= m) return;
+    const i32 i = hubs[hi];
+    if (dest[i] == NO_MOVE) return;
+    const u32 gmask = (u32)(goff[hi + 1] - goff[hi] - 1);
+    hub_pv[hi] = PACKED ? ptable_get(gtab + goff[hi], gmask, part[i])
+                        : table_get(gkeys + goff[hi], gvals + goff[hi], gmask, part[i]);
+}
+
+// Movers' edges into hubs: weight w of edge (i, u) leaves key part[i] and
+// joins key dest[i] in hub u's table (run before the moves are applied).
+// Same degree split as the uncut delta.  A failed update sets *stale.
+template <bool PACKED, class WT>
+__device__ __forceinline__ void hub_update_edge(const i32* col, WT W, const i32* hidx, const i64* goff, i32* gkeys,
+                                                i64* gvals, u64* gtab, i64 j, i32 oi, i32 ni, i64* stale) {
+    const i32 h = hidx[col[j]];
+    if (h < 0) return;
+    const i64 w = W(j);
+    const u32 gmask = (u32)(goff[h + 1] - goff[h] - 1);
+    bool ok;
+    if (PACKED) {
+        ok = ptable_update(gtab + goff[h], gmask, oi, -w);
+        ok = ptable_update(gtab + goff[h], gmask, ni, w) && ok;
+    } else {
+        ok = table_update(gkeys + goff[h], gvals + goff[h], gmask, oi, -w);
+        ok = table_update(gkeys + goff[h], gvals + goff[h], gmask, ni, w) && ok;
+    }
+    if (!ok) *stale = 1;
+}
+template <bool PACKED, class WT>
+__global__ void __launch_bounds__(256) k_hub_update_thread(const i64* __restrict__ rp, const i32* __restrict__ col,
+                                                           WT W, const i32* __restrict__ moves, i64 k,
+                                                           const i32* __restrict__ part, const i32* __restrict__ dest,
+                                                           const i32* __restrict__ hidx, const i64* __restrict__ goff,
+                                                           i32* gkeys, i64* gvals, u64* gtab, i64* stale) {
+    const i64 x = thread_item();
+    if (x >= k) return;
+    const i32 i = moves[x];
+    const i64 s = rp[i], e = rp[i + 1];
+    if (e - s >= LARGE_CUTOFF) return;
+    const i32 oi = part[i], ni = dest[i];
+    if (ni < 0) return;  // not applied (k_apply_moves)
+    for (i64 j = s; j < e; j++) hub_update_edge<PACKED>(col, W, hidx, goff, gkeys, gvals, gtab, j, oi, ni, stale);
+}
+template <bool PACKED, class WT>
+__global__ void __launch_bounds__(256) k_hub_update_wave(const i64* __restrict__ rp, const i32* __restrict__ col,
+                                                         WT W, const i32* __restrict__ moves, i64 k, i64 nw,
+                                                         const i32* __restrict__ part, const i32* __restrict__ dest,
+                                                         const i32* __restrict__ hidx, const i64* __restrict__ goff,
+                                                         i32* gkeys, i64* gvals, u64* gtab, i64* stale) {
+    const int lane = threadIdx.x & 63;
+    const i64 w = block_id() * 4 + (threadIdx.x >> 6);
+    if (w >= nw) return;
+    const i32 i = moves[k - 1 - w];
+    const i64 s = rp[i], e = rp[i + 1];
+    if (e - s < LARGE_CUTOFF) return;
+    const i32 oi = part[i], ni = dest[i];
+    if (ni < 0) return;  // not applied (k_apply_moves)
+    for (i64 base = s; base < e; base += 64) {  // wave-uniform loop control
+        const i64 j = base + lane;
+        if (j < e) hub_update_edge<PACKED>(col, W, hidx, goff, gkeys, gvals, gtab, j, oi, ni, stale);
+    }
 }
 
 template <bool PACKED>
@@ -494,7 +567,7 @@ __global__ void __launch_bounds__(256) k_hub_scan(const i32* __restrict__ hubs, 
                                                   const i64* __restrict__ goff, const i32* __restrict__ part,
                                                   const i32* __restrict__ dest, const i64* __restrict__ vw,
                                                   const i64* __restrict__ vol, const i32* __restrict__ cons, float lam,
-                                                  float ratio, i32* gkeys, i64* gvals, u64* gtab,
+                                                  float ratio, const i32* gkeys, const i64* gvals, const u64* gtab,
                                                   const i64* __restrict__ hub_pv, u64* hub_best) {
     const i64 bb = block_id();
     if (bb >= nmap) return;
@@ -518,17 +591,14 @@ __global__ void __launch_bounds__(256) k_hub_scan(const i32* __restrict__ hubs, 
         if (PACKED) {
             const u64 e = gtab[q];
             if (e == PEMPTY) continue;
-            gtab[q] = PEMPTY;
             key = (i32)(e >> 32);
             agg = (i64)(e & 0xffffffffull);
         } else {
             key = gkeys[q];
             if (key == EMPTY_KEY) continue;
             agg = gvals[q];
-            gkeys[q] = EMPTY_KEY;
-            gvals[q] = 0;
         }
-        if (agg > 0 && (float)agg >= bth && (cons == nullptr || cons[key] == cons[p])) {
+        if (key != p && agg > 0 && (float)agg >= bth && (cons == nullptr || cons[key] == cons[p])) {
             const float jc = fmaf(-(float)vol[key], multi, (float)agg);
             if (jc >= bth) {
                 const u64 k2 = pack_cand(jc, key);
@@ -602,20 +672,32 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
                 k_hub_raw<<<grid_blocks(lp.nitems), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, hubs, m, lp.items, part,
                                                                        vol, cons, lam, ratio, dest, lp.hub_best);
             } else {
-                tk = g.wk == W_UNIT ? c.kt_begin(KS_SCORE_HUB) : -1;
-                if (lp.packed) {
-                    k_hub_insert<true><<<grid_blocks(lp.igrid), 256, 0, c.stream>>>(
-                        g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid, lp.goff, part, dest, nullptr, nullptr,
-                        lp.gtab, lp.hub_pv, c.kt_counter(tk));
+                if (!lp.hub_live) {  // first table sweep of the call (or stale tables): build
+                    tk = g.wk == W_UNIT ? c.kt_begin(KS_SCORE_HUB) : -1;
+                    if (lp.packed) {
+                        fill_i64(c, (i64*)lp.gtab.p, lp.gslots, (i64)PEMPTY);
+                        k_hub_insert<true><<<grid_blocks(lp.igrid), 256, 0, c.stream>>>(
+                            g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid, lp.goff, part, nullptr, nullptr,
+                            lp.gtab, c.kt_counter(tk));
+                    } else {
+                        fill_i32(c, lp.gkeys, lp.gslots, EMPTY_KEY);
+                        fill_i64(c, lp.gvals, lp.gslots, 0);
+                        k_hub_insert<false><<<grid_blocks(lp.igrid), 256, 0, c.stream>>>(
+                            g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid, lp.goff, part, lp.gkeys, lp.gvals,
+                            nullptr, c.kt_counter(tk));
+                    }
                     c.kt_end(tk, 0.0);
+                    lp.hub_live = true;
+                }
+                if (lp.packed) {
+                    k_hub_pv<true><<<grid_items(m, 256), 256, 0, c.stream>>>(hubs, m, lp.goff, part, dest, nullptr,
+                                                                            nullptr, lp.gtab, lp.hub_pv);
                     k_hub_scan<true><<<grid_blocks(lp.sgrid), 256, 0, c.stream>>>(
                         hubs, m, lp.sitems, lp.smap, lp.sgrid, lp.goff, part, dest, g.vw, vol, cons, lam, ratio,
                         nullptr, nullptr, lp.gtab, lp.hub_pv, lp.hub_best);
                 } else {
-                    k_hub_insert<false><<<grid_blocks(lp.igrid), 256, 0, c.stream>>>(
-                        g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid, lp.goff, part, dest, lp.gkeys,
-                        lp.gvals, nullptr, lp.hub_pv, c.kt_counter(tk));
-                    c.kt_end(tk, 0.0);
+                    k_hub_pv<false><<<grid_items(m, 256), 256, 0, c.stream>>>(hubs, m, lp.goff, part, dest, lp.gkeys,
+                                                                             lp.gvals, nullptr, lp.hub_pv);
                     k_hub_scan<false><<<grid_blocks(lp.sgrid), 256, 0, c.stream>>>(
                         hubs, m, lp.sitems, lp.smap, lp.sgrid, lp.goff, part, dest, g.vw, vol, cons, lam, ratio,
                         lp.gkeys, lp.gvals, nullptr, lp.hub_pv, lp.hub_best);
@@ -956,10 +1038,35 @@ __global__ void k_set_mflag(const i32* moves, i64 k, i32* mflag) {
 
 // perform_moves (src/local_mover.cpp:1241-1279).  moves is in order1 order;
 // n_large = vertices of the level with degree >= LARGE_CUTOFF.
-static void perform_moves(Ctx& c, MoveScratch& s, const DGraph& g, RefineData& curr, const i32* moves, i64 k,
-                          i64 n_large, bool& tables) {
+static void perform_moves(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, RefineData& curr,
+                          const i32* moves, i64 k, i64 n_large, bool& tables) {
     const i64 nw = std::min(k, n_large);
     const int tk = c.kt_begin(KS_MOVES);
+    fill_i64(c, s.scal.p + 3, 1, 0);
+    const i64 m7 = lp.sc_start[8] - lp.sc_start[7];
+    // many movers: updating costs more than rebuilding the tables next sweep
+    if (lp.hub_live && m7 > 0 && (c.hub_update_div <= 0 || k * c.hub_update_div > g.n)) lp.hub_live = false;
+    if (lp.hub_live && m7 > 0) {  // keep the hub tables current (before part changes)
+        i64* stale = s.scal.p + 3;
+        PGGC_WDISPATCH(g, {
+            if (lp.packed) {
+                k_hub_update_thread<true><<<grid_items(k, 256), 256, 0, c.stream>>>(
+                    g.rp, g.col, W, moves, k, s.part, s.dest, lp.hidx, lp.goff, nullptr, nullptr, lp.gtab, stale);
+                if (nw > 0)
+                    k_hub_update_wave<true><<<grid_items(nw, 4), 256, 0, c.stream>>>(
+                        g.rp, g.col, W, moves, k, nw, s.part, s.dest, lp.hidx, lp.goff, nullptr, nullptr, lp.gtab,
+                        stale);
+            } else {
+                k_hub_update_thread<false><<<grid_items(k, 256), 256, 0, c.stream>>>(
+                    g.rp, g.col, W, moves, k, s.part, s.dest, lp.hidx, lp.goff, lp.gkeys, lp.gvals, nullptr, stale);
+                if (nw > 0)
+                    k_hub_update_wave<false><<<grid_items(nw, 4), 256, 0, c.stream>>>(
+                        g.rp, g.col, W, moves, k, nw, s.part, s.dest, lp.hidx, lp.goff, lp.gkeys, lp.gvals, nullptr,
+                        stale);
+            }
+        });
+        PGGC_LAUNCH_CHECK();
+    }
     {
         Buf<i64> du(c, std::max<i64>(k, 1));
         PGGC_WDISPATCH(g, {
@@ -987,7 +1094,8 @@ static void perform_moves(Ctx& c, MoveScratch& s, const DGraph& g, RefineData& c
     }
     PGGC_LAUNCH_CHECK();
     sum_sq_u128(c, curr.vol, curr.label_count, s.scal.p + 1);
-    const i64* h = c.read(s.scal, 3);
+    const i64* h = c.read(s.scal, 4);
+    if (h[3]) lp.hub_live = false;  // a hub table ran out of slots: rebuild next sweep
     curr.uncut += h[0];
     curr.obj = objective_value(curr.uncut, curr.lambda, (u64)h[1], (u64)h[2]);
 }
@@ -1094,6 +1202,7 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
     const i64 n = g.n;
     s.ensure(c, n);
     plan_level(c, g, lp);
+    lp.hub_live = false;  // the clustering changed since the last call
     Buf<i64> cv;
     RefineData curr = begin_curr(c, best, cv, n);
     PGGC_HIP(hipMemcpyAsync(s.part, best_part, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
@@ -1106,16 +1215,19 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
     SweepTrace tr(c, g, is_initial ? 0 : 1);
     // active-set predictor: on levels of at least min_nnz nonzeros, from the
     // second sweep of a call that starts at singletons (whose first sweep
-    // moves most vertices), from the first sweep otherwise
+    // moves most vertices), from the first sweep otherwise, and only while
+    // the previous sweep left at least n/16 candidates (below that the
+    // active set is mostly the movers' neighbourhood and little is to gain)
     Gnn* G = (s.gnn && s.gnn->cfg.mode != GNN_OFF && g.nnz >= s.gnn->cfg.min_nnz) ? s.gnn : nullptr;
     const bool train = G && G->cfg.mode == GNN_TRAIN;
     if (G) gnn_begin(c, *G, n, train);
+    i64 kprev = n;
     for (int x = 0; x < 2; x++) {
         const float fr = SA ? ratios[x] : 0.0f;
         for (int cnt = 0; cnt < limits[x]; cnt++) {
             iter++;
             const float lam = (float)curr.lambda;
-            const bool use = G && !(is_initial && iter == 1);
+            const bool use = G && !(is_initial && iter == 1) && kprev * 16 >= n;
             const bool full = use && G->cfg.full_every > 0 && iter % G->cfg.full_every == 0;
             if (full) gnn_restore(c, *G, n, s.dest);
             if (use && !full)
@@ -1137,6 +1249,7 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
                 stats->sweeps++;
                 stats->cands += k;
             }
+            kprev = k;
             if (G) gnn_stamp(c, *G, s.moves, km, iter);
             if (train && use && !full) gnn_backward(c, *G, lp, g, nullptr, iter, true);
             if (k == 0) {
@@ -1148,7 +1261,7 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
                 break;
             }
             if (iter > 1 || !is_initial) set_new_cluster_ids(c, s, g, curr, s.moves, km, constrained ? cons : nullptr);
-            perform_moves(c, s, g, curr, s.moves, km, g.n - lp.ord.off_large, tables);
+            perform_moves(c, s, lp, g, curr, s.moves, km, g.n - lp.ord.off_large, tables);
             if (stats) stats->moves += km;
             if (curr.obj > best.obj) {
                 copy_state(c, best, curr);
@@ -1228,6 +1341,7 @@ void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i
     const i64 n = g.n;
     s.ensure(c, n);
     plan_level(c, g, lp);
+    lp.hub_live = false;  // the clustering changed since the last call
     Buf<i64> cv;
     RefineData curr = begin_curr(c, best, cv, n);
     PGGC_HIP(hipMemcpyAsync(s.part, best_part, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
@@ -1266,7 +1380,7 @@ void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i
         }
         k_set_mflag<<<grid_items(tr, 256), 256, 0, c.stream>>>(s.moves, tr, s.mflag);
         PGGC_LAUNCH_CHECK();
-        perform_moves(c, s, g, curr, s.moves, tr, g.n - lp.ord.off_large, tables);
+        perform_moves(c, s, lp, g, curr, s.moves, tr, g.n - lp.ord.off_large, tables);
         if (stats) stats->moves += tr;
         if (curr.obj > best.obj) {
             copy_state(c, best, curr);
@@ -1285,6 +1399,7 @@ i64 sweep_once(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32
     (void)L;
     s.ensure(c, n);
     plan_level(c, g, lp);
+    lp.hub_live = false;
     PGGC_HIP(hipMemcpyAsync(s.part, part, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
     PGGC_HIP(hipMemcpyAsync(s.dest, dest, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
     PGGC_HIP(hipMemcpyAsync(s.gain, gain, n * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
