@@ -35,6 +35,7 @@ def parse():
     p.add_argument("--cpu-scale", type=int, default=16)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-leiden", action="store_true", help="(default) skip the pLeiden+ report")
+    p.add_argument("--no-gnn", action="store_true", help="skip the active-set predictor A/B step")
     p.add_argument("--leiden", type=int, default=0, metavar="ITERS",
                    help="also run pLeiden+ with ex_iters=ITERS (to stable it creeps up by ~1e-8 per outer "
                         "iteration for hundreds of iterations at scale 24)")
@@ -124,6 +125,25 @@ def main():
         extra["leiden_plus"] = {"ex_iters": a.leiden, "seconds": time.perf_counter() - t1, "Q": ql,
                                 "labels": infol["labels"], "sweeps": infol["sweeps"]}
         print(f"# pLeiden+: {extra['leiden_plus']}", file=sys.stderr, flush=True)
+
+    if not a.no_gnn:
+        # A/B: one more step with the active-set predictor pruning (shipped
+        # weights and threshold), against the plain steps timed above
+        from pennsylvaniagpugraphclustering_amd import predictor
+        d = predictor.enable(ctx=ctx)
+        C.gnn_reset_stats(ctx=ctx)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        qg, _, ig = step()
+        torch.cuda.synchronize()
+        tg = time.perf_counter() - t1
+        st = C.gnn_stats(ctx=ctx)
+        C.gnn_configure("off", ctx=ctx)
+        extra["gnn"] = {"tau": d["tau"], "min_nnz": d.get("min_nnz"), "seconds": tg, "plain_seconds": dt,
+                        "speedup": dt / tg, "Q": qg, "dQ_vs_plain": qg - q, "sweeps": ig["sweeps"],
+                        "outer_iterations": ig["outer"], "predicted_sweeps": st["sweeps"],
+                        "pruned_fraction": st["pruned"] / max(st["active"], 1), "restores": st["restores"]}
+        print(f"# predictor A/B: {extra['gnn']}", file=sys.stderr, flush=True)
 
     cpu = None
     if not a.no_cpu:

@@ -673,15 +673,18 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
                                                                        vol, cons, lam, ratio, dest, lp.hub_best);
             } else {
                 if (!lp.hub_live) {  // first table sweep of the call (or stale tables): build
-                    tk = g.wk == W_UNIT ? c.kt_begin(KS_SCORE_HUB) : -1;
                     if (lp.packed) {
                         fill_i64(c, (i64*)lp.gtab.p, lp.gslots, (i64)PEMPTY);
+                    } else {
+                        fill_i32(c, lp.gkeys, lp.gslots, EMPTY_KEY);
+                        fill_i64(c, lp.gvals, lp.gslots, 0);
+                    }
+                    tk = g.wk == W_UNIT ? c.kt_begin(KS_SCORE_HUB) : -1;
+                    if (lp.packed) {
                         k_hub_insert<true><<<grid_blocks(lp.igrid), 256, 0, c.stream>>>(
                             g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid, lp.goff, part, nullptr, nullptr,
                             lp.gtab, c.kt_counter(tk));
                     } else {
-                        fill_i32(c, lp.gkeys, lp.gslots, EMPTY_KEY);
-                        fill_i64(c, lp.gvals, lp.gslots, 0);
                         k_hub_insert<false><<<grid_blocks(lp.igrid), 256, 0, c.stream>>>(
                             g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid, lp.goff, part, lp.gkeys, lp.gvals,
                             nullptr, c.kt_counter(tk));
