@@ -61,9 +61,21 @@ struct LevelPlan {
 // Device scratch sized for the top level, shared by every level.
 struct Gnn;
 
+// What the afterburner reads of a neighbour, in one 16-byte load: its gain,
+// destination and cluster of this sweep and its vertex weight as the fp32
+// value the gain arithmetic uses.
+struct __align__(16) AbInfo {
+    float gain;
+    i32 dest;
+    i32 part;
+    float vwf;
+};
+
 struct MoveScratch {
     i64 cap = 0;
     Gnn* gnn = nullptr;  // active-set predictor (null or mode off: plain sweeps)
+    Buf<AbInfo> abi;
+    Buf<u64> mvp;  // per vertex: cluster | moved << 31, and the cluster after this sweep's moves
     Buf<i32> part, dest, mflag, cand, moves, flag, tmp, tmp2;
     Buf<float> gain;
     Buf<i64> prio, scal;

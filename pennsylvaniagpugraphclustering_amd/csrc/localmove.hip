@@ -48,6 +48,8 @@ void MoveScratch::ensure(Ctx& c, i64 n) {
     tmp = Buf<i32>(c, cap);
     tmp2 = Buf<i32>(c, cap + 1);
     gain = Buf<float>(c, cap);
+    abi = Buf<AbInfo>(c, cap);
+    mvp = Buf<u64>(c, cap);
     prio = Buf<i64>(c, cap + 1);
     scal = Buf<i64>(c, 16);
     fill_i32(c, mflag, cap, 0);
@@ -752,18 +754,21 @@ static i64 gather_candidates(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph
 }
 
 // ------------------------------------------------------------- afterburner
+// afterburner_filter (src/local_mover.cpp:374-433): the change in the
+// candidate's gain if every neighbour with a larger gain (ties broken by the
+// xorshift hash) also makes its move
 template <class WT>
-__device__ __forceinline__ void ab_term(const i32* col, WT W, const i64* vw, const i32* part, const i32* dest,
-                                        const float* gain, i64 j, i32 p, i32 best, float multi, float igain, i32 hi,
-                                        float& acc) {
+__device__ __forceinline__ void ab_term(const i32* col, WT W, const AbInfo* info, i64 j, i32 p, i32 best,
+                                        float multi, float igain, i32 hi, float& acc) {
     const i32 v = col[j];
-    const float vg = gain[v];
+    const AbInfo q4 = info[v];
+    const float vg = q4.gain;
     if ((vg - igain) >= AFTERBURNER_EPS || (fabsf(vg - igain) < AFTERBURNER_EPS && (i32)xorshift32((u32)v) < hi)) {
-        i32 vpart = dest[v];
-        const float q = fmaf(-multi, (float)vw[v], (float)W(j));
+        i32 vpart = q4.dest;
+        const float q = fmaf(-multi, q4.vwf, (float)W(j));
         acc = acc - ((vpart == p) ? q : 0.0f);
         acc = acc + ((vpart == best && best != NEW_PART) ? q : 0.0f);
-        vpart = part[v];
+        vpart = q4.part;
         acc = acc + ((vpart == p) ? q : 0.0f);
         acc = acc - ((vpart == best) ? q : 0.0f);
     }
@@ -771,18 +776,18 @@ __device__ __forceinline__ void ab_term(const i32* col, WT W, const i64* vw, con
 
 template <class WT>
 __global__ void __launch_bounds__(256) k_ab_serial(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
-                                                   const i64* __restrict__ vw, const i32* __restrict__ cand, i64 k,
-                                                   const i32* __restrict__ part, const i32* __restrict__ dest,
-                                                   const float* __restrict__ gain, float lam, i32* keep, i32* mflag) {
+                                                   const AbInfo* __restrict__ info, const i32* __restrict__ cand,
+                                                   i64 k, float lam, i32* keep, i32* mflag) {
     const i64 x = thread_item();
     if (x >= k) return;
     const i32 i = cand[x];
-    const i32 best = dest[i], p = part[i];
-    const float multi = (float)vw[i] * lam;
-    const float igain = gain[i];
+    const AbInfo me = info[i];
+    const i32 best = me.dest, p = me.part;
+    const float multi = me.vwf * lam;
+    const float igain = me.gain;
     const i32 hi = (i32)xorshift32((u32)i);
     float acc = 0.0f;
-    for (i64 j = rp[i]; j < rp[i + 1]; j++) ab_term(col, W, vw, part, dest, gain, j, p, best, multi, igain, hi, acc);
+    for (i64 j = rp[i]; j < rp[i + 1]; j++) ab_term(col, W, info, j, p, best, multi, igain, hi, acc);
     const int kp = (igain + acc >= 0.0f) ? 1 : 0;
     keep[x] = kp;
     mflag[i] = kp;  // candidates start with mflag 0: an unconditional store
@@ -790,23 +795,23 @@ __global__ void __launch_bounds__(256) k_ab_serial(const i64* __restrict__ rp, c
 
 template <class WT>
 __global__ void __launch_bounds__(256) k_ab_wave(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
-                                                 const i64* __restrict__ vw, const i32* __restrict__ cand, i64 k0,
-                                                 i64 k1, const i32* __restrict__ part, const i32* __restrict__ dest,
-                                                 const float* __restrict__ gain, float lam, i32* keep, i32* mflag) {
+                                                 const AbInfo* __restrict__ info, const i32* __restrict__ cand, i64 k0,
+                                                 i64 k1, float lam, i32* keep, i32* mflag) {
     const int lane = threadIdx.x & 63;
     const i64 x = k0 + block_id() * 4 + (threadIdx.x >> 6);
     if (x >= k1) return;
     {
         const i32 i = cand[x];
-        const i32 best = dest[i], p = part[i];
-        const float multi = (float)vw[i] * lam;
-        const float igain = gain[i];
+        const AbInfo me = info[i];
+        const i32 best = me.dest, p = me.part;
+        const float multi = me.vwf * lam;
+        const float igain = me.gain;
         const i32 hi = (i32)xorshift32((u32)i);
         float acc = 0.0f;
         const i64 e = rp[i + 1];
         for (i64 base = rp[i]; base < e; base += 64) {  // wave-uniform loop control
             const i64 j = base + lane;
-            if (j < e) ab_term(col, W, vw, part, dest, gain, j, p, best, multi, igain, hi, acc);
+            if (j < e) ab_term(col, W, info, j, p, best, multi, igain, hi, acc);
         }
         xlane_fence(acc);
 #pragma unroll
@@ -821,24 +826,24 @@ __global__ void __launch_bounds__(256) k_ab_wave(const i64* __restrict__ rp, con
 
 template <class WT>
 __global__ void __launch_bounds__(1024) k_ab_block(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
-                                                   const i64* __restrict__ vw, const i32* __restrict__ cand, i64 k0,
-                                                   i64 k1, const i32* __restrict__ part, const i32* __restrict__ dest,
-                                                   const float* __restrict__ gain, float lam, i32* keep, i32* mflag) {
+                                                   const AbInfo* __restrict__ info, const i32* __restrict__ cand,
+                                                   i64 k0, i64 k1, float lam, i32* keep, i32* mflag) {
     __shared__ float sh[1024];
     const int t = threadIdx.x;
     const i64 x = k0 + block_id();
     if (x >= k1) return;
     {
         const i32 i = cand[x];
-        const i32 best = dest[i], p = part[i];
-        const float multi = (float)vw[i] * lam;
-        const float igain = gain[i];
+        const AbInfo me = info[i];
+        const i32 best = me.dest, p = me.part;
+        const float multi = me.vwf * lam;
+        const float igain = me.gain;
         const i32 hi = (i32)xorshift32((u32)i);
         float acc = 0.0f;
         const i64 e = rp[i + 1];
         for (i64 base = rp[i]; base < e; base += 1024) {  // block-uniform loop control
             const i64 j = base + t;
-            if (j < e) ab_term(col, W, vw, part, dest, gain, j, p, best, multi, igain, hi, acc);
+            if (j < e) ab_term(col, W, info, j, p, best, multi, igain, hi, acc);
         }
         sh[t] = acc;
         __syncthreads();
@@ -861,22 +866,58 @@ __global__ void __launch_bounds__(1024) k_ab_block(const i64* __restrict__ rp, c
     }
 }
 
+__global__ void __launch_bounds__(256) k_ab_pack(i64 n, const float* __restrict__ gain, const i32* __restrict__ dest,
+                                                 const i32* __restrict__ part, const i64* __restrict__ vw,
+                                                 AbInfo* __restrict__ info) {
+    const i64 i = thread_item();
+    if (i >= n) return;
+    AbInfo a;
+    a.gain = gain[i];
+    a.dest = dest[i];
+    a.part = part[i];
+    a.vwf = (float)vw[i];
+    info[i] = a;
+}
+
+// algorithmic bytes of the filter: per candidate edge the column, weight and
+// the neighbour's gain, destination, cluster and vertex weight; per
+// candidate its id, row bounds and keep flag
+template <class WT>
+__global__ void __launch_bounds__(256) k_ab_bytes(const i64* __restrict__ rp, const i32* __restrict__ cand, i64 k,
+                                                  u64* acct) {
+    const i64 x = thread_item();
+    i64 b = 0;
+    if (x < k) {
+        const i32 i = cand[x];
+        b = (rp[i + 1] - rp[i]) * (20 + WT::bytes) + 16;
+    }
+    b = block_sum_i64<256>(b);
+    if (threadIdx.x == 0 && b) atomicAdd((unsigned long long*)acct, (unsigned long long)b);
+}
+
 static i64 afterburner(Ctx& c, MoveScratch& s, const DGraph& g, const i32* cand, i64 k, const i64* b, float lam,
                        i32* moves) {
     const int tk = c.kt_begin(KS_AFTERBURNER);
+    k_ab_pack<<<grid_items(g.n, 256), 256, 0, c.stream>>>(g.n, s.gain, s.dest, s.part, g.vw, s.abi);
+    PGGC_LAUNCH_CHECK();
     PGGC_WDISPATCH(g, {
         if (b[0] > 0)
-            k_ab_serial<<<grid_items(b[0], 256), 256, 0, c.stream>>>(
-                g.rp, g.col, W, g.vw, cand, b[0], s.part, s.dest, s.gain, lam, s.flag, s.mflag);
+            k_ab_serial<<<grid_items(b[0], 256), 256, 0, c.stream>>>(g.rp, g.col, W, s.abi, cand, b[0], lam, s.flag,
+                                                                     s.mflag);
         if (b[1] > b[0])
-            k_ab_wave<<<grid_items(b[1] - b[0], 4), 256, 0, c.stream>>>(
-                g.rp, g.col, W, g.vw, cand, b[0], b[1], s.part, s.dest, s.gain, lam, s.flag, s.mflag);
+            k_ab_wave<<<grid_items(b[1] - b[0], 4), 256, 0, c.stream>>>(g.rp, g.col, W, s.abi, cand, b[0], b[1], lam,
+                                                                        s.flag, s.mflag);
         if (k > b[1])
-            k_ab_block<<<grid_blocks(k - b[1]), 1024, 0, c.stream>>>(
-                g.rp, g.col, W, g.vw, cand, b[1], k, s.part, s.dest, s.gain, lam, s.flag, s.mflag);
+            k_ab_block<<<grid_blocks(k - b[1]), 1024, 0, c.stream>>>(g.rp, g.col, W, s.abi, cand, b[1], k, lam,
+                                                                     s.flag, s.mflag);
     });
     c.kt_end(tk, 0.0);
     PGGC_LAUNCH_CHECK();
+    if (tk >= 0) {  // timing runs only: count the bytes outside the timed region
+        PGGC_WDISPATCH(g, (k_ab_bytes<decltype(W)><<<grid_items(k, 256), 256, 0, c.stream>>>(g.rp, cand, k,
+                                                                                            c.kt_counter(tk))));
+        PGGC_LAUNCH_CHECK();
+    }
     return compact_flags(c, s.flag, cand, k, moves, s.scal);
 }
 
@@ -938,13 +979,26 @@ static void set_new_cluster_ids(Ctx& c, MoveScratch& s, const DGraph& g, RefineD
 // twice for edges to vertices that stay.  Move lists are in order1 order, so
 // movers of degree >= LARGE_CUTOFF form a suffix: they take one wave each
 // (counted from the end); the others one thread each.  out[x] = delta of mover x.
+// mvp[v]: low word part[v] | mflag[v] << 31 (cluster ids are >= 0), high
+// word the cluster after the moves (dest[v] for a mover, else part[v]), so
+// the delta reads one 8-byte word per edge
+__global__ void k_mv_pack(i64 n, const i32* __restrict__ part, const i32* __restrict__ dest,
+                          const i32* __restrict__ mflag, u64* __restrict__ mvp) {
+    const i64 i = thread_item();
+    if (i >= n) return;
+    const i32 ov = part[i];
+    const u32 mv = mflag[i] ? 1u : 0u;
+    const i32 nv = mv ? dest[i] : ov;
+    mvp[i] = (u64)((u32)ov | (mv << 31)) | ((u64)(u32)nv << 32);
+}
+
 template <class WT>
-__device__ __forceinline__ i64 delta_term(const i32* col, WT W, const i32* part, const i32* dest, const i32* mflag,
-                                          i64 j, i32 oi, i32 ni) {
-    const i32 v = col[j];
-    const i32 ov = part[v];
-    const int mv = mflag[v];
-    const i32 nv = mv ? dest[v] : ov;
+__device__ __forceinline__ i64 delta_term(const i32* col, WT W, const u64* mvp, i64 j, i32 oi, i32 ni) {
+    const u64 q = mvp[col[j]];
+    const u32 lo = (u32)q;
+    const i32 ov = (i32)(lo & 0x7fffffffu);
+    const bool mv = (lo >> 31) != 0;
+    const i32 nv = (i32)(u32)(q >> 32);
     const i64 dd = (i64)(ni == nv) - (i64)(oi == ov);
     return dd * W(j) * (mv ? 1 : 2);
 }
@@ -953,7 +1007,7 @@ __global__ void __launch_bounds__(256) k_delta_uncut_thread(const i64* __restric
                                                             WT W, const i32* __restrict__ moves, i64 k,
                                                             const i32* __restrict__ part,
                                                             const i32* __restrict__ dest,
-                                                            const i32* __restrict__ mflag, i64* out) {
+                                                            const u64* __restrict__ mvp, i64* out) {
     const i64 x = thread_item();
     if (x >= k) return;
     const i32 i = moves[x];
@@ -961,14 +1015,14 @@ __global__ void __launch_bounds__(256) k_delta_uncut_thread(const i64* __restric
     if (e - s >= LARGE_CUTOFF) return;
     const i32 oi = part[i], ni = dest[i];
     i64 acc = 0;
-    for (i64 j = s; j < e; j++) acc += delta_term(col, W, part, dest, mflag, j, oi, ni);
+    for (i64 j = s; j < e; j++) acc += delta_term(col, W, mvp, j, oi, ni);
     out[x] = acc;
 }
 template <class WT>
 __global__ void __launch_bounds__(256) k_delta_uncut_wave(const i64* __restrict__ rp, const i32* __restrict__ col,
                                                           WT W, const i32* __restrict__ moves, i64 k, i64 nw,
                                                           const i32* __restrict__ part, const i32* __restrict__ dest,
-                                                          const i32* __restrict__ mflag, i64* out) {
+                                                          const u64* __restrict__ mvp, i64* out) {
     const int lane = threadIdx.x & 63;
     const i64 w = block_id() * 4 + (threadIdx.x >> 6);
     if (w >= nw) return;
@@ -980,7 +1034,7 @@ __global__ void __launch_bounds__(256) k_delta_uncut_wave(const i64* __restrict_
     i64 acc = 0;
     for (i64 base = s; base < e; base += 64) {  // wave-uniform loop control
         const i64 j = base + lane;
-        if (j < e) acc += delta_term(col, W, part, dest, mflag, j, oi, ni);
+        if (j < e) acc += delta_term(col, W, mvp, j, oi, ni);
     }
     acc = wave_sum_i64(acc);
     if (lane == 0) out[x] = acc;
@@ -1072,12 +1126,14 @@ static void perform_moves(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g
     }
     {
         Buf<i64> du(c, std::max<i64>(k, 1));
+        k_mv_pack<<<grid_items(g.n, 256), 256, 0, c.stream>>>(g.n, s.part, s.dest, s.mflag, s.mvp);
+        PGGC_LAUNCH_CHECK();
         PGGC_WDISPATCH(g, {
             k_delta_uncut_thread<<<grid_items(k, 256), 256, 0, c.stream>>>(g.rp, g.col, W, moves, k, s.part, s.dest,
-                                                                          s.mflag, du);
+                                                                          s.mvp, du);
             if (nw > 0)
                 k_delta_uncut_wave<<<grid_items(nw, 4), 256, 0, c.stream>>>(g.rp, g.col, W, moves, k, nw, s.part,
-                                                                           s.dest, s.mflag, du);
+                                                                           s.dest, s.mvp, du);
         });
         PGGC_LAUNCH_CHECK();
         sum_i64(c, du, k, s.scal);
