@@ -336,20 +336,52 @@ __global__ void __launch_bounds__(BS) k_cagg(const i64* rp, const i32* col, WT W
             __syncthreads();
         }
         if (!GLOBAL) {
-            for (i64 s = t; s < S; s += BS)
-                if (keys[s] == EMPTY_KEY) keys[s] = 0x7fffffff;
-            __syncthreads();
-            bitonic_sort_lds<BS>(keys, vals, (int)S);
-            int c = 0;
-            for (i64 sb = 0; sb < S; sb += BS) {  // block-uniform loop control
-                const i64 s = sb + t;
-                if (s < S) c += keys[s] != 0x7fffffff;
+            // Sort only the distinct entries: move them to the front of the
+            // table (through registers, so no second LDS buffer), then rank
+            // them directly when one block's threads suffice (keys are
+            // distinct) or bitonic-sort the next power of two above them.
+            constexpr int R = SLOTS / BS;
+            i32 ek[R];
+            i64 ev[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const i64 q = t + (i64)r * BS;
+                ek[r] = q < S ? keys[q] : EMPTY_KEY;
+                ev[r] = q < S ? vals[q] : 0;
             }
-            c = (int)block_sum_i64<BS>(c);
-            i64 base = toff[u];
-            for (i64 s = t; s < c; s += BS) {
-                tcol[base + s] = keys[s];
-                tw[base + s] = vals[s];
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                if (ek[r] != EMPTY_KEY) {
+                    const int pos = atomicAdd(&s_cnt, 1);
+                    keys[pos] = ek[r];
+                    vals[pos] = ev[r];
+                }
+            }
+            __syncthreads();
+            const int c = s_cnt;
+            const i64 base = toff[u];
+            if (c <= BS) {
+                if (t < c) {
+                    const i32 mk = keys[t];
+                    int rank = 0;
+                    for (int j = 0; j < c; j++) rank += keys[j] < mk ? 1 : 0;
+                    tcol[base + rank] = mk;
+                    tw[base + rank] = vals[t];
+                }
+            } else {
+                int P = 1;
+                while (P < c) P <<= 1;  // P <= S: the table has at least twice the distinct keys
+                for (int q = c + t; q < P; q += BS) {
+                    keys[q] = 0x7fffffff;
+                    vals[q] = 0;
+                }
+                __syncthreads();
+                bitonic_sort_lds<BS>(keys, vals, P);
+                for (int q = t; q < c; q += BS) {
+                    tcol[base + q] = keys[q];
+                    tw[base + q] = vals[q];
+                }
             }
             if (t == 0) ccnt[u] = c;
         } else {
