@@ -23,6 +23,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of a timed region, measured with rocprofv3 FETCH_SIZE
+# and WRITE_SIZE counters scaled by a plain copy kernel of known size
+# (profiles/rmat24_afterburner_traffic.md); keyed by (region, R-MAT scale)
+MEASURED_TRAFFIC = {("afterburner", 24): 12.81e9}
+
 
 def parse():
     p = argparse.ArgumentParser()
@@ -36,6 +41,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-leiden", action="store_true", help="(default) skip the pLeiden+ report")
     p.add_argument("--no-gnn", action="store_true", help="skip the active-set predictor A/B step")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run a plain 1 GiB read + 1 GiB write elementwise kernel 5 times (the reference point "
+                        "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
     p.add_argument("--leiden", type=int, default=0, metavar="ITERS",
                    help="also run pLeiden+ with ex_iters=ITERS (to stable it creeps up by ~1e-8 per outer "
                         "iteration for hundreds of iterations at scale 24)")
@@ -57,7 +65,7 @@ def cpu_baseline(scale, seed=2):
                       f"{dt:.2f} s, Q={q:.6f}"}
 
 
-def roofline(kstats):
+def roofline(kstats, scale):
     """Roofline object for the dominant timed kernel group that carries an
     algorithmic byte count (HIP events on the library stream)."""
     best = None
@@ -69,7 +77,7 @@ def roofline(kstats):
     n, ms, by = kstats[best]
     achieved = by / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": best, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launches": n,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get((best, scale)), "launches": n,
             "avg_launch_us": ms / n * 1000.0, "bytes_per_launch": by / n}
 
 
@@ -79,6 +87,14 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
     import torch
+    if a.copy_calibration:
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"read_bytes": 1 << 30, "write_bytes": 1 << 30, "repeats": 5}}))
+        return
     from pennsylvaniagpugraphclustering_amd import graphs
     from pennsylvaniagpugraphclustering_amd import clustering as C
 
@@ -172,7 +188,7 @@ def main():
         "levels": info["levels"],
         "outer_iterations": info["outer"],
         "edges_per_s_per_sweep": m * info["sweeps"] / dt,
-        "roofline": roofline(kstats),
+        "roofline": roofline(kstats, a.scale),
         "kernel_groups_ms_per_step": {k: v[1] / max(a.steps, 1) for k, v in kstats.items() if v[0]},
         "cpu_baseline": cpu,
     }
