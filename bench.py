@@ -41,6 +41,10 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-leiden", action="store_true", help="(default) skip the pLeiden+ report")
     p.add_argument("--no-gnn", action="store_true", help="skip the active-set predictor A/B step")
+    p.add_argument("--batch", type=int, default=512,
+                   help="batch leg: this many planted-partition graphs (n log-uniform in [2^10, 2^18]), pLouvain "
+                        "ex_iters 0, one by one and as one batch (0 skips it)")
+    p.add_argument("--batch-streams", type=int, default=8)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 1 GiB read + 1 GiB write elementwise kernel 5 times (the reference point "
                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -79,6 +83,37 @@ def roofline(kstats, scale):
     return {"bound": "hbm", "kernel": best, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get((best, scale)), "launches": n,
             "avg_launch_us": ms / n * 1000.0, "bytes_per_launch": by / n}
+
+
+def batch_leg(a, ctx, C, graphs, torch):
+    """Many small instances: `a.batch` planted-partition graphs with seeds
+    100.., n = round(2^U(10,18)), blocks uniform in [n/256, n/32], expected
+    intra degree 10 and inter degree 3 (graphs.sbm_torch), pLouvain ex_iters
+    0.  Timed one by one on the context, then as one pggc_cluster_batch call
+    on a.batch_streams streams; the objectives must agree."""
+    import numpy as np
+    rng = np.random.default_rng(100)
+    bg = []
+    for k in range(a.batch):
+        n = int(round(2 ** rng.uniform(10, 18)))
+        b = int(rng.integers(max(1, -(-n // 256)), max(1, n // 32) + 1))
+        rp_, col_ = graphs.sbm_torch(n, b, 10, 3, seed=100 + k)
+        bg.append(C.DeviceGraph(rp_, col_, None, None, objective="Mod", ctx=ctx))
+    edges = sum(x.nnz // 2 for x in bg)
+    ctx.set_verbose(0)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    seq = [C.cluster(x, "louvain", "Mod", 1.0, 0, seed=k, ctx=ctx) for k, x in enumerate(bg)]
+    torch.cuda.synchronize()
+    ts = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    res = C.cluster_batch(bg, "louvain", "Mod", 1.0, 0, seeds=list(range(len(bg))), streams=a.batch_streams, ctx=ctx)
+    tb = time.perf_counter() - t2
+    ctx.set_verbose(1)
+    return {"graphs": len(bg), "vertices": sum(x.n for x in bg), "edges": edges, "one_by_one_s": ts,
+            "batch_s": tb, "streams": a.batch_streams, "speedup": ts / tb, "batch_edges_per_s": edges / tb,
+            "objectives_equal": all(r[0] == q[0] for r, q in zip(res, seq)),
+            "mean_Q": float(np.mean([r[0] for r in res]))}
 
 
 def main():
@@ -160,6 +195,10 @@ def main():
                         "outer_iterations": ig["outer"], "predicted_sweeps": st["sweeps"],
                         "pruned_fraction": st["pruned"] / max(st["active"], 1), "restores": st["restores"]}
         print(f"# predictor A/B: {extra['gnn']}", file=sys.stderr, flush=True)
+
+    if a.batch > 0:
+        extra["batch"] = batch_leg(a, ctx, C, graphs, torch)
+        print(f"# batch: {extra['batch']}", file=sys.stderr, flush=True)
 
     cpu = None
     if not a.no_cpu:

@@ -265,6 +265,25 @@ int pggc_leiden(pggc_ctx* ctx, const pggc_graph* g, pggc_state* st, int plus, in
 int pggc_cluster(pggc_ctx* ctx, const pggc_graph* g, int algo, int objective, double lambda_mult, int ex_iters,
                  uint64_t seed, int32_t* part_out, double* objective_out, int64_t* info);
 
+/* Batched runs: `count` independent pggc_cluster calls (different graphs, or
+ * seeded trials of one graph as in run_clustering's trial loop,
+ * app/cluster_main.cpp:189-204, or meme population members) in one call.
+ * Up to `streams` worker contexts, each with a host thread of its own and
+ * all on the calling context's stream, take instances in order: the host
+ * work of small instances (launches, readbacks) overlaps while their kernels
+ * run in submission order.  Each instance gives exactly what pggc_cluster
+ * gives for it alone.  graphs[count] (host array of device graphs), seeds[count]
+ * (host, nullable: instance index), parts_out[count] (host array of device
+ * pointers), objectives_out[count] and info_out[8*count] (host, nullable).
+ * The call first waits for the context's stream, so inputs produced on it
+ * are complete, and returns when every instance has finished.
+ * The context's knobs and pruning predictor (not training) apply to every
+ * instance; progress, verbose and trace callbacks do not.  The first failing
+ * instance stops the batch and is named in the error message. */
+int pggc_cluster_batch(pggc_ctx* ctx, int count, const pggc_graph* graphs, int algo, int objective,
+                       double lambda_mult, int ex_iters, const uint64_t* seeds, int32_t* const* parts_out,
+                       double* objectives_out, int64_t* info_out, int streams);
+
 /* Memetic loop building blocks (app/memetic_clustering.cpp).
  * gen_signature (:71-106): bit j of sig_out[(nnz+31)/32] (device) is set when
  * CSR edge j joins two clusters of part. */

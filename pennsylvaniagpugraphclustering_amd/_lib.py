@@ -103,6 +103,7 @@ SIGNATURES = {
     "pggc_leiden": (c.c_int, [P, c.POINTER(Graph), c.POINTER(State), c.c_int, c.c_int, P, P]),
     "pggc_cluster": (c.c_int, [P, c.POINTER(Graph), c.c_int, c.c_int, c.c_double, c.c_int, c.c_uint64, P,
                                c.POINTER(c.c_double), I64P]),
+    "pggc_cluster_batch": (c.c_int, [P, c.c_int, P, c.c_int, c.c_int, c.c_double, c.c_int, P, P, P, P, c.c_int]),
 }
 
 _lib = None

@@ -305,6 +305,25 @@ def leiden_part(g, st, plus=False, improve=False, input=None, ctx=None):
 INFO_KEYS = ["labels", "uncut", "levels", "sweeps", "moves", "outer", "cands", "total_nnz"]
 
 
+def cluster_batch(gs, algo="louvain", objective="Mod", mult=1.0, ex_iters=0, seeds=None, streams=8, ctx=None):
+    """pggc_cluster_batch: independent runs on the graphs `gs` (or seeded
+    trials of one graph repeated in `gs`), overlapped on `streams` worker
+    streams.  Returns a list of (objective, part device tensor, info dict)."""
+    ctx = _ctx(ctx)
+    k = len(gs)
+    garr = (Graph * max(k, 1))(*[g.c() for g in gs])
+    outs = [torch.empty(max(g.n, 1), dtype=torch.int32, device=_dev()) for g in gs]
+    parr = (ctypes.c_void_p * max(k, 1))(*[o.data_ptr() for o in outs])
+    sarr = (ctypes.c_uint64 * max(k, 1))(*(seeds if seeds is not None else range(k)))
+    qs = (ctypes.c_double * max(k, 1))()
+    info = (ctypes.c_int64 * (8 * max(k, 1)))()
+    check(ctx.lib.pggc_cluster_batch(ctx.h, k, garr, ALGOS[algo], OBJECTIVES[objective], mult, ex_iters, sarr, parr,
+                                     qs, info, int(streams)))
+    torch.cuda.synchronize()
+    return [(qs[x], outs[x][: gs[x].n], dict(zip(INFO_KEYS, (int(v) for v in info[8 * x: 8 * x + 8]))))
+            for x in range(k)]
+
+
 def cluster(g, algo="louvain", objective="Mod", mult=1.0, ex_iters=0, seed=0, ctx=None):
     """run_clustering (app/cluster_main.cpp:116-165) for one trial.
     Returns (objective value, part device tensor, info dict)."""

@@ -1,11 +1,14 @@
 // Multilevel drivers (src/clustering_methods.cpp), the run_clustering outer
 // loop (app/cluster_main.cpp:101-165) and the C ABI declared in pggc.h.
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "device.h"
@@ -21,6 +24,8 @@ struct pggc_ctx {
     Ctx c;
     MoveScratch ms;
     Gnn gnn;
+    // worker contexts of pggc_cluster_batch (on this context's stream)
+    std::vector<pggc_ctx*> workers;
 };
 
 namespace pggc {
@@ -122,7 +127,7 @@ constexpr int LEIDEN_MAX_STALLS = 8;
 static void louvain_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& rfd, bool constrained,
                          i32* constraint, i32* part_out, RunStats& rs, LevelPlan* top_plan) {
     if (constrained) {  // cluster_data::reset (interface/cluster_data.hpp:49-55)
-        PGGC_HIP(hipMemcpyAsync(rfd.vol, top.vw, top.n * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
+        copy_dev(c, rfd.vol, top.vw, top.n * sizeof(i64));
         rfd.uncut = 0;
         rfd.label_count = (i32)top.n;
         update_objective(c, rfd);
@@ -152,7 +157,7 @@ static void louvain_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData&
             c.sync();
             rs.contract_s += since(t0);
             og->vw = Buf<i64>(c, std::max<i64>(nc, 1));
-            PGGC_HIP(hipMemcpyAsync(og->vw, rfd.vol, nc * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
+            copy_dev(c, og->vw, rfd.vol, nc * sizeof(i64));
             if (constrained) {
                 Buf<i32> nxt(c, std::max<i64>(nc, 1));
                 downsample_max(c, g.n, pp, nc, cons, nxt);
@@ -181,7 +186,7 @@ static void louvain_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData&
         project(c, g.n, parts[i + 1], parts[i]);
         local_move(c, ms, lp, g, parts[i], rfd, false, false, nullptr, true, &rs.mv);
     }
-    PGGC_HIP(hipMemcpyAsync(part_out, parts[0].p, top.n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+    copy_dev(c, part_out, parts[0].p, top.n * sizeof(i32));
     c.sync();
 }
 
@@ -193,7 +198,7 @@ static void leiden_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& 
     levels.push_back(top_level(top));
     Buf<i32> part(c, std::max<i64>(top.n, 1));
     if (!improve) iota_i32(c, part, top.n);
-    else PGGC_HIP(hipMemcpyAsync(part, input, top.n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+    else copy_dev(c, part, input, top.n * sizeof(i32));
     int stalled = 0;  // consecutive refinements that contracted nothing
     while (true) {
         const size_t li = levels.size() - 1;
@@ -258,13 +263,13 @@ static void leiden_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& 
         project(c, g.n, parts[i + 1], parts[i]);
         if (plus) local_move(c, ms, lp, g, parts[i], rfd, false, false, nullptr, true, &rs.mv);
     }
-    PGGC_HIP(hipMemcpyAsync(part_out, parts[0].p, top.n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+    copy_dev(c, part_out, parts[0].p, top.n * sizeof(i32));
     c.sync();
 }
 
 // get_objective (app/objective_helpers.hpp:9-22, interface/cluster_data.hpp)
 static void state_init(Ctx& c, const DGraph& g, int objective, double mult, RefineData& rd) {
-    PGGC_HIP(hipMemcpyAsync(rd.vol, g.vw, g.n * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
+    copy_dev(c, rd.vol, g.vw, g.n * sizeof(i64));
     rd.top_nnz = g.nnz;
     rd.label_count = (i32)g.n;
     rd.uncut = 0;
@@ -389,6 +394,7 @@ int pggc_ctx_create(pggc_ctx** out, void* stream) {
         x->c.stream = (hipStream_t)stream;
         PGGC_HIP(hipGetDevice(&x->c.device));
         PGGC_HIP(hipHostMalloc((void**)&x->c.pinned, 64 * sizeof(i64), hipHostMallocDefault));
+        PGGC_HIP(hipHostGetDevicePointer((void**)&x->c.pinned_dev, x->c.pinned, 0));
         x->ms.gnn = &x->gnn;
         *out = x.release();
     });
@@ -398,6 +404,8 @@ int pggc_ctx_destroy(pggc_ctx* ctx) {
     return guard([&] {
         if (!ctx) return;
         (void)hipStreamSynchronize(ctx->c.stream);
+        for (pggc_ctx* w : ctx->workers) (void)pggc_ctx_destroy(w);
+        ctx->workers.clear();
         ctx->gnn = Gnn();
         ctx->ms = MoveScratch();
         for (auto& kv : ctx->c.live) (void)hipFree(kv.first);
@@ -544,17 +552,17 @@ int pggc_gnn_eval(pggc_ctx* ctx, const pggc_graph* gg, const int32_t* part, cons
         G.ensure(c, n, true);
         Buf<i32> d2(c, std::max<i64>(n, 1));
         if (n > 0) {
-            PGGC_HIP(hipMemcpyAsync(d2.p, dest, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
-            PGGC_HIP(hipMemcpyAsync(G.stamp.p, stamp, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+            copy_dev(c, d2.p, dest, n * sizeof(i32));
+            copy_dev(c, G.stamp.p, stamp, n * sizeof(i32));
             fill_i32(c, (i32*)G.logit.p, n, 0);
         }
         const i64 sw = G.sweeps;
         gnn_predict(c, G, lp, g, part, d2, gain, vol, cur, first != 0, false, true);
         G.sweeps = sw;
         if (rows_out && n > 0)
-            PGGC_HIP(hipMemcpyAsync(rows_out, G.in.p, n * GNN_IN * sizeof(u16), hipMemcpyDeviceToDevice, c.stream));
+            copy_dev(c, rows_out, G.in.p, n * GNN_IN * sizeof(u16));
         if (logit_out && n > 0)
-            PGGC_HIP(hipMemcpyAsync(logit_out, G.logit.p, n * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+            copy_dev(c, logit_out, G.logit.p, n * sizeof(float));
         if (labels) {
             double l0 = 0.0, l1 = 0.0;
             PGGC_HIP(hipMemcpyAsync(&l0, G.dloss.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -680,8 +688,8 @@ int pggc_orderings(pggc_ctx* ctx, const pggc_graph* gg, int32_t* order1, int32_t
         Orders o;
         make_orders(c, g, o);
         if (g.n > 0) {
-            PGGC_HIP(hipMemcpyAsync(order1, o.o1, g.n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
-            PGGC_HIP(hipMemcpyAsync(order2, o.o2, g.n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+            copy_dev(c, order1, o.o1, g.n * sizeof(i32));
+            copy_dev(c, order2, o.o2, g.n * sizeof(i32));
         }
         c.sync();
         offs[0] = o.off_large;
@@ -727,14 +735,14 @@ int pggc_contract(pggc_ctx* ctx, const pggc_graph* gg, const int32_t* vcmap, int
         OwnedGraph og;
         contract(c, g, vcmap, nc, og);
         *nnz_out = og.nnz;
-        PGGC_HIP(hipMemcpyAsync(out_rp, og.rp, (nc + 1) * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
+        copy_dev(c, out_rp, og.rp, (nc + 1) * sizeof(i64));
         if (og.nnz > cap) {
             c.sync();
             throw Error(PGGC_ECAPACITY, "coarse nnz exceeds output capacity");
         }
         if (og.nnz > 0) {
-            PGGC_HIP(hipMemcpyAsync(out_col, og.col, og.nnz * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
-            PGGC_HIP(hipMemcpyAsync(out_w, og.w, og.nnz * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
+            copy_dev(c, out_col, og.col, og.nnz * sizeof(i32));
+            copy_dev(c, out_w, og.w, og.nnz * sizeof(i64));
         }
         c.sync();
     });
@@ -866,7 +874,7 @@ int pggc_cluster(pggc_ctx* ctx, const pggc_graph* gg, int algo, int objective, d
                 term = false;
             }
             if (term) break;
-            PGGC_HIP(hipMemcpyAsync(cons, part_out, g.n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+            copy_dev(c, cons, part_out, g.n * sizeof(i32));
             if (algo == PGGC_LOUVAIN) louvain_part(c, ctx->ms, g, rd, true, cons, part_out, rs, &top);
             else leiden_part(c, ctx->ms, g, rd, algo == PGGC_LEIDEN_PLUS, true, cons, part_out, rs, &top);
             outer++;
@@ -886,6 +894,93 @@ int pggc_cluster(pggc_ctx* ctx, const pggc_graph* gg, int algo, int objective, d
             info[6] = rs.mv.cands;
             info[7] = rs.total_nnz;
         }
+    });
+}
+
+// Many independent clusterings in one call: worker contexts, each with a
+// host thread of its own, take instances from a shared counter.  Each
+// instance runs exactly as pggc_cluster would run it alone.
+int pggc_cluster_batch(pggc_ctx* ctx, int count, const pggc_graph* graphs, int algo, int objective, double mult,
+                       int ex_iters, const uint64_t* seeds, int32_t* const* parts_out, double* objectives_out,
+                       int64_t* info_out, int streams) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (count < 0 || streams < 1) throw Error(PGGC_EINVAL, "count must be >= 0 and streams >= 1");
+        if (count > 0 && (!graphs || !parts_out || !objectives_out))
+            throw Error(PGGC_EINVAL, "graphs, parts_out and objectives_out are required");
+        if (ctx->gnn.cfg.mode == GNN_TRAIN) throw Error(PGGC_EINVAL, "predictor training is not available in batches");
+        const int S = std::min(streams, count);
+        if (S == 0) return;
+        // inputs may still be in flight on the caller's stream; the workers
+        // run on streams of their own
+        ctx->c.sync();
+        // Workers share the caller's stream: kernels run in submission order
+        // while the host threads overlap their launches and readbacks.  (With
+        // a stream per worker, concurrently executing instances gave
+        // run-to-run different clusterings on gfx950; see the design notes.)
+        while ((int)ctx->workers.size() < S) {
+            pggc_ctx* w = nullptr;
+            const int rc = pggc_ctx_create(&w, ctx->c.stream);
+            if (rc != PGGC_OK) throw Error(rc, g_err);
+            ctx->workers.push_back(w);
+        }
+        // the caller's knobs and predictor settings apply to every instance
+        std::vector<float> prm;
+        const bool prune = ctx->gnn.cfg.mode == GNN_PRUNE;
+        if (prune) {
+            prm.resize(GNN_P);
+            PGGC_HIP(hipMemcpyAsync(prm.data(), ctx->gnn.prm.p, GNN_P * sizeof(float), hipMemcpyDeviceToHost,
+                                    ctx->c.stream));
+            ctx->c.sync();
+        }
+        for (int k = 0; k < S; k++) {
+            pggc_ctx* w = ctx->workers[k];
+            w->c.hub_update_div = ctx->c.hub_update_div;
+            if (prune) {
+                w->gnn.set_params(w->c, prm.data());
+                w->gnn.cfg = ctx->gnn.cfg;
+            } else {
+                w->gnn.cfg.mode = GNN_OFF;
+            }
+        }
+        std::atomic<int> next{0};
+        std::mutex mu;
+        int first_rc = PGGC_OK;
+        std::string first_msg;
+        auto work = [&](int k) {
+            if (hipSetDevice(ctx->c.device) != hipSuccess) {
+                std::lock_guard<std::mutex> l(mu);
+                if (first_rc == PGGC_OK) {
+                    first_rc = PGGC_EHIP;
+                    first_msg = "hipSetDevice failed in a batch worker";
+                }
+                return;
+            }
+            pggc_ctx* w = ctx->workers[k];
+            while (true) {
+                {
+                    std::lock_guard<std::mutex> l(mu);
+                    if (first_rc != PGGC_OK) return;
+                }
+                const int x = next.fetch_add(1);
+                if (x >= count) return;
+                const int rc = pggc_cluster(w, &graphs[x], algo, objective, mult, ex_iters,
+                                            seeds ? seeds[x] : (uint64_t)x, parts_out[x], &objectives_out[x],
+                                            info_out ? info_out + 8 * (int64_t)x : nullptr);
+                if (rc != PGGC_OK) {
+                    std::lock_guard<std::mutex> l(mu);
+                    if (first_rc == PGGC_OK) {
+                        first_rc = rc;
+                        first_msg = "instance " + std::to_string(x) + ": " + g_err;
+                    }
+                    return;
+                }
+            }
+        };
+        std::vector<std::thread> th;
+        for (int k = 0; k < S; k++) th.emplace_back(work, k);
+        for (auto& t : th) t.join();
+        if (first_rc != PGGC_OK) throw Error(first_rc, first_msg);
     });
 }
 

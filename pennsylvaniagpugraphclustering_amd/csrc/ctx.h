@@ -52,7 +52,8 @@ struct Ctx {
     std::multimap<size_t, void*> free_blocks;
     std::unordered_map<void*, size_t> live;
     size_t bytes_live = 0, bytes_cached = 0;
-    i64* pinned = nullptr;  // 64 pinned int64 readback slots
+    i64* pinned = nullptr;      // 64 pinned int64 readback slots
+    i64* pinned_dev = nullptr;  // the same slots as seen from the device
     u64 seed = 0;           // leidenR seed stream
     u64 draws = 0;
     int verbose = 0;        // >0: per-iteration progress lines on stderr
@@ -129,6 +130,7 @@ struct Buf {
 void fill_i32(Ctx& c, i32* p, i64 n, i32 v);
 void fill_i64(Ctx& c, i64* p, i64 n, i64 v);
 void iota_i32(Ctx& c, i32* p, i64 n);
+void copy_dev(Ctx& c, void* dst, const void* src, size_t bytes);  // on c.stream
 // out[0..n] = exclusive scan of in[0..n); out[n] = total.  in may alias out.
 void scan_i64(Ctx& c, const i64* in, i64* out, i64 n);
 void scan_i32_to_i64(Ctx& c, const i32* in, i64* out, i64 n);

@@ -567,7 +567,7 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
     PGGC_LAUNCH_CHECK();
     scan_i64(c, cnt, mstart, nc);
     scan_i64(c, bound, toff, nc);
-    PGGC_HIP(hipMemcpyAsync(pos, mstart, nc * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
+    copy_dev(c, pos, mstart, nc * sizeof(i64));
     k_cscatter<<<grid_items(n, 256), 256, 0, c.stream>>>(vcmap, n, pos, members);
     PGGC_LAUNCH_CHECK();
     const i64 tb = c.read1(toff.p + nc);

@@ -1182,7 +1182,7 @@ static void relabel(Ctx& c, MoveScratch& s, i64 n, i32* labels, RefineData& rd) 
     k_relabel_vol<<<grid_items(L, 256), 256, 0, c.stream>>>(s.flag, pos, L, rd.vol, nvol);
     PGGC_LAUNCH_CHECK();
     i64 t = c.read1(pos.p + L);
-    PGGC_HIP(hipMemcpyAsync(rd.vol, nvol, t * sizeof(i64), hipMemcpyDeviceToDevice, c.stream));
+    copy_dev(c, rd.vol, nvol, t * sizeof(i64));
     k_relabel_part<<<grid_items(n, 256), 256, 0, c.stream>>>(pos, n, labels);
     PGGC_LAUNCH_CHECK();
     rd.label_count = (i32)t;
@@ -1190,8 +1190,7 @@ static void relabel(Ctx& c, MoveScratch& s, i64 n, i32* labels, RefineData& rd) 
 
 static void copy_state(Ctx& c, RefineData& dst, const RefineData& src) {
     if (src.label_count > dst.cap) throw Error(-4, "label capacity exceeded");
-    PGGC_HIP(hipMemcpyAsync(dst.vol, src.vol, (size_t)src.label_count * sizeof(i64), hipMemcpyDeviceToDevice,
-                            c.stream));
+    copy_dev(c, dst.vol, src.vol, (size_t)src.label_count * sizeof(i64));
     dst.top_nnz = src.top_nnz;
     dst.uncut = src.uncut;
     dst.obj = src.obj;
@@ -1264,7 +1263,7 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
     lp.hub_live = false;  // the clustering changed since the last call
     Buf<i64> cv;
     RefineData curr = begin_curr(c, best, cv, n);
-    PGGC_HIP(hipMemcpyAsync(s.part, best_part, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+    copy_dev(c, s.part, best_part, n * sizeof(i32));
     fill_i32(c, s.dest, n, NULL_PART);
     bool tables = !is_initial;
     const float ratios[2] = {0.75f, 0.25f};
@@ -1324,7 +1323,7 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
             if (stats) stats->moves += km;
             if (curr.obj > best.obj) {
                 copy_state(c, best, curr);
-                PGGC_HIP(hipMemcpyAsync(best_part, s.part, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+                copy_dev(c, best_part, s.part, n * sizeof(i32));
             }
             tr.end(k, km);
         }
@@ -1403,7 +1402,7 @@ void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i
     lp.hub_live = false;  // the clustering changed since the last call
     Buf<i64> cv;
     RefineData curr = begin_curr(c, best, cv, n);
-    PGGC_HIP(hipMemcpyAsync(s.part, best_part, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+    copy_dev(c, s.part, best_part, n * sizeof(i32));
     fill_i32(c, s.dest, n, NULL_PART);
     bool tables = !is_initial;
     const i32* ccons = constrained ? cons : nullptr;
@@ -1443,7 +1442,7 @@ void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i
         if (stats) stats->moves += tr;
         if (curr.obj > best.obj) {
             copy_state(c, best, curr);
-            PGGC_HIP(hipMemcpyAsync(best_part, s.part, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+            copy_dev(c, best_part, s.part, n * sizeof(i32));
         }
         fill_i32(c, s.dest, n, NULL_PART);
         trc.end(k, tr);
@@ -1459,9 +1458,9 @@ i64 sweep_once(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32
     s.ensure(c, n);
     plan_level(c, g, lp);
     lp.hub_live = false;
-    PGGC_HIP(hipMemcpyAsync(s.part, part, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
-    PGGC_HIP(hipMemcpyAsync(s.dest, dest, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
-    PGGC_HIP(hipMemcpyAsync(s.gain, gain, n * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+    copy_dev(c, s.part, part, n * sizeof(i32));
+    copy_dev(c, s.dest, dest, n * sizeof(i32));
+    copy_dev(c, s.gain, gain, n * sizeof(float));
     const float lam = (float)lambda;
     score_all(c, s, lp, g, s.part, vol, cons, raw, lam, ratio, s.dest, s.gain);
     i64 b[2];
@@ -1473,8 +1472,8 @@ i64 sweep_once(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32
         k_clear_mflag<<<grid_items(km, 256), 256, 0, c.stream>>>(moves_out, km, s.mflag);
         PGGC_LAUNCH_CHECK();
     }
-    PGGC_HIP(hipMemcpyAsync(dest, s.dest, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
-    PGGC_HIP(hipMemcpyAsync(gain, s.gain, n * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+    copy_dev(c, dest, s.dest, n * sizeof(i32));
+    copy_dev(c, gain, s.gain, n * sizeof(float));
     c.sync();
     return km;
 }

@@ -127,8 +127,17 @@ void Ctx::kt_free() {
     kt.pool.clear();
 }
 
+// Small readbacks: a one-wave kernel on the context stream stores the values
+// into the pinned host slots, then the host waits for the stream.
+__global__ void k_readback(const i64* __restrict__ dev, i64* host, int count) {
+    const int t = threadIdx.x;
+    if (t < count) host[t] = dev[t];
+}
+
 const i64* Ctx::read(const i64* dev, int count) {
-    PGGC_HIP(hipMemcpyAsync(pinned, dev, sizeof(i64) * count, hipMemcpyDeviceToHost, stream));
+    if (count < 1 || count > 64) throw Error(-5, "read: 1..64 values");
+    k_readback<<<1, 64, 0, stream>>>(dev, pinned_dev, count);
+    PGGC_LAUNCH_CHECK();
     PGGC_HIP(hipStreamSynchronize(stream));
     return pinned;
 }
@@ -152,6 +161,31 @@ void fill_i64(Ctx& c, i64* p, i64 n, i64 v) {
     k_fill<i64><<<grid_for(n, 256 * 4, 8192), 256, 0, c.stream>>>(p, n, v);
     PGGC_LAUNCH_CHECK();
 }
+// device-to-device copy as a kernel on the context stream (16-byte words
+// when both ends are 16-byte aligned and the size is a multiple of 16, else
+// 4-byte words; every copy in the library is a multiple of 4 bytes)
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, i64 n) {
+    const i64 i = thread_item();
+    if (i < n) dst[i] = src[i];
+}
+__global__ void k_copy4(const u32* __restrict__ src, u32* __restrict__ dst, i64 n) {
+    const i64 i = thread_item();
+    if (i < n) dst[i] = src[i];
+}
+void copy_dev(Ctx& c, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0 || dst == src) return;
+    if (bytes % 4 != 0 || ((uintptr_t)dst | (uintptr_t)src) % 4 != 0)
+        throw Error(-5, "copy_dev: size and addresses must be multiples of 4 bytes");
+    if (bytes % 16 == 0 && ((uintptr_t)dst | (uintptr_t)src) % 16 == 0) {
+        const i64 n = (i64)(bytes / 16);
+        k_copy16<<<grid_items(n, 256), 256, 0, c.stream>>>((const uint4*)src, (uint4*)dst, n);
+    } else {
+        const i64 n = (i64)(bytes / 4);
+        k_copy4<<<grid_items(n, 256), 256, 0, c.stream>>>((const u32*)src, (u32*)dst, n);
+    }
+    PGGC_LAUNCH_CHECK();
+}
+
 void iota_i32(Ctx& c, i32* p, i64 n) {
     if (n <= 0) return;
     k_iota<<<grid_for(n, 256 * 4, 8192), 256, 0, c.stream>>>(p, n);
@@ -354,8 +388,8 @@ void radix_sort_pairs(Ctx& c, u64* keys, i32* vals, i64 n, int bits) {
         std::swap(va, vb);
     }
     if (ka != keys) {
-        PGGC_HIP(hipMemcpyAsync(keys, ka, n * sizeof(u64), hipMemcpyDeviceToDevice, c.stream));
-        PGGC_HIP(hipMemcpyAsync(vals, va, n * sizeof(i32), hipMemcpyDeviceToDevice, c.stream));
+        copy_dev(c, keys, ka, n * sizeof(u64));
+        copy_dev(c, vals, va, n * sizeof(i32));
     }
 }
 

@@ -206,6 +206,50 @@ def rmat_torch(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=2, weights=No
     return rp, col, ww
 
 
+def _csr_from_pairs_torch(lo, hi, n, device):
+    """Symmetric deduplicated CSR (rp int64, col int32) from undirected pairs lo < hi."""
+    import torch
+    key = torch.unique(lo * n + hi)
+    lo, hi = key // n, key % n
+    k2, _ = torch.sort(torch.cat([lo * n + hi, hi * n + lo]))
+    col = (k2 % n).to(torch.int32)
+    cnt = torch.bincount(k2 // n, minlength=n)
+    rp = torch.zeros(n + 1, dtype=torch.int64, device=device)
+    rp[1:] = torch.cumsum(cnt, 0)
+    return rp, col
+
+
+def sbm_torch(n, blocks, d_in, d_out, seed, device="cuda"):
+    """Planted-partition graph built with torch on `device`: vertex i is in
+    block i*blocks//n; each vertex draws Poisson(d_in/2) partners uniformly
+    from its own block and Poisson(d_out/2) uniformly from the whole graph
+    (partners in its own block dropped), so expected intra and inter degrees
+    are about d_in and d_out after symmetrising.  Self loops and repeated
+    pairs are removed.  A fast stand-in for :func:`sbm` (whose per-pair
+    Bernoulli draws run on the host) for batches of many graphs; the two give
+    different graphs.  Returns (rp int64, col int32) on the device."""
+    import torch
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    v = torch.arange(n, dtype=torch.int64, device=device)
+    blk = v * blocks // n
+    start = (blk * n + blocks - 1) // blocks  # first vertex of each vertex's block
+    end = ((blk + 1) * n + blocks - 1) // blocks
+    k_in = torch.poisson(torch.full((n,), d_in / 2.0, device=device), generator=gen).to(torch.int64)
+    src = torch.repeat_interleave(v, k_in)
+    span = (end - start)[src]
+    dst = start[src] + (torch.rand(src.numel(), generator=gen, device=device) * span).to(torch.int64).clamp_max(span - 1)
+    k_out = torch.poisson(torch.full((n,), d_out / 2.0, device=device), generator=gen).to(torch.int64)
+    s2 = torch.repeat_interleave(v, k_out)
+    d2 = torch.randint(0, n, (s2.numel(),), generator=gen, device=device)
+    keep2 = blk[d2] != blk[s2]
+    src = torch.cat([src, s2[keep2]])
+    dst = torch.cat([dst, d2[keep2]])
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    return _csr_from_pairs_torch(torch.minimum(src, dst), torch.maximum(src, dst), n, device)
+
+
 def karate():
     """Zachary's karate club (34 vertices, 78 edges)."""
     edges = [
