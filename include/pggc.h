@@ -132,12 +132,17 @@ typedef void (*pggc_sweep_fn)(void* user, const int64_t* rec, int nrec);
 int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user);
 
 /* Tuning knobs of the HIP path (no counterpart in the reference, whose
- * equivalents are compile-time constants).  Results do not depend on them.
+ * equivalents are compile-time constants).
  *   PGGC_KNOB_HUB_UPDATE_DIV: the hash tables of the highest-degree vertices
  *     are kept current from the movers while a sweep moves at most
  *     n / value vertices and rebuilt otherwise; 0 rebuilds every sweep.
- *     Default 32. */
+ *     Default 32.
+ *   PGGC_KNOB_GNN_MAX_SWEEPS: the active-set predictor runs on at most this
+ *     many sweeps of a local-move call; 0 (default) runs it on every
+ *     eligible sweep.  (It changes which vertices are pruned, so with
+ *     pruning on it changes results.) */
 #define PGGC_KNOB_HUB_UPDATE_DIV 0
+#define PGGC_KNOB_GNN_MAX_SWEEPS 1
 int pggc_ctx_set_knob(pggc_ctx* ctx, int knob, int64_t value);
 int pggc_ctx_get_knob(pggc_ctx* ctx, int knob, int64_t* value);
 

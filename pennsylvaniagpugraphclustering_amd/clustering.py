@@ -84,10 +84,11 @@ class Context:
             out[name] = (n.value, ms.value, by.value)
         return out
 
-    KNOBS = {"hub_update_div": 0}
+    KNOBS = {"hub_update_div": 0, "gnn_max_sweeps": 1}
 
     def set_knob(self, name, value):
-        """Tuning knob of the HIP path (results do not depend on it)."""
+        """Tuning knob of the HIP path (hub_update_div: performance only;
+        gnn_max_sweeps: predicted sweeps per local-move call)."""
         check(self.lib.pggc_ctx_set_knob(self.h, self.KNOBS[name], int(value)))
 
     def get_knob(self, name):

@@ -580,9 +580,10 @@ int pggc_gnn_eval(pggc_ctx* ctx, const pggc_graph* gg, const int32_t* part, cons
 int pggc_ctx_set_knob(pggc_ctx* ctx, int knob, int64_t value) {
     return guard([&] {
         CTX_CHECK(ctx);
-        if (knob != PGGC_KNOB_HUB_UPDATE_DIV) throw Error(PGGC_EINVAL, "unknown knob");
         if (value < 0) throw Error(PGGC_EINVAL, "knob value must be >= 0");
-        ctx->c.hub_update_div = value;
+        if (knob == PGGC_KNOB_HUB_UPDATE_DIV) ctx->c.hub_update_div = value;
+        else if (knob == PGGC_KNOB_GNN_MAX_SWEEPS) ctx->c.gnn_max_sweeps = value;
+        else throw Error(PGGC_EINVAL, "unknown knob");
     });
 }
 
@@ -590,8 +591,9 @@ int pggc_ctx_get_knob(pggc_ctx* ctx, int knob, int64_t* value) {
     return guard([&] {
         CTX_CHECK(ctx);
         if (!value) throw Error(PGGC_EINVAL, "null output");
-        if (knob != PGGC_KNOB_HUB_UPDATE_DIV) throw Error(PGGC_EINVAL, "unknown knob");
-        *value = ctx->c.hub_update_div;
+        if (knob == PGGC_KNOB_HUB_UPDATE_DIV) *value = ctx->c.hub_update_div;
+        else if (knob == PGGC_KNOB_GNN_MAX_SWEEPS) *value = ctx->c.gnn_max_sweeps;
+        else throw Error(PGGC_EINVAL, "unknown knob");
     });
 }
 
@@ -936,6 +938,7 @@ int pggc_cluster_batch(pggc_ctx* ctx, int count, const pggc_graph* graphs, int a
         for (int k = 0; k < S; k++) {
             pggc_ctx* w = ctx->workers[k];
             w->c.hub_update_div = ctx->c.hub_update_div;
+            w->c.gnn_max_sweeps = ctx->c.gnn_max_sweeps;
             if (prune) {
                 w->gnn.set_params(w->c, prm.data());
                 w->gnn.cfg = ctx->gnn.cfg;

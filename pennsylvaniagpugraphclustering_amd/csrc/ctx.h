@@ -69,6 +69,9 @@ struct Ctx {
     // movers while a sweep moves at most n / hub_update_div vertices and
     // rebuilt otherwise (0: always rebuilt)
     i64 hub_update_div = 32;
+    // the active-set predictor runs on at most this many sweeps of a
+    // local-move call (0: on every eligible sweep)
+    i64 gnn_max_sweeps = 0;
     KTimer kt;
 
     void* alloc(size_t bytes);

@@ -1280,17 +1280,21 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
     const bool train = G && G->cfg.mode == GNN_TRAIN;
     if (G) gnn_begin(c, *G, n, train);
     i64 kprev = n;
+    i64 npred = 0;  // predicted sweeps in this call
     for (int x = 0; x < 2; x++) {
         const float fr = SA ? ratios[x] : 0.0f;
         for (int cnt = 0; cnt < limits[x]; cnt++) {
             iter++;
             const float lam = (float)curr.lambda;
-            const bool use = G && !(is_initial && iter == 1) && kprev * 16 >= n;
+            const bool use = G && !(is_initial && iter == 1) && kprev * 16 >= n &&
+                             (c.gnn_max_sweeps <= 0 || npred < c.gnn_max_sweeps);
             const bool full = use && G->cfg.full_every > 0 && iter % G->cfg.full_every == 0;
             if (full) gnn_restore(c, *G, n, s.dest);
-            if (use && !full)
+            if (use && !full) {
                 gnn_predict(c, *G, lp, g, s.part, s.dest, s.gain, curr.vol, iter, iter == 1,
                             G->cfg.mode == GNN_PRUNE, train);
+                npred++;
+            }
             tr.begin(s.dest, iter == 1 && !use);
             score_all(c, s, lp, g, s.part, curr.vol, ccons, !tables, lam, fr, s.dest, s.gain);
             i64 b[2];
