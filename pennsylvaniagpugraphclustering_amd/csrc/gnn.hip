@@ -735,11 +735,11 @@ void gnn_predict(Ctx& c, Gnn& G, LevelPlan& lp, const DGraph& g, const i32* part
         if ((m = st[5] - st[4]) > 0)
             k_gnn_agg_group<64><<<grid_items(m, 4), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[4], m, G.pk,
                                                                         vol, G.in);
-        if ((m = st[7] - st[5]) > 0)
-            k_gnn_agg_wave<<<grid_items(m, 4), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[5], m, G.pk, vol,
+        if ((m = st[CL_HUB] - st[CL_TABLE]) > 0)
+            k_gnn_agg_wave<<<grid_items(m, 4), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[CL_TABLE], m, G.pk, vol,
                                                                    G.in);
-        if ((m = st[8] - st[7]) > 0)
-            k_gnn_agg_block<<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[7], m, G.pk, vol,
+        if ((m = st[CL_HUB + 1] - st[CL_HUB]) > 0)
+            k_gnn_agg_block<<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[CL_HUB], m, G.pk, vol,
                                                                   G.in);
     });
     PGGC_LAUNCH_CHECK();
@@ -764,11 +764,11 @@ void gnn_predict(Ctx& c, Gnn& G, LevelPlan& lp, const DGraph& g, const i32* part
     if ((m = st[5] - st[4]) > 0)
         k_gnn_logit_group<64><<<grid_items(m, 4), 256, 0, c.stream>>>(g.rp, g.col, L + st[4], m, dest, G.s, G.t, tau,
                                                                       pr, cur, G.logit, G.pruned);
-    if ((m = st[7] - st[5]) > 0)
-        k_gnn_logit_wave<<<grid_items(m, 4), 256, 0, c.stream>>>(g.rp, g.col, L + st[5], m, dest, G.s, G.t, tau, pr,
+    if ((m = st[CL_HUB] - st[CL_TABLE]) > 0)
+        k_gnn_logit_wave<<<grid_items(m, 4), 256, 0, c.stream>>>(g.rp, g.col, L + st[CL_TABLE], m, dest, G.s, G.t, tau, pr,
                                                                  cur, G.logit, G.pruned);
-    if ((m = st[8] - st[7]) > 0)
-        k_gnn_logit_block<<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, L + st[7], m, dest, G.s, G.t, tau, pr,
+    if ((m = st[CL_HUB + 1] - st[CL_HUB]) > 0)
+        k_gnn_logit_block<<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, L + st[CL_HUB], m, dest, G.s, G.t, tau, pr,
                                                                 cur, G.logit, G.pruned);
     PGGC_LAUNCH_CHECK();
     if (prune) {
@@ -822,10 +822,10 @@ void gnn_backward(Ctx& c, Gnn& G, LevelPlan& lp, const DGraph& g, const i32* lab
         k_gnn_ds_group<32><<<grid_items(m, 8), 256, 0, c.stream>>>(g.rp, g.col, L + st[3], m, G.r, G.ds);
     if ((m = st[5] - st[4]) > 0)
         k_gnn_ds_group<64><<<grid_items(m, 4), 256, 0, c.stream>>>(g.rp, g.col, L + st[4], m, G.r, G.ds);
-    if ((m = st[7] - st[5]) > 0)
-        k_gnn_ds_wave<<<grid_items(m, 4), 256, 0, c.stream>>>(g.rp, g.col, L + st[5], m, G.r, G.ds);
-    if ((m = st[8] - st[7]) > 0)
-        k_gnn_ds_block<<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, L + st[7], m, G.r, G.ds);
+    if ((m = st[CL_HUB] - st[CL_TABLE]) > 0)
+        k_gnn_ds_wave<<<grid_items(m, 4), 256, 0, c.stream>>>(g.rp, g.col, L + st[CL_TABLE], m, G.r, G.ds);
+    if ((m = st[CL_HUB + 1] - st[CL_HUB]) > 0)
+        k_gnn_ds_block<<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, L + st[CL_HUB], m, G.r, G.ds);
     PGGC_LAUNCH_CHECK();
     const i64 ntiles = (n + 15) / 16;
     const int nb = (int)std::min<i64>(GNN_GRAD_BLOCKS, std::max<i64>(1, (ntiles + 3) / 4));

@@ -56,9 +56,9 @@ void MoveScratch::ensure(Ctx& c, i64 n) {
 }
 
 // ------------------------------------------------------------- level plan
-constexpr int NCLS = 8;
 __host__ __device__ __forceinline__ int score_class(i64 d) {
-    return d <= 4 ? 0 : d <= 8 ? 1 : d <= 16 ? 2 : d <= 32 ? 3 : d <= 64 ? 4 : d <= 512 ? 5 : d <= 2048 ? 6 : 7;
+    return d <= 4 ? 0 : d <= 8 ? 1 : d <= 16 ? 2 : d <= 32 ? 3 : d <= 64 ? 4 : d <= 128 ? 5 : d <= 256 ? 6
+         : d <= 512 ? 7 : d <= 2048 ? 8 : 9;
 }
 constexpr int HCH_E = 2048;  // hub edges per insert work item
 constexpr int HCH_S = 4096;  // hub table slots per scan work item
@@ -169,7 +169,7 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
     k_score_classes<<<grid_items(g.n, 256), 256, 0, c.stream>>>(g.rp, g.n, cls);
     PGGC_LAUNCH_CHECK();
     partition_classes(c, cls, g.n, NCLS, p.sc_list, p.sc_start);
-    i64 m7 = p.sc_start[8] - p.sc_start[7];
+    i64 m7 = p.sc_start[CL_HUB + 1] - p.sc_start[CL_HUB];
     p.gslots = 0;
     if (m7 > 0) {
         p.goff = Buf<i64>(c, m7 + 1);
@@ -177,7 +177,7 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
         p.sitems = Buf<i64>(c, m7 + 1);
         p.hub_pv = Buf<i64>(c, m7);
         p.hub_best = Buf<u64>(c, m7);
-        k_global_sizes<<<grid_items(m7, 256), 256, 0, c.stream>>>(g.rp, p.sc_list.p + p.sc_start[7], m7, p.goff,
+        k_global_sizes<<<grid_items(m7, 256), 256, 0, c.stream>>>(g.rp, p.sc_list.p + p.sc_start[CL_HUB], m7, p.goff,
                                                                         p.items);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, p.goff, p.goff, m7);
@@ -188,7 +188,7 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
         p.gslots = c.read1(p.goff.p + m7);
         p.nitems = c.read1(p.items.p + m7);
         p.nsitems = c.read1(p.sitems.p + m7);
-        const i32* hubs = p.sc_list.p + p.sc_start[7];
+        const i32* hubs = p.sc_list.p + p.sc_start[CL_HUB];
         // packed slots need every hub row weight below 2^32
         p.packed = true;
         if (g.wk != W_UNIT) {
@@ -649,25 +649,35 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
         c.kt_end(tk, 0.0);
         tk = c.kt_begin(KS_SCORE_TABLE);
         if (raw) {
-            if ((m = st[6] - st[5]) > 0)
-                k_score_raw<64><<<grid_blocks(m), 64, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[5], m, part, vol,
-                                                                     cons, lam, ratio, dest, gain);
-            if ((m = st[7] - st[6]) > 0)
-                k_score_raw<256><<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[6], m, part, vol,
-                                                                       cons, lam, ratio, dest, gain);
+            if ((m = st[CL_BIG] - st[CL_TABLE]) > 0)
+                k_score_raw<64><<<grid_blocks(m), 64, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[CL_TABLE], m,
+                                                                     part, vol, cons, lam, ratio, dest, gain);
+            if ((m = st[CL_HUB] - st[CL_BIG]) > 0)
+                k_score_raw<256><<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[CL_BIG], m,
+                                                                       part, vol, cons, lam, ratio, dest, gain);
         } else {
+            // one wave per vertex; the LDS table is sized for the class, so
+            // the many low-degree rows keep more waves resident per CU
             if ((m = st[6] - st[5]) > 0)
-                k_score_table<64, 1024, false><<<grid_blocks(m), 64, 0, c.stream>>>(
+                k_score_table<64, 256, false><<<grid_blocks(m), 64, 0, c.stream>>>(
                     g.rp, g.col, W, g.vw, L + st[5], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr,
                     nullptr, nullptr);
             if ((m = st[7] - st[6]) > 0)
-                k_score_table<512, 4096, false><<<grid_blocks(m), 512, 0, c.stream>>>(
+                k_score_table<64, 512, false><<<grid_blocks(m), 64, 0, c.stream>>>(
                     g.rp, g.col, W, g.vw, L + st[6], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr,
+                    nullptr, nullptr);
+            if ((m = st[8] - st[7]) > 0)
+                k_score_table<64, 1024, false><<<grid_blocks(m), 64, 0, c.stream>>>(
+                    g.rp, g.col, W, g.vw, L + st[7], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr,
+                    nullptr, nullptr);
+            if ((m = st[CL_HUB] - st[CL_BIG]) > 0)
+                k_score_table<512, 4096, false><<<grid_blocks(m), 512, 0, c.stream>>>(
+                    g.rp, g.col, W, g.vw, L + st[CL_BIG], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr,
                     nullptr, nullptr);
         }
         c.kt_end(tk, 0.0);
-        if ((m = st[8] - st[7]) > 0) {
-            const i32* hubs = L + st[7];
+        if ((m = st[CL_HUB + 1] - st[CL_HUB]) > 0) {
+            const i32* hubs = L + st[CL_HUB];
             fill_i64(c, lp.hub_pv, m, 0);
             fill_i64(c, (i64*)lp.hub_best.p, m, 0);
             if (raw) {
@@ -1100,7 +1110,7 @@ static void perform_moves(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g
     const i64 nw = std::min(k, n_large);
     const int tk = c.kt_begin(KS_MOVES);
     fill_i64(c, s.scal.p + 3, 1, 0);
-    const i64 m7 = lp.sc_start[8] - lp.sc_start[7];
+    const i64 m7 = lp.sc_start[CL_HUB + 1] - lp.sc_start[CL_HUB];
     // many movers: updating costs more than rebuilding the tables next sweep
     if (lp.hub_live && m7 > 0 && (c.hub_update_div <= 0 || k * c.hub_update_div > g.n)) lp.hub_live = false;
     if (lp.hub_live && m7 > 0) {  // keep the hub tables current (before part changes)
