@@ -87,7 +87,7 @@ void partition_classes(Ctx& c, const i32* cls, i64 n, int K, i32* out, i64* star
         case 3: partition_k<3>(c, cls, n, out, starts); break;
         case 4: partition_k<4>(c, cls, n, out, starts); break;
         case 8: partition_k<8>(c, cls, n, out, starts); break;
-        case 10: partition_k<10>(c, cls, n, out, starts); break;
+        case 11: partition_k<11>(c, cls, n, out, starts); break;
         default: throw Error(-1, "partition_classes: unsupported class count");
     }
 }

@@ -30,9 +30,10 @@ struct MoveStats {
 
 // Scoring classes by degree: 0..4 lane groups (d <= 4, 8, 16, 32, 64),
 // CL_TABLE..CL_BIG-1 one wave per vertex with an LDS table sized for the class
-// (d <= 128, 256, 512), CL_BIG one 512-thread block (d <= 2048), CL_HUB hubs.
-constexpr int NCLS = 10;
-constexpr int CL_TABLE = 5, CL_BIG = 8, CL_HUB = 9;
+// (d <= 128, 256, 512), CL_BIG..CL_HUB-1 one block per vertex (256 threads for
+// d <= 1024, 512 for d <= 2048), CL_HUB hubs.
+constexpr int NCLS = 11;
+constexpr int CL_TABLE = 5, CL_BIG = 8, CL_HUB = 10;
 
 // Per-level cached work partitioning (reused by every local_move on a level).
 struct LevelPlan {
@@ -40,7 +41,7 @@ struct LevelPlan {
     i64 n = -1;
     Orders ord;
     Buf<i32> sc_list;           // vertices grouped by scoring class
-    i64 sc_start[11] = {0};
+    i64 sc_start[NCLS + 1] = {0};
     // hubs (largest degree class): HBM hash tables of neighbour-cluster
     // weights (own cluster included), edge and slot work items per hub.
     // A table is built at the first table sweep of a local-move call and

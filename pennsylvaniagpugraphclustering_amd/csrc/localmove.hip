@@ -58,7 +58,7 @@ void MoveScratch::ensure(Ctx& c, i64 n) {
 // ------------------------------------------------------------- level plan
 __host__ __device__ __forceinline__ int score_class(i64 d) {
     return d <= 4 ? 0 : d <= 8 ? 1 : d <= 16 ? 2 : d <= 32 ? 3 : d <= 64 ? 4 : d <= 128 ? 5 : d <= 256 ? 6
-         : d <= 512 ? 7 : d <= 2048 ? 8 : 9;
+         : d <= 512 ? 7 : d <= 1024 ? 8 : d <= 2048 ? 9 : 10;
 }
 constexpr int HCH_E = 2048;  // hub edges per insert work item
 constexpr int HCH_S = 4096;  // hub table slots per scan work item
@@ -670,9 +670,13 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
                 k_score_table<64, 1024, false><<<grid_blocks(m), 64, 0, c.stream>>>(
                     g.rp, g.col, W, g.vw, L + st[7], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr,
                     nullptr, nullptr);
-            if ((m = st[CL_HUB] - st[CL_BIG]) > 0)
+            if ((m = st[9] - st[8]) > 0)
+                k_score_table<256, 2048, false><<<grid_blocks(m), 256, 0, c.stream>>>(
+                    g.rp, g.col, W, g.vw, L + st[8], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr,
+                    nullptr, nullptr);
+            if ((m = st[10] - st[9]) > 0)
                 k_score_table<512, 4096, false><<<grid_blocks(m), 512, 0, c.stream>>>(
-                    g.rp, g.col, W, g.vw, L + st[CL_BIG], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr,
+                    g.rp, g.col, W, g.vw, L + st[9], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr,
                     nullptr, nullptr);
         }
         c.kt_end(tk, 0.0);
