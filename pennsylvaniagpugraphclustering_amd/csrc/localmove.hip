@@ -60,7 +60,7 @@ __host__ __device__ __forceinline__ int score_class(i64 d) {
     return d <= 4 ? 0 : d <= 8 ? 1 : d <= 16 ? 2 : d <= 32 ? 3 : d <= 64 ? 4 : d <= 128 ? 5 : d <= 256 ? 6
          : d <= 512 ? 7 : d <= 1024 ? 8 : d <= 2048 ? 9 : 10;
 }
-constexpr int HCH_E = 2048;  // hub edges per insert work item
+constexpr int HCH_E = 1024;  // hub edges per insert work item (1024 measured best of 512, 1024, 2048)
 constexpr int HCH_S = 4096;  // hub table slots per scan work item
 constexpr int HLDS = 2 * HCH_E;  // LDS pre-aggregation slots per insert item
 __host__ __device__ __forceinline__ i64 tslots(i64 d) {
