@@ -409,7 +409,7 @@ __global__ void __launch_bounds__(BS) k_cagg(const i64* rp, const i32* col, WT W
 // merges its edges in LDS and then inserts the distinct pairs into the row's
 // HBM hash table; a second pass splits each table into slot items of BCH_S
 // slots and compacts the occupied slots.
-constexpr int BCH_E = 2048;      // member edges per insert item (8 per thread)
+constexpr int BCH_E = 1024;      // member edges per insert item (4 per thread)
 constexpr int BCH_S = 4096;      // table slots per extract item (16 per thread)
 constexpr int BLDS = 2 * BCH_E;  // LDS pre-aggregation slots
 
