@@ -254,13 +254,15 @@ __global__ void k_cscatter(const i32* vcmap, i64 n, i64* pos, i32* members) {
 }
 
 constexpr i64 C_SMALL = 256;   // bound <= C_SMALL: 64-thread block, 512-slot LDS table
+constexpr i64 C_HALF = 1024;   // bound <= C_HALF: 256-thread block, 2048-slot LDS table
 constexpr i64 C_MID = 2048;    // bound <= C_MID: 256-thread block, 4096-slot LDS table
 
+// classes: 0..2 LDS merge (small, half, mid), 3 big rows, 4 empty rows
 __global__ void k_cclass(const i64* bound, i64 nc, i32* cls) {
     const i64 u = thread_item();
     if (u >= nc) return;
     i64 b = bound[u];
-    cls[u] = b == 0 ? 3 : (b <= C_SMALL ? 0 : (b <= C_MID ? 1 : 2));
+    cls[u] = b == 0 ? 4 : (b <= C_SMALL ? 0 : (b <= C_HALF ? 1 : (b <= C_MID ? 2 : 3)));
 }
 
 __host__ __device__ __forceinline__ i64 table_slots(i64 bound, i64 nc) {
@@ -578,10 +580,10 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
     Buf<i32> cls(c, std::max<i64>(nc, 1)), list(c, std::max<i64>(nc, 1));
     k_cclass<<<grid_items(nc, 256), 256, 0, c.stream>>>(bound, nc, cls);
     PGGC_LAUNCH_CHECK();
-    i64 st[5];
-    partition_k<4>(c, cls, nc, list, st);
+    i64 st[6];
+    partition_k<5>(c, cls, nc, list, st);
     fill_i64(c, ccnt, nc + 1, 0);
-    const i64 n0 = st[1] - st[0], n1 = st[2] - st[1], n2 = st[3] - st[2];
+    const i64 n0 = st[1] - st[0], n1 = st[2] - st[1], n1b = st[3] - st[2], n2 = st[4] - st[3];
     int tk = c.kt_begin(KS_CONTRACT_AGG);
     PGGC_WDISPATCH(g, {
         if (n0 > 0)
@@ -589,14 +591,18 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
                 g.rp, g.col, W, vcmap, nc, list.p + st[0], n0, mstart, members, bound, nullptr, nullptr, nullptr,
                 toff, tcol, tw, ccnt);
         if (n1 > 0)
-            k_cagg<256, 4096, false><<<grid_blocks(n1), 256, 0, c.stream>>>(
+            k_cagg<256, 2048, false><<<grid_blocks(n1), 256, 0, c.stream>>>(
                 g.rp, g.col, W, vcmap, nc, list.p + st[1], n1, mstart, members, bound, nullptr, nullptr, nullptr,
+                toff, tcol, tw, ccnt);
+        if (n1b > 0)
+            k_cagg<256, 4096, false><<<grid_blocks(n1b), 256, 0, c.stream>>>(
+                g.rp, g.col, W, vcmap, nc, list.p + st[2], n1b, mstart, members, bound, nullptr, nullptr, nullptr,
                 toff, tcol, tw, ccnt);
     });
     c.kt_end(tk, 0.0);
     PGGC_LAUNCH_CHECK();
     if (n2 > 0) {
-        const i32* big = list.p + st[2];
+        const i32* big = list.p + st[3];
         Buf<i64> goff(c, n2 + 1), items(c, n2 + 1), mdp(c, n + 1);
         k_goff_sizes<<<grid_items(n2, 256), 256, 0, c.stream>>>(big, n2, bound, nc, goff);
         PGGC_LAUNCH_CHECK();
@@ -644,7 +650,7 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
         PGGC_LAUNCH_CHECK();
     }
     if (n2 > 0) {
-        const i32* big = list.p + st[2];
+        const i32* big = list.p + st[3];
         Buf<i64> boff(c, n2 + 1);
         k_big_count<<<grid_items(n2, 256), 256, 0, c.stream>>>(big, n2, out.rp, boff);
         PGGC_LAUNCH_CHECK();
