@@ -526,8 +526,10 @@ __global__ void k_big_count(const i32* list, i64 m, const i64* crp, i64* cnt) {
     i32 u = list[x];
     cnt[x] = crp[u + 1] - crp[u];
 }
-__global__ void k_big_flatten(const i32* list, i64 m, const i64* crp, const i64* boff, const i32* ocol, u64* key,
-                              i32* val, i64* target) {
+// key = (big row index << cb) | column (columns < 2^cb), so the radix sort
+// needs only cb + bits(big rows) bits
+__global__ void k_big_flatten(const i32* list, i64 m, const i64* crp, const i64* boff, const i32* ocol, int cb,
+                              u64* key, i32* val, i64* target) {
     const i64 x = block_id();
     if (x >= m) return;
     {
@@ -535,7 +537,7 @@ __global__ void k_big_flatten(const i32* list, i64 m, const i64* crp, const i64*
         i64 s = crp[u], e = crp[u + 1], b = boff[x];
         for (i64 j = s + threadIdx.x; j < e; j += blockDim.x) {
             i64 f = b + (j - s);
-            key[f] = ((u64)x << 32) | (u32)ocol[j];
+            key[f] = ((u64)x << cb) | (u32)ocol[j];
             val[f] = (i32)f;
             target[f] = j;
         }
@@ -546,10 +548,12 @@ __global__ void k_big_gather(const u64* key, const i32* val, const i64* target, 
     for (i64 f = (i64)blockIdx.x * blockDim.x + threadIdx.x; f < E; f += (i64)gridDim.x * blockDim.x)
         wtmp[f] = ow_in[target[val[f]]];
 }
-__global__ void k_big_scatter(const u64* key, const i64* target, i64 E, const i64* wtmp, i32* ocol, i64* ow) {
+__global__ void k_big_scatter(const u64* key, const i64* target, i64 E, const i64* wtmp, int cb, i32* ocol,
+                              i64* ow) {
+    const u64 mask = ((u64)1 << cb) - 1;
     for (i64 f = (i64)blockIdx.x * blockDim.x + threadIdx.x; f < E; f += (i64)gridDim.x * blockDim.x) {
         i64 j = target[f];
-        ocol[j] = (i32)(key[f] & 0xffffffffu);
+        ocol[j] = (i32)(key[f] & mask);
         ow[j] = wtmp[f];
     }
 }
@@ -660,13 +664,15 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
             Buf<u64> key(c, E);
             Buf<i32> val(c, E);
             Buf<i64> target(c, E), wtmp(c, E);
-            k_big_flatten<<<grid_blocks(n2), 256, 0, c.stream>>>(big, n2, out.rp, boff, out.col, key, val,
+            const int cb = bits_for(nc - 1);
+            k_big_flatten<<<grid_blocks(n2), 256, 0, c.stream>>>(big, n2, out.rp, boff, out.col, cb, key, val,
                                                                          target);
             PGGC_LAUNCH_CHECK();
-            radix_sort_pairs(c, key, val, E, 32 + bits_for(n2));
+            radix_sort_pairs(c, key, val, E, cb + bits_for(n2));
             k_big_gather<<<grid_for(E, 256 * 4, 8192), 256, 0, c.stream>>>(key, val, target, E, out.w, wtmp);
             PGGC_LAUNCH_CHECK();
-            k_big_scatter<<<grid_for(E, 256 * 4, 8192), 256, 0, c.stream>>>(key, target, E, wtmp, out.col, out.w);
+            k_big_scatter<<<grid_for(E, 256 * 4, 8192), 256, 0, c.stream>>>(key, target, E, wtmp, cb, out.col,
+                                                                            out.w);
             PGGC_LAUNCH_CHECK();
         }
     }
