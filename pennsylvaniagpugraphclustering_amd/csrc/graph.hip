@@ -499,20 +499,46 @@ __global__ void __launch_bounds__(256) k_cbig_extract(const i32* big, i64 nbig, 
     }
 }
 
-// Copy temporary rows into the compact coarse CSR.
+// Copy temporary rows into the compact coarse CSR.  Consecutive lanes copy
+// consecutive output entries (EPT steps of EB entries per block), so each
+// wave-wide load and store covers whole cache lines.  The row of an entry is
+// a binary search over the chunk's row starts, staged in LDS as offsets from
+// the chunk start clamped to [0, EC]; a chunk spanning more than EC rows (runs
+// of empty rows) searches the row pointers in global memory instead.  The
+// search interval carries over between steps since entries only move right.
 __global__ void __launch_bounds__(EB) k_crow_copy(const i64* crp, i64 nc, i64 nnzc, const i64* toff, const i32* tcol,
                                                    const i64* tw, i32* ocol, i64* ow) {
+    __shared__ i32 s_st[EC + 1];
     ChunkRows cr;
     chunk_rows(crp, nc, nnzc, cr);
-    i64 j0 = cr.eb + (i64)threadIdx.x * EPT;
-    if (j0 >= cr.ee) return;
-    i64 r = upper_row(crp, cr.rlo, cr.rhi, j0);
-    i64 nxt = crp[r + 1];
+    const i64 nr = cr.rhi - cr.rlo;
+    const bool staged = nr <= EC;
+    if (staged) {
+        for (int i = threadIdx.x; i <= (int)nr; i += EB) {
+            const i64 v = crp[cr.rlo + i] - cr.eb;
+            s_st[i] = (i32)(v < 0 ? 0 : (v > EC ? EC : v));
+        }
+    }
+    __syncthreads();
+    int lo = 0;
+    i64 r = cr.rlo;
     for (int k = 0; k < EPT; k++) {
-        i64 j = j0 + k;
+        const i64 j = cr.eb + (i64)k * EB + threadIdx.x;
         if (j >= cr.ee) break;
-        if (j >= nxt) r = advance_row(crp, r, cr.rhi, j, nxt);
-        i64 src = toff[r] + (j - crp[r]);
+        if (staged) {
+            // s_st[lo] <= o < s_st[hi]: s_st[0] = 0 and s_st[nr] >= ee - eb
+            const int o = (int)(j - cr.eb);
+            int hi = (int)nr;
+            while (hi - lo > 1) {
+                const int m = (lo + hi) >> 1;
+                if (s_st[m] <= o) lo = m;
+                else hi = m;
+            }
+            r = cr.rlo + lo;
+        } else {
+            r = upper_row(crp, r, cr.rhi, j);
+        }
+        const i64 src = toff[r] + (j - crp[r]);
         ocol[j] = tcol[src];
         ow[j] = tw[src];
     }
