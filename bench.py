@@ -48,6 +48,9 @@ def parse():
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 1 GiB read + 1 GiB write elementwise kernel 5 times (the reference point "
                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
+    p.add_argument("--knob", action="append", default=[], metavar="NAME=VALUE",
+                   help="set a tuning knob of the HIP path before the run (hub_update_div, gnn_max_sweeps); "
+                        "repeatable")
     p.add_argument("--leiden", type=int, default=0, metavar="ITERS",
                    help="also run pLeiden+ with ex_iters=ITERS (to stable it creeps up by ~1e-8 per outer "
                         "iteration for hundreds of iterations at scale 24)")
@@ -140,6 +143,10 @@ def main():
     gen_s = time.perf_counter() - t0
     ctx = C.default_context()
     ctx.set_verbose(1)
+    for kv in a.knob:
+        name, value = kv.split("=")
+        ctx.set_knob(name, int(value))
+        print(f"# knob {name} = {ctx.get_knob(name)}", file=sys.stderr, flush=True)
     g = C.DeviceGraph(rp, col, None, None, objective="Mod", ctx=ctx)
     m = g.nnz // 2
     print(f"# graph R-MAT s{a.scale}: n={g.n} nnz={g.nnz} m={m} (generated in {gen_s:.1f}s)", file=sys.stderr,

@@ -145,10 +145,11 @@ __device__ __forceinline__ void lds_agg_flush(const i32* lk, const i64* lv, i32*
         if (key != EMPTY_KEY) table_add_global(gkeys, gvals, gmask, key, w);
     }
 }
-// Batched flush into a packed table: each thread first loads the home slots
-// of all its keys (independent loads in flight together), then adds to the
-// slots that already hold the key (no-return atomics), then claims empty
-// slots with CAS; only keys that meet another key fall back to probing.
+// Batched flush into a packed table: each thread first tries to claim the
+// home slots of all its keys with CAS (independent atomics in flight
+// together; a key new to the table then costs one atomic), then adds to the
+// slots that turned out to hold the key already (no-return atomics); only
+// keys that meet another key fall back to probing.
 template <int BS, int LSLOTS>
 __device__ __forceinline__ void lds_agg_flush_packed(const i32* lk, const i64* lv, u64* tab, u32 gmask) {
     __syncthreads();
@@ -164,20 +165,9 @@ __device__ __forceinline__ void lds_agg_flush_packed(const i32* lk, const i64* l
     }
 #pragma unroll
     for (int r = 0; r < R; r++)
-        cur[r] = key[r] != EMPTY_KEY ? __hip_atomic_load(&tab[h[r]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+        cur[r] = key[r] != EMPTY_KEY ? (u64)atomicCAS((unsigned long long*)&tab[h[r]], (unsigned long long)PEMPTY,
+                                                      (unsigned long long)(((u64)(u32)key[r] << 32) | w[r]))
                                      : PEMPTY;
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        if (key[r] != EMPTY_KEY && (u32)(cur[r] >> 32) == (u32)key[r]) {
-            atomicAdd((unsigned long long*)&tab[h[r]], (unsigned long long)w[r]);
-            key[r] = EMPTY_KEY;
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < R; r++)
-        if (key[r] != EMPTY_KEY && cur[r] == PEMPTY)
-            cur[r] = atomicCAS((unsigned long long*)&tab[h[r]], (unsigned long long)PEMPTY,
-                               (unsigned long long)(((u64)(u32)key[r] << 32) | w[r]));
 #pragma unroll
     for (int r = 0; r < R; r++) {
         if (key[r] == EMPTY_KEY || cur[r] == PEMPTY) continue;  // done, or the CAS claimed the slot

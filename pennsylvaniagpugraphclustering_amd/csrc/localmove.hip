@@ -203,14 +203,12 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
             for (i64 x = 0; x < m7; x++)
                 if (hs[x] >= ((i64)1 << 32) - 1) p.packed = false;
         }
+        // filled at each build (hub_live is false until the first one)
         if (p.packed) {
             p.gtab = Buf<u64>(c, p.gslots);
-            fill_i64(c, (i64*)p.gtab.p, p.gslots, (i64)PEMPTY);
         } else {
             p.gkeys = Buf<i32>(c, p.gslots);
             p.gvals = Buf<i64>(c, p.gslots);
-            fill_i32(c, p.gkeys, p.gslots, EMPTY_KEY);
-            fill_i64(c, p.gvals, p.gslots, 0);
         }
         xcd_maps(c, p, m7);
         p.hidx = Buf<i32>(c, std::max<i64>(g.n, 1));
