@@ -2,6 +2,8 @@
 // compaction, stable LSD radix sort, exact integer reductions.
 #include <algorithm>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "ctx.h"
 #include "device.h"
 
@@ -311,86 +313,22 @@ i64 compact_flags(Ctx& c, const i32* flag, const i32* src, i64 n, i32* out, i64*
 }
 
 // ---------------------------------------------------------------- radix sort
-constexpr int RT = 256;
-constexpr int RR = 8;  // rounds per tile
-constexpr int RTILE = RT * RR;
-
-__global__ void __launch_bounds__(RT) k_radix_hist(const u64* keys, i64 n, int shift, i64* counts, i64 nb) {
-    __shared__ u32 h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    i64 base = (i64)blockIdx.x * RTILE;
-    for (int r = 0; r < RR; r++) {
-        i64 x = base + (i64)r * RT + threadIdx.x;
-        if (x < n) atomicAdd(&h[(keys[x] >> shift) & 255], 1u);
-    }
-    __syncthreads();
-    counts[(i64)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
-}
-
-__global__ void __launch_bounds__(RT) k_radix_scatter(const u64* kin, const i32* vin, u64* kout, i32* vout, i64 n,
-                                                       int shift, const i64* offs, i64 nb) {
-    __shared__ i64 run[256];
-    __shared__ i64 bd[256];
-    __shared__ u32 wc[4][256];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    run[t] = offs[(i64)t * nb + blockIdx.x];
-    const u64 lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    i64 base = (i64)blockIdx.x * RTILE;
-    for (int r = 0; r < RR; r++) {
-        i64 x = base + (i64)r * RT + t;
-        bool valid = x < n;
-        u64 key = valid ? kin[x] : 0;
-        i32 val = valid ? vin[x] : 0;
-        u32 d = (u32)(key >> shift) & 255u;
-        wc[0][t] = 0; wc[1][t] = 0; wc[2][t] = 0; wc[3][t] = 0;
-        __syncthreads();
-        u64 peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            u64 bal = __ballot(((d >> b) & 1u) != 0);
-            peers &= ((d >> b) & 1u) ? bal : ~bal;
-        }
-        u32 rank = (u32)__popcll(peers & lt);
-        if (valid && rank == 0) wc[w][d] = (u32)__popcll(peers);
-        __syncthreads();
-        u32 c0 = wc[0][t], c1 = wc[1][t], c2 = wc[2][t], c3 = wc[3][t];
-        wc[0][t] = 0; wc[1][t] = c0; wc[2][t] = c0 + c1; wc[3][t] = c0 + c1 + c2;
-        bd[t] = run[t];
-        run[t] += (i64)(c0 + c1 + c2 + c3);
-        __syncthreads();
-        if (valid) {
-            i64 dst = bd[d] + wc[w][d] + rank;
-            kout[dst] = key;
-            vout[dst] = val;
-        }
-        __syncthreads();
-    }
-}
-
+// Stable LSD sort of (key, value) pairs on the low `bits` key bits: rocPRIM's
+// onesweep radix sort (a plain library sort, compiled from the ROCm headers),
+// with its temporary storage and the second buffers from the context's pool.
 void radix_sort_pairs(Ctx& c, u64* keys, i32* vals, i64 n, int bits) {
-    if (n <= 1) return;
-    i64 nb = ceil_div(n, RTILE);
+    if (n <= 1 || bits <= 0) return;
     Buf<u64> k2(c, n);
     Buf<i32> v2(c, n);
-    Buf<i64> counts(c, 256 * nb + 1);
-    u64 *ka = keys, *kb = k2;
-    i32 *va = vals, *vb = v2;
-    int passes = (bits + 7) / 8;
-    for (int p = 0; p < passes; p++) {
-        int shift = p * 8;
-        k_radix_hist<<<(int)nb, RT, 0, c.stream>>>(ka, n, shift, counts, nb);
-        PGGC_LAUNCH_CHECK();
-        scan_i64(c, counts, counts, 256 * nb);
-        k_radix_scatter<<<(int)nb, RT, 0, c.stream>>>(ka, va, kb, vb, n, shift, counts, nb);
-        PGGC_LAUNCH_CHECK();
-        std::swap(ka, kb);
-        std::swap(va, vb);
-    }
-    if (ka != keys) {
-        copy_dev(c, keys, ka, n * sizeof(u64));
-        copy_dev(c, vals, va, n * sizeof(i32));
-    }
+    rocprim::double_buffer<u64> kd(keys, k2.p);
+    rocprim::double_buffer<i32> vd(vals, v2.p);
+    const unsigned int end_bit = (unsigned int)std::min(bits, 64);
+    size_t tb = 0;
+    PGGC_HIP(rocprim::radix_sort_pairs(nullptr, tb, kd, vd, (size_t)n, 0u, end_bit, c.stream));
+    Buf<u64> tmp(c, std::max<size_t>((tb + 7) / 8, 1));
+    PGGC_HIP(rocprim::radix_sort_pairs((void*)tmp.p, tb, kd, vd, (size_t)n, 0u, end_bit, c.stream));
+    if (kd.current() != keys) copy_dev(c, keys, kd.current(), n * sizeof(u64));
+    if (vd.current() != vals) copy_dev(c, vals, vd.current(), n * sizeof(i32));
 }
 
 // ------------------------------------------------------------ exact reductions
