@@ -86,6 +86,30 @@ def test_contraction(gpu_ctx, name, frac):
     assert np.array_equal(npy(w), rw)
 
 
+@pytest.mark.parametrize("extra", [0, 3000])
+def test_contraction_with_runs_of_empty_rows(gpu_ctx, extra):
+    """A perfect matching whose pair (2k, 2k+1) maps to clusters 3k and 3k+1
+    leaves every third coarse row empty: a chunk of 4096 coarse entries spans
+    about 6144 rows, more than the row starts the copy stages in LDS, so the
+    copy searches the row offsets in global memory.  `extra` random edges
+    make some rows longer.  Bit-identical to the oracle."""
+    n = 200000
+    rng = np.random.default_rng(5)
+    src = np.concatenate([np.arange(0, n, 2), rng.integers(0, n, extra)])
+    dst = np.concatenate([np.arange(1, n, 2), rng.integers(0, n, extra)])
+    g = graphs.from_edges(n, src, dst)
+    u = np.arange(n)
+    vc = (3 * (u // 2) + (u % 2)).astype(np.int32)
+    nc = 3 * (n // 2)
+    print(f"contract matching n={n} extra={extra} nc={nc}", flush=True)
+    rp, col, w = C().build_coarse_graph(dgraph(g, "Mod"), vc, nc)
+    rrp, rcol, rw = O.contract(g.rp, g.col, None, vc, nc)
+    assert int(np.diff(rrp).astype(bool).sum()) <= 2 * nc // 3 + 2 * extra
+    assert np.array_equal(npy(rp), rrp)
+    assert np.array_equal(npy(col), rcol)
+    assert np.array_equal(npy(w), rw)
+
+
 def test_contraction_of_coarse_graph(gpu_ctx):
     """Contraction input with int64 weights (a coarse level)."""
     g = graphs.random_graph(5000, 12, seed=3, weights=(1, 9))
