@@ -552,13 +552,31 @@ __global__ void __launch_bounds__(256) k_hub_update_wave(const i64* __restrict__
     if (w >= nw) return;
     const i32 i = moves[k - 1 - w];
     const i64 s = rp[i], e = rp[i + 1];
-    if (e - s < LARGE_CUTOFF) return;
+    if (e - s < LARGE_CUTOFF || e - s >= MASSIVE_CUTOFF) return;
     const i32 oi = part[i], ni = dest[i];
     if (ni < 0) return;  // not applied (k_apply_moves)
     for (i64 base = s; base < e; base += 64) {  // wave-uniform loop control
         const i64 j = base + lane;
         if (j < e) hub_update_edge<PACKED>(col, W, hidx, goff, gkeys, gvals, gtab, j, oi, ni, stale);
     }
+}
+// movers of degree >= MASSIVE_CUTOFF (the last nb entries of moves at most):
+// a 1024-thread block each, so one huge row is not a single wave's serial tail
+template <bool PACKED, class WT>
+__global__ void __launch_bounds__(1024) k_hub_update_block(const i64* __restrict__ rp, const i32* __restrict__ col,
+                                                           WT W, const i32* __restrict__ moves, i64 k, i64 nb,
+                                                           const i32* __restrict__ part, const i32* __restrict__ dest,
+                                                           const i32* __restrict__ hidx, const i64* __restrict__ goff,
+                                                           i32* gkeys, i64* gvals, u64* gtab, i64* stale) {
+    const i64 w = block_id();
+    if (w >= nb) return;
+    const i32 i = moves[k - 1 - w];
+    const i64 s = rp[i], e = rp[i + 1];
+    if (e - s < MASSIVE_CUTOFF) return;
+    const i32 oi = part[i], ni = dest[i];
+    if (ni < 0) return;  // not applied (k_apply_moves)
+    for (i64 j = s + threadIdx.x; j < e; j += 1024)
+        hub_update_edge<PACKED>(col, W, hidx, goff, gkeys, gvals, gtab, j, oi, ni, stale);
 }
 
 template <bool PACKED>
@@ -1041,7 +1059,7 @@ __global__ void __launch_bounds__(256) k_delta_uncut_wave(const i64* __restrict_
     const i64 x = k - 1 - w;
     const i32 i = moves[x];
     const i64 s = rp[i], e = rp[i + 1];
-    if (e - s < LARGE_CUTOFF) return;
+    if (e - s < LARGE_CUTOFF || e - s >= MASSIVE_CUTOFF) return;
     const i32 oi = part[i], ni = dest[i];
     i64 acc = 0;
     for (i64 base = s; base < e; base += 64) {  // wave-uniform loop control
@@ -1050,6 +1068,24 @@ __global__ void __launch_bounds__(256) k_delta_uncut_wave(const i64* __restrict_
     }
     acc = wave_sum_i64(acc);
     if (lane == 0) out[x] = acc;
+}
+template <class WT>
+__global__ void __launch_bounds__(1024) k_delta_uncut_block(const i64* __restrict__ rp, const i32* __restrict__ col,
+                                                            WT W, const i32* __restrict__ moves, i64 k, i64 nb,
+                                                            const i32* __restrict__ part,
+                                                            const i32* __restrict__ dest,
+                                                            const u64* __restrict__ mvp, i64* out) {
+    const i64 w = block_id();
+    if (w >= nb) return;
+    const i64 x = k - 1 - w;
+    const i32 i = moves[x];
+    const i64 s = rp[i], e = rp[i + 1];
+    if (e - s < MASSIVE_CUTOFF) return;  // block-uniform
+    const i32 oi = part[i], ni = dest[i];
+    i64 acc = 0;
+    for (i64 j = s + threadIdx.x; j < e; j += 1024) acc += delta_term(col, W, mvp, j, oi, ni);
+    acc = block_sum_i64<1024>(acc);
+    if (threadIdx.x == 0) out[x] = acc;
 }
 
 __global__ void k_apply_moves(const i32* moves, i64 k, const i64* vw, i32* part, const i32* dest, i64* vol) {
@@ -1087,9 +1123,24 @@ __global__ void __launch_bounds__(256) k_reset_affected_wave(const i64* __restri
     if (w >= nw) return;
     const i32 i = moves[k - 1 - w];
     const i64 s = rp[i], e = rp[i + 1];
-    if (e - s < LARGE_CUTOFF) return;
+    if (e - s < LARGE_CUTOFF || e - s >= MASSIVE_CUTOFF) return;
     for (i64 j = s + lane; j < e; j += 64) dest[col[j]] = NULL_PART;
     if (lane == 0) {
+        dest[i] = NULL_PART;
+        mflag[i] = 0;
+    }
+}
+__global__ void __launch_bounds__(1024) k_reset_affected_block(const i64* __restrict__ rp,
+                                                               const i32* __restrict__ col,
+                                                               const i32* __restrict__ moves, i64 k, i64 nb,
+                                                               i32* dest, i32* mflag) {
+    const i64 w = block_id();
+    if (w >= nb) return;
+    const i32 i = moves[k - 1 - w];
+    const i64 s = rp[i], e = rp[i + 1];
+    if (e - s < MASSIVE_CUTOFF) return;
+    for (i64 j = s + threadIdx.x; j < e; j += 1024) dest[col[j]] = NULL_PART;
+    if (threadIdx.x == 0) {
         dest[i] = NULL_PART;
         mflag[i] = 0;
     }
@@ -1105,11 +1156,15 @@ __global__ void k_set_mflag(const i32* moves, i64 k, i32* mflag) {
     mflag[moves[x]] = 1;
 }
 
-// perform_moves (src/local_mover.cpp:1241-1279).  moves is in order1 order;
-// n_large = vertices of the level with degree >= LARGE_CUTOFF.
+// perform_moves (src/local_mover.cpp:1241-1279).  moves is in order1 order,
+// so movers by degree class: below LARGE_CUTOFF (a thread each), below
+// MASSIVE_CUTOFF (a wave each), then the rest (a 1024-thread block each).
+// n_large / n_massive = vertices of the level with degree >= LARGE_CUTOFF /
+// MASSIVE_CUTOFF, which bound the movers of the last two classes.
 static void perform_moves(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, RefineData& curr,
-                          const i32* moves, i64 k, i64 n_large, bool& tables) {
+                          const i32* moves, i64 k, i64 n_large, i64 n_massive, bool& tables) {
     const i64 nw = std::min(k, n_large);
+    const i64 nb = std::min(k, n_massive);
     const int tk = c.kt_begin(KS_MOVES);
     fill_i64(c, s.scal.p + 3, 1, 0);
     const i64 m7 = lp.sc_start[CL_HUB + 1] - lp.sc_start[CL_HUB];
@@ -1125,12 +1180,20 @@ static void perform_moves(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g
                     k_hub_update_wave<true><<<grid_items(nw, 4), 256, 0, c.stream>>>(
                         g.rp, g.col, W, moves, k, nw, s.part, s.dest, lp.hidx, lp.goff, nullptr, nullptr, lp.gtab,
                         stale);
+                if (nb > 0)
+                    k_hub_update_block<true><<<grid_blocks(nb), 1024, 0, c.stream>>>(
+                        g.rp, g.col, W, moves, k, nb, s.part, s.dest, lp.hidx, lp.goff, nullptr, nullptr, lp.gtab,
+                        stale);
             } else {
                 k_hub_update_thread<false><<<grid_items(k, 256), 256, 0, c.stream>>>(
                     g.rp, g.col, W, moves, k, s.part, s.dest, lp.hidx, lp.goff, lp.gkeys, lp.gvals, nullptr, stale);
                 if (nw > 0)
                     k_hub_update_wave<false><<<grid_items(nw, 4), 256, 0, c.stream>>>(
                         g.rp, g.col, W, moves, k, nw, s.part, s.dest, lp.hidx, lp.goff, lp.gkeys, lp.gvals, nullptr,
+                        stale);
+                if (nb > 0)
+                    k_hub_update_block<false><<<grid_blocks(nb), 1024, 0, c.stream>>>(
+                        g.rp, g.col, W, moves, k, nb, s.part, s.dest, lp.hidx, lp.goff, lp.gkeys, lp.gvals, nullptr,
                         stale);
             }
         });
@@ -1145,6 +1208,9 @@ static void perform_moves(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g
                                                                           s.mvp, du);
             if (nw > 0)
                 k_delta_uncut_wave<<<grid_items(nw, 4), 256, 0, c.stream>>>(g.rp, g.col, W, moves, k, nw, s.part,
+                                                                           s.dest, s.mvp, du);
+            if (nb > 0)
+                k_delta_uncut_block<<<grid_blocks(nb), 1024, 0, c.stream>>>(g.rp, g.col, W, moves, k, nb, s.part,
                                                                            s.dest, s.mvp, du);
         });
         PGGC_LAUNCH_CHECK();
@@ -1161,6 +1227,9 @@ static void perform_moves(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g
         k_reset_affected_thread<<<grid_items(k, 256), 256, 0, c.stream>>>(g.rp, g.col, moves, k, s.dest, s.mflag);
         if (nw > 0)
             k_reset_affected_wave<<<grid_items(nw, 4), 256, 0, c.stream>>>(g.rp, g.col, moves, k, nw, s.dest,
+                                                                           s.mflag);
+        if (nb > 0)
+            k_reset_affected_block<<<grid_blocks(nb), 1024, 0, c.stream>>>(g.rp, g.col, moves, k, nb, s.dest,
                                                                            s.mflag);
     }
     PGGC_LAUNCH_CHECK();
@@ -1335,7 +1404,7 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
                 break;
             }
             if (iter > 1 || !is_initial) set_new_cluster_ids(c, s, g, curr, s.moves, km, constrained ? cons : nullptr);
-            perform_moves(c, s, lp, g, curr, s.moves, km, g.n - lp.ord.off_large, tables);
+            perform_moves(c, s, lp, g, curr, s.moves, km, g.n - lp.ord.off_large, g.n - lp.ord.off_large2, tables);
             if (stats) stats->moves += km;
             if (curr.obj > best.obj) {
                 copy_state(c, best, curr);
@@ -1454,7 +1523,7 @@ void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i
         }
         k_set_mflag<<<grid_items(tr, 256), 256, 0, c.stream>>>(s.moves, tr, s.mflag);
         PGGC_LAUNCH_CHECK();
-        perform_moves(c, s, lp, g, curr, s.moves, tr, g.n - lp.ord.off_large, tables);
+        perform_moves(c, s, lp, g, curr, s.moves, tr, g.n - lp.ord.off_large, g.n - lp.ord.off_large2, tables);
         if (stats) stats->moves += tr;
         if (curr.obj > best.obj) {
             copy_state(c, best, curr);
