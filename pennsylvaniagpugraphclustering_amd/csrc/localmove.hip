@@ -817,7 +817,9 @@ __global__ void __launch_bounds__(256) k_ab_serial(const i64* __restrict__ rp, c
     const float igain = me.gain;
     const i32 hi = (i32)xorshift32((u32)i);
     float acc = 0.0f;
-    for (i64 j = rp[i]; j < rp[i + 1]; j++) ab_term(col, W, info, j, p, best, multi, igain, hi, acc);
+    const i64 e = rp[i + 1];
+#pragma unroll 4  // independent neighbour gathers in flight; the adds keep their order
+    for (i64 j = rp[i]; j < e; j++) ab_term(col, W, info, j, p, best, multi, igain, hi, acc);
     const int kp = (igain + acc >= 0.0f) ? 1 : 0;
     keep[x] = kp;
     mflag[i] = kp;  // candidates start with mflag 0: an unconditional store
@@ -839,6 +841,7 @@ __global__ void __launch_bounds__(256) k_ab_wave(const i64* __restrict__ rp, con
         const i32 hi = (i32)xorshift32((u32)i);
         float acc = 0.0f;
         const i64 e = rp[i + 1];
+#pragma unroll 4
         for (i64 base = rp[i]; base < e; base += 64) {  // wave-uniform loop control
             const i64 j = base + lane;
             if (j < e) ab_term(col, W, info, j, p, best, multi, igain, hi, acc);
@@ -1045,6 +1048,7 @@ __global__ void __launch_bounds__(256) k_delta_uncut_thread(const i64* __restric
     if (e - s >= LARGE_CUTOFF) return;
     const i32 oi = part[i], ni = dest[i];
     i64 acc = 0;
+#pragma unroll 4
     for (i64 j = s; j < e; j++) acc += delta_term(col, W, mvp, j, oi, ni);
     out[x] = acc;
 }
