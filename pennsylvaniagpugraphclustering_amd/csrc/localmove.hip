@@ -335,7 +335,34 @@ __global__ void __launch_bounds__(BS) k_score_table(const i64* __restrict__ rp, 
         __syncthreads();
         const i32 p = part[i];
         i64 pv = 0;
-        for (i64 eb = 0; eb < d; eb += BS) {  // block-uniform loop control
+        // LDS classes hold d <= SLOTS / 2: a thread first gathers the
+        // clusters of all its (at most R) edges, so the gathers are in flight
+        // together, then inserts them
+        constexpr int R = GLOBAL ? 0 : (SLOTS / 2 + BS - 1) / BS;
+        i64 e_done = 0;
+        if (!GLOBAL) {
+            i32 kk[R > 0 ? R : 1];
+            i64 wv[R > 0 ? R : 1];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const i64 e = (i64)r * BS + t;
+                kk[r] = EMPTY_KEY;
+                wv[r] = 0;
+                if (e < d) {
+                    const i32 v = col[s + e];
+                    kk[r] = raw ? v : part[v];
+                    wv[r] = W(s + e);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                if (kk[r] == EMPTY_KEY) continue;
+                if (!raw && kk[r] == p) pv += wv[r];
+                else table_add_lds(keys, vals, mask, kk[r], wv[r]);
+            }
+            e_done = (i64)R * BS;
+        }
+        for (i64 eb = e_done; eb < d; eb += BS) {  // block-uniform loop control
             const i64 e = eb + t;
             if (e >= d) continue;
             i32 v = col[s + e];
@@ -539,6 +566,7 @@ __global__ void __launch_bounds__(256) k_hub_update_thread(const i64* __restrict
     if (e - s >= LARGE_CUTOFF) return;
     const i32 oi = part[i], ni = dest[i];
     if (ni < 0) return;  // not applied (k_apply_moves)
+#pragma unroll 4
     for (i64 j = s; j < e; j++) hub_update_edge<PACKED>(col, W, hidx, goff, gkeys, gvals, gtab, j, oi, ni, stale);
 }
 template <bool PACKED, class WT>
@@ -1115,6 +1143,7 @@ __global__ void __launch_bounds__(256) k_reset_affected_thread(const i64* __rest
     const i32 i = moves[x];
     const i64 s = rp[i], e = rp[i + 1];
     if (e - s >= LARGE_CUTOFF) return;
+#pragma unroll 4
     for (i64 j = s; j < e; j++) dest[col[j]] = NULL_PART;
     dest[i] = NULL_PART;
     mflag[i] = 0;
