@@ -320,20 +320,34 @@ __global__ void __launch_bounds__(BS) k_cagg(const i64* rp, const i32* col, WT W
             pre[t] = ex;
             rs[t] = st;
             __syncthreads();
-            for (i64 eb = 0; eb < tot; eb += BS) {  // block-uniform loop control
-                const i64 e = eb + t;
-                if (e >= tot) continue;
-                int lo = 0, hi = BS;
-                while (hi - lo > 1) {
-                    int md = (lo + hi) >> 1;
-                    if (pre[md] <= e) lo = md;
-                    else hi = md;
+            // groups of CR edges per thread: locate and gather all of them
+            // (independent loads in flight together), then insert
+            constexpr int CR = 4;
+            for (i64 eb = 0; eb < tot; eb += (i64)BS * CR) {  // block-uniform loop control
+                i32 cv[CR];
+                i64 cw[CR];
+#pragma unroll
+                for (int r = 0; r < CR; r++) {
+                    const i64 e = eb + (i64)r * BS + t;
+                    cv[r] = u;  // no entry
+                    cw[r] = 0;
+                    if (e < tot) {
+                        int lo = 0, hi = BS;
+                        while (hi - lo > 1) {
+                            int md = (lo + hi) >> 1;
+                            if (pre[md] <= e) lo = md;
+                            else hi = md;
+                        }
+                        const i64 j = rs[lo] + (e - pre[lo]);
+                        cv[r] = vcmap[col[j]];
+                        cw[r] = W(j);
+                    }
                 }
-                i64 j = rs[lo] + (e - pre[lo]);
-                i32 v = vcmap[col[j]];
-                if (v != u) {
-                    if (GLOBAL) table_add_global(keys, vals, mask, v, W(j));
-                    else table_add_lds(keys, vals, mask, v, W(j));
+#pragma unroll
+                for (int r = 0; r < CR; r++) {
+                    if (cv[r] == u) continue;
+                    if (GLOBAL) table_add_global(keys, vals, mask, cv[r], cw[r]);
+                    else table_add_lds(keys, vals, mask, cv[r], cw[r]);
                 }
             }
             __syncthreads();
