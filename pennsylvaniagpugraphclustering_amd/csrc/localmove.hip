@@ -68,6 +68,13 @@ __host__ __device__ __forceinline__ i64 tslots(i64 d) {
     while (s < 2 * d) s <<= 1;
     return s;
 }
+// HBM hub tables: power of two >= 1.5 d, so at most 2/3 full (a hub has at
+// most d distinct neighbour clusters); every probe loop ends at an empty slot
+__host__ __device__ __forceinline__ i64 hub_slots(i64 d) {
+    i64 s = 2;
+    while (2 * s < 3 * d) s <<= 1;
+    return s;
+}
 
 __global__ void k_score_classes(const i64* rp, i64 n, i32* cls) {
     const i64 i = thread_item();
@@ -79,7 +86,7 @@ __global__ void k_global_sizes(const i64* rp, const i32* list, i64 m, i64* sz, i
     if (x >= m) return;
     i32 i = list[x];
     i64 d = rp[i + 1] - rp[i];
-    sz[x] = tslots(d);
+    sz[x] = hub_slots(d);
     items[x] = (d + HCH_E - 1) / HCH_E;
 }
 __global__ void k_slot_items(const i64* goff, i64 m, i64* sitems) {
