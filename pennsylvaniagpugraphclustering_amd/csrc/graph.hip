@@ -447,7 +447,7 @@ template <class WT>
 __global__ void __launch_bounds__(256) k_cbig_insert(const i64* rp, const i32* col, WT W, const i32* vcmap,
                                                       const i32* members, const i64* mstart, const i64* mdp,
                                                       const i32* big, i64 nbig, const i64* items, const i64* goff,
-                                                      i32* gkeys, i64* gvals) {
+                                                      i32* gkeys, i64* gvals, u64* gtab) {
     const i64 b = block_id();
     if (b >= items[nbig]) return;
     const i64 ui = upper_row(items, 0, nbig, b);
@@ -474,12 +474,15 @@ __global__ void __launch_bounds__(256) k_cbig_insert(const i64* rp, const i32* c
             if (cv != u) table_add_lds(lk, lv, BLDS - 1, cv, W(j));
         }
     }
-    lds_agg_flush<256, BLDS>(lk, lv, gkeys + goff[ui], gvals + goff[ui], (u32)(goff[ui + 1] - goff[ui] - 1));
+    const u32 gmask = (u32)(goff[ui + 1] - goff[ui] - 1);
+    if (gtab) lds_agg_flush_packed<256, BLDS>(lk, lv, gtab + goff[ui], gmask);  // kernel-uniform
+    else lds_agg_flush<256, BLDS>(lk, lv, gkeys + goff[ui], gvals + goff[ui], gmask);
 }
 
+// gtab non-null: packed tables (key << 32 | weight), else gkeys / gvals
 __global__ void __launch_bounds__(256) k_cbig_extract(const i32* big, i64 nbig, const i64* items, const i64* goff,
-                                                       const i32* gkeys, const i64* gvals, const i64* toff,
-                                                       i32* tcol, i64* tw, i64* ccnt) {
+                                                       const i32* gkeys, const i64* gvals, const u64* gtab,
+                                                       const i64* toff, i32* tcol, i64* tw, i64* ccnt) {
     __shared__ int s_cnt;
     __shared__ i64 s_base;
     const i64 b = block_id();
@@ -496,7 +499,12 @@ __global__ void __launch_bounds__(256) k_cbig_extract(const i32* big, i64 nbig, 
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const i64 sl = s_lo + threadIdx.x + (i64)q * 256;
-        kk[q] = sl < s_hi ? gkeys[sl] : EMPTY_KEY;
+        if (gtab) {
+            const u64 e = sl < s_hi ? gtab[sl] : PEMPTY;
+            kk[q] = e == PEMPTY ? EMPTY_KEY : (i32)(e >> 32);
+        } else {
+            kk[q] = sl < s_hi ? gkeys[sl] : EMPTY_KEY;
+        }
         pos[q] = kk[q] != EMPTY_KEY ? atomicAdd(&s_cnt, 1) : -1;
     }
     __syncthreads();
@@ -508,7 +516,7 @@ __global__ void __launch_bounds__(256) k_cbig_extract(const i32* big, i64 nbig, 
         if (pos[q] >= 0) {
             const i64 sl = s_lo + threadIdx.x + (i64)q * 256;
             tcol[base + pos[q]] = kk[q];
-            tw[base + pos[q]] = gvals[sl];
+            tw[base + pos[q]] = gtab ? (i64)(gtab[sl] & 0xffffffffull) : gvals[sl];
         }
     }
 }
@@ -652,10 +660,19 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
         PGGC_LAUNCH_CHECK();
         scan_i64(c, goff, goff, n2);
         const i64 gs = c.read1(goff.p + n2);
-        Buf<i32> gkeys(c, gs);
-        Buf<i64> gvals(c, gs);
-        fill_i32(c, gkeys, gs, EMPTY_KEY);
-        fill_i64(c, gvals, gs, 0);
+        // packed tables (one CAS per new key) when every coarse weight fits
+        // 32 bits: with unit weights a coarse weight counts fine edges
+        const bool packed = g.wk == W_UNIT && g.nnz < ((i64)1 << 32) - 1;
+        Buf<i32> gkeys(c, packed ? 1 : gs);
+        Buf<i64> gvals(c, packed ? 1 : gs);
+        Buf<u64> gtab(c, packed ? gs : 1);
+        if (packed) {
+            fill_i64(c, (i64*)gtab.p, gs, (i64)PEMPTY);
+        } else {
+            fill_i32(c, gkeys, gs, EMPTY_KEY);
+            fill_i64(c, gvals, gs, 0);
+        }
+        u64* gt = packed ? gtab.p : nullptr;
         k_member_degs<<<grid_items(n, 256), 256, 0, c.stream>>>(members, g.rp, n, mdp);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, mdp, mdp, n);
@@ -666,14 +683,16 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
         tk = c.kt_begin(KS_CONTRACT_BIG);
         if (I > 0) {
             PGGC_WDISPATCH(g, (k_cbig_insert<<<grid_blocks(I), 256, 0, c.stream>>>(
-                                  g.rp, g.col, W, vcmap, members, mstart, mdp, big, n2, items, goff, gkeys, gvals)));
+                                  g.rp, g.col, W, vcmap, members, mstart, mdp, big, n2, items, goff, gkeys, gvals,
+                                  gt)));
             PGGC_LAUNCH_CHECK();
         }
         k_items<<<grid_items(n2, 256), 256, 0, c.stream>>>(big, n2, bound, goff, nc, 1, items);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, items, items, n2);
         const i64 I2 = c.read1(items.p + n2);
-        k_cbig_extract<<<grid_blocks(I2), 256, 0, c.stream>>>(big, n2, items, goff, gkeys, gvals, toff, tcol, tw, ccnt);
+        k_cbig_extract<<<grid_blocks(I2), 256, 0, c.stream>>>(big, n2, items, goff, gkeys, gvals, gt, toff, tcol, tw,
+                                                              ccnt);
         c.kt_end(tk, 0.0);
         PGGC_LAUNCH_CHECK();
     }
