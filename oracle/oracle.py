@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# PGGC_ORACLE_LIB selects another build of the oracle (the tests of the
+# pinning tests load variants with a planted mistake this way)
+_LIB_PATH = os.environ.get("PGGC_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -55,9 +57,34 @@ def lib():
         L.or_local_move.restype = c.c_int64
         L.or_local_move.argtypes = [c.c_int64, i64p, i32p, vp, vp, i64p, i32p, i64p, c.c_int64, i64p,
                                     c.c_double, c.c_int, c.c_int, c.c_int, ctypes.POINTER(c.c_double)]
+        L.or_set_threads.restype = None
+        L.or_set_threads.argtypes = [c.c_int]
+        L.or_get_threads.restype = c.c_int
+        L.or_get_threads.argtypes = []
+        u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+        L.or_philox.restype = None
+        L.or_philox.argtypes = [u32p, u32p, u32p]
+        L.or_rmat.restype = c.c_void_p
+        L.or_rmat.argtypes = [c.c_int, c.c_int, c.c_double, c.c_double, c.c_double, c.c_uint64, c.c_int32,
+                              c.c_int32]
+        L.or_graph_info.restype = None
+        L.or_graph_info.argtypes = [c.c_void_p, c.POINTER(c.c_int64), c.POINTER(c.c_int64), c.POINTER(c.c_int)]
+        L.or_graph_copy.restype = None
+        L.or_graph_copy.argtypes = [c.c_void_p, i64p, i32p, vp]
+        L.or_graph_free.restype = None
+        L.or_graph_free.argtypes = [c.c_void_p]
+        L.or_random_labels.restype = None
+        L.or_random_labels.argtypes = [c.c_int64, c.c_int64, c.c_uint64, i32p]
+        L.or_sweep_detail.restype = c.c_int64
+        L.or_sweep_detail.argtypes = L.or_sweep.argtypes + [i32p, f32p]
+        L.or_local_move_trace.restype = c.c_int64
+        L.or_local_move_trace.argtypes = L.or_local_move.argtypes + [
+            np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS"), c.c_int64, i64p]
         L.or_leiden_refine.restype = c.c_int64
         L.or_leiden_refine.argtypes = [c.c_int64, i64p, i32p, vp, vp, i64p, i32p, i64p, c.c_int64, c.c_double,
                                        c.c_int, c.c_uint64, c.c_uint64, i32p]
+        L.or_leiden_refine_detail.restype = c.c_int64
+        L.or_leiden_refine_detail.argtypes = L.or_leiden_refine.argtypes + [i32p, i32p]
         _lib = L
     return _lib
 
@@ -176,3 +203,92 @@ def leiden_refine(rp, col, w, vw, louv, vol, lam, uniform, seed, draw0=0):
     nc = lib().or_leiden_refine(n, rp, col, w32, w64, vw, louv, vol, len(vol), lam, int(uniform), seed, draw0,
                                 out)
     return out, int(nc)
+
+
+def set_threads(k):
+    """OpenMP threads of the oracle's per-vertex loops (results do not depend
+    on it)."""
+    lib().or_set_threads(int(k))
+
+
+def get_threads():
+    return int(lib().or_get_threads())
+
+
+def philox(ctr, key):
+    """Philox4x32-10 of a 4-word counter and 2-word key."""
+    out = np.empty(4, np.uint32)
+    lib().or_philox(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)
+    return out
+
+
+def rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=2, weights=None):
+    """Counter-based R-MAT graph (Philox4x32-10 draws), symmetrised,
+    deduplicated, without self loops, rows sorted: (rp, col, w or None)."""
+    wmin, wmax = (0, 0) if weights is None else weights
+    L = lib()
+    h = L.or_rmat(scale, edge_factor, a, b, c, seed, wmin, wmax)
+    try:
+        n, nnz, wd = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        L.or_graph_info(h, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(wd))
+        rp = np.empty(n.value + 1, np.int64)
+        col = np.empty(nnz.value, np.int32)
+        w = np.empty(nnz.value, np.int32) if wd.value else None
+        L.or_graph_copy(h, rp, col, _ptr(w))
+    finally:
+        L.or_graph_free(h)
+    return rp, col, w
+
+
+def random_labels(n, L, seed):
+    """labels[i] = Philox4x32-10(counter (i, 0, 0, 2), key seed)[0] mod L."""
+    out = np.empty(n, np.int32)
+    lib().or_random_labels(n, L, seed, out)
+    return out
+
+
+def sweep_detail(rp, col, w, vw, part, vol, lam, ratio, raw, dest, gain, constraint=None):
+    """sweep() plus the candidate list and each candidate's afterburner change."""
+    n = len(rp) - 1
+    dest = dest.copy()
+    gain = gain.copy()
+    moves = np.empty(max(n, 1), np.int32)
+    cands = np.empty(max(n, 1), np.int32)
+    change = np.empty(max(n, 1), np.float32)
+    nc = np.zeros(1, np.int64)
+    w32, w64 = _w(w)
+    k = lib().or_sweep_detail(n, rp, col, w32, w64, vw, part, vol, len(vol), lam, ratio, int(raw),
+                              _ptr(constraint), dest, gain, moves, nc, cands, change)
+    m = int(nc[0])
+    return moves[:k].copy(), dest, gain, cands[:m].copy(), change[:m].copy()
+
+
+def local_move_trace(rp, col, w, vw, part, vol, L, uncut, lam, obj, is_initial, strict=False, SA=True):
+    """local_move() plus its per-sweep records (ratio, candidates, moves,
+    objective after the sweep) as an array of shape (sweeps, 4)."""
+    n = len(rp) - 1
+    part = part.copy()
+    v = np.zeros(n, np.int64)
+    v[:L] = vol[:L]
+    u = np.array([uncut], np.int64)
+    o = ctypes.c_double(obj)
+    tr = np.zeros(4 * 64, np.float64)
+    nt = np.zeros(1, np.int64)
+    w32, w64 = _w(w)
+    L2 = lib().or_local_move_trace(n, rp, col, w32, w64, vw, part, v, L, u, lam, int(is_initial), int(strict),
+                                   int(SA), ctypes.byref(o), tr, 64, nt)
+    k = min(int(nt[0]), 64)
+    return part, v[:L2].copy(), int(L2), int(u[0]), o.value, tr[:4 * k].reshape(k, 4)
+
+
+def leiden_refine_detail(rp, col, w, vw, louv, vol, lam, uniform, seed, draw0=0):
+    """leiden_refine() plus the forest before the gamma breaks (root per
+    vertex) and the order its trees are sorted by."""
+    n = len(rp) - 1
+    out = np.empty(n, np.int32)
+    trees = np.empty(n, np.int32)
+    order = np.empty(n, np.int32)
+    w32, w64 = _w(w)
+    nc = lib().or_leiden_refine_detail(n, rp, col, w32, w64, vw, louv, vol, len(vol), lam, int(uniform), seed,
+                                       draw0, out, trees, order)
+    return out, int(nc), trees, order

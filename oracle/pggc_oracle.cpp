@@ -1,6 +1,13 @@
-// PGGC CPU oracle: a serial restatement of the reference's Louvain/Leiden hot
-// path, used ONLY as test infrastructure (parity checker and CPU baseline).
+// PGGC CPU oracle: a restatement of the reference's Louvain/Leiden hot path,
+// used ONLY as test infrastructure (parity checker and CPU baseline).
 // Nothing in the product library links or calls this file.
+//
+// Threads: or_set_threads(k) runs the per-vertex loops (scoring, afterburner,
+// uncut delta, contraction rows, evaluator, leidenR neighbour choice) on k
+// OpenMP threads.  Every parallel loop writes per-vertex results or reduces
+// integers, and every order-dependent step (candidate lists, label
+// assignment, FP32 folds) stays sequential, so results are bit-identical for
+// every thread count.
 //
 // Every function cites the reference file:line whose behaviour it restates.
 // Where the reference is nondeterministic (CAS-built hash tables, team
@@ -25,6 +32,13 @@
 #include <numeric>
 #include <unordered_map>
 #include <vector>
+
+#include <omp.h>
+#include <parallel/algorithm>
+
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0  // tests build variants with one planted mistake each
+#endif
 
 namespace orc {
 
@@ -85,6 +99,62 @@ static inline u32 seed_draw(u64 seed, u64 k) {
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z ^= z >> 31;
     return (u32)z;
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11): ten rounds of two 32x32->64 multiplies with the
+// published multipliers and Weyl key increments.  The synthetic graphs are
+// drawn from it so that the CPU and the GPU build identical graphs.
+struct P4 {
+    u32 v[4];
+};
+static inline P4 philox4x32_10(P4 c, u32 k0, u32 k1) {
+    for (int r = 0; r < 10; r++) {
+        const u64 p0 = (u64)0xD2511F53u * c.v[0];
+        const u64 p1 = (u64)0xCD9E8D57u * c.v[2];
+        P4 o;
+        o.v[0] = (u32)(p1 >> 32) ^ c.v[1] ^ k0;
+        o.v[1] = (u32)p1;
+        o.v[2] = (u32)(p0 >> 32) ^ c.v[3] ^ k1;
+        o.v[3] = (u32)p0;
+        c = o;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// Streams of the generators (counter word 3).
+constexpr u32 STREAM_RMAT = 0, STREAM_WEIGHT = 1, STREAM_LABEL = 2;
+
+// R-MAT quadrant thresholds on a u32 draw: r < t[0] -> (0,0), < t[1] ->
+// (0,1), < t[2] -> (1,0), else (1,1) (Chakrabarti, Zhan, Faloutsos, SDM'04).
+static void rmat_thresholds(double a, double b, double c, u64 t[3]) {
+    t[0] = (u64)std::floor(a * 4294967296.0);
+    t[1] = (u64)std::floor((a + b) * 4294967296.0);
+    t[2] = (u64)std::floor((a + b + c) * 4294967296.0);
+}
+// Edge draw e: one Philox block per 4 levels, counter (e lo, e hi, block,
+// stream 0), key (seed lo, seed hi); level l (MSB first) uses word l % 4.
+static inline void rmat_edge(u64 e, int scale, const u64 t[3], u64 seed, u32& src, u32& dst) {
+    u32 s = 0, d = 0;
+    P4 blk{};
+    for (int l = 0; l < scale; l++) {
+        if ((l & 3) == 0) blk = philox4x32_10(P4{{(u32)e, (u32)(e >> 32), (u32)(l >> 2), STREAM_RMAT}},
+                                              (u32)seed, (u32)(seed >> 32));
+        const u64 r = blk.v[l & 3];
+        const u32 bs = r >= t[1];
+        const u32 bd = (r >= t[0] && r < t[1]) || r >= t[2];
+        s = (s << 1) | bs;
+        d = (d << 1) | bd;
+    }
+    src = s;
+    dst = d;
+}
+// weight of the undirected pair lo < hi: uniform in [wmin, wmax]
+static inline i32 pair_weight(u32 lo, u32 hi, u64 seed, i32 wmin, i32 wmax) {
+    const P4 r = philox4x32_10(P4{{lo, hi, 0u, STREAM_WEIGHT}}, (u32)seed, (u32)(seed >> 32));
+    return wmin + (i32)(r.v[0] % (u32)(wmax - wmin + 1));
 }
 
 struct Graph {
@@ -180,13 +250,15 @@ static Orders make_orders(const Graph& g) {
 // app/verify_objective.cpp:64-89 + interface/cluster_data.hpp:76-87
 static void evaluate(const Graph& g, const i32* labels, i64 L, std::vector<i64>& vol, i64& uncut) {
     vol.assign(L, 0);
-    uncut = 0;
+    i64 u = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : u)
     for (i64 i = 0; i < g.n; i++) {
         i32 l = labels[i];
         for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++)
-            if (labels[g.col[j]] == l) uncut += g.wt(j);
-        vol[l] += g.vw[i];
+            if (labels[g.col[j]] == l) u += g.wt(j);
     }
+    uncut = u;
+    for (i64 i = 0; i < g.n; i++) vol[labels[i]] += g.vw[i];
 }
 
 // ------------------------------------------------------------- A10 contraction
@@ -207,28 +279,45 @@ static Graph contract(const Graph& g, const std::vector<i32>& vcmap, i64 nc) {
     c.n = nc;
     c.uniform = false;
     c.rp.assign(nc + 1, 0);
-    std::vector<i64> acc(nc, 0);
-    std::vector<i32> touched;
-    std::vector<char> seen(nc, 0);
-    for (i64 u = 0; u < nc; u++) {
-        touched.clear();
-        for (i64 k = cnt[u]; k < cnt[u + 1]; k++) {
-            i32 i = members[k];
-            for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++) {
-                i32 v = vcmap[g.col[j]];
-                if (v == u) continue;
-                if (!seen[v]) { seen[v] = 1; touched.push_back(v); }
-                acc[v] += g.wt(j);
+    // pass 0 counts the distinct coarse neighbours of every coarse row, pass 1
+    // writes them sorted with their summed weights
+    for (int pass = 0; pass < 2; pass++) {
+#pragma omp parallel
+        {
+            std::vector<i64> acc(nc, 0);
+            std::vector<i32> touched;
+            std::vector<char> seen(nc, 0);
+#pragma omp for schedule(dynamic, 256)
+            for (i64 u = 0; u < nc; u++) {
+                touched.clear();
+                for (i64 k = cnt[u]; k < cnt[u + 1]; k++) {
+                    i32 i = members[k];
+                    for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++) {
+                        i32 v = vcmap[g.col[j]];
+                        if (v == u) continue;
+                        if (!seen[v]) { seen[v] = 1; touched.push_back(v); }
+                        acc[v] += g.wt(j);
+                    }
+                }
+                if (pass == 0) {
+                    c.rp[u + 1] = (i64)touched.size();
+                } else {
+                    std::sort(touched.begin(), touched.end());
+                    i64 o = c.rp[u];
+                    for (i32 v : touched) {
+                        c.col[o] = v;
+                        c.w[o] = acc[v];
+                        o++;
+                    }
+                }
+                for (i32 v : touched) { acc[v] = 0; seen[v] = 0; }
             }
         }
-        std::sort(touched.begin(), touched.end());
-        for (i32 v : touched) {
-            c.col.push_back(v);
-            c.w.push_back(acc[v]);
-            acc[v] = 0;
-            seen[v] = 0;
+        if (pass == 0) {
+            for (i64 u = 0; u < nc; u++) c.rp[u + 1] += c.rp[u];
+            c.col.resize(c.rp[nc]);
+            c.w.resize(c.rp[nc]);
         }
-        c.rp[u + 1] = (i64)c.col.size();
     }
     return c;
 }
@@ -315,9 +404,14 @@ static void score_vertex(Mover& M, State& st, i64 i, float lam, float ratio, boo
 // vertex, then list the ones with dest != NO_MOVE in order1 order.
 static std::vector<i32> candidates(Mover& M, State& st, float ratio, bool constrained, const i32* constraint) {
     float lam = (float)st.lambda;
-    Agg agg;
-    agg.ensure(std::max<size_t>(st.vol.size(), (size_t)M.g.n) + 1);
-    for (i64 i = 0; i < M.g.n; i++) score_vertex(M, st, i, lam, ratio, constrained, constraint, agg);
+    const size_t cap = std::max<size_t>(st.vol.size(), (size_t)M.g.n) + 1;
+#pragma omp parallel
+    {
+        static thread_local Agg agg;
+        agg.ensure(cap);
+#pragma omp for schedule(dynamic, 1024)
+        for (i64 i = 0; i < M.g.n; i++) score_vertex(M, st, i, lam, ratio, constrained, constraint, agg);
+    }
     std::vector<i32> out;
     for (i64 x = 0; x < M.g.n; x++) {
         i32 i = M.ord.o1[x];
@@ -337,14 +431,26 @@ static inline void ab_term(const Mover& M, i64 j, i32 p, i32 best, float multi,
     const Graph& g = M.g;
     i32 v = g.col[j];
     float vg = M.gain[v];
-    if ((vg - igain) >= AFTERBURNER_EPS ||
-        (std::fabs(vg - igain) < AFTERBURNER_EPS && (i32)xorshift32((u32)v) < hi)) {
+#if ORC_MUTANT == 1  // planted mistake: unsigned hash comparison
+    const bool lower = xorshift32((u32)v) < (u32)hi;
+#else
+    const bool lower = (i32)xorshift32((u32)v) < hi;
+#endif
+    if ((vg - igain) >= AFTERBURNER_EPS || (std::fabs(vg - igain) < AFTERBURNER_EPS && lower)) {
         i32 vpart = M.dest[v];
         float q = fmaf(-multi, (float)g.vw[v], (float)g.wt(j));
+#if ORC_MUTANT == 2  // planted mistake: dest and part terms of p swapped
+        acc = acc + ((vpart == p) ? q : 0.0f);
+#else
         acc = acc - ((vpart == p) ? q : 0.0f);
+#endif
         acc = acc + ((vpart == best && best != NEW_PART) ? q : 0.0f);
         vpart = M.part[v];
+#if ORC_MUTANT == 2
+        acc = acc - ((vpart == p) ? q : 0.0f);
+#else
         acc = acc + ((vpart == p) ? q : 0.0f);
+#endif
         acc = acc - ((vpart == best) ? q : 0.0f);
     }
 }
@@ -376,13 +482,17 @@ static float ab_change(const Mover& M, i64 i, float lam) {
     return tree_fold(M, i, p, best, multi, igain, hi, d < MASSIVE_CUTOFF ? 64 : 1024);
 }
 
-static std::vector<i32> afterburner(Mover& M, State& st, const std::vector<i32>& cands) {
+static std::vector<i32> afterburner(Mover& M, State& st, const std::vector<i32>& cands,
+                                    std::vector<float>* change_out = nullptr) {
     float lam = (float)st.lambda;
+    const i64 k = (i64)cands.size();
+    std::vector<float> ch(k);
+#pragma omp parallel for schedule(dynamic, 512)
+    for (i64 x = 0; x < k; x++) ch[x] = ab_change(M, cands[x], lam);
     std::vector<i32> out;
-    for (i32 i : cands) {
-        float ch = ab_change(M, i, lam);
-        if (M.gain[i] + ch >= 0) out.push_back(i);  // :332-334
-    }
+    for (i64 x = 0; x < k; x++)
+        if (M.gain[cands[x]] + ch[x] >= 0) out.push_back(cands[x]);  // :332-334
+    if (change_out) *change_out = std::move(ch);
     return out;
 }
 
@@ -394,7 +504,8 @@ static void set_new_cluster_ids(Mover& M, State& st, const std::vector<i32>& mov
     if (nv.empty()) return;
     i32 curr = st.label_count;
     std::vector<char> has(curr, 0);
-    for (i64 i = 0; i < M.g.n; i++) has[M.part[i]] = 1;
+#pragma omp parallel for schedule(static)
+    for (i64 i = 0; i < M.g.n; i++) __atomic_store_n(&has[M.part[i]], (char)1, __ATOMIC_RELAXED);
     i64 k = 0;
     for (i32 l = 0; l < curr && k < (i64)nv.size(); l++) {
         if (!has[l]) {
@@ -428,7 +539,10 @@ static void perform_moves(Mover& M, State& st, const std::vector<i32>& moves) {
     std::vector<char> mv(g.n, 0);
     for (i32 i : moves) mv[i] = 1;
     i64 delta = 0;
-    for (i32 i : moves) {
+    const i64 k = (i64)moves.size();
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : delta)
+    for (i64 x = 0; x < k; x++) {
+        const i32 i = moves[x];
         i32 oi = M.part[i], ni = M.dest[i];
         for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++) {
             i32 v = g.col[j];
@@ -446,9 +560,12 @@ static void perform_moves(Mover& M, State& st, const std::vector<i32>& moves) {
         M.tables = true;
         std::fill(M.dest.begin(), M.dest.end(), NULL_PART);
     } else {
-        for (i32 i : moves) {
-            M.dest[i] = NULL_PART;
-            for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++) M.dest[g.col[j]] = NULL_PART;
+#pragma omp parallel for schedule(dynamic, 256)
+        for (i64 x = 0; x < k; x++) {
+            const i32 i = moves[x];
+            __atomic_store_n(&M.dest[i], NULL_PART, __ATOMIC_RELAXED);
+            for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++)
+                __atomic_store_n(&M.dest[g.col[j]], NULL_PART, __ATOMIC_RELAXED);
         }
     }
     st.uncut += delta;
@@ -477,6 +594,19 @@ struct Stats {
     i64 sweeps = 0, moves = 0, cands = 0;
 };
 
+// Per-sweep record of local_move / local_move_strict, kept while a test asks
+// for it: ratio, candidates, accepted moves, and the objective after the
+// sweep (the current state, not the best one).
+struct SweepRec {
+    float ratio;
+    i64 cands, moves;
+    double obj;
+};
+static std::vector<SweepRec>* g_trace = nullptr;
+static void trace_sweep(float ratio, i64 cands, i64 moves, double obj) {
+    if (g_trace) g_trace->push_back(SweepRec{ratio, cands, moves, obj});
+}
+
 // local_move (src/local_mover.cpp:1360-1417)
 static void local_move(const Graph& g, std::vector<i32>& best_part, State& best, bool is_initial, bool constrained,
                        i32* constraint, bool SA, Stats* stats) {
@@ -487,7 +617,11 @@ static void local_move(const Graph& g, std::vector<i32>& best_part, State& best,
     M.dest.assign(g.n, NULL_PART);
     M.gain.assign(g.n, 0.0f);
     M.tables = !is_initial;
+#if ORC_MUTANT == 3  // planted mistake: second filter ratio
+    const float ratios[2] = {0.75f, 0.5f};
+#else
     const float ratios[2] = {0.75f, 0.25f};
+#endif
     const int limits[2] = {4, 2};
     int iter = 0;
     for (int x = 0; x < 2; x++) {
@@ -496,13 +630,24 @@ static void local_move(const Graph& g, std::vector<i32>& best_part, State& best,
             iter++;
             std::vector<i32> cands = candidates(M, curr, fr, constrained, constraint);
             if (stats) { stats->sweeps++; stats->cands += (i64)cands.size(); }
-            if (cands.empty()) break;
+            if (cands.empty()) {
+                trace_sweep(fr, 0, 0, curr.obj);
+                break;
+            }
             std::vector<i32> moves = afterburner(M, curr, cands);
-            if (moves.empty()) break;
+            if (moves.empty()) {
+                trace_sweep(fr, (i64)cands.size(), 0, curr.obj);
+                break;
+            }
             if (iter > 1 || !is_initial) set_new_cluster_ids(M, curr, moves, constrained, constraint);
             perform_moves(M, curr, moves);
             if (stats) stats->moves += (i64)moves.size();
+            trace_sweep(fr, (i64)cands.size(), (i64)moves.size(), curr.obj);
+#if ORC_MUTANT == 4  // planted mistake: the last state is kept, not the best
+            if (true) {
+#else
             if (curr.obj > best.obj) {
+#endif
                 best.copy_from(curr);
                 best_part = M.part;
             }
@@ -600,6 +745,7 @@ static void local_move_strict(const Graph& g, std::vector<i32>& best_part, State
 // src/local_mover.cpp:1354-1358, consumed by src/leidenR.cpp:279)
 static std::vector<i64> own_weight(const Graph& g, const std::vector<i32>& labels) {
     std::vector<i64> pv(g.n, 0);
+#pragma omp parallel for schedule(dynamic, 4096)
     for (i64 i = 0; i < g.n; i++)
         for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++)
             if (labels[g.col[j]] == labels[i]) pv[i] += g.wt(j);
@@ -642,7 +788,7 @@ static void ensure_gamma(const Graph& g, std::vector<i32>& vcmap, const std::vec
     i64 n = g.n;
     std::vector<i32> ids(n);
     std::iota(ids.begin(), ids.end(), 0);
-    std::sort(ids.begin(), ids.end(), [&](i32 a, i32 b) {
+    __gnu_parallel::sort(ids.begin(), ids.end(), [&](i32 a, i32 b) {
         if (vcmap[a] != vcmap[b]) return vcmap[a] < vcmap[b];
         if (order[a] != order[b]) return order[a] < order[b];
         return a < b;
@@ -656,6 +802,7 @@ static void ensure_gamma(const Graph& g, std::vector<i32>& vcmap, const std::vec
         for (i64 x = 0; x < n; x++) { tsize[x] = run; run += g.vw[ids[x]]; }
     }
     std::vector<i64> inner(n, 0);
+#pragma omp parallel for schedule(dynamic, 4096)
     for (i64 i = 0; i < n; i++) {
         i64 r = 0;
         for (i64 j = g.rp[i]; j < g.rp[i + 1]; j++) {
@@ -669,7 +816,11 @@ static void ensure_gamma(const Graph& g, std::vector<i32>& vcmap, const std::vec
         i32 i = ids[x];
         i32 c = vcmap[i];
         i64 prev = tsize[x] - tsize[cbeg[c]];
+#if ORC_MUTANT == 5  // planted mistake: the vertex's own weight left out of the inner test
+        if ((float)inner[i] < gamma * (float)prev) breakers[c] = std::min(breakers[c], x);
+#else
         if ((float)inner[i] < gamma * (float)g.vw[i] * (float)prev) breakers[c] = std::min(breakers[c], x);
+#endif
     }
     std::vector<i64> outer(n);
     {
@@ -688,7 +839,11 @@ static void ensure_gamma(const Graph& g, std::vector<i32>& vcmap, const std::vec
         i64 other = total_deg[constraint[i]] - cur;
         i64 out = outer[i];
         if (cb > 0) out -= outer[ids[cb - 1]];
+#if ORC_MUTANT == 6  // planted mistake: the outer test breaks before the vertex, not after it
+        if ((float)out < gamma * (float)cur * (float)other) breakers[c] = std::min(breakers[c], x);
+#else
         if ((float)out < gamma * (float)cur * (float)other) breakers[c] = std::min(breakers[c], x + 1);
+#endif
     }
     for (i64 x = 0; x < n; x++) {
         i32 i = ids[x];
@@ -708,11 +863,19 @@ static i64 contigitize(std::vector<i32>& vcmap, i64 n) {
 
 // coarsen_leidenR (src/leidenR.cpp:263-500); `uniform` selects the unit-weight
 // neighbour rule used only on the top level of unit-weight graphs (:173).
+// Trees and random order of the last leiden_refine call, kept while a test
+// asks for them.
+struct RefineDetail {
+    std::vector<i32> trees, order;
+};
+static RefineDetail* g_refine = nullptr;
+
 static std::vector<i32> leiden_refine(const Graph& g, const std::vector<i32>& constraint, const State& rfd,
                                       const std::vector<i64>& pvals, bool uniform, Rng& rng, i64& nc) {
     i64 n = g.n;
     float gamma = (float)rfd.lambda;
     std::vector<i32> vcmap(n, ORD_MAX), hn(n), well(n);
+#pragma omp parallel for schedule(static)
     for (i64 i = 0; i < n; i++) {  // :281-287
         i64 wd = g.vw[i];
         well[i] = ((float)pvals[i] >= gamma * (float)wd * (float)(rfd.vol[constraint[i]] - wd)) ? 1 : 0;
@@ -720,6 +883,7 @@ static std::vector<i32> leiden_refine(const Graph& g, const std::vector<i32>& co
     if (uniform) {
         // :296-393: first satisfactory neighbour cyclically from a hashed start
         i32 seed = (i32)rng.draw();
+#pragma omp parallel for schedule(dynamic, 4096)
         for (i64 i = 0; i < n; i++) {
             i64 s = g.rp[i], e = g.rp[i + 1], width = e - s;
             float multi = gamma * (float)g.vw[i];
@@ -747,6 +911,7 @@ static std::vector<i32> leiden_refine(const Graph& g, const std::vector<i32>& co
         // degree >= 128 keeps the first maximum (MaxFirstLoc team path).
         // (w - ...)^4 is evaluated as ((double)val^2)^2 in double.
         i32 seed = (i32)rng.draw();
+#pragma omp parallel for schedule(dynamic, 4096)
         for (i64 i = 0; i < n; i++) {
             i64 s = g.rp[i], e = g.rp[i + 1];
             float multi = gamma * (float)g.vw[i];
@@ -771,6 +936,7 @@ static std::vector<i32> leiden_refine(const Graph& g, const std::vector<i32>& co
     std::vector<i32> order(n);
     {
         i32 seed = (i32)rng.draw();
+#pragma omp parallel for schedule(static)
         for (i64 i = 0; i < n; i++) {  // :482-486
             i32 o = (i32)murmur3_i32((i32)((u32)i + (u32)seed));
             i32 r = o % SPLIT;
@@ -780,6 +946,10 @@ static std::vector<i32> leiden_refine(const Graph& g, const std::vector<i32>& co
     for (i64 i = 0; i < n; i++)  // :487-494
         if (order[i] == order[hn[i]]) hn[i] = (i32)i;
     find_trees(vcmap, n, hn, order);
+    if (g_refine) {
+        g_refine->trees = vcmap;
+        g_refine->order = order;
+    }
     ensure_gamma(g, vcmap, constraint, order, rfd.vol, pvals, gamma);
     nc = contigitize(vcmap, n);
     return vcmap;
@@ -910,6 +1080,52 @@ static std::vector<i32> louvain_part(const Graph& top, State& rfd, bool constrai
     return parts[0];
 }
 
+// Symmetric CSR of an R-MAT draw: 2^scale vertices, edge_factor * 2^scale
+// draws, self loops dropped, repeated pairs merged, both directions stored,
+// rows sorted; weights (wmax > 0) are drawn per undirected pair.
+static Graph build_rmat(int scale, int edge_factor, double a, double b, double c, u64 seed, i32 wmin, i32 wmax) {
+    const i64 n = (i64)1 << scale;
+    const i64 m = (i64)edge_factor << scale;
+    u64 t[3];
+    rmat_thresholds(a, b, c, t);
+    std::vector<u64> key(m);
+#pragma omp parallel for schedule(static)
+    for (i64 e = 0; e < m; e++) {
+        u32 s, d;
+        rmat_edge((u64)e, scale, t, seed, s, d);
+        const u32 lo = std::min(s, d), hi = std::max(s, d);
+        key[e] = lo == hi ? ~(u64)0 : ((u64)lo << 32) | hi;
+    }
+    __gnu_parallel::sort(key.begin(), key.end());
+    i64 u = 0;
+    for (i64 e = 0; e < m; e++) {
+        if (key[e] == ~(u64)0) break;
+        if (u == 0 || key[e] != key[u - 1]) key[u++] = key[e];
+    }
+    key.resize(u);
+    Graph g;
+    g.n = n;
+    g.rp.assign(n + 1, 0);
+    for (u64 k : key) {
+        g.rp[(k >> 32) + 1]++;
+        g.rp[(u32)k + 1]++;
+    }
+    for (i64 i = 0; i < n; i++) g.rp[i + 1] += g.rp[i];
+    g.col.resize(g.rp[n]);
+    g.uniform = wmax <= 0;
+    if (!g.uniform) g.w.resize(g.rp[n]);
+    // pairs in (lo, hi) order reach every row in increasing column order
+    std::vector<i64> pos(g.rp.begin(), g.rp.end() - 1);
+    for (u64 k : key) {
+        const u32 lo = (u32)(k >> 32), hi = (u32)k;
+        const i64 a0 = pos[lo]++, a1 = pos[hi]++;
+        g.col[a0] = (i32)hi;
+        g.col[a1] = (i32)lo;
+        if (!g.uniform) g.w[a0] = g.w[a1] = pair_weight(lo, hi, seed, wmin, wmax);
+    }
+    return g;
+}
+
 }  // namespace orc
 
 // =====================================================================
@@ -930,6 +1146,42 @@ static Graph make_graph(i64 n, const i64* rp, const i32* col, const i32* w32, co
 }
 
 extern "C" {
+
+void or_set_threads(int k) { omp_set_num_threads(k > 0 ? k : 1); }
+int or_get_threads(void) { return omp_get_max_threads(); }
+
+void or_philox(const u32* ctr, const u32* key, u32* out) {
+    P4 r = philox4x32_10(P4{{ctr[0], ctr[1], ctr[2], ctr[3]}}, key[0], key[1]);
+    std::copy(r.v, r.v + 4, out);
+}
+
+// R-MAT graph held by the oracle until or_graph_free
+void* or_rmat(int scale, int edge_factor, double a, double b, double c, u64 seed, i32 wmin, i32 wmax) {
+    return new Graph(build_rmat(scale, edge_factor, a, b, c, seed, wmin, wmax));
+}
+void or_graph_info(void* h, i64* n, i64* nnz, int* weighted) {
+    const Graph* g = (const Graph*)h;
+    *n = g->n;
+    *nnz = g->nnz();
+    *weighted = g->uniform ? 0 : 1;
+}
+void or_graph_copy(void* h, i64* rp, i32* col, i32* w) {
+    const Graph* g = (const Graph*)h;
+    std::copy(g->rp.begin(), g->rp.end(), rp);
+    std::copy(g->col.begin(), g->col.end(), col);
+    if (w && !g->uniform)
+        for (i64 j = 0; j < g->nnz(); j++) w[j] = (i32)g->w[j];
+}
+void or_graph_free(void* h) { delete (Graph*)h; }
+
+// labels[i] = Philox(counter (i lo, i hi, 0, stream 2), key seed) word 0 mod L
+void or_random_labels(i64 n, i64 L, u64 seed, i32* out) {
+#pragma omp parallel for schedule(static)
+    for (i64 i = 0; i < n; i++) {
+        P4 r = philox4x32_10(P4{{(u32)i, (u32)((u64)i >> 32), 0u, STREAM_LABEL}}, (u32)seed, (u32)(seed >> 32));
+        out[i] = (i32)(r.v[0] % (u32)L);
+    }
+}
 
 u32 or_murmur3_i32(i32 key) { return murmur3_i32(key); }
 u32 or_xorshift32(u32 x) { return xorshift32(x); }
@@ -978,9 +1230,23 @@ i64 or_contract(i64 n, const i64* rp, const i32* col, const i32* w32, const i64*
 // in/out: returns the number of accepted moves, written (order1 order) to
 // moves_out; dest/gain are updated in place.  raw != 0 scores the raw
 // adjacency (the first sweep of an initial call).
+i64 or_sweep_detail(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w64, const i64* vw,
+                    const i32* part, const i64* vol, i64 L, double lambda, float ratio, int raw, const i32* constraint,
+                    i32* dest, float* gain, i32* moves_out, i64* ncand_out, i32* cands_out, float* change_out);
+
 i64 or_sweep(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w64, const i64* vw, const i32* part,
              const i64* vol, i64 L, double lambda, float ratio, int raw, const i32* constraint, i32* dest, float* gain,
              i32* moves_out, i64* ncand_out) {
+    return or_sweep_detail(n, rp, col, w32, w64, vw, part, vol, L, lambda, ratio, raw, constraint, dest, gain,
+                           moves_out, ncand_out, nullptr, nullptr);
+}
+
+// or_sweep, also returning the candidate list (order1 order) and each
+// candidate's afterburner change (the sum over its priority neighbours,
+// src/local_mover.cpp:314-331) in cands_out / change_out (n entries each)
+i64 or_sweep_detail(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w64, const i64* vw,
+                    const i32* part, const i64* vol, i64 L, double lambda, float ratio, int raw, const i32* constraint,
+                    i32* dest, float* gain, i32* moves_out, i64* ncand_out, i32* cands_out, float* change_out) {
     Graph g = make_graph(n, rp, col, w32, w64, vw);
     State st;
     st.vol.assign(vol, vol + L);
@@ -992,7 +1258,10 @@ i64 or_sweep(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w6
     M.gain.assign(gain, gain + n);
     M.tables = !raw;
     std::vector<i32> c = candidates(M, st, ratio, constraint != nullptr, constraint);
-    std::vector<i32> mv = c.empty() ? std::vector<i32>() : afterburner(M, st, c);
+    std::vector<float> ch;
+    std::vector<i32> mv = c.empty() ? std::vector<i32>() : afterburner(M, st, c, &ch);
+    if (cands_out) std::copy(c.begin(), c.end(), cands_out);
+    if (change_out) std::copy(ch.begin(), ch.end(), change_out);
     std::copy(M.dest.begin(), M.dest.end(), dest);
     std::copy(M.gain.begin(), M.gain.end(), gain);
     std::copy(mv.begin(), mv.end(), moves_out);
@@ -1086,9 +1355,24 @@ double or_cluster(i64 n, const i64* rp, const i32* col, const i32* w32, const i6
 
 // local_move on one level from a given state (for parity of A8 on its own)
 // vol_inout sized >= n; returns label_count; part_inout relabelled.
+i64 or_local_move_trace(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w64, const i64* vw,
+                        i32* part_inout, i64* vol_inout, i64 L, i64* uncut_inout, double lambda, int is_initial,
+                        int strict, int SA, double* obj_inout, double* trace_out, i64 trace_cap, i64* ntrace);
+
 i64 or_local_move(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w64, const i64* vw,
                   i32* part_inout, i64* vol_inout, i64 L, i64* uncut_inout, double lambda, int is_initial,
                   int strict, int SA, double* obj_inout) {
+    return or_local_move_trace(n, rp, col, w32, w64, vw, part_inout, vol_inout, L, uncut_inout, lambda, is_initial,
+                               strict, SA, obj_inout, nullptr, 0, nullptr);
+}
+
+// or_local_move with its per-sweep records: trace_out[4*k..4*k+3] = ratio,
+// candidates, accepted moves, objective after sweep k (at most trace_cap)
+i64 or_local_move_trace(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w64, const i64* vw,
+                        i32* part_inout, i64* vol_inout, i64 L, i64* uncut_inout, double lambda, int is_initial,
+                        int strict, int SA, double* obj_inout, double* trace_out, i64 trace_cap, i64* ntrace) {
+    std::vector<SweepRec> tr;
+    g_trace = trace_out ? &tr : nullptr;
     Graph g = make_graph(n, rp, col, w32, w64, vw);
     State st;
     st.vol.assign(vol_inout, vol_inout + n);
@@ -1100,6 +1384,17 @@ i64 or_local_move(i64 n, const i64* rp, const i32* col, const i32* w32, const i6
     std::vector<i32> part(part_inout, part_inout + n);
     if (strict) local_move_strict(g, part, st, is_initial != 0, false, nullptr, nullptr);
     else local_move(g, part, st, is_initial != 0, false, nullptr, SA != 0, nullptr);
+    g_trace = nullptr;
+    if (trace_out) {
+        const i64 k = std::min<i64>((i64)tr.size(), trace_cap);
+        for (i64 x = 0; x < k; x++) {
+            trace_out[4 * x] = tr[x].ratio;
+            trace_out[4 * x + 1] = (double)tr[x].cands;
+            trace_out[4 * x + 2] = (double)tr[x].moves;
+            trace_out[4 * x + 3] = tr[x].obj;
+        }
+        *ntrace = (i64)tr.size();
+    }
     std::copy(part.begin(), part.end(), part_inout);
     std::copy(st.vol.begin(), st.vol.begin() + st.label_count, vol_inout);
     *uncut_inout = st.uncut;
@@ -1108,9 +1403,25 @@ i64 or_local_move(i64 n, const i64* rp, const i32* col, const i32* w32, const i6
 }
 
 // leidenR on one level: louv = Louvain labels (constraint), vol[L] its volumes
+i64 or_leiden_refine_detail(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w64, const i64* vw,
+                            const i32* louv, const i64* vol, i64 L, double lambda, int uniform, u64 seed, u64 draw0,
+                            i32* vcmap_out, i32* trees_out, i32* order_out);
+
 i64 or_leiden_refine(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w64, const i64* vw,
                      const i32* louv, const i64* vol, i64 L, double lambda, int uniform, u64 seed, u64 draw0,
                      i32* vcmap_out) {
+    return or_leiden_refine_detail(n, rp, col, w32, w64, vw, louv, vol, L, lambda, uniform, seed, draw0, vcmap_out,
+                                   nullptr, nullptr);
+}
+
+// or_leiden_refine, also returning the forest before the gamma breaks
+// (trees_out[i] = root of i's tree, find_trees, src/leidenR.cpp:212-260) and
+// the order it is sorted by (order_out, after find_trees' flip)
+i64 or_leiden_refine_detail(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w64, const i64* vw,
+                            const i32* louv, const i64* vol, i64 L, double lambda, int uniform, u64 seed, u64 draw0,
+                            i32* vcmap_out, i32* trees_out, i32* order_out) {
+    RefineDetail det;
+    g_refine = trees_out ? &det : nullptr;
     Graph g = make_graph(n, rp, col, w32, w64, vw);
     State st;
     st.vol.assign(vol, vol + L);
@@ -1121,6 +1432,11 @@ i64 or_leiden_refine(i64 n, const i64* rp, const i32* col, const i32* w32, const
     Rng rng{seed, draw0};
     i64 nc = 0;
     std::vector<i32> m = leiden_refine(g, lv, st, pv, uniform != 0, rng, nc);
+    g_refine = nullptr;
+    if (trees_out) {
+        std::copy(det.trees.begin(), det.trees.end(), trees_out);
+        std::copy(det.order.begin(), det.order.end(), order_out);
+    }
     std::copy(m.begin(), m.end(), vcmap_out);
     return nc;
 }
