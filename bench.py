@@ -1,20 +1,31 @@
 """Benchmark: end-to-end pLouvain to stable (ex_iters = -1) on the R-MAT
-scale-24 graph (edge factor 16, a/b/c/d = .57/.19/.19/.05, symmetrised,
-deduplicated, no self loops, unit weights, int64 row offsets, int32 columns,
-seed 2), one MI355X.
+scale-24 graph (edge factor 16, a/b/c/d = .57/.19/.19/.05, seed 2,
+symmetrised, deduplicated, no self loops, unit weights, int64 row offsets,
+int32 columns), one MI355X.  The graph is built on the GPU by the library's
+counter-based generator; the CPU oracle builds the identical graph.
 
 One step = one full clustering run (multilevel local moving, contraction and
 the ex_iters loop) from the device-resident CSR to the device-resident
 clustering.  value = undirected edges / seconds per step.
 
-Prints one JSON line.  The cpu_baseline leg times the CPU oracle (a serial
-restatement, 1 core) on a bounded sample (an R-MAT graph of smaller scale)
-and reports its edges/s.
+Also reported on the same graph:
+- edges_per_s_per_sweep: m / the time of the first full top-level sweep
+  (the table sweep after the raw sweep of the first local-move call);
+- cpu_baseline: the oracle (the OpenMP CPU restatement, all host threads) on
+  the same graph copied from HBM, timed on the same full top-level sweep
+  (its call stopped after two sweeps), value m / that time;
+- roofline of the kernel group that takes the most time, from HIP events
+  around its launches and the algorithmic bytes the library counts for it;
+- the pLeiden+ leg (--leiden ITERS, a bounded ex_iters), the active-set
+  predictor A/B leg and the batch leg.
+
+Prints one JSON line.
 """
 import argparse
-import ctypes
+import hashlib
 import json
 import os
+import platform
 import sys
 import time
 
@@ -22,11 +33,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-
-# HBM bytes per launch of a timed region, measured with rocprofv3 FETCH_SIZE
-# and WRITE_SIZE counters scaled by a plain copy kernel of known size
-# (profiles/rmat24_afterburner_traffic.md); keyed by (region, R-MAT scale)
-MEASURED_TRAFFIC = {("afterburner", 24): 12.81e9}
+GOLDEN = os.path.join(ROOT, "tests", "golden", "rmat{scale}_seed2.json")
+TRAFFIC = os.path.join(ROOT, "profiles", "traffic_rmat{scale}.json")
 
 
 def parse():
@@ -37,9 +45,8 @@ def parse():
     p.add_argument("--scale", type=int, default=24)
     p.add_argument("--algo", default="louvain", choices=["louvain", "leiden", "leiden+"])
     p.add_argument("--ex-iters", type=int, default=-1)
-    p.add_argument("--cpu-scale", type=int, default=16)
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--no-leiden", action="store_true", help="(default) skip the pLeiden+ report")
+    p.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: all host threads, at most 16)")
     p.add_argument("--no-gnn", action="store_true", help="skip the active-set predictor A/B step")
     p.add_argument("--batch", type=int, default=512,
                    help="batch leg: this many planted-partition graphs (n log-uniform in [2^10, 2^18]), pLouvain "
@@ -49,32 +56,79 @@ def parse():
                    help="only run a plain 1 GiB read + 1 GiB write elementwise kernel 5 times (the reference point "
                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
     p.add_argument("--knob", action="append", default=[], metavar="NAME=VALUE",
-                   help="set a tuning knob of the HIP path before the run (hub_update_div, gnn_max_sweeps); "
-                        "repeatable")
-    p.add_argument("--leiden", type=int, default=0, metavar="ITERS",
-                   help="also run pLeiden+ with ex_iters=ITERS (to stable it creeps up by ~1e-8 per outer "
-                        "iteration for hundreds of iterations at scale 24)")
+                   help="set a tuning knob of the HIP path before the run (repeatable)")
+    p.add_argument("--leiden", type=int, default=2, metavar="ITERS",
+                   help="pLeiden+ leg with ex_iters=ITERS (-1: to stable, which creeps up by ~1e-8 per outer "
+                        "iteration for hundreds of iterations at scale 24; 0 or less than -1 skips it)")
     return p.parse_args()
 
 
-def cpu_baseline(scale, seed=2):
-    """Serial CPU oracle on an R-MAT graph of the given scale (same
-    construction, numpy stream): pLouvain to stable, edges/s."""
+def sha(t):
+    return hashlib.sha256(t.detach().cpu().numpy().tobytes()).hexdigest()
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def first_sweep_times(C, g, ctx):
+    """GPU: the first local-move call from singletons on the top level, with
+    the per-sweep trace; returns the sweep records."""
+    recs = []
+    ctx.set_sweep_trace(recs)
+    try:
+        st, vol = C.state_init(g, "Mod", 1.0, ctx=ctx)
+        import torch
+        part = torch.arange(g.n, dtype=torch.int32, device="cuda")
+        C.local_move(g, part, st, True, ctx=ctx)
+    finally:
+        ctx.set_sweep_trace(None)
+    return recs
+
+
+def cpu_baseline(rp, col, threads):
+    """The oracle (OpenMP restatement) on the same graph: the first local-move
+    call from singletons on the top level, stopped after two sweeps (the raw
+    sweep and the first full table sweep); value = m / the full sweep's time."""
+    import numpy as np
     from oracle import oracle as O
-    from pennsylvaniagpugraphclustering_amd import graphs
-    g = graphs.rmat(scale, 16, seed=seed)
-    vw = g.vertex_weights("Mod")
-    t0 = time.perf_counter()
-    q, part, info = O.cluster(g.rp, g.col, None, vw, "Mod", 1.0, "louvain", -1, seed=0)
-    dt = time.perf_counter() - t0
-    return {"value": (g.nnz // 2) / dt, "unit": "edges/s", "cores": 1, "kind": "port",
-            "sample": f"R-MAT scale {scale} (n={g.n}, m={g.nnz // 2}), pLouvain ex_iters=-1, "
-                      f"{dt:.2f} s, Q={q:.6f}"}
+    n = rp.numel() - 1
+    hrp = rp.cpu().numpy()
+    hcol = col.cpu().numpy()
+    vw = np.diff(hrp).astype(np.int64)
+    m = int(hrp[-1]) // 2
+    nthreads = threads or min(os.cpu_count() or 1, 16)
+    O.set_threads(nthreads)
+    O.set_sweep_limit(2)
+    try:
+        lam = 1.0 / float(hrp[-1])
+        obj0 = -lam * float(np.dot(vw.astype(np.float64), vw.astype(np.float64)))
+        t0 = time.perf_counter()
+        _, _, _, _, _, tr = O.local_move_trace(hrp, hcol, None, vw, np.arange(n, dtype=np.int32), vw, n, 0, lam,
+                                               obj0, True)
+        total = time.perf_counter() - t0
+    finally:
+        O.set_sweep_limit(0)
+    t_full = float(tr[1, 4]) if tr.shape[0] > 1 else float("nan")
+    return {"value": m / t_full, "unit": "edges/s per full top-level sweep", "cores": nthreads, "kind": "port",
+            "cpu": cpu_model(),
+            "sample": f"oracle (OpenMP CPU restatement) on the same R-MAT graph (n={n}, m={m}): first top-level "
+                      f"local-move call of pLouvain stopped after 2 sweeps; raw sweep {tr[0, 4]:.2f} s, full "
+                      f"table sweep {t_full:.2f} s ({int(tr[1, 1]) if tr.shape[0] > 1 else 0} candidates); "
+                      f"{total:.1f} s with set-up"}
 
 
 def roofline(kstats, scale):
-    """Roofline object for the dominant timed kernel group that carries an
-    algorithmic byte count (HIP events on the library stream)."""
+    """Roofline object for the kernel group that takes the most time (HIP
+    events on the library stream, algorithmic bytes counted by the library).
+    traffic: HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes recorded in profiles/ for this group and workload, if present."""
     best = None
     for name, (n, ms, by) in kstats.items():
         if n and by > 0 and (best is None or ms > kstats[best][1]):
@@ -83,9 +137,16 @@ def roofline(kstats, scale):
         return None
     n, ms, by = kstats[best]
     achieved = by / (ms * 1e-3) / 1e9
-    return {"bound": "hbm", "kernel": best, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get((best, scale)), "launches": n,
-            "avg_launch_us": ms / n * 1000.0, "bytes_per_launch": by / n}
+    out = {"bound": "hbm", "kernel": best, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launches": n, "avg_launch_us": ms / n * 1000.0,
+           "bytes_per_launch": by / n}
+    path = TRAFFIC.format(scale=scale)
+    if os.path.exists(path):
+        t = json.load(open(path)).get(best)
+        if t:
+            out["traffic"] = t["hbm_bytes_per_launch"]
+            out["traffic_source"] = {k: t[k] for k in ("commit", "profile") if k in t}
+    return out
 
 
 def batch_leg(a, ctx, C, graphs, torch):
@@ -136,12 +197,11 @@ def main():
     from pennsylvaniagpugraphclustering_amd import graphs
     from pennsylvaniagpugraphclustering_amd import clustering as C
 
-    dev = torch.device("cuda")
     t0 = time.perf_counter()
-    rp, col, _ = graphs.rmat_torch(a.scale, 16, seed=2, device=dev)
+    ctx = C.default_context()
+    rp, col, _ = graphs.rmat_device(a.scale, 16, seed=2, ctx=ctx)
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
-    ctx = C.default_context()
     ctx.set_verbose(1)
     for kv in a.knob:
         name, value = kv.split("=")
@@ -149,8 +209,8 @@ def main():
         print(f"# knob {name} = {ctx.get_knob(name)}", file=sys.stderr, flush=True)
     g = C.DeviceGraph(rp, col, None, None, objective="Mod", ctx=ctx)
     m = g.nnz // 2
-    print(f"# graph R-MAT s{a.scale}: n={g.n} nnz={g.nnz} m={m} (generated in {gen_s:.1f}s)", file=sys.stderr,
-          flush=True)
+    print(f"# graph R-MAT s{a.scale}: n={g.n} nnz={g.nnz} m={m} (built on the GPU in {gen_s:.1f}s)",
+          file=sys.stderr, flush=True)
 
     def step():
         return C.cluster(g, a.algo, "Mod", 1.0, a.ex_iters, seed=0, ctx=ctx)
@@ -168,20 +228,43 @@ def main():
     dt = (time.perf_counter() - t0) / max(a.steps, 1)
     kstats = ctx.kernel_stats()
     ctx.set_timing(False)
+    groups = {}
     for name, (n, ms, by) in kstats.items():
         if n:
+            groups[name] = {"ms_per_step": ms / max(a.steps, 1), "launches_per_step": n / max(a.steps, 1),
+                            "GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None,
+                            "GB_per_step": by / max(a.steps, 1) / 1e9}
             print(f"# kernel group {name}: {n} launches, {ms / max(a.steps, 1):.1f} ms/step, "
-                  f"{ms / n * 1000:.1f} us avg" + (f", {by / (ms * 1e-3) / 1e9:.0f} GB/s" if by else ""),
-                  file=sys.stderr, flush=True)
+                  f"{ms / n * 1000:.1f} us avg, {groups[name]['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
 
     extra = {}
-    if a.leiden and not a.no_leiden:
+    golden = GOLDEN.format(scale=a.scale)
+    if os.path.exists(golden) and a.algo == "louvain" and a.ex_iters == -1:
+        gr = json.load(open(golden)).get("runs", {}).get("louvain_ex-1")
+        if gr:
+            extra["matches_oracle"] = (sha(part) == gr["sha_part"] and float(q).hex() == gr["objective_hex"])
+            extra["oracle_seconds"] = gr.get("oracle_seconds")
+            extra["oracle_threads"] = gr.get("oracle_threads")
+
+    recs = first_sweep_times(C, g, ctx)
+    full = [r for r in recs if r["n"] == g.n and r["sweep"] == 1]
+    sweep_s = full[0]["us"] * 1e-6 if full else None
+    print(f"# first top-level sweeps: {recs[:3]}", file=sys.stderr, flush=True)
+
+    if a.leiden > 0 or a.leiden == -1:
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        ql, _, infol = C.cluster(g, "leiden+", "Mod", 1.0, a.leiden, seed=0, ctx=ctx)
+        ql, partl, infol = C.cluster(g, "leiden+", "Mod", 1.0, a.leiden, seed=0, ctx=ctx)
         torch.cuda.synchronize()
-        extra["leiden_plus"] = {"ex_iters": a.leiden, "seconds": time.perf_counter() - t1, "Q": ql,
-                                "labels": infol["labels"], "sweeps": infol["sweeps"]}
+        tl = time.perf_counter() - t1
+        extra["leiden_plus"] = {"ex_iters": a.leiden, "seconds": tl, "Q": ql, "labels": infol["labels"],
+                                "sweeps": infol["sweeps"], "outer_iterations": infol["outer"],
+                                "time_vs_plouvain": tl / dt}
+        if os.path.exists(golden):
+            gr = json.load(open(golden)).get("runs", {}).get(f"leiden+_ex{a.leiden}")
+            if gr:
+                extra["leiden_plus"]["matches_oracle"] = (sha(partl) == gr["sha_part"] and
+                                                          float(ql).hex() == gr["objective_hex"])
         print(f"# pLeiden+: {extra['leiden_plus']}", file=sys.stderr, flush=True)
 
     if not a.no_gnn:
@@ -209,7 +292,7 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(a.cpu_scale)
+        cpu = cpu_baseline(rp, col, a.cpu_threads)
         print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
     out = {
@@ -225,17 +308,19 @@ def main():
         "vs_baseline": None,
         "dtype": "int64+fp32",
         "data": "synthetic",
-        "config": {"workload": f"R-MAT scale {a.scale} ef16 (.57,.19,.19,.05) seed 2, unit weights, "
-                               f"{a.algo} ex_iters={a.ex_iters}, objective Mod",
+        "config": {"workload": f"R-MAT scale {a.scale} ef16 (.57,.19,.19,.05) seed 2 (Philox4x32-10 draws), "
+                               f"unit weights, {a.algo} ex_iters={a.ex_iters}, objective Mod",
                    "n": g.n, "nnz": g.nnz, "undirected_edges": m},
         "modularity": q,
         "labels": info["labels"],
         "sweeps": info["sweeps"],
         "levels": info["levels"],
         "outer_iterations": info["outer"],
-        "edges_per_s_per_sweep": m * info["sweeps"] / dt,
+        "time_to_stable_s": dt,
+        "edges_per_s_per_sweep": m / sweep_s if sweep_s else None,
+        "first_full_sweep_ms": sweep_s * 1000.0 if sweep_s else None,
         "roofline": roofline(kstats, a.scale),
-        "kernel_groups_ms_per_step": {k: v[1] / max(a.steps, 1) for k, v in kstats.items() if v[0]},
+        "kernel_groups": groups,
         "cpu_baseline": cpu,
     }
     out.update(extra)

@@ -99,15 +99,16 @@ int pggc_ctx_set_verbose(pggc_ctx* ctx, int level);
  * launches, summed milliseconds and summed algorithmic bytes of one slot. */
 #define PGGC_KS_SCORE_GROUP 0    /* neighbour-cluster scoring, d <= 64 */
 #define PGGC_KS_SCORE_TABLE 1    /* scoring with LDS tables, 64 < d <= 2048 */
-#define PGGC_KS_SCORE_HUB 2      /* hub edge aggregation (k_hub_insert), unit-weight levels;
-                                    bytes counted on the device */
+#define PGGC_KS_SCORE_HUB 2      /* hub scoring (HBM table build or scan), d > 2048 */
 #define PGGC_KS_AFTERBURNER 3    /* conflict filter */
-#define PGGC_KS_MOVES 4          /* exact uncut delta + volume moves */
+#define PGGC_KS_MOVES 4          /* hub-table updates, exact uncut delta, volume moves */
 #define PGGC_KS_CONTRACT_AGG 5   /* coarse-row merge in LDS */
 #define PGGC_KS_CONTRACT_BIG 6   /* coarse-row merge in HBM tables */
 #define PGGC_KS_CONTRACT_COPY 7  /* coarse-row copy-out */
 #define PGGC_KS_EVAL 8           /* evaluator edge pass */
-#define PGGC_KS_COUNT 9
+#define PGGC_KS_LEIDEN 9         /* leidenR refinement */
+#define PGGC_KS_ORDER 10         /* ordering pass */
+#define PGGC_KS_COUNT 11
 int pggc_ctx_set_timing(pggc_ctx* ctx, int on);
 int pggc_ctx_kernel_stats(pggc_ctx* ctx, int slot, int64_t* launches, double* ms, double* bytes);
 
@@ -301,6 +302,43 @@ int pggc_signature_diff(pggc_ctx* ctx, const uint32_t* a, const uint32_t* b, int
  * order, *labels_out (host) = their count */
 int pggc_intersect(pggc_ctx* ctx, int64_t n, const int32_t* c1, const int32_t* c2, int64_t l2, int32_t* out,
                    int64_t* labels_out);
+
+/* ---------------------------------------------------------------------
+ * Device-side graph construction.
+ *
+ * pggc_graph_from_edges replaces construct_from_edgelist
+ * (app/io/io_mtx.cpp:95-201): from an edge list in device memory (src[m],
+ * dst[m] int32 in [0, n), w[m] int32 >= 0 or NULL) it drops self loops and
+ * zero weights, merges repeated undirected pairs (keeping the weight of the
+ * first occurrence in input order), stores both directions and sorts every
+ * row.  Negative weights or ids outside [0, n) give PGGC_EINVAL.
+ *
+ * pggc_rmat builds the R-MAT graph of the synthetic workloads on the device:
+ * edge_factor * 2^scale draws, each of `scale` quadrant choices taken from a
+ * Philox4x32-10 draw (counter = edge index lo, hi, block, 0; key = seed;
+ * level l uses word l % 4 of block l / 4; a draw r picks (0,0) when r <
+ * floor(a 2^32), (0,1) when r < floor((a+b) 2^32), (1,0) when r < floor((a+b+c)
+ * 2^32), else (1,1), first level = most significant bit), then the
+ * construction above.  With wmax > 0 each undirected pair lo < hi weighs
+ * wmin + Philox(lo, hi, 0, 1; seed) word 0 mod (wmax - wmin + 1).  The
+ * reference reads its graphs from files; no generator of its own replaces
+ * this.
+ *
+ * The graph is owned by the returned handle until pggc_devgraph_free; its
+ * view (vtx_w = NULL) stays valid until then. */
+typedef struct pggc_devgraph pggc_devgraph;
+int pggc_graph_from_edges(pggc_ctx* ctx, int64_t n, int64_t m, const int32_t* src, const int32_t* dst,
+                          const int32_t* w, pggc_devgraph** out);
+int pggc_rmat(pggc_ctx* ctx, int scale, int edge_factor, double a, double b, double c, uint64_t seed, int32_t wmin,
+              int32_t wmax, pggc_devgraph** out);
+int pggc_devgraph_view(const pggc_devgraph* g, pggc_graph* view);
+/* device-to-device copy into caller buffers (row_ptr[n+1], col[nnz], w[nnz]
+ * when weighted) on the context's stream */
+int pggc_devgraph_copy(pggc_ctx* ctx, const pggc_devgraph* g, int64_t* row_ptr, int32_t* col, int32_t* w);
+int pggc_devgraph_free(pggc_devgraph* g);
+/* out[i] = Philox4x32-10(counter i lo, i hi, 0, 2; key seed) word 0 mod L:
+ * the fixed random labellings of the contraction workloads */
+int pggc_random_labels(pggc_ctx* ctx, int64_t n, int64_t L, uint64_t seed, int32_t* out);
 
 #ifdef __cplusplus
 }

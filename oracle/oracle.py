@@ -57,6 +57,8 @@ def lib():
         L.or_local_move.restype = c.c_int64
         L.or_local_move.argtypes = [c.c_int64, i64p, i32p, vp, vp, i64p, i32p, i64p, c.c_int64, i64p,
                                     c.c_double, c.c_int, c.c_int, c.c_int, ctypes.POINTER(c.c_double)]
+        L.or_set_sweep_limit.restype = None
+        L.or_set_sweep_limit.argtypes = [c.c_int]
         L.or_set_threads.restype = None
         L.or_set_threads.argtypes = [c.c_int]
         L.or_get_threads.restype = c.c_int
@@ -211,6 +213,12 @@ def set_threads(k):
     lib().or_set_threads(int(k))
 
 
+def set_sweep_limit(k):
+    """Stop every local_move call after k sweeps (0 = the reference's
+    schedule).  Only for bounded CPU timing samples."""
+    lib().or_set_sweep_limit(int(k))
+
+
 def get_threads():
     return int(lib().or_get_threads())
 
@@ -265,20 +273,20 @@ def sweep_detail(rp, col, w, vw, part, vol, lam, ratio, raw, dest, gain, constra
 
 def local_move_trace(rp, col, w, vw, part, vol, L, uncut, lam, obj, is_initial, strict=False, SA=True):
     """local_move() plus its per-sweep records (ratio, candidates, moves,
-    objective after the sweep) as an array of shape (sweeps, 4)."""
+    objective after the sweep, wall seconds) as an array of shape (sweeps, 5)."""
     n = len(rp) - 1
     part = part.copy()
     v = np.zeros(n, np.int64)
     v[:L] = vol[:L]
     u = np.array([uncut], np.int64)
     o = ctypes.c_double(obj)
-    tr = np.zeros(4 * 64, np.float64)
+    tr = np.zeros(5 * 64, np.float64)
     nt = np.zeros(1, np.int64)
     w32, w64 = _w(w)
     L2 = lib().or_local_move_trace(n, rp, col, w32, w64, vw, part, v, L, u, lam, int(is_initial), int(strict),
                                    int(SA), ctypes.byref(o), tr, 64, nt)
     k = min(int(nt[0]), 64)
-    return part, v[:L2].copy(), int(L2), int(u[0]), o.value, tr[:4 * k].reshape(k, 4)
+    return part, v[:L2].copy(), int(L2), int(u[0]), o.value, tr[:5 * k].reshape(k, 5)
 
 
 def leiden_refine_detail(rp, col, w, vw, louv, vol, lam, uniform, seed, draw0=0):

@@ -24,6 +24,7 @@
 // the afterburner term q and the weighted leidenR score) use fmaf explicitly.
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -600,12 +601,20 @@ struct Stats {
 struct SweepRec {
     float ratio;
     i64 cands, moves;
-    double obj;
+    double obj, seconds;
 };
 static std::vector<SweepRec>* g_trace = nullptr;
+static std::chrono::steady_clock::time_point g_sweep_t0;
 static void trace_sweep(float ratio, i64 cands, i64 moves, double obj) {
-    if (g_trace) g_trace->push_back(SweepRec{ratio, cands, moves, obj});
+    if (g_trace) {
+        const double sec =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - g_sweep_t0).count();
+        g_trace->push_back(SweepRec{ratio, cands, moves, obj, sec});
+    }
 }
+// sweeps per local_move call (0 = the reference's schedule; a positive value
+// stops a call early, for bounded CPU timing samples only)
+static int g_sweep_limit = 0;
 
 // local_move (src/local_mover.cpp:1360-1417)
 static void local_move(const Graph& g, std::vector<i32>& best_part, State& best, bool is_initial, bool constrained,
@@ -628,6 +637,8 @@ static void local_move(const Graph& g, std::vector<i32>& best_part, State& best,
         float fr = SA ? ratios[x] : 0.0f;
         for (int c = 0; c < limits[x]; c++) {
             iter++;
+            if (g_sweep_limit > 0 && iter > g_sweep_limit) break;
+            g_sweep_t0 = std::chrono::steady_clock::now();
             std::vector<i32> cands = candidates(M, curr, fr, constrained, constraint);
             if (stats) { stats->sweeps++; stats->cands += (i64)cands.size(); }
             if (cands.empty()) {
@@ -1148,6 +1159,7 @@ static Graph make_graph(i64 n, const i64* rp, const i32* col, const i32* w32, co
 extern "C" {
 
 void or_set_threads(int k) { omp_set_num_threads(k > 0 ? k : 1); }
+void or_set_sweep_limit(int k) { g_sweep_limit = k > 0 ? k : 0; }
 int or_get_threads(void) { return omp_get_max_threads(); }
 
 void or_philox(const u32* ctr, const u32* key, u32* out) {
@@ -1366,8 +1378,9 @@ i64 or_local_move(i64 n, const i64* rp, const i32* col, const i32* w32, const i6
                                strict, SA, obj_inout, nullptr, 0, nullptr);
 }
 
-// or_local_move with its per-sweep records: trace_out[4*k..4*k+3] = ratio,
-// candidates, accepted moves, objective after sweep k (at most trace_cap)
+// or_local_move with its per-sweep records: trace_out[5*k..5*k+4] = ratio,
+// candidates, accepted moves, objective after sweep k and its wall seconds
+// (at most trace_cap records)
 i64 or_local_move_trace(i64 n, const i64* rp, const i32* col, const i32* w32, const i64* w64, const i64* vw,
                         i32* part_inout, i64* vol_inout, i64 L, i64* uncut_inout, double lambda, int is_initial,
                         int strict, int SA, double* obj_inout, double* trace_out, i64 trace_cap, i64* ntrace) {
@@ -1388,10 +1401,11 @@ i64 or_local_move_trace(i64 n, const i64* rp, const i32* col, const i32* w32, co
     if (trace_out) {
         const i64 k = std::min<i64>((i64)tr.size(), trace_cap);
         for (i64 x = 0; x < k; x++) {
-            trace_out[4 * x] = tr[x].ratio;
-            trace_out[4 * x + 1] = (double)tr[x].cands;
-            trace_out[4 * x + 2] = (double)tr[x].moves;
-            trace_out[4 * x + 3] = tr[x].obj;
+            trace_out[5 * x] = tr[x].ratio;
+            trace_out[5 * x + 1] = (double)tr[x].cands;
+            trace_out[5 * x + 2] = (double)tr[x].moves;
+            trace_out[5 * x + 3] = tr[x].obj;
+            trace_out[5 * x + 4] = tr[x].seconds;
         }
         *ntrace = (i64)tr.size();
     }

@@ -104,6 +104,13 @@ SIGNATURES = {
     "pggc_cluster": (c.c_int, [P, c.POINTER(Graph), c.c_int, c.c_int, c.c_double, c.c_int, c.c_uint64, P,
                                c.POINTER(c.c_double), I64P]),
     "pggc_cluster_batch": (c.c_int, [P, c.c_int, P, c.c_int, c.c_int, c.c_double, c.c_int, P, P, P, P, c.c_int]),
+    "pggc_graph_from_edges": (c.c_int, [P, c.c_int64, c.c_int64, P, P, P, c.POINTER(c.c_void_p)]),
+    "pggc_rmat": (c.c_int, [P, c.c_int, c.c_int, c.c_double, c.c_double, c.c_double, c.c_uint64, c.c_int32,
+                            c.c_int32, c.POINTER(c.c_void_p)]),
+    "pggc_devgraph_view": (c.c_int, [P, c.POINTER(Graph)]),
+    "pggc_devgraph_copy": (c.c_int, [P, P, P, P, P]),
+    "pggc_devgraph_free": (c.c_int, [P]),
+    "pggc_random_labels": (c.c_int, [P, c.c_int64, c.c_int64, c.c_uint64, P]),
 }
 
 _lib = None

@@ -67,8 +67,8 @@ class Context:
         check(self.lib.pggc_ctx_memory(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
-    KERNEL_SLOTS = ("score_group", "score_table", "hub_insert", "afterburner", "moves", "contract_agg",
-                    "contract_big", "contract_copy", "eval")
+    KERNEL_SLOTS = ("score_group", "score_table", "score_hub", "afterburner", "moves", "contract_agg",
+                    "contract_big", "contract_copy", "eval", "leiden", "order")
 
     def set_timing(self, on=True):
         """Record HIP events on the context stream around the timed kernel

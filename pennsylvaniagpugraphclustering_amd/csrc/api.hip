@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "device.h"
+#include "gen.h"
 #include "gnn.h"
 #include "leiden.h"
 #include "localmove.h"
@@ -19,6 +20,10 @@
 #include "pggc.h"
 
 using namespace pggc;
+
+struct pggc_devgraph {
+    DevCSR g;
+};
 
 struct pggc_ctx {
     Ctx c;
@@ -1012,6 +1017,78 @@ int pggc_intersect(pggc_ctx* ctx, int64_t n, const int32_t* c1, const int32_t* c
         if (n < 0 || l2 < 1 || (n > 0 && (!c1 || !c2 || !out))) throw Error(PGGC_EINVAL, "bad intersection arguments");
         const i64 L = intersect_clusterings(ctx->c, n, c1, c2, l2, out);
         if (labels_out) *labels_out = L;
+    });
+}
+
+int pggc_graph_from_edges(pggc_ctx* ctx, int64_t n, int64_t m, const int32_t* src, const int32_t* dst,
+                          const int32_t* w, pggc_devgraph** out) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (!out) throw Error(PGGC_EINVAL, "null output");
+        std::unique_ptr<pggc_devgraph> h(new pggc_devgraph());
+        try {
+            edges_csr(ctx->c, n, m, src, dst, w, h->g);
+        } catch (...) {
+            h->g.release();
+            throw;
+        }
+        *out = h.release();
+    });
+}
+
+int pggc_rmat(pggc_ctx* ctx, int scale, int edge_factor, double a, double b, double c, uint64_t seed, int32_t wmin,
+              int32_t wmax, pggc_devgraph** out) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (!out) throw Error(PGGC_EINVAL, "null output");
+        std::unique_ptr<pggc_devgraph> h(new pggc_devgraph());
+        try {
+            rmat_csr(ctx->c, scale, edge_factor, a, b, c, seed, wmin, wmax, h->g);
+        } catch (...) {
+            h->g.release();
+            throw;
+        }
+        *out = h.release();
+    });
+}
+
+int pggc_devgraph_view(const pggc_devgraph* g, pggc_graph* view) {
+    return guard([&] {
+        if (!g || !view) throw Error(PGGC_EINVAL, "null argument");
+        view->n = g->g.n;
+        view->nnz = g->g.nnz;
+        view->row_ptr = g->g.rp;
+        view->col = g->g.col;
+        view->w = g->g.w;
+        view->w64 = nullptr;
+        view->vtx_w = nullptr;
+    });
+}
+
+int pggc_devgraph_copy(pggc_ctx* ctx, const pggc_devgraph* g, int64_t* row_ptr, int32_t* col, int32_t* w) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (!g || !row_ptr || (g->g.nnz > 0 && !col) || (g->g.w && g->g.nnz > 0 && !w))
+            throw Error(PGGC_EINVAL, "null argument");
+        copy_dev(ctx->c, row_ptr, g->g.rp, (size_t)(g->g.n + 1) * sizeof(i64));
+        copy_dev(ctx->c, col, g->g.col, (size_t)g->g.nnz * sizeof(i32));
+        if (g->g.w) copy_dev(ctx->c, w, g->g.w, (size_t)g->g.nnz * sizeof(i32));
+    });
+}
+
+int pggc_devgraph_free(pggc_devgraph* g) {
+    return guard([&] {
+        if (!g) return;
+        g->g.release();
+        delete g;
+    });
+}
+
+int pggc_random_labels(pggc_ctx* ctx, int64_t n, int64_t L, uint64_t seed, int32_t* out) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (n > 0 && !out) throw Error(PGGC_EINVAL, "null output");
+        random_labels(ctx->c, n, L, seed, out);
     });
 }
 

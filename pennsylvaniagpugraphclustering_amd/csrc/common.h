@@ -125,6 +125,32 @@ __host__ __device__ __forceinline__ u32 seed_draw(u64 seed, u64 k) {
     return (u32)z;
 }
 
+// Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw, SC'11):
+// ten rounds of two 32x32->64 multiplies with the published multipliers and
+// Weyl key increments.  The synthetic graph generators draw from it so that
+// a CPU build of the same draws reproduces the same graph.
+struct Philox4 {
+    u32 v[4];
+};
+__host__ __device__ __forceinline__ Philox4 philox4x32_10(u32 c0, u32 c1, u32 c2, u32 c3, u32 k0, u32 k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const u64 p0 = (u64)0xD2511F53u * c0;
+        const u64 p1 = (u64)0xCD9E8D57u * c2;
+        const u32 n0 = (u32)(p1 >> 32) ^ c1 ^ k0;
+        const u32 n2 = (u32)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (u32)p1;
+        c3 = (u32)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return Philox4{{c0, c1, c2, c3}};
+}
+// streams of the generators (counter word 3)
+constexpr u32 STREAM_RMAT = 0, STREAM_WEIGHT = 1, STREAM_LABEL = 2;
+
 // Monotone map float -> u32 (for packed argmax keys).
 __device__ __forceinline__ u32 ord_f32(float f) {
     u32 b = __float_as_uint(f);

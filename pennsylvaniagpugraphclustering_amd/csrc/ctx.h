@@ -16,14 +16,16 @@ namespace pggc {
 // moves (measurement hooks; off unless enabled).
 enum KSlot {
     KS_SCORE_GROUP = 0,  // k_score_group<G>
-    KS_SCORE_TABLE,      // k_score_table (LDS tables)
-    KS_SCORE_HUB,        // k_hub_insert on unit-weight levels (bytes counted on the device)
-    KS_AFTERBURNER,      // k_ab_serial / k_ab_wave / k_ab_block
-    KS_MOVES,            // k_delta_uncut + k_apply_moves
+    KS_SCORE_TABLE,      // k_score_table / k_score_raw (LDS tables, block arg-max)
+    KS_SCORE_HUB,        // hub scoring: table build (k_hub_insert), k_hub_pv, k_hub_scan, k_hub_decide
+    KS_AFTERBURNER,      // k_ab_pack + k_ab_serial / k_ab_wave / k_ab_block
+    KS_MOVES,            // hub-table updates + k_delta_uncut + k_apply_moves
     KS_CONTRACT_AGG,     // k_cagg (LDS merge of coarse rows)
     KS_CONTRACT_BIG,     // k_cbig_insert + k_cbig_extract
     KS_CONTRACT_COPY,    // k_crow_copy
     KS_EVAL,             // k_eval_uncut
+    KS_LEIDEN,           // leidenR refinement (all its kernels)
+    KS_ORDER,            // ordering pass (classes + two stable partitions)
     KS_COUNT
 };
 
@@ -142,6 +144,10 @@ void scan_i32_to_i64(Ctx& c, const i32* in, i64* out, i64 n);
 i64 compact_flags(Ctx& c, const i32* flag, const i32* src, i64 n, i32* out, i64* count_dev);
 // Stable LSD radix sort of (key, val) pairs on the low `bits` key bits.
 void radix_sort_pairs(Ctx& c, u64* keys, i32* vals, i64 n, int bits);
+void radix_sort_pairs(Ctx& c, u64* keys, i64* vals, i64 n, int bits);
+void radix_sort_keys(Ctx& c, u64* keys, i64 n, int bits);
+// *dst += *src on the stream (algorithmic-byte counters of timed regions)
+void acct_add(Ctx& c, u64* dst, const i64* src);
 // sum of int64 values (device result in *out)
 void sum_i64(Ctx& c, const i64* in, i64 n, i64* out);
 // exact sum of squares of int64 values: out[0] = high word, out[1] = low word

@@ -103,6 +103,9 @@ __global__ void k_order_classes(const i64* rp, i64 n, i32* c1, i32* c2) {
 
 void make_orders(Ctx& c, const DGraph& g, Orders& o) {
     i64 n = g.n;
+    // row bounds and two class bytes per vertex read, then per partition
+    // the classes read twice (count, write) and the ids written
+    const int tk = c.kt_begin(KS_ORDER);
     if (o.o1.n < (size_t)n) {
         o.o1 = Buf<i32>(c, std::max<i64>(n, 1));
         o.o2 = Buf<i32>(c, std::max<i64>(n, 1));
@@ -117,6 +120,7 @@ void make_orders(Ctx& c, const DGraph& g, Orders& o) {
     o.off_large2 = s1[2];
     o.off_mid = s2[1];
     o.off_mid2 = s2[2];
+    c.kt_end(tk, 8.0 * (n + 1) + 8.0 * n + 2.0 * 12.0 * n);
 }
 
 // ------------------------------------------------- edge-chunk row reductions
@@ -577,7 +581,7 @@ __global__ void k_big_count(const i32* list, i64 m, const i64* crp, i64* cnt) {
 // key = (big row index << cb) | column (columns < 2^cb), so the radix sort
 // needs only cb + bits(big rows) bits
 __global__ void k_big_flatten(const i32* list, i64 m, const i64* crp, const i64* boff, const i32* ocol, int cb,
-                              u64* key, i32* val, i64* target) {
+                              u64* key, i64* val, i64* target) {
     const i64 x = block_id();
     if (x >= m) return;
     {
@@ -586,12 +590,12 @@ __global__ void k_big_flatten(const i32* list, i64 m, const i64* crp, const i64*
         for (i64 j = s + threadIdx.x; j < e; j += blockDim.x) {
             i64 f = b + (j - s);
             key[f] = ((u64)x << cb) | (u32)ocol[j];
-            val[f] = (i32)f;
+            val[f] = f;  // 64-bit: big rows can hold 2^31 or more entries
             target[f] = j;
         }
     }
 }
-__global__ void k_big_gather(const u64* key, const i32* val, const i64* target, i64 E, const i64* ow_in,
+__global__ void k_big_gather(const u64* key, const i64* val, const i64* target, i64 E, const i64* ow_in,
                              i64* wtmp) {
     for (i64 f = (i64)blockIdx.x * blockDim.x + threadIdx.x; f < E; f += (i64)gridDim.x * blockDim.x)
         wtmp[f] = ow_in[target[val[f]]];
@@ -604,6 +608,33 @@ __global__ void k_big_scatter(const u64* key, const i64* target, i64 E, const i6
         ocol[j] = (i32)(key[f] & mask);
         ow[j] = wtmp[f];
     }
+}
+
+// Algorithmic bytes of the coarse-row merge (timing runs only, counted after
+// it ran): per coarse row its list entry, member range and output count; per
+// member its id and row bounds; per member edge the column, weight and the
+// neighbour's coarse id; per distinct coarse neighbour the column and int64
+// weight written.
+__global__ void __launch_bounds__(256) k_contract_bytes(const i32* __restrict__ list, i64 m,
+                                                        const i64* __restrict__ mstart, const i64* __restrict__ bound,
+                                                        const i64* __restrict__ ccnt, int wb, i64* acc) {
+    const i64 x = thread_item();
+    i64 b = 0;
+    if (x < m) {
+        const i32 u = list[x];
+        b = 28 + 20 * (mstart[u + 1] - mstart[u]) + bound[u] * (8 + wb) + 12 * ccnt[u];
+    }
+    b = block_sum_i64<256>(b);
+    if (threadIdx.x == 0 && b) atomicAdd((unsigned long long*)acc, (unsigned long long)b);
+}
+static void contract_bytes(Ctx& c, int tk, const i32* list, i64 m, const i64* mstart, const i64* bound,
+                           const i64* ccnt, int wb) {
+    if (tk < 0 || m <= 0) return;
+    Buf<i64> acc(c, 1);
+    fill_i64(c, acc, 1, 0);
+    k_contract_bytes<<<grid_items(m, 256), 256, 0, c.stream>>>(list, m, mstart, bound, ccnt, wb, acc);
+    PGGC_LAUNCH_CHECK();
+    acct_add(c, c.kt_counter(tk), acc);
 }
 
 static int bits_for(i64 v) {
@@ -653,6 +684,8 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
     });
     c.kt_end(tk, 0.0);
     PGGC_LAUNCH_CHECK();
+    const int wb = g.wk == W_UNIT ? 0 : g.wk == W_I32 ? 4 : 8;
+    contract_bytes(c, tk, list.p + st[0], n0 + n1 + n1b, mstart, bound, ccnt, wb);
     if (n2 > 0) {
         const i32* big = list.p + st[3];
         Buf<i64> goff(c, n2 + 1), items(c, n2 + 1), mdp(c, n + 1);
@@ -695,6 +728,7 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
                                                               ccnt);
         c.kt_end(tk, 0.0);
         PGGC_LAUNCH_CHECK();
+        contract_bytes(c, tk, big, n2, mstart, bound, ccnt, wb);
     }
     out.n = nc;
     out.rp = Buf<i64>(c, nc + 1);
@@ -721,8 +755,7 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
         i64 E = c.read1(boff.p + n2);
         if (E > 0) {
             Buf<u64> key(c, E);
-            Buf<i32> val(c, E);
-            Buf<i64> target(c, E), wtmp(c, E);
+            Buf<i64> val(c, E), target(c, E), wtmp(c, E);
             const int cb = bits_for(nc - 1);
             k_big_flatten<<<grid_blocks(n2), 256, 0, c.stream>>>(big, n2, out.rp, boff, out.col, cb, key, val,
                                                                          target);

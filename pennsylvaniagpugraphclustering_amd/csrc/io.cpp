@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cctype>
 #include <chrono>
+#include <climits>
 #include <cstdint>
 #include <cstdio>
 #include <fstream>
@@ -162,6 +163,7 @@ void read_metis(const std::string& path, bool& unit_edges, pggc_host_graph& g) {
                 continue;
             }
             if (*f == '\n' && rows < n) g.rp[++rows] = nz;
+            if (*f == '-') throw Fail{PGGC_EINVAL, "Negative value in metis file (edge weights must be >= 0)"};
             f++;
         }
         if (f >= end) break;
@@ -171,6 +173,8 @@ void read_metis(const std::string& path, bool& unit_edges, pggc_host_graph& g) {
             if (x < 1 || (int64_t)x > n) throw Fail{PGGC_EINVAL, "Vertex id " + std::to_string(x) + " out of range"};
             g.col[nz] = (int32_t)(x - 1);
         } else if (keep_w) {
+            if (x > (uint64_t)INT32_MAX)
+                throw Fail{PGGC_EINVAL, "Edge weight " + std::to_string(x) + " exceeds 2147483647"};
             g.w[nz] = (int32_t)x;
         }
         if (has_ew) {
@@ -259,12 +263,16 @@ void read_mtx(const std::string& path, bool& unit_edges, pggc_host_graph& g) {
                 continue;
             }
             if (*f == '\n' && state != 0) throw Fail{PGGC_EINVAL, "Line with exactly one value detected"};
+            if (*f == '-') throw Fail{PGGC_EINVAL, "Negative value in matrix market file (edge weights must be >= 0)"};
             f++;
         }
         if (f >= end) break;
         if (read >= m) throw Fail{PGGC_EINVAL, "More entries than the header's " + std::to_string(m)};
         if (state == 2) {
-            val[read++] = (int32_t)read_uint(f, end);
+            const uint64_t x = read_uint(f, end);
+            if (x > (uint64_t)INT32_MAX)
+                throw Fail{PGGC_EINVAL, "Edge weight " + std::to_string(x) + " exceeds 2147483647"};
+            val[read++] = (int32_t)x;
             state = 0;
             skip_line(f, end);
             continue;

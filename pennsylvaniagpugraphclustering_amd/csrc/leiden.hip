@@ -257,6 +257,13 @@ i64 leiden_refine(Ctx& c, const DGraph& g, const Orders& ord, const i32* louv, c
     const i64 n = g.n;
     const float gamma = (float)lambda;
     const dim3 G = grid_items(n, 256);
+    // algorithmic bytes: two adjacency passes (neighbour choice: column,
+    // weight, the neighbour's constraint, flag and vertex weight; inner
+    // weights: column, weight, the neighbour's tree and order) and about 40
+    // int32/int64 vertex arrays read or written by the flag, order, forest,
+    // sort, scan, break and relabel steps
+    const int wb = g.wk == W_UNIT ? 0 : g.wk == W_I32 ? 4 : 8;
+    const int tk = c.kt_begin(KS_LEIDEN);
     Buf<i32> well(c, n + 1), hn(c, n + 1), order(c, n + 1), vcmap(c, n + 1);
     k_well<<<G, 256, 0, c.stream>>>(g.vw, louv, vol, pv, n, gamma, well);
     PGGC_LAUNCH_CHECK();
@@ -317,6 +324,7 @@ i64 leiden_refine(Ctx& c, const DGraph& g, const Orders& ord, const i32* louv, c
     scan_i32_to_i64(c, well, tsz, n);
     k_root_relabel<<<G, 256, 0, c.stream>>>(n, tsz, vcmap, vcmap_out);
     PGGC_LAUNCH_CHECK();
+    c.kt_end(tk, 2.0 * 8.0 * (n + 1) + (double)g.nnz * (32 + 2 * wb) + 250.0 * n);
     return c.read1(tsz.p + n);
 }
 

@@ -675,10 +675,82 @@ __global__ void k_hub_decide(const i32* hubs, i64 m, const i32* part, const i64*
 }
 
 
+// Algorithmic bytes of one scoring sweep, counted (on timing runs only)
+// before it runs, since dest decides which vertices it scores.  A scored
+// vertex reads its row bounds, destination, cluster, vertex weight and its
+// cluster's volume, every edge's column (and weight) and, after the first
+// sweep, the neighbour's cluster, and writes its destination and gain:
+// 48 + d (4 + weight bytes + 4) bytes.  A hub whose table is kept current reads
+// the table instead of its edges: 48 + 8 per slot, plus the edge bytes on the
+// sweep that builds the table.  Every listed vertex also costs its list entry
+// and destination read.  acc3[0..2] = group, table and hub classes.
+struct ScoreBounds {
+    i64 s[NCLS + 1];
+};
+template <int WB>
+__global__ void __launch_bounds__(256) k_score_bytes(const i64* __restrict__ rp, const i32* __restrict__ list,
+                                                     ScoreBounds sb, const i32* __restrict__ dest, int raw, int build,
+                                                     const i64* __restrict__ goff, i64* acc3) {
+    const i64 x = thread_item();
+    i64 b0 = 0, b1 = 0, b2 = 0;
+    if (x < sb.s[NCLS]) {
+        const i32 i = list[x];
+        const int grp = x < sb.s[CL_TABLE] ? 0 : (x < sb.s[CL_HUB] ? 1 : 2);
+        i64 by = 8;
+        if (dest[i] != NO_MOVE) {
+            const i64 eb = (rp[i + 1] - rp[i]) * (4 + WB + (raw ? 0 : 4));
+            if (grp < 2 || raw) {
+                by += 48 + eb;
+            } else {
+                const i64 h = x - sb.s[CL_HUB];
+                by += 48 + 8 * (goff[h + 1] - goff[h]) + (build ? eb : 0);
+            }
+        }
+        if (grp == 0) b0 = by;
+        else if (grp == 1) b1 = by;
+        else b2 = by;
+    }
+    b0 = block_sum_i64<256>(b0);
+    b1 = block_sum_i64<256>(b1);
+    b2 = block_sum_i64<256>(b2);
+    if (threadIdx.x == 0) {
+        if (b0) atomicAdd((unsigned long long*)&acc3[0], (unsigned long long)b0);
+        if (b1) atomicAdd((unsigned long long*)&acc3[1], (unsigned long long)b1);
+        if (b2) atomicAdd((unsigned long long*)&acc3[2], (unsigned long long)b2);
+    }
+}
+
+// alpha + beta * degree bytes per listed vertex (per candidate of the
+// afterburner, per mover of perform_moves), plus `extra` once
+__global__ void __launch_bounds__(256) k_list_bytes(const i64* __restrict__ rp, const i32* __restrict__ list, i64 k,
+                                                    i64 alpha, i64 beta, i64 extra, i64* acc) {
+    const i64 x = thread_item();
+    i64 b = 0;
+    if (x < k) {
+        const i32 i = list[x];
+        b = alpha + beta * (rp[i + 1] - rp[i]);
+    }
+    if (x == 0) b += extra;
+    b = block_sum_i64<256>(b);
+    if (threadIdx.x == 0 && b) atomicAdd((unsigned long long*)acc, (unsigned long long)b);
+}
+
 static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32* part, const i64* vol,
                       const i32* cons, bool raw, float lam, float ratio, i32* dest, float* gain) {
     const i64* st = lp.sc_start;
     const i32* L = lp.sc_list.p;
+    Buf<i64> acct;
+    if (c.kt.on && g.n > 0) {  // timing runs only
+        acct = Buf<i64>(c, 3);
+        fill_i64(c, acct, 3, 0);
+        ScoreBounds sb;
+        for (int k = 0; k <= NCLS; k++) sb.s[k] = st[k];
+        const int build = (!raw && !lp.hub_live) ? 1 : 0;
+        const i64* goff = st[CL_HUB + 1] > st[CL_HUB] ? lp.goff.p : nullptr;
+        PGGC_WDISPATCH(g, (k_score_bytes<decltype(W)::bytes><<<grid_items(g.n, 256), 256, 0, c.stream>>>(
+                              g.rp, L, sb, dest, raw ? 1 : 0, build, goff, acct)));
+        PGGC_LAUNCH_CHECK();
+    }
     PGGC_WDISPATCH(g, {
         i64 m;
         int tk = c.kt_begin(KS_SCORE_GROUP);
@@ -698,6 +770,7 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
             k_score_group<64><<<grid_items(m, 4), 256, 0, c.stream>>>(
                 g.rp, g.col, W, g.vw, L + st[4], m, part, vol, cons, raw, lam, ratio, dest, gain);
         c.kt_end(tk, 0.0);
+        if (tk >= 0) acct_add(c, c.kt_counter(tk), acct.p);
         tk = c.kt_begin(KS_SCORE_TABLE);
         if (raw) {
             if ((m = st[CL_BIG] - st[CL_TABLE]) > 0)
@@ -731,8 +804,10 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
                     nullptr, nullptr);
         }
         c.kt_end(tk, 0.0);
+        if (tk >= 0) acct_add(c, c.kt_counter(tk), acct.p + 1);
         if ((m = st[CL_HUB + 1] - st[CL_HUB]) > 0) {
             const i32* hubs = L + st[CL_HUB];
+            tk = c.kt_begin(KS_SCORE_HUB);
             fill_i64(c, lp.hub_pv, m, 0);
             fill_i64(c, (i64*)lp.hub_best.p, m, 0);
             if (raw) {
@@ -746,17 +821,15 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
                         fill_i32(c, lp.gkeys, lp.gslots, EMPTY_KEY);
                         fill_i64(c, lp.gvals, lp.gslots, 0);
                     }
-                    tk = g.wk == W_UNIT ? c.kt_begin(KS_SCORE_HUB) : -1;
                     if (lp.packed) {
                         k_hub_insert<true><<<grid_blocks(lp.igrid), 256, 0, c.stream>>>(
                             g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid, lp.goff, part, nullptr, nullptr,
-                            lp.gtab, c.kt_counter(tk));
+                            lp.gtab, nullptr);
                     } else {
                         k_hub_insert<false><<<grid_blocks(lp.igrid), 256, 0, c.stream>>>(
                             g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid, lp.goff, part, lp.gkeys, lp.gvals,
-                            nullptr, c.kt_counter(tk));
+                            nullptr, nullptr);
                     }
-                    c.kt_end(tk, 0.0);
                     lp.hub_live = true;
                 }
                 if (lp.packed) {
@@ -775,6 +848,8 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
             }
             k_hub_decide<<<grid_items(m, 256), 256, 0, c.stream>>>(hubs, m, part, g.vw, vol, lam, ratio, lp.hub_pv,
                                                                        lp.hub_best, dest, gain);
+            c.kt_end(tk, 0.0);
+            if (tk >= 0) acct_add(c, c.kt_counter(tk), acct.p + 2);
         }
     });
     PGGC_LAUNCH_CHECK();
@@ -947,22 +1022,6 @@ __global__ void __launch_bounds__(256) k_ab_pack(i64 n, const float* __restrict_
     info[i] = a;
 }
 
-// algorithmic bytes of the filter: per candidate edge the column, weight and
-// the neighbour's gain, destination, cluster and vertex weight; per
-// candidate its id, row bounds and keep flag
-template <class WT>
-__global__ void __launch_bounds__(256) k_ab_bytes(const i64* __restrict__ rp, const i32* __restrict__ cand, i64 k,
-                                                  u64* acct) {
-    const i64 x = thread_item();
-    i64 b = 0;
-    if (x < k) {
-        const i32 i = cand[x];
-        b = (rp[i + 1] - rp[i]) * (20 + WT::bytes) + 16;
-    }
-    b = block_sum_i64<256>(b);
-    if (threadIdx.x == 0 && b) atomicAdd((unsigned long long*)acct, (unsigned long long)b);
-}
-
 static i64 afterburner(Ctx& c, MoveScratch& s, const DGraph& g, const i32* cand, i64 k, const i64* b, float lam,
                        i32* moves) {
     const int tk = c.kt_begin(KS_AFTERBURNER);
@@ -982,9 +1041,16 @@ static i64 afterburner(Ctx& c, MoveScratch& s, const DGraph& g, const i32* cand,
     c.kt_end(tk, 0.0);
     PGGC_LAUNCH_CHECK();
     if (tk >= 0) {  // timing runs only: count the bytes outside the timed region
-        PGGC_WDISPATCH(g, (k_ab_bytes<decltype(W)><<<grid_items(k, 256), 256, 0, c.stream>>>(g.rp, cand, k,
-                                                                                            c.kt_counter(tk))));
+        // the pack reads gain, dest, part and vertex weight and writes a
+        // 16-byte record per vertex; the filter reads per candidate edge the
+        // column, weight and the neighbour's record, per candidate its id, row
+        // bounds and keep flag
+        Buf<i64> acc(c, 1);
+        fill_i64(c, acc, 1, 0);
+        k_list_bytes<<<grid_items(std::max<i64>(k, 1), 256), 256, 0, c.stream>>>(
+            g.rp, cand, k, 24, 20 + (g.wk == W_UNIT ? 0 : g.wk == W_I32 ? 4 : 8), 36 * g.n, acc);
         PGGC_LAUNCH_CHECK();
+        acct_add(c, c.kt_counter(tk), acc);
     }
     return compact_flags(c, s.flag, cand, k, moves, s.scal);
 }
@@ -1210,7 +1276,8 @@ static void perform_moves(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g
     const i64 m7 = lp.sc_start[CL_HUB + 1] - lp.sc_start[CL_HUB];
     // many movers: updating costs more than rebuilding the tables next sweep
     if (lp.hub_live && m7 > 0 && (c.hub_update_div <= 0 || k * c.hub_update_div > g.n)) lp.hub_live = false;
-    if (lp.hub_live && m7 > 0) {  // keep the hub tables current (before part changes)
+    const bool hub_upd = lp.hub_live && m7 > 0;
+    if (hub_upd) {  // keep the hub tables current (before part changes)
         i64* stale = s.scal.p + 3;
         PGGC_WDISPATCH(g, {
             if (lp.packed) {
@@ -1259,6 +1326,20 @@ static void perform_moves(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g
     k_apply_moves<<<grid_items(k, 256), 256, 0, c.stream>>>(moves, k, g.vw, s.part, s.dest, curr.vol);
     c.kt_end(tk, 0.0);
     PGGC_LAUNCH_CHECK();
+    if (tk >= 0) {  // timing runs only
+        // per vertex the packed record (part, dest, flag read, 8 bytes
+        // written); per mover row bounds, part, dest, the partial delta and
+        // its sum, vertex weight, two volume updates and the part write; per
+        // mover edge column, weight and the neighbour's record, plus the
+        // column, weight and hub index again when hub tables are updated
+        const int wb = g.wk == W_UNIT ? 0 : g.wk == W_I32 ? 4 : 8;
+        Buf<i64> acc(c, 1);
+        fill_i64(c, acc, 1, 0);
+        k_list_bytes<<<grid_items(std::max<i64>(k, 1), 256), 256, 0, c.stream>>>(
+            g.rp, moves, k, 72, 12 + wb + (hub_upd ? 8 + wb : 0), 20 * g.n, acc);
+        PGGC_LAUNCH_CHECK();
+        acct_add(c, c.kt_counter(tk), acc);
+    }
     if (!tables) {
         tables = true;
         fill_i32(c, s.dest, g.n, NULL_PART);

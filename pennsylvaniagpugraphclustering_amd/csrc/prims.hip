@@ -316,19 +316,34 @@ i64 compact_flags(Ctx& c, const i32* flag, const i32* src, i64 n, i32* out, i64*
 // Stable LSD sort of (key, value) pairs on the low `bits` key bits: rocPRIM's
 // onesweep radix sort (a plain library sort, compiled from the ROCm headers),
 // with its temporary storage and the second buffers from the context's pool.
-void radix_sort_pairs(Ctx& c, u64* keys, i32* vals, i64 n, int bits) {
+template <class V>
+void radix_sort_pairs_t(Ctx& c, u64* keys, V* vals, i64 n, int bits) {
     if (n <= 1 || bits <= 0) return;
     Buf<u64> k2(c, n);
-    Buf<i32> v2(c, n);
+    Buf<V> v2(c, n);
     rocprim::double_buffer<u64> kd(keys, k2.p);
-    rocprim::double_buffer<i32> vd(vals, v2.p);
+    rocprim::double_buffer<V> vd(vals, v2.p);
     const unsigned int end_bit = (unsigned int)std::min(bits, 64);
     size_t tb = 0;
     PGGC_HIP(rocprim::radix_sort_pairs(nullptr, tb, kd, vd, (size_t)n, 0u, end_bit, c.stream));
     Buf<u64> tmp(c, std::max<size_t>((tb + 7) / 8, 1));
     PGGC_HIP(rocprim::radix_sort_pairs((void*)tmp.p, tb, kd, vd, (size_t)n, 0u, end_bit, c.stream));
     if (kd.current() != keys) copy_dev(c, keys, kd.current(), n * sizeof(u64));
-    if (vd.current() != vals) copy_dev(c, vals, vd.current(), n * sizeof(i32));
+    if (vd.current() != vals) copy_dev(c, vals, vd.current(), n * sizeof(V));
+}
+void radix_sort_pairs(Ctx& c, u64* keys, i32* vals, i64 n, int bits) { radix_sort_pairs_t(c, keys, vals, n, bits); }
+void radix_sort_pairs(Ctx& c, u64* keys, i64* vals, i64 n, int bits) { radix_sort_pairs_t(c, keys, vals, n, bits); }
+
+void radix_sort_keys(Ctx& c, u64* keys, i64 n, int bits) {
+    if (n <= 1 || bits <= 0) return;
+    Buf<u64> k2(c, n);
+    rocprim::double_buffer<u64> kd(keys, k2.p);
+    const unsigned int end_bit = (unsigned int)std::min(bits, 64);
+    size_t tb = 0;
+    PGGC_HIP(rocprim::radix_sort_keys(nullptr, tb, kd, (size_t)n, 0u, end_bit, c.stream));
+    Buf<u64> tmp(c, std::max<size_t>((tb + 7) / 8, 1));
+    PGGC_HIP(rocprim::radix_sort_keys((void*)tmp.p, tb, kd, (size_t)n, 0u, end_bit, c.stream));
+    if (kd.current() != keys) copy_dev(c, keys, kd.current(), n * sizeof(u64));
 }
 
 // ------------------------------------------------------------ exact reductions
@@ -341,6 +356,15 @@ __global__ void __launch_bounds__(256) k_sum_i64(const i64* in, i64 n, i64* out)
     }
     s = block_sum_i64<256>(s);
     if (threadIdx.x == 0) atomicAdd((unsigned long long*)out, (unsigned long long)s);
+}
+
+__global__ void k_acct_add(u64* dst, const i64* src) {
+    if (threadIdx.x == 0) *dst += (u64)*src;
+}
+void acct_add(Ctx& c, u64* dst, const i64* src) {
+    if (!dst) return;
+    k_acct_add<<<1, 64, 0, c.stream>>>(dst, src);
+    PGGC_LAUNCH_CHECK();
 }
 
 void sum_i64(Ctx& c, const i64* in, i64 n, i64* out) {

@@ -6,7 +6,8 @@ weights).  This is the layout the reference loads (app/io/io_mtx.cpp:95-201
 symmetrises, deduplicates and sorts; app/io/io_metis.cpp reads it as given).
 
 Small graphs are generated with numpy (deterministic across machines); the
-R-MAT generator can also run on a torch device for the large workloads.
+large R-MAT workloads are built on the GPU by the library (rmat_device), from
+counter-based draws that the CPU oracle reproduces.
 """
 import numpy as np
 
@@ -156,54 +157,74 @@ def rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=2, weights=None):
     return from_edges(n, src, dst, w)
 
 
-def rmat_torch(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=2, weights=None, device="cuda"):
-    """Large R-MAT built with torch on `device` (same construction as
-    :func:`rmat`, different random stream).  Returns numpy-free torch tensors
-    (rp int64, col int32, w int32|None) resident on the device."""
+def _devgraph_tensors(h, ctx):
+    """Copy a library-owned device graph into torch tensors and free it."""
+    import ctypes
     import torch
-    gen = torch.Generator(device=device)
-    gen.manual_seed(seed)
-    n = 1 << scale
-    m = edge_factor << scale
-    src = torch.zeros(m, dtype=torch.int64, device=device)
-    dst = torch.zeros(m, dtype=torch.int64, device=device)
-    for lvl in range(scale):
-        r = torch.rand(m, generator=gen, device=device)
-        bs = r >= a + b
-        bd = ((r >= a) & (r < a + b)) | (r >= a + b + c)
-        sh = scale - 1 - lvl
-        src |= bs.to(torch.int64) << sh
-        dst |= bd.to(torch.int64) << sh
-        del r, bs, bd
-    keep = src != dst
-    src, dst = src[keep], dst[keep]
-    lo = torch.minimum(src, dst)
-    hi = torch.maximum(src, dst)
-    del src, dst, keep
-    key = torch.unique(lo * n + hi)
-    del lo, hi
-    lo = key // n
-    hi = key % n
-    del key
-    w_und = None
-    if weights is not None:
-        w_und = torch.randint(weights[0], weights[1] + 1, (lo.numel(),), generator=gen, device=device,
-                              dtype=torch.int32)
-    k2 = torch.cat([lo * n + hi, hi * n + lo])
-    ww = None if w_und is None else torch.cat([w_und, w_und])
-    del lo, hi, w_und
-    k2, order = torch.sort(k2)
-    if ww is not None:
-        ww = ww[order]
-    del order
-    s = k2 // n
-    col = (k2 % n).to(torch.int32)
-    del k2
-    cnt = torch.bincount(s, minlength=n)
-    del s
-    rp = torch.zeros(n + 1, dtype=torch.int64, device=device)
-    rp[1:] = torch.cumsum(cnt, 0)
-    return rp, col, ww
+    from . import _lib
+    lib = ctx.lib
+    try:
+        v = _lib.Graph()
+        _lib.check(lib.pggc_devgraph_view(h, ctypes.byref(v)))
+        dev = torch.device("cuda")
+        rp = torch.empty(v.n + 1, dtype=torch.int64, device=dev)
+        col = torch.empty(max(v.nnz, 1), dtype=torch.int32, device=dev)
+        w = torch.empty(max(v.nnz, 1), dtype=torch.int32, device=dev) if v.w else None
+        _lib.check(lib.pggc_devgraph_copy(ctx.h, h, ctypes.c_void_p(rp.data_ptr()), ctypes.c_void_p(col.data_ptr()),
+                                          None if w is None else ctypes.c_void_p(w.data_ptr())))
+        torch.cuda.current_stream().synchronize()
+    finally:
+        lib.pggc_devgraph_free(h)
+    nnz = int(v.nnz)
+    return rp, col[:nnz], None if w is None else w[:nnz]
+
+
+def rmat_device(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=2, weights=None, ctx=None):
+    """R-MAT graph built on the GPU by the library (pggc_rmat): Philox4x32-10
+    draws, symmetrised, deduplicated, no self loops, sorted rows; weights
+    uniform in [lo, hi] per undirected pair when `weights` = (lo, hi).  The
+    oracle's generator builds the identical graph on the CPU.  Returns device
+    tensors (rp int64, col int32, w int32 or None)."""
+    import ctypes
+    from . import _lib
+    from .clustering import default_context
+    ctx = ctx or default_context()
+    wmin, wmax = (0, 0) if weights is None else weights
+    h = ctypes.c_void_p()
+    _lib.check(ctx.lib.pggc_rmat(ctx.h, scale, edge_factor, a, b, c, seed, wmin, wmax, ctypes.byref(h)))
+    return _devgraph_tensors(h, ctx)
+
+
+def from_edges_device(n, src, dst, w=None, ctx=None):
+    """construct_from_edgelist on the GPU (pggc_graph_from_edges): device
+    int32 src/dst (and int32 weights) -> (rp, col, w) device tensors."""
+    import ctypes
+    import torch
+    from . import _lib
+    from .clustering import default_context
+    ctx = ctx or default_context()
+    src = src.to(torch.int32).contiguous()
+    dst = dst.to(torch.int32).contiguous()
+    w = None if w is None else w.to(torch.int32).contiguous()
+    h = ctypes.c_void_p()
+    _lib.check(ctx.lib.pggc_graph_from_edges(ctx.h, n, src.numel(), ctypes.c_void_p(src.data_ptr()),
+                                             ctypes.c_void_p(dst.data_ptr()),
+                                             None if w is None else ctypes.c_void_p(w.data_ptr()),
+                                             ctypes.byref(h)))
+    return _devgraph_tensors(h, ctx)
+
+
+def random_labels_device(n, L, seed, ctx=None):
+    """labels[i] = Philox4x32-10(counter (i, 0, 0, 2), key seed)[0] mod L on
+    the GPU (pggc_random_labels)."""
+    import ctypes
+    import torch
+    from . import _lib
+    from .clustering import default_context
+    ctx = ctx or default_context()
+    out = torch.empty(max(n, 1), dtype=torch.int32, device="cuda")
+    _lib.check(ctx.lib.pggc_random_labels(ctx.h, n, L, seed, ctypes.c_void_p(out.data_ptr())))
+    return out[:n]
 
 
 def _csr_from_pairs_torch(lo, hi, n, device):

@@ -43,7 +43,7 @@ def make_graph(spec):
     """Device graph of a spec: ("rmat", scale, seed) or ("sbm", n, seed)."""
     kind, size, seed = spec
     if kind == "rmat":
-        rp, col, _ = graphs.rmat_torch(size, 16, seed=seed, device="cuda")
+        rp, col, _ = graphs.rmat_device(size, 16, seed=seed)
         return C.DeviceGraph(rp, col, None, None, objective="Mod")
     g = graphs.sbm(size, max(2, size // 128), 12, 4, seed=seed, permute_seed=seed)
     return C.DeviceGraph.from_csr(g, "Mod")

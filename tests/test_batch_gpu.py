@@ -23,7 +23,7 @@ def batch_graphs():
         b = int(rng.integers(max(1, n // 256), max(2, n // 32) + 1))
         rp, col = graphs.sbm_torch(n, b, 10, 3, seed=100 + k)
         out.append(C().DeviceGraph(rp, col, None, None, objective="Mod"))
-    rp, col, _ = graphs.rmat_torch(14, 16, seed=7)
+    rp, col, _ = graphs.rmat_device(14, 16, seed=7)
     out.append(C().DeviceGraph(rp, col, None, None, objective="Mod"))
     out.append(C().DeviceGraph.from_csr(graphs.rmat(12, 16, seed=3, weights=(1, 50)), "WMod"))
     return out
@@ -44,7 +44,7 @@ def test_batch_equals_single_runs(gpu_ctx, algo, objective):
 
 
 def test_batch_of_seeded_trials(gpu_ctx):
-    rp, col, _ = graphs.rmat_torch(15, 16, seed=2)
+    rp, col, _ = graphs.rmat_device(15, 16, seed=2)
     g = C().DeviceGraph(rp, col, None, None, objective="Mod")
     seeds = list(range(100, 116))
     res = C().cluster_batch([g] * len(seeds), "leiden+", "Mod", 1.0, 0, seeds=seeds, streams=8)

@@ -1,0 +1,72 @@
+"""GPU parity of the device-side graph construction (pggc_graph_from_edges,
+the construct_from_edgelist of app/io/io_mtx.cpp:95-201) and of the
+counter-based generators (pggc_rmat, pggc_random_labels) against the CPU
+oracle's generator and the numpy construction, bit for bit."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pennsylvaniagpugraphclustering_amd import graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def npy(t):
+    return None if t is None else t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("scale,ef,weights,seed", [(1, 1, None, 3), (6, 4, None, 1), (10, 16, None, 2),
+                                                    (12, 16, (1, 10), 4), (16, 16, None, 2),
+                                                    (14, 8, (1, 100), 3)])
+def test_rmat_matches_oracle_generator(gpu_ctx, scale, ef, weights, seed):
+    print(f"rmat s{scale} ef{ef} w={weights} seed={seed}", flush=True)
+    rp, col, w = graphs.rmat_device(scale, ef, seed=seed, weights=weights)
+    orp, ocol, ow = O.rmat(scale, ef, seed=seed, weights=weights)
+    assert np.array_equal(npy(rp), orp)
+    assert np.array_equal(npy(col), ocol)
+    if weights is None:
+        assert w is None and ow is None
+    else:
+        assert np.array_equal(npy(w), ow)
+        assert npy(w).min() >= weights[0] and npy(w).max() <= weights[1]
+
+
+def test_random_labels_match_oracle(gpu_ctx):
+    for n, L, seed in [(1, 1, 0), (1000, 7, 3), (1 << 20, 65536, 2)]:
+        lab = npy(graphs.random_labels_device(n, L, seed))
+        assert np.array_equal(lab, O.random_labels(n, L, seed))
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_from_edges_matches_numpy_construction(gpu_ctx, weighted):
+    """Self loops, zero weights and repeated pairs (in both directions) in
+    the input; the repeated pair keeps the weight of its first occurrence."""
+    import torch
+    rng = np.random.default_rng(5)
+    for n, m in [(1, 0), (2, 5), (50, 400), (3000, 20000), (70000, 300000)]:
+        src = rng.integers(0, n, m)
+        dst = rng.integers(0, n, m)
+        if m:
+            src[: m // 10] = dst[: m // 10]  # self loops
+        w = rng.integers(0, 20, m) if weighted else None
+        print(f"from_edges n={n} m={m} weighted={weighted}", flush=True)
+        rp, col, wd = graphs.from_edges_device(n, torch.from_numpy(src).cuda(), torch.from_numpy(dst).cuda(),
+                                               None if w is None else torch.from_numpy(w).cuda())
+        keep = np.ones(m, bool) if w is None else w != 0
+        ref = graphs.from_edges(n, src[keep], dst[keep], None if w is None else w[keep])
+        assert np.array_equal(npy(rp), ref.rp)
+        assert np.array_equal(npy(col), ref.col)
+        if weighted and ref.nnz > 0:
+            assert np.array_equal(npy(wd), ref.w)
+
+
+def test_from_edges_rejects_bad_input(gpu_ctx):
+    import torch
+    from pennsylvaniagpugraphclustering_amd import _lib
+    src = torch.tensor([0, 1, 5], dtype=torch.int32, device="cuda")
+    dst = torch.tensor([1, 2, 0], dtype=torch.int32, device="cuda")
+    with pytest.raises(_lib.PggcError):
+        graphs.from_edges_device(4, src, dst)  # id 5 outside [0, 4)
+    w = torch.tensor([1, -2, 3], dtype=torch.int32, device="cuda")
+    with pytest.raises(_lib.PggcError):
+        graphs.from_edges_device(6, src, dst, w)  # negative weight

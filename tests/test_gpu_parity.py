@@ -252,7 +252,7 @@ def test_determinism_rmat20(gpu_ctx):
     two runs give the same clustering and objective, and the objective equals
     a fresh evaluation of the returned clustering."""
     import torch
-    rp, col, _ = graphs.rmat_torch(20, 16, seed=2, device="cuda")
+    rp, col, _ = graphs.rmat_device(20, 16, seed=2)
     dg = C().DeviceGraph(rp, col, None, None, objective="Mod")
     print("determinism rmat20", flush=True)
     q1, p1, i1 = C().cluster(dg, "louvain", "Mod", 1.0, 0, seed=1)
@@ -293,53 +293,3 @@ def test_sbm_131072_planted_partition(gpu_ctx):
     assert q == rq
     for k in ["labels", "uncut", "sweeps", "moves", "outer"]:
         assert info[k] == rinfo[k], k
-
-
-def _torch_contract(rp, col, labels, nc):
-    """Independent contraction by sorting (coarse row, coarse column) keys:
-    unit edge weights, so the summed weight of a coarse edge is its count."""
-    import torch
-    n = rp.numel() - 1
-    row = torch.repeat_interleave(torch.arange(n, device=rp.device, dtype=torch.int64), rp.diff())
-    cu = labels[row].to(torch.int64)
-    del row
-    cv = labels[col.to(torch.int64)].to(torch.int64)
-    keep = cu != cv
-    key = cu[keep] * nc + cv[keep]
-    del cu, cv, keep
-    ukey, cnt = torch.unique(key, sorted=True, return_counts=True)
-    del key
-    urow = ukey // nc
-    orp = torch.zeros(nc + 1, dtype=torch.int64, device=rp.device)
-    orp[1:] = torch.cumsum(torch.bincount(urow, minlength=nc), 0)
-    return orp, (ukey % nc).to(torch.int32), cnt.to(torch.int64)
-
-
-def test_contraction_rmat24_65536_clusters(gpu_ctx):
-    """R-MAT scale 24 (16.7M vertices, edge factor 16, seed 2): the coarse
-    graph of a fixed random 65,536-cluster labelling is bit-identical to an
-    independent sort-based contraction, and the evaluator's volumes, uncut
-    weight and exact sum of squared volumes match direct reductions."""
-    import torch
-    rp, col, _ = graphs.rmat_torch(24, 16, seed=2, device="cuda")
-    n, nc = rp.numel() - 1, 65536
-    gen = torch.Generator(device="cuda").manual_seed(11)
-    lab = torch.randint(0, nc, (n,), generator=gen, device="cuda", dtype=torch.int32)
-    lab[:nc] = torch.arange(nc, device="cuda", dtype=torch.int32)
-    dg = C().DeviceGraph(rp, col, None, None, objective="Mod")
-    print(f"contract rmat24 n={n} nnz={col.numel()} nc={nc}", flush=True)
-    crp, ccol, cw = C().build_coarse_graph(dg, lab, nc)
-    erp, ecol, ew = _torch_contract(rp, col, lab, nc)
-    assert torch.equal(crp, erp)
-    assert torch.equal(ccol, ecol)
-    assert torch.equal(cw, ew)
-    del crp, ccol, cw, erp, ecol, ew
-    print("evaluate rmat24", flush=True)
-    vol, uncut, sq = C().evaluate(dg, lab, nc)
-    deg = rp.diff()
-    evol = torch.zeros(nc, dtype=torch.int64, device="cuda").index_add_(0, lab.to(torch.int64), deg)
-    assert torch.equal(vol, evol)
-    row = torch.repeat_interleave(torch.arange(n, device="cuda", dtype=torch.int64), deg)
-    euncut = int((lab[row] == lab[col.to(torch.int64)]).sum().item())
-    assert uncut == euncut
-    assert sq == sum(int(v) * int(v) for v in evol.cpu().tolist())

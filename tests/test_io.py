@@ -97,6 +97,28 @@ def test_mtx_rejects_bipartite_and_array(tmp_path):
     assert "non-coordinate" in str(e.value)
 
 
+@pytest.mark.parametrize("fmt", ["metis", "mtx"])
+def test_negative_and_oversized_weights_rejected(tmp_path, fmt):
+    """Edge weights must be in [0, 2^31 - 1]: a negative weight or one that
+    does not fit int32 is an error, not a silent wrap or a dropped sign."""
+    for bad, msg in [("-3", "Negative"), ("2147483648", "exceeds")]:
+        if fmt == "metis":
+            p = tmp_path / "w.graph"
+            p.write_text(f"2 1 1\n2 {bad}\n1 {bad}\n")
+        else:
+            p = tmp_path / "w.mtx"
+            p.write_text(f"%%MatrixMarket matrix coordinate integer symmetric\n2 2 1\n2 1 {bad}\n")
+        with pytest.raises(_lib.PggcError) as e:
+            graphs.read_graph(p, "WMod")
+        assert msg in str(e.value)
+    # the largest int32 weight is read as is
+    if fmt == "metis":
+        p.write_text("2 1 1\n2 2147483647\n1 2147483647\n")
+    else:
+        p.write_text("%%MatrixMarket matrix coordinate integer symmetric\n2 2 1\n2 1 2147483647\n")
+    assert list(graphs.read_graph(p, "WMod").w) == [2147483647, 2147483647]
+
+
 def test_partition_file_round_trip(tmp_path):
     lib = _lib.load()
     part = np.random.default_rng(0).integers(0, 1000, 777).astype(np.int32)
