@@ -75,6 +75,13 @@ def lib():
         L.or_graph_copy.argtypes = [c.c_void_p, i64p, i32p, vp]
         L.or_graph_free.restype = None
         L.or_graph_free.argtypes = [c.c_void_p]
+        L.or_lfr.restype = c.c_void_p
+        L.or_lfr.argtypes = [c.c_int64, c.c_double, c.c_double, c.c_int32, c.c_double, c.c_int32, c.c_int32,
+                             c.c_double, c.c_int32, c.c_int32, c.c_uint64, vp, vp, vp, c.c_int64,
+                             c.POINTER(c.c_int64)]
+        L.or_power_law_table.restype = c.c_int64
+        L.or_power_law_table.argtypes = [c.c_double, c.c_int32, c.c_int32, c.c_double, c.POINTER(c.c_int32), u64p,
+                                         c.c_int64]
         L.or_random_labels.restype = None
         L.or_random_labels.argtypes = [c.c_int64, c.c_int64, c.c_uint64, i32p]
         L.or_sweep_detail.restype = c.c_int64
@@ -246,6 +253,44 @@ def rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=2, weights=None):
     finally:
         L.or_graph_free(h)
     return rp, col, w
+
+
+def _take_graph(L, h):
+    try:
+        n, nnz, wd = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        L.or_graph_info(h, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(wd))
+        rp = np.empty(n.value + 1, np.int64)
+        col = np.empty(nnz.value, np.int32)
+        w = np.empty(nnz.value, np.int32) if wd.value else None
+        L.or_graph_copy(h, rp, col, _ptr(w))
+    finally:
+        L.or_graph_free(h)
+    return rp, col, w
+
+
+def lfr(n, tau1=2.5, kmean=20.0, kmax=1000, tau2=1.5, cmin=20, cmax=2000, mu=0.4, weights=None, seed=3):
+    """LFR-style benchmark graph (the oracle's generator; Philox4x32-10
+    draws): (rp, col, w or None, planted community per vertex, drawn degree
+    per generated vertex, community sizes)."""
+    wmin, wmax = (0, 0) if weights is None else weights
+    L = lib()
+    gt = np.empty(n, np.int32)
+    deg = np.empty(n, np.int32)
+    cap = n + 1
+    sizes = np.empty(cap, np.int64)
+    nc = ctypes.c_int64()
+    h = L.or_lfr(n, tau1, kmean, kmax, tau2, cmin, cmax, mu, wmin, wmax, seed, gt.ctypes.data, deg.ctypes.data,
+                 sizes.ctypes.data, cap, ctypes.byref(nc))
+    rp, col, w = _take_graph(L, h)
+    return rp, col, w, gt, deg, sizes[: nc.value].copy()
+
+
+def power_law_table(tau, lo, top, mean=0.0):
+    """(k0, thresholds) of the integer power law the generators draw from."""
+    thr = np.empty(top - lo + 2, np.uint64)
+    k0 = ctypes.c_int32()
+    k = lib().or_power_law_table(tau, lo, top, mean, ctypes.byref(k0), thr, len(thr))
+    return k0.value, thr[:k].copy()
 
 
 def random_labels(n, L, seed):

@@ -127,6 +127,8 @@ static inline P4 philox4x32_10(P4 c, u32 k0, u32 k1) {
 
 // Streams of the generators (counter word 3).
 constexpr u32 STREAM_RMAT = 0, STREAM_WEIGHT = 1, STREAM_LABEL = 2;
+constexpr u32 STREAM_LFR_DEG = 3, STREAM_LFR_COMM = 4, STREAM_LFR_ISTUB = 5, STREAM_LFR_ESTUB = 6,
+              STREAM_LFR_PERM = 7;
 
 // R-MAT quadrant thresholds on a u32 draw: r < t[0] -> (0,0), < t[1] ->
 // (0,1), < t[2] -> (1,0), else (1,1) (Chakrabarti, Zhan, Faloutsos, SDM'04).
@@ -1091,29 +1093,9 @@ static std::vector<i32> louvain_part(const Graph& top, State& rfd, bool constrai
     return parts[0];
 }
 
-// Symmetric CSR of an R-MAT draw: 2^scale vertices, edge_factor * 2^scale
-// draws, self loops dropped, repeated pairs merged, both directions stored,
-// rows sorted; weights (wmax > 0) are drawn per undirected pair.
-static Graph build_rmat(int scale, int edge_factor, double a, double b, double c, u64 seed, i32 wmin, i32 wmax) {
-    const i64 n = (i64)1 << scale;
-    const i64 m = (i64)edge_factor << scale;
-    u64 t[3];
-    rmat_thresholds(a, b, c, t);
-    std::vector<u64> key(m);
-#pragma omp parallel for schedule(static)
-    for (i64 e = 0; e < m; e++) {
-        u32 s, d;
-        rmat_edge((u64)e, scale, t, seed, s, d);
-        const u32 lo = std::min(s, d), hi = std::max(s, d);
-        key[e] = lo == hi ? ~(u64)0 : ((u64)lo << 32) | hi;
-    }
-    __gnu_parallel::sort(key.begin(), key.end());
-    i64 u = 0;
-    for (i64 e = 0; e < m; e++) {
-        if (key[e] == ~(u64)0) break;
-        if (u == 0 || key[e] != key[u - 1]) key[u++] = key[e];
-    }
-    key.resize(u);
+// Symmetric CSR of sorted, deduplicated pair keys (lo << 32 | hi, lo < hi):
+// both directions stored, rows sorted; weights (wmax > 0) drawn per pair.
+static Graph csr_from_pairs(i64 n, const std::vector<u64>& key, u64 seed, i32 wmin, i32 wmax) {
     Graph g;
     g.n = n;
     g.rp.assign(n + 1, 0);
@@ -1135,6 +1117,216 @@ static Graph build_rmat(int scale, int edge_factor, double a, double b, double c
         if (!g.uniform) g.w[a0] = g.w[a1] = pair_weight(lo, hi, seed, wmin, wmax);
     }
     return g;
+}
+
+// sort, drop the invalid key ~0 and repeats
+static void sort_unique(std::vector<u64>& key) {
+    __gnu_parallel::sort(key.begin(), key.end());
+    i64 u = 0;
+    for (size_t e = 0; e < key.size(); e++) {
+        if (key[e] == ~(u64)0) break;
+        if (u == 0 || key[e] != key[u - 1]) key[u++] = key[e];
+    }
+    key.resize(u);
+}
+
+// Symmetric CSR of an R-MAT draw: 2^scale vertices, edge_factor * 2^scale
+// draws, self loops dropped, repeated pairs merged, both directions stored,
+// rows sorted; weights (wmax > 0) are drawn per undirected pair.
+static Graph build_rmat(int scale, int edge_factor, double a, double b, double c, u64 seed, i32 wmin, i32 wmax) {
+    const i64 n = (i64)1 << scale;
+    const i64 m = (i64)edge_factor << scale;
+    u64 t[3];
+    rmat_thresholds(a, b, c, t);
+    std::vector<u64> key(m);
+#pragma omp parallel for schedule(static)
+    for (i64 e = 0; e < m; e++) {
+        u32 s, d;
+        rmat_edge((u64)e, scale, t, seed, s, d);
+        const u32 lo = std::min(s, d), hi = std::max(s, d);
+        key[e] = lo == hi ? ~(u64)0 : ((u64)lo << 32) | hi;
+    }
+    sort_unique(key);
+    return csr_from_pairs(n, key, seed, wmin, wmax);
+}
+
+// ---------------------------------------------------------------- LFR-style
+// Benchmark graph in the manner of Lancichinetti, Fortunato and Radicchi
+// (Phys. Rev. E 78, 046110, 2008), stated for exact CPU/GPU agreement:
+//  - integer power laws: X with density ~ x^-tau on [x0, top + 1), value
+//    floor(X); the degree law's x0 is solved by bisection for the mean
+//    degree, the community-size law starts at cmin; a u32 Philox draw r maps
+//    to the first value whose cumulative threshold floor(CDF * 2^32) exceeds r;
+//  - community sizes are drawn in sequence until they cover n; the last takes
+//    what is left and joins the one before if that is below cmin;
+//  - vertices fill the communities in id order; internal degree
+//    round((1 - mu) * degree) (mu in thousandths), at most size - 1;
+//  - stubs: internal ones sorted (stably) by (community, draw), paired in
+//    order inside each community; external ones sorted by draw and paired
+//    in order over the graph; pairs of one vertex and repeated pairs vanish in
+//    the simple-graph build;
+//  - vertex ids are permuted by a stable sort on one draw per vertex.
+struct PLTable {
+    i32 k0 = 0;
+    std::vector<u64> thr;
+};
+
+static void pl_probs(double tau, double x0, i32 top, std::vector<double>& p, i32& k0) {
+    k0 = (i32)std::floor(x0);
+    const double a = std::pow(x0, 1.0 - tau);
+    const double b = std::pow((double)top + 1.0, 1.0 - tau);
+    const double z = a - b;
+    p.assign((size_t)(top - k0 + 1), 0.0);
+    double prev = a;
+    for (i32 k = k0; k <= top; k++) {
+        const double nxt = std::pow((double)k + 1.0, 1.0 - tau);
+        p[(size_t)(k - k0)] = (prev - nxt) / z;
+        prev = nxt;
+    }
+}
+
+static double pl_mean(double tau, double x0, i32 top) {
+    std::vector<double> p;
+    i32 k0;
+    pl_probs(tau, x0, top, p, k0);
+    double m = 0.0;
+    for (size_t i = 0; i < p.size(); i++) m += (double)(k0 + (i32)i) * p[i];
+    return m;
+}
+
+static PLTable power_law_table(double tau, i32 lo, i32 top, double mean) {
+    double x0 = (double)lo;
+    if (mean > 0.0) {
+        double a = (double)lo, b = (double)top;
+        for (int it = 0; it < 200; it++) {
+            const double mid = 0.5 * (a + b);
+            if (pl_mean(tau, mid, top) < mean) a = mid;
+            else b = mid;
+        }
+        x0 = 0.5 * (a + b);
+    }
+    std::vector<double> p;
+    PLTable t;
+    pl_probs(tau, x0, top, p, t.k0);
+    t.thr.resize(p.size());
+    double cum = 0.0;
+    for (size_t i = 0; i < p.size(); i++) {
+        cum += p[i];
+        double f = std::floor(cum * 4294967296.0);
+        if (f < 0.0) f = 0.0;
+        if (f > 4294967296.0) f = 4294967296.0;
+        t.thr[i] = (u64)f;
+    }
+    t.thr.back() = (u64)1 << 32;
+    return t;
+}
+
+static i32 pl_draw(const PLTable& t, u32 r) {
+    const i32 i = (i32)(std::upper_bound(t.thr.begin(), t.thr.end(), (u64)r) - t.thr.begin());
+    return t.k0 + i;
+}
+
+static inline u32 draw32(u64 x, u32 stream, u64 seed) {
+    return philox4x32_10(P4{{(u32)x, (u32)(x >> 32), 0u, stream}}, (u32)seed, (u32)(seed >> 32)).v[0];
+}
+
+struct LfrSpec {
+    i64 n;
+    double tau1, kmean, tau2, mu;
+    i32 kmax, cmin, cmax, wmin, wmax;
+    u64 seed;
+};
+
+static std::vector<i64> lfr_sizes(const LfrSpec& p) {
+    const PLTable t = power_law_table(p.tau2, p.cmin, p.cmax, 0.0);
+    std::vector<i64> sz;
+    i64 sum = 0;
+    for (u64 j = 0; sum < p.n; j++) {
+        const i64 s = pl_draw(t, draw32(j, STREAM_LFR_COMM, p.seed));
+        sz.push_back(s);
+        sum += s;
+    }
+    if (!sz.empty()) {
+        const i64 rem = p.n - (sum - sz.back());
+        sz.back() = rem;
+        if (rem < p.cmin && sz.size() > 1) {
+            sz[sz.size() - 2] += rem;
+            sz.pop_back();
+        }
+    }
+    return sz;
+}
+
+// stable sort of (key, vertex) records by key
+static void stable_by_key(std::vector<std::pair<u64, i32>>& r) {
+    __gnu_parallel::stable_sort(r.begin(), r.end(),
+                                [](const std::pair<u64, i32>& a, const std::pair<u64, i32>& b) { return a.first < b.first; });
+}
+
+static Graph build_lfr(const LfrSpec& p, std::vector<i32>* gt, std::vector<i32>* degree) {
+    const i64 n = p.n;
+    const PLTable dt = power_law_table(p.tau1, 1, p.kmax, p.kmean);
+    const std::vector<i64> sz = lfr_sizes(p);
+    const i64 nc = (i64)sz.size();
+    std::vector<i64> cs(nc + 1, 0);
+    for (i64 k = 0; k < nc; k++) cs[k + 1] = cs[k] + sz[k];
+    const i64 mu_m = std::llround(p.mu * 1000.0);
+    std::vector<i32> kin(n), kext(n), comm(n);
+    for (i64 c = 0; c < nc; c++) {
+        for (i64 v = cs[c]; v < cs[c + 1]; v++) {
+            const i64 deg = pl_draw(dt, draw32((u64)v, STREAM_LFR_DEG, p.seed));
+            i64 ki = (deg * (1000 - mu_m) + 500) / 1000;
+            ki = std::min(ki, sz[c] - 1);
+            kin[v] = (i32)ki;
+            kext[v] = (i32)(deg - ki);
+            comm[v] = (i32)c;
+        }
+    }
+    if (degree) {
+        degree->resize(n);
+        for (i64 v = 0; v < n; v++) (*degree)[v] = kin[v] + kext[v];
+    }
+    // vertex permutation: rank of each vertex's draw
+    std::vector<i32> pi(n);
+    {
+        std::vector<std::pair<u64, i32>> r(n);
+        for (i64 v = 0; v < n; v++) r[v] = {draw32((u64)v, STREAM_LFR_PERM, p.seed), (i32)v};
+        stable_by_key(r);
+        for (i64 k = 0; k < n; k++) pi[r[k].second] = (i32)k;
+    }
+    if (gt) {
+        gt->resize(n);
+        for (i64 v = 0; v < n; v++) (*gt)[pi[v]] = comm[v];
+    }
+    std::vector<u64> key;
+    auto add_pair = [&](i32 a, i32 b) {
+        const u32 u = (u32)pi[a], w = (u32)pi[b];
+        if (u != w) key.push_back(((u64)std::min(u, w) << 32) | std::max(u, w));
+    };
+    {  // internal stubs, paired inside each community
+        std::vector<i64> oin(n + 1, 0);
+        for (i64 v = 0; v < n; v++) oin[v + 1] = oin[v] + kin[v];
+        std::vector<std::pair<u64, i32>> r(oin[n]);
+        for (i64 v = 0; v < n; v++)
+            for (i64 q = oin[v]; q < oin[v + 1]; q++)
+                r[q] = {((u64)(u32)comm[v] << 32) | draw32((u64)q, STREAM_LFR_ISTUB, p.seed), (i32)v};
+        stable_by_key(r);
+        for (i64 c = 0; c < nc; c++) {
+            const i64 s0 = oin[cs[c]], len = oin[cs[c + 1]] - s0;
+            for (i64 q = 0; q + 1 < len; q += 2) add_pair(r[s0 + q].second, r[s0 + q + 1].second);
+        }
+    }
+    {  // external stubs, paired over the whole graph
+        std::vector<i64> oext(n + 1, 0);
+        for (i64 v = 0; v < n; v++) oext[v + 1] = oext[v] + kext[v];
+        std::vector<std::pair<u64, i32>> r(oext[n]);
+        for (i64 v = 0; v < n; v++)
+            for (i64 q = oext[v]; q < oext[v + 1]; q++) r[q] = {draw32((u64)q, STREAM_LFR_ESTUB, p.seed), (i32)v};
+        stable_by_key(r);
+        for (i64 q = 0; q + 1 < (i64)r.size(); q += 2) add_pair(r[q].second, r[q + 1].second);
+    }
+    sort_unique(key);
+    return csr_from_pairs(n, key, p.seed, p.wmin, p.wmax);
 }
 
 }  // namespace orc
@@ -1170,6 +1362,29 @@ void or_philox(const u32* ctr, const u32* key, u32* out) {
 // R-MAT graph held by the oracle until or_graph_free
 void* or_rmat(int scale, int edge_factor, double a, double b, double c, u64 seed, i32 wmin, i32 wmax) {
     return new Graph(build_rmat(scale, edge_factor, a, b, c, seed, wmin, wmax));
+}
+// LFR-style graph held by the oracle until or_graph_free; gt (n entries,
+// nullable) receives the planted communities, degree (n, nullable) the drawn
+// degrees in generation order, sizes (nullable, sizes_cap entries) the
+// community sizes; returns the number of communities
+void* or_lfr(i64 n, double tau1, double kmean, i32 kmax, double tau2, i32 cmin, i32 cmax, double mu, i32 wmin,
+             i32 wmax, u64 seed, i32* gt, i32* degree, i64* sizes, i64 sizes_cap, i64* ncomm) {
+    const LfrSpec p{n, tau1, kmean, tau2, mu, kmax, cmin, cmax, wmin, wmax, seed};
+    std::vector<i32> g, d;
+    Graph* out = new Graph(build_lfr(p, gt ? &g : nullptr, degree ? &d : nullptr));
+    if (gt) std::copy(g.begin(), g.end(), gt);
+    if (degree) std::copy(d.begin(), d.end(), degree);
+    const std::vector<i64> sz = lfr_sizes(p);
+    *ncomm = (i64)sz.size();
+    if (sizes) std::copy(sz.begin(), sz.begin() + std::min<i64>(sizes_cap, (i64)sz.size()), sizes);
+    return out;
+}
+// the integer power-law table: k0 and up to cap thresholds; returns the count
+i64 or_power_law_table(double tau, i32 lo, i32 top, double mean, i32* k0, u64* thr, i64 cap) {
+    const PLTable t = power_law_table(tau, lo, top, mean);
+    *k0 = t.k0;
+    std::copy(t.thr.begin(), t.thr.begin() + std::min<i64>(cap, (i64)t.thr.size()), thr);
+    return (i64)t.thr.size();
 }
 void or_graph_info(void* h, i64* n, i64* nnz, int* weighted) {
     const Graph* g = (const Graph*)h;

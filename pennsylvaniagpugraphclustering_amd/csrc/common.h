@@ -150,6 +150,10 @@ __host__ __device__ __forceinline__ Philox4 philox4x32_10(u32 c0, u32 c1, u32 c2
 }
 // streams of the generators (counter word 3)
 constexpr u32 STREAM_RMAT = 0, STREAM_WEIGHT = 1, STREAM_LABEL = 2;
+// LFR-style generator: degrees, community sizes, internal and external stub
+// keys, vertex permutation
+constexpr u32 STREAM_LFR_DEG = 3, STREAM_LFR_COMM = 4, STREAM_LFR_ISTUB = 5, STREAM_LFR_ESTUB = 6,
+              STREAM_LFR_PERM = 7;
 
 // Monotone map float -> u32 (for packed argmax keys).
 __device__ __forceinline__ u32 ord_f32(float f) {

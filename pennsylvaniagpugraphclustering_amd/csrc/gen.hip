@@ -307,6 +307,261 @@ void edges_csr(Ctx& c, i64 n, i64 m, const i32* src, const i32* dst, const i32* 
     c.trim();
 }
 
+// ------------------------------------------------------------------ LFR
+// Power-law tables are built on the host in double precision (the CPU
+// restatement builds the same tables with the same operations); the device
+// only compares integer draws against them, so CPU and GPU draws agree bit
+// for bit.
+static void pl_probs(double tau, double x0, i32 top, std::vector<double>& p, i32& k0) {
+    k0 = (i32)std::floor(x0);
+    const double a = std::pow(x0, 1.0 - tau);
+    const double b = std::pow((double)top + 1.0, 1.0 - tau);
+    const double z = a - b;
+    p.assign((size_t)(top - k0 + 1), 0.0);
+    double prev = a;  // x^(1-tau) at the lower end of the current bin
+    for (i32 k = k0; k <= top; k++) {
+        const double nxt = std::pow((double)k + 1.0, 1.0 - tau);
+        p[(size_t)(k - k0)] = (prev - nxt) / z;
+        prev = nxt;
+    }
+}
+
+static double pl_mean(double tau, double x0, i32 top) {
+    std::vector<double> p;
+    i32 k0;
+    pl_probs(tau, x0, top, p, k0);
+    double m = 0.0;
+    for (size_t i = 0; i < p.size(); i++) m += (double)(k0 + (i32)i) * p[i];
+    return m;
+}
+
+PowerLawTable power_law_table(double tau, i32 lo, i32 top, double mean) {
+    if (!(tau > 0.0) || std::fabs(tau - 1.0) < 1e-6 || lo < 1 || top < lo)
+        throw Error(-1, "power law: need tau > 0, tau != 1 and 1 <= lo <= top");
+    double x0 = (double)lo;
+    if (mean > 0.0) {
+        double a = (double)lo, b = (double)top;
+        if (mean < pl_mean(tau, a, top) || mean > pl_mean(tau, b, top))
+            throw Error(-1, "power law: the requested mean is outside the reachable range");
+        for (int it = 0; it < 200; it++) {  // the mean grows with the lower end
+            const double mid = 0.5 * (a + b);
+            if (pl_mean(tau, mid, top) < mean) a = mid;
+            else b = mid;
+        }
+        x0 = 0.5 * (a + b);
+    }
+    std::vector<double> p;
+    PowerLawTable t;
+    pl_probs(tau, x0, top, p, t.k0);
+    t.thr.resize(p.size());
+    double cum = 0.0;
+    for (size_t i = 0; i < p.size(); i++) {
+        cum += p[i];
+        double f = std::floor(cum * 4294967296.0);
+        if (f < 0.0) f = 0.0;
+        if (f > 4294967296.0) f = 4294967296.0;
+        t.thr[i] = (u64)f;
+    }
+    t.thr.back() = (u64)1 << 32;
+    return t;
+}
+
+// first i with r < thr[i] (thr non-decreasing, thr[K-1] = 2^32)
+__host__ __device__ __forceinline__ i32 pl_index(const u64* thr, i32 K, u32 r) {
+    i32 lo = 0, hi = K - 1;
+    while (lo < hi) {
+        const i32 m = (lo + hi) >> 1;
+        if ((u64)r < thr[m]) hi = m;
+        else lo = m + 1;
+    }
+    return lo;
+}
+
+std::vector<i64> lfr_community_sizes(const LfrParams& p) {
+    const PowerLawTable t = power_law_table(p.tau2, p.cmin, p.cmax, 0.0);
+    std::vector<i64> sz;
+    i64 sum = 0;
+    for (u64 j = 0; sum < p.n; j++) {
+        const Philox4 r = philox4x32_10((u32)j, (u32)(j >> 32), 0u, STREAM_LFR_COMM, (u32)p.seed,
+                                        (u32)(p.seed >> 32));
+        const i64 s = t.k0 + pl_index(t.thr.data(), (i32)t.thr.size(), r.v[0]);
+        sz.push_back(s);
+        sum += s;
+    }
+    if (!sz.empty()) {
+        // the last community takes what is left; a remainder below cmin joins
+        // the community before it
+        const i64 rem = p.n - (sum - sz.back());
+        sz.back() = rem;
+        if (rem < p.cmin && sz.size() > 1) {
+            sz[sz.size() - 2] += rem;
+            sz.pop_back();
+        }
+    }
+    return sz;
+}
+
+// Vertex v lies in community c with cs[c] <= v < cs[c+1].  Degree from the
+// degree table; internal degree round((1 - mu) * degree) (mu in thousandths),
+// at most the community size - 1; the rest of the degree is external.
+__global__ void __launch_bounds__(256) k_lfr_degrees(i64 n, i64 nc, const i64* __restrict__ cs,
+                                                     const u64* __restrict__ thr, i32 K, i32 k0, i64 mu_m, u64 seed,
+                                                     i32* __restrict__ kin, i32* __restrict__ kext,
+                                                     i32* __restrict__ comm) {
+    const i64 v = thread_item();
+    if (v >= n) return;
+    const i64 c = upper_row(cs, 0, nc, v);
+    const i64 size = cs[c + 1] - cs[c];
+    const Philox4 r = philox4x32_10((u32)v, (u32)((u64)v >> 32), 0u, STREAM_LFR_DEG, (u32)seed, (u32)(seed >> 32));
+    const i64 deg = k0 + pl_index(thr, K, r.v[0]);
+    i64 ki = (deg * (1000 - mu_m) + 500) / 1000;
+    if (ki > size - 1) ki = size - 1;
+    kin[v] = (i32)ki;
+    kext[v] = (i32)(deg - ki);
+    comm[v] = (i32)c;
+}
+
+// stub p of the vertex v with off[v] <= p < off[v+1]: key = (community << 32
+// when WITH_COMM) | Philox draw of p
+template <bool WITH_COMM>
+__global__ void __launch_bounds__(256) k_lfr_stubs(i64 S, i64 n, const i64* __restrict__ off,
+                                                   const i32* __restrict__ comm, u64 seed, u32 stream,
+                                                   u64* __restrict__ keys, i32* __restrict__ vals) {
+    const i64 p = thread_item();
+    if (p >= S) return;
+    const i64 v = upper_row(off, 0, n, p);
+    const Philox4 r = philox4x32_10((u32)p, (u32)((u64)p >> 32), 0u, stream, (u32)seed, (u32)(seed >> 32));
+    keys[p] = WITH_COMM ? (((u64)(u32)comm[v] << 32) | r.v[0]) : (u64)r.v[0];
+    vals[p] = (i32)v;
+}
+
+__device__ __forceinline__ u64 lfr_pair_key(const i32* pi, i32 a, i32 b, int B, u64 invalid) {
+    const u32 u = (u32)pi[a], w = (u32)pi[b];
+    if (u == w) return invalid;
+    const u32 lo = u < w ? u : w, hi = u < w ? w : u;
+    return ((u64)lo << B) | hi;
+}
+
+// internal stubs sorted by (community, draw): the stubs of community c fill
+// [oin[cs[c]], oin[cs[c+1]]), paired (first, second), (third, fourth), ...
+__global__ void __launch_bounds__(256) k_lfr_pair_int(i64 S, const u64* __restrict__ keys, const i32* __restrict__ vals,
+                                                      const i64* __restrict__ oin, const i64* __restrict__ cs,
+                                                      const i32* __restrict__ pi, int B, u64 invalid,
+                                                      u64* __restrict__ ek) {
+    const i64 p = thread_item();
+    if (p >= S) return;
+    const i64 c = (i64)(keys[p] >> 32);
+    const i64 s0 = oin[cs[c]], len = oin[cs[c + 1]] - s0;
+    const i64 q = p - s0;
+    ek[p] = ((q & 1) == 0 && q + 1 < len) ? lfr_pair_key(pi, vals[p], vals[p + 1], B, invalid) : invalid;
+}
+
+// external stubs sorted by draw, paired consecutively over the whole graph
+__global__ void __launch_bounds__(256) k_lfr_pair_ext(i64 S, const i32* __restrict__ vals,
+                                                      const i32* __restrict__ pi, int B, u64 invalid,
+                                                      u64* __restrict__ ek) {
+    const i64 p = thread_item();
+    if (p >= S) return;
+    ek[p] = ((p & 1) == 0 && p + 1 < S) ? lfr_pair_key(pi, vals[p], vals[p + 1], B, invalid) : invalid;
+}
+
+__global__ void __launch_bounds__(256) k_lfr_perm_keys(i64 n, u64 seed, u64* __restrict__ keys,
+                                                       i32* __restrict__ vals) {
+    const i64 v = thread_item();
+    if (v >= n) return;
+    const Philox4 r = philox4x32_10((u32)v, (u32)((u64)v >> 32), 0u, STREAM_LFR_PERM, (u32)seed, (u32)(seed >> 32));
+    keys[v] = r.v[0];
+    vals[v] = (i32)v;
+}
+__global__ void __launch_bounds__(256) k_rank_scatter(i64 n, const i32* __restrict__ order, i32* __restrict__ pi) {
+    const i64 r = thread_item();
+    if (r >= n) return;
+    pi[order[r]] = (i32)r;
+}
+__global__ void __launch_bounds__(256) k_lfr_gt(i64 n, const i32* __restrict__ pi, const i32* __restrict__ comm,
+                                                i32* __restrict__ gt) {
+    const i64 v = thread_item();
+    if (v >= n) return;
+    gt[pi[v]] = comm[v];
+}
+
+void lfr_csr(Ctx& c, const LfrParams& p, DevCSR& out, i32* gt) {
+    if (p.n < 1 || p.n >= 0x7fffffff) throw Error(-1, "LFR: need 1 <= n < 2^31 - 1");
+    if (p.kmax < 1 || p.cmin < 1 || p.cmax < p.cmin) throw Error(-1, "LFR: need kmax >= 1 and 1 <= cmin <= cmax");
+    if (!(p.mu >= 0.0 && p.mu <= 1.0)) throw Error(-1, "LFR: mixing mu must be in [0, 1]");
+    if (p.wmax > 0 && (p.wmin < 1 || p.wmin > p.wmax)) throw Error(-1, "LFR: weights need 1 <= wmin <= wmax");
+    const PowerLawTable dt = power_law_table(p.tau1, 1, p.kmax, p.kmean);
+    const std::vector<i64> sz = lfr_community_sizes(p);
+    const i64 n = p.n, nc = (i64)sz.size();
+    std::vector<i64> cs(nc + 1, 0);
+    for (i64 k = 0; k < nc; k++) cs[k + 1] = cs[k] + sz[k];
+    const i64 mu_m = std::llround(p.mu * 1000.0);
+    const i32 K = (i32)dt.thr.size();
+    Buf<i64> dcs(c, nc + 1);
+    Buf<u64> dthr(c, K);
+    PGGC_HIP(hipMemcpyAsync(dcs.p, cs.data(), (nc + 1) * sizeof(i64), hipMemcpyHostToDevice, c.stream));
+    PGGC_HIP(hipMemcpyAsync(dthr.p, dt.thr.data(), K * sizeof(u64), hipMemcpyHostToDevice, c.stream));
+    c.sync();  // the host vectors are sources of the copies
+    Buf<i32> kin(c, n), kext(c, n), comm(c, n);
+    k_lfr_degrees<<<grid_items(n, 256), 256, 0, c.stream>>>(n, nc, dcs, dthr, K, dt.k0, mu_m, p.seed, kin, kext, comm);
+    PGGC_LAUNCH_CHECK();
+    Buf<i64> oin(c, n + 1), oext(c, n + 1);
+    scan_i32_to_i64(c, kin, oin, n);
+    scan_i32_to_i64(c, kext, oext, n);
+    const i64 Sin = c.read1(oin.p + n), Sext = c.read1(oext.p + n);
+    Buf<i32> pi(c, n);
+    {
+        Buf<u64> pk(c, n);
+        Buf<i32> pv(c, n);
+        k_lfr_perm_keys<<<grid_items(n, 256), 256, 0, c.stream>>>(n, p.seed, pk, pv);
+        PGGC_LAUNCH_CHECK();
+        radix_sort_pairs(c, pk, pv, n, 32);
+        k_rank_scatter<<<grid_items(n, 256), 256, 0, c.stream>>>(n, pv, pi);
+        PGGC_LAUNCH_CHECK();
+    }
+    if (gt) {
+        k_lfr_gt<<<grid_items(n, 256), 256, 0, c.stream>>>(n, pi, comm, gt);
+        PGGC_LAUNCH_CHECK();
+    }
+    const int B = key_bits(std::max<i64>(n, 2));
+    const u64 invalid = ((u64)1 << (2 * B)) - 1;
+    const i64 M = Sin + Sext;
+    Buf<u64> ek(c, std::max<i64>(M, 1));
+    if (Sin > 0) {
+        Buf<u64> sk(c, Sin);
+        Buf<i32> sv(c, Sin);
+        k_lfr_stubs<true><<<grid_items(Sin, 256), 256, 0, c.stream>>>(Sin, n, oin, comm, p.seed, STREAM_LFR_ISTUB,
+                                                                       sk, sv);
+        PGGC_LAUNCH_CHECK();
+        radix_sort_pairs(c, sk, sv, Sin, 32 + key_bits(std::max<i64>(nc, 2)));
+        k_lfr_pair_int<<<grid_items(Sin, 256), 256, 0, c.stream>>>(Sin, sk, sv, oin, dcs, pi, B, invalid, ek);
+        PGGC_LAUNCH_CHECK();
+    }
+    if (Sext > 0) {
+        Buf<u64> sk(c, Sext);
+        Buf<i32> sv(c, Sext);
+        k_lfr_stubs<false><<<grid_items(Sext, 256), 256, 0, c.stream>>>(Sext, n, oext, comm, p.seed,
+                                                                        STREAM_LFR_ESTUB, sk, sv);
+        PGGC_LAUNCH_CHECK();
+        radix_sort_pairs(c, sk, sv, Sext, 32);
+        k_lfr_pair_ext<<<grid_items(Sext, 256), 256, 0, c.stream>>>(Sext, sv, pi, B, invalid, ek.p + Sin);
+        PGGC_LAUNCH_CHECK();
+    }
+    kin.reset();
+    kext.reset();
+    comm.reset();
+    i64 U = 0;
+    Buf<u64> uk(c, std::max<i64>(M, 1));
+    if (M > 0) {
+        radix_sort_keys(c, ek, M, 2 * B);
+        U = unique_pairs(c, ek, nullptr, M, invalid, uk, nullptr);
+    }
+    ek.reset();
+    pairs_to_csr(c, n, uk, nullptr, U, B, p.wmax > 0 ? 2 : 0, p.seed, p.wmin, p.wmax, out);
+    uk.reset();
+    c.trim();
+}
+
 void random_labels(Ctx& c, i64 n, i64 L, u64 seed, i32* out) {
     if (n < 0 || L < 1 || L > 0x7fffffff) throw Error(-1, "random labels: need n >= 0 and 1 <= L < 2^31");
     if (n == 0) return;

@@ -110,3 +110,136 @@ def test_random_labels_in_range_and_uniform():
     words = philox_np([np.arange(10, dtype=np.uint64), np.zeros(10, np.uint64), np.zeros(10, np.uint64),
                        np.full(10, 2, np.uint64)], 2, 0)
     assert np.array_equal(lab[:10], (words[0] % np.uint64(64)).astype(np.int32))
+
+
+# ---------------------------------------------------------------- LFR-style
+def draw_np(idx, stream, seed):
+    idx = np.asarray(idx, np.uint64)
+    z = np.zeros_like(idx)
+    return philox_np([idx & np.uint64(M32), idx >> np.uint64(32), z, np.full_like(idx, stream)],
+                     seed & M32, seed >> 32)[0]
+
+
+def lfr_np(n, tau1, kmean, kmax, tau2, cmin, cmax, mu, seed):
+    """numpy restatement of the LFR-style generator (same draws, stable
+    sorts and pairing rules), on the oracle's power-law tables (pinned by
+    the table tests below): (src, dst, planted community per final id)."""
+    k0d, thd = O.power_law_table(tau1, 1, kmax, kmean)
+    k0c, thc = O.power_law_table(tau2, cmin, cmax)
+    sizes, s, j = [], 0, 0
+    while s < n:
+        k = k0c + int(np.searchsorted(thc, draw_np([j], 4, seed)[0], side="right"))
+        sizes.append(k)
+        s += k
+        j += 1
+    rem = n - (s - sizes[-1])
+    sizes[-1] = rem
+    if rem < cmin and len(sizes) > 1:
+        sizes[-2] += rem
+        sizes.pop()
+    sizes = np.array(sizes, np.int64)
+    comm = np.repeat(np.arange(len(sizes)), sizes)
+    deg = k0d + np.searchsorted(thd, draw_np(np.arange(n), 3, seed), side="right")
+    mu_m = int(np.floor(mu * 1000 + 0.5))
+    kin = np.minimum((deg * (1000 - mu_m) + 500) // 1000, sizes[comm] - 1)
+    kext = deg - kin
+    pi = np.empty(n, np.int64)
+    pi[np.argsort(draw_np(np.arange(n), 7, seed), kind="stable")] = np.arange(n)
+    src, dst = [], []
+    # internal stubs: sorted by (community, draw), paired inside each community
+    owner = np.repeat(np.arange(n), kin)
+    key = (comm[owner].astype(np.uint64) << np.uint64(32)) | draw_np(np.arange(len(owner)), 5, seed)
+    o = np.argsort(key, kind="stable")
+    sv = owner[o]
+    oin = np.concatenate([[0], np.cumsum(kin)])
+    cs = np.concatenate([[0], np.cumsum(sizes)])
+    for c in range(len(sizes)):
+        s0, s1 = oin[cs[c]], oin[cs[c + 1]]
+        m = (s1 - s0) // 2 * 2
+        src.append(sv[s0:s0 + m:2])
+        dst.append(sv[s0 + 1:s0 + m:2])
+    # external stubs: sorted by draw, paired over the graph
+    owner = np.repeat(np.arange(n), kext)
+    o = np.argsort(draw_np(np.arange(len(owner)), 6, seed), kind="stable")
+    sv = owner[o]
+    m = len(sv) // 2 * 2
+    src.append(sv[0:m:2])
+    dst.append(sv[1:m:2])
+    src = pi[np.concatenate(src)]
+    dst = pi[np.concatenate(dst)]
+    gt = np.empty(n, np.int32)
+    gt[pi] = comm
+    return src, dst, gt, sizes, deg
+
+
+@pytest.mark.parametrize("n,kmax,cmin,cmax,mu,seed", [(1, 10, 20, 200, 0.4, 3), (150, 30, 20, 100, 0.3, 1),
+                                                       (20000, 300, 20, 500, 0.4, 3),
+                                                       (60000, 1000, 20, 2000, 0.6, 5)])
+def test_oracle_lfr_matches_numpy_restatement(n, kmax, cmin, cmax, mu, seed):
+    kmean = min(20.0, kmax / 3)
+    src, dst, gt, sizes, deg = lfr_np(n, 2.5, kmean, kmax, 1.5, cmin, cmax, mu, seed)
+    ref = graphs.from_edges(n, src, dst)
+    rp, col, w, ogt, odeg, osizes = O.lfr(n, 2.5, kmean, kmax, 1.5, cmin, cmax, mu, seed=seed)
+    assert w is None
+    assert np.array_equal(osizes, sizes) and np.array_equal(odeg, deg) and np.array_equal(ogt, gt)
+    assert np.array_equal(rp, ref.rp) and np.array_equal(col, ref.col)
+
+
+def test_oracle_lfr_weights_per_pair():
+    rp, col, w, _, _, _ = O.lfr(3000, kmax=100, cmax=300, weights=(1, 100), seed=3)
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    lo = np.minimum(rows, col).astype(np.uint64)
+    hi = np.maximum(rows, col).astype(np.uint64)
+    words = philox_np([lo, hi, np.zeros_like(lo), np.ones_like(lo)], 3, 0)
+    assert np.array_equal(w, (words[0] % np.uint64(100) + np.uint64(1)).astype(np.int32))
+    assert w.min() >= 1 and w.max() <= 100
+
+
+def _pl_tail(x, tau):
+    return np.power(np.asarray(x, np.float64), 1.0 - tau)
+
+
+def test_power_law_table_closed_form():
+    """Community sizes: thresholds are floor(F(k + 1) 2^32) with
+    F(x) = (lo^(1-tau) - x^(1-tau)) / (lo^(1-tau) - (top+1)^(1-tau)), the
+    CDF of the continuous power law on [lo, top + 1)."""
+    for tau, lo, top in [(1.5, 20, 2000), (2.5, 1, 1000), (3.0, 5, 50)]:
+        k0, thr = O.power_law_table(tau, lo, top)
+        assert k0 == lo and len(thr) == top - lo + 1
+        ks = np.arange(lo, top + 1)
+        F = (_pl_tail(lo, tau) - _pl_tail(ks + 1, tau)) / (_pl_tail(lo, tau) - _pl_tail(top + 1, tau))
+        want = np.floor(F * 2.0 ** 32)
+        assert np.max(np.abs(thr[:-1].astype(np.float64) - want[:-1])) <= 1.0
+        assert int(thr[-1]) == 1 << 32 and np.all(np.diff(thr.astype(np.int64)) >= 0)
+
+
+def test_degree_table_mean_and_shape():
+    """The degree law's lower end is solved for the mean: the table's mean
+    equals kmean, and above the first value P(k) / P(k+1) is the closed form
+    (k^(1-tau) - (k+1)^(1-tau)) / ((k+1)^(1-tau) - (k+2)^(1-tau))."""
+    tau, kmax, kmean = 2.5, 1000, 20.0
+    k0, thr = O.power_law_table(tau, 1, kmax, kmean)
+    p = np.diff(np.concatenate([[0], thr.astype(np.float64)])) / 2.0 ** 32
+    ks = k0 + np.arange(len(p))
+    assert abs(float(np.dot(ks, p)) - kmean) < 1e-6
+    k = ks[1:60]
+    ratio = (_pl_tail(k, tau) - _pl_tail(k + 1, tau)) / (_pl_tail(k + 1, tau) - _pl_tail(k + 2, tau))
+    assert np.allclose(p[1:60] / p[2:61], ratio, rtol=1e-5)
+
+
+def test_lfr_statistics():
+    """Sizes cover n within [cmin, cmax] (the merged last one may pass cmax
+    by less than cmin), drawn degrees average kmean, the realised mixing is
+    close to mu, and the planted partition has high modularity."""
+    n = 200000
+    rp, col, w, gt, deg, sizes = O.lfr(n, mu=0.4, seed=3)
+    assert sizes.sum() == n and sizes.min() >= 20 and sizes.max() < 2000 + 20
+    assert abs(deg.mean() - 20.0) < 0.5 and deg.max() <= 1000
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    mix = np.mean(gt[rows] != gt[col])
+    assert 0.38 < mix < 0.48
+    nnz = len(col)
+    vol = np.bincount(gt, weights=np.diff(rp).astype(np.float64))
+    q = np.mean(gt[rows] == gt[col]) - float(np.sum(vol * vol)) / float(nnz) ** 2
+    assert q > 0.5
