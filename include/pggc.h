@@ -133,7 +133,8 @@ typedef void (*pggc_sweep_fn)(void* user, const int64_t* rec, int nrec);
 int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user);
 
 /* Tuning knobs of the HIP path (no counterpart in the reference, whose
- * equivalents are compile-time constants).
+ * equivalents are compile-time constants: src/header/ordering.h:17-19,
+ * src/local_mover.cpp:985,1312).
  *   PGGC_KNOB_HUB_UPDATE_DIV: the hash tables of the highest-degree vertices
  *     are kept current from the movers while a sweep moves at most
  *     n / value vertices and rebuilt otherwise; 0 rebuilds every sweep.
@@ -141,11 +142,41 @@ int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user);
  *   PGGC_KNOB_GNN_MAX_SWEEPS: the active-set predictor runs on at most this
  *     many sweeps of a local-move call; 0 (default) runs it on every
  *     eligible sweep.  (It changes which vertices are pruned, so with
- *     pruning on it changes results.) */
+ *     pruning on it changes results.)
+ * Execution shape of the scoring kernels; clustering results are identical
+ * for every value, only time changes (values other than the listed ones are
+ * rejected):
+ *   PGGC_KNOB_TABLE_THREADS: threads per vertex of the LDS-hash-table
+ *     scoring of vertices with 64 < degree <= 512: 64 (default), 128, 256.
+ *   PGGC_KNOB_TABLE_SLOTS_PER_EDGE: LDS hash-table slots per edge of a degree
+ *     class's bound, 64 < degree <= 2048: 2 (default) or 4.
+ *   PGGC_KNOB_HUB_LDS_MAX_DEGREE: vertices of degree 2049..value are scored
+ *     from an 8192-slot LDS table instead of an HBM hub table: 2048
+ *     (default, none) or 4096.
+ *   PGGC_KNOB_HUB_EDGES_PER_ITEM: edges per hub-table build work item:
+ *     512, 1024 (default), 2048.
+ *   PGGC_KNOB_HUB_SLOTS_PER_ITEM: hub-table slots per scan work item:
+ *     2048, 4096 (default), 8192. */
 #define PGGC_KNOB_HUB_UPDATE_DIV 0
 #define PGGC_KNOB_GNN_MAX_SWEEPS 1
+#define PGGC_KNOB_TABLE_THREADS 2
+#define PGGC_KNOB_TABLE_SLOTS_PER_EDGE 3
+#define PGGC_KNOB_HUB_LDS_MAX_DEGREE 4
+#define PGGC_KNOB_HUB_EDGES_PER_ITEM 5
+#define PGGC_KNOB_HUB_SLOTS_PER_ITEM 6
+#define PGGC_KNOB_COUNT 7
 int pggc_ctx_set_knob(pggc_ctx* ctx, int knob, int64_t value);
 int pggc_ctx_get_knob(pggc_ctx* ctx, int knob, int64_t* value);
+/* Autotuner of the execution-shape knobs (no counterpart in the reference):
+ * coordinate descent in the order hub LDS max degree, table threads, table
+ * slots per edge, hub edges per item, hub slots per item, hub update divisor;
+ * each candidate value is timed on the first local-move call on the top
+ * level of g from singletons (best of reps runs, after a plan build), and the
+ * fastest value is kept on ctx.  report (host, nullable): up to cap rows of
+ * (knob, value, microseconds); *nrows (nullable) receives the rows timed.
+ * The active-set predictor must be off. */
+int pggc_autotune(pggc_ctx* ctx, const pggc_graph* g, int objective, int reps, int64_t* report, int cap,
+                  int* nrows);
 
 /* ---------------------------------------------------------------------
  * Active-set predictor.  No counterpart in the reference: a two-layer graph
@@ -331,6 +362,27 @@ int pggc_graph_from_edges(pggc_ctx* ctx, int64_t n, int64_t m, const int32_t* sr
                           const int32_t* w, pggc_devgraph** out);
 int pggc_rmat(pggc_ctx* ctx, int scale, int edge_factor, double a, double b, double c, uint64_t seed, int32_t wmin,
               int32_t wmax, pggc_devgraph** out);
+/* pggc_lfr builds an LFR-style benchmark graph (Lancichinetti, Fortunato,
+ * Radicchi, Phys. Rev. E 78, 046110, 2008) on the device, with no
+ * counterpart in the reference (which reads graphs from files):
+ *  - degrees: the integer power law floor(X), X with density ~ x^-tau1 on
+ *    [x0, kmax + 1), x0 solved so that the mean degree is kmean;
+ *  - community sizes: floor(X), X ~ x^-tau2 on [cmin, cmax + 1), drawn in
+ *    sequence until they cover n (the last takes what is left and joins the
+ *    one before when that is below cmin); vertices fill them in order;
+ *  - internal degree round((1 - mu) degree), at most the community size - 1;
+ *  - internal stubs paired after a stable sort by (community, draw), external
+ *    stubs after a stable sort by draw over the whole graph; pairs of one
+ *    vertex and repeated pairs vanish in the construction above;
+ *  - vertex ids permuted by a stable sort on one draw per vertex;
+ *  - with wmax > 0, per-pair weights as for pggc_rmat.
+ * Every draw is a u32 Philox4x32-10 word compared against integer tables
+ * built on the host, so a CPU implementation of the same steps builds the
+ * same graph.  gt_out (device, n entries, nullable) receives each vertex's
+ * planted community. */
+int pggc_lfr(pggc_ctx* ctx, int64_t n, double tau1, double kmean, int32_t kmax, double tau2, int32_t cmin,
+             int32_t cmax, double mu, int32_t wmin, int32_t wmax, uint64_t seed, int32_t* gt_out,
+             pggc_devgraph** out);
 int pggc_devgraph_view(const pggc_devgraph* g, pggc_graph* view);
 /* device-to-device copy into caller buffers (row_ptr[n+1], col[nnz], w[nnz]
  * when weighted) on the context's stream */

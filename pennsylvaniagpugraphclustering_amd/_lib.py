@@ -69,6 +69,7 @@ SIGNATURES = {
     "pggc_ctx_set_sweep_trace": (c.c_int, [P, P, P]),
     "pggc_ctx_set_knob": (c.c_int, [P, c.c_int, c.c_int64]),
     "pggc_ctx_get_knob": (c.c_int, [P, c.c_int, P]),
+    "pggc_autotune": (c.c_int, [P, c.POINTER(Graph), c.c_int, c.c_int, P, c.c_int, P]),
     "pggc_gnn_configure": (c.c_int, [P, c.c_int, c.c_double, c.c_int64, c.c_int, c.c_double, c.c_double]),
     "pggc_gnn_set_params": (c.c_int, [P, P, c.c_int]),
     "pggc_gnn_get_params": (c.c_int, [P, P, c.c_int]),
@@ -107,6 +108,8 @@ SIGNATURES = {
     "pggc_graph_from_edges": (c.c_int, [P, c.c_int64, c.c_int64, P, P, P, c.POINTER(c.c_void_p)]),
     "pggc_rmat": (c.c_int, [P, c.c_int, c.c_int, c.c_double, c.c_double, c.c_double, c.c_uint64, c.c_int32,
                             c.c_int32, c.POINTER(c.c_void_p)]),
+    "pggc_lfr": (c.c_int, [P, c.c_int64, c.c_double, c.c_double, c.c_int32, c.c_double, c.c_int32, c.c_int32,
+                           c.c_double, c.c_int32, c.c_int32, c.c_uint64, P, c.POINTER(c.c_void_p)]),
     "pggc_devgraph_view": (c.c_int, [P, c.POINTER(Graph)]),
     "pggc_devgraph_copy": (c.c_int, [P, P, P, P, P]),
     "pggc_devgraph_free": (c.c_int, [P]),

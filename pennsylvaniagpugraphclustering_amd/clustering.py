@@ -84,17 +84,36 @@ class Context:
             out[name] = (n.value, ms.value, by.value)
         return out
 
-    KNOBS = {"hub_update_div": 0, "gnn_max_sweeps": 1}
+    KNOBS = {"hub_update_div": 0, "gnn_max_sweeps": 1, "table_threads": 2, "table_slots_per_edge": 3,
+             "hub_lds_max_degree": 4, "hub_edges_per_item": 5, "hub_slots_per_item": 6}
+
+    # values the execution-shape knobs accept (results are identical for all)
+    KNOB_VALUES = {"table_threads": (64, 128, 256), "table_slots_per_edge": (2, 4),
+                   "hub_lds_max_degree": (2048, 4096), "hub_edges_per_item": (512, 1024, 2048),
+                   "hub_slots_per_item": (2048, 4096, 8192), "hub_update_div": (16, 32, 64)}
 
     def set_knob(self, name, value):
-        """Tuning knob of the HIP path (hub_update_div: performance only;
-        gnn_max_sweeps: predicted sweeps per local-move call)."""
+        """Tuning knob of the HIP path (gnn_max_sweeps: predicted sweeps per
+        local-move call; every other knob changes time only)."""
         check(self.lib.pggc_ctx_set_knob(self.h, self.KNOBS[name], int(value)))
 
     def get_knob(self, name):
         v = ctypes.c_int64()
         check(self.lib.pggc_ctx_get_knob(self.h, self.KNOBS[name], ctypes.byref(v)))
         return v.value
+
+    def autotune(self, g, objective="Mod", reps=2):
+        """pggc_autotune: coordinate descent over the execution-shape knobs,
+        timing the first top-level local-move call of g; the fastest values
+        stay set.  Returns (chosen {knob: value}, [(knob, value, us), ...])."""
+        names = {v: k for k, v in self.KNOBS.items()}
+        cap = 64
+        rep = (ctypes.c_int64 * (3 * cap))()
+        rows = ctypes.c_int()
+        check(self.lib.pggc_autotune(self.h, ctypes.byref(g.c()), OBJECTIVES[objective], int(reps), rep, cap,
+                                     ctypes.byref(rows)))
+        table = [(names[rep[3 * i]], int(rep[3 * i + 1]), int(rep[3 * i + 2])) for i in range(min(rows.value, cap))]
+        return {k: self.get_knob(k) for k in self.KNOB_VALUES}, table
 
     SWEEP_FIELDS = ("n", "nnz", "phase", "sweep", "active", "cands", "moves", "us")
     _SweepFn = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.c_int)

@@ -1,5 +1,6 @@
 // Multilevel drivers (src/clustering_methods.cpp), the run_clustering outer
 // loop (app/cluster_main.cpp:101-165) and the C ABI declared in pggc.h.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -582,13 +583,67 @@ int pggc_gnn_eval(pggc_ctx* ctx, const pggc_graph* gg, const int32_t* part, cons
     });
 }
 
+// candidate values of the execution-shape knobs (what the kernels are
+// instantiated for); the other knobs take any value >= 0
+static const std::vector<int64_t>* knob_candidates(int knob) {
+    static const std::vector<int64_t> threads{64, 128, 256}, spe{2, 4}, lds{2048, 4096}, edges{512, 1024, 2048},
+        slots{2048, 4096, 8192}, div{16, 32, 64};
+    switch (knob) {
+        case PGGC_KNOB_TABLE_THREADS: return &threads;
+        case PGGC_KNOB_TABLE_SLOTS_PER_EDGE: return &spe;
+        case PGGC_KNOB_HUB_LDS_MAX_DEGREE: return &lds;
+        case PGGC_KNOB_HUB_EDGES_PER_ITEM: return &edges;
+        case PGGC_KNOB_HUB_SLOTS_PER_ITEM: return &slots;
+        case PGGC_KNOB_HUB_UPDATE_DIV: return &div;  // the autotuner's range (any value >= 0 is accepted)
+        default: return nullptr;
+    }
+}
+
+static void set_knob(Ctx& c, int knob, int64_t value) {
+    if (value < 0) throw Error(PGGC_EINVAL, "knob value must be >= 0");
+    const std::vector<int64_t>* cand = knob_candidates(knob);
+    const bool shape = knob == PGGC_KNOB_TABLE_THREADS || knob == PGGC_KNOB_TABLE_SLOTS_PER_EDGE ||
+                       knob == PGGC_KNOB_HUB_LDS_MAX_DEGREE || knob == PGGC_KNOB_HUB_EDGES_PER_ITEM ||
+                       knob == PGGC_KNOB_HUB_SLOTS_PER_ITEM;
+    if (shape && std::find(cand->begin(), cand->end(), value) == cand->end())
+        throw Error(PGGC_EINVAL, "knob value is not one the kernels are built for");
+    switch (knob) {
+        case PGGC_KNOB_HUB_UPDATE_DIV: c.hub_update_div = value; break;
+        case PGGC_KNOB_GNN_MAX_SWEEPS: c.gnn_max_sweeps = value; break;
+        case PGGC_KNOB_TABLE_THREADS: c.table_threads = (int)value; break;
+        case PGGC_KNOB_TABLE_SLOTS_PER_EDGE: c.table_spe = (int)value; break;
+        case PGGC_KNOB_HUB_LDS_MAX_DEGREE: c.hub_lds_max = value; break;
+        case PGGC_KNOB_HUB_EDGES_PER_ITEM: c.hub_edges_item = value; break;
+        case PGGC_KNOB_HUB_SLOTS_PER_ITEM: c.hub_slots_item = value; break;
+        default: throw Error(PGGC_EINVAL, "unknown knob");
+    }
+    if (shape) c.knob_epoch++;  // level plans depend on these
+}
+
+static int64_t get_knob(const Ctx& c, int knob) {
+    switch (knob) {
+        case PGGC_KNOB_HUB_UPDATE_DIV: return c.hub_update_div;
+        case PGGC_KNOB_GNN_MAX_SWEEPS: return c.gnn_max_sweeps;
+        case PGGC_KNOB_TABLE_THREADS: return c.table_threads;
+        case PGGC_KNOB_TABLE_SLOTS_PER_EDGE: return c.table_spe;
+        case PGGC_KNOB_HUB_LDS_MAX_DEGREE: return c.hub_lds_max;
+        case PGGC_KNOB_HUB_EDGES_PER_ITEM: return c.hub_edges_item;
+        case PGGC_KNOB_HUB_SLOTS_PER_ITEM: return c.hub_slots_item;
+        default: throw Error(PGGC_EINVAL, "unknown knob");
+    }
+}
+
+static void copy_knobs(Ctx& dst, const Ctx& src) {
+    for (int k = 0; k < PGGC_KNOB_COUNT; k++) {
+        const int64_t v = get_knob(src, k);
+        if (get_knob(dst, k) != v) set_knob(dst, k, v);
+    }
+}
+
 int pggc_ctx_set_knob(pggc_ctx* ctx, int knob, int64_t value) {
     return guard([&] {
         CTX_CHECK(ctx);
-        if (value < 0) throw Error(PGGC_EINVAL, "knob value must be >= 0");
-        if (knob == PGGC_KNOB_HUB_UPDATE_DIV) ctx->c.hub_update_div = value;
-        else if (knob == PGGC_KNOB_GNN_MAX_SWEEPS) ctx->c.gnn_max_sweeps = value;
-        else throw Error(PGGC_EINVAL, "unknown knob");
+        set_knob(ctx->c, knob, value);
     });
 }
 
@@ -596,9 +651,64 @@ int pggc_ctx_get_knob(pggc_ctx* ctx, int knob, int64_t* value) {
     return guard([&] {
         CTX_CHECK(ctx);
         if (!value) throw Error(PGGC_EINVAL, "null output");
-        if (knob == PGGC_KNOB_HUB_UPDATE_DIV) *value = ctx->c.hub_update_div;
-        else if (knob == PGGC_KNOB_GNN_MAX_SWEEPS) *value = ctx->c.gnn_max_sweeps;
-        else throw Error(PGGC_EINVAL, "unknown knob");
+        *value = get_knob(ctx->c, knob);
+    });
+}
+
+// Coordinate descent over the execution-shape knobs: each knob in turn
+// takes every candidate value, the first local-move call on the top level of
+// g (from singletons, the most expensive call of a run) is timed, and the
+// fastest value is kept.  A plan is built per value before the timed runs.
+int pggc_autotune(pggc_ctx* ctx, const pggc_graph* gg, int objective, int reps, int64_t* report, int cap,
+                  int* nrows) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (reps < 1) throw Error(PGGC_EINVAL, "reps must be >= 1");
+        if (ctx->gnn.cfg.mode != GNN_OFF) throw Error(PGGC_EINVAL, "turn the active-set predictor off to tune");
+        DGraph g = view_of(gg);
+        if (!g.vw) throw Error(PGGC_EINVAL, "vertex weights required");
+        Ctx& c = ctx->c;
+        const i64 n = g.n;
+        Buf<i64> vol(c, std::max<i64>(n, 1));
+        Buf<i32> part(c, std::max<i64>(n, 1));
+        int rows = 0;
+        const int order[] = {PGGC_KNOB_HUB_LDS_MAX_DEGREE, PGGC_KNOB_TABLE_THREADS, PGGC_KNOB_TABLE_SLOTS_PER_EDGE,
+                             PGGC_KNOB_HUB_EDGES_PER_ITEM, PGGC_KNOB_HUB_SLOTS_PER_ITEM, PGGC_KNOB_HUB_UPDATE_DIV};
+        for (int knob : order) {
+            const std::vector<int64_t>& cand = *knob_candidates(knob);
+            int64_t best_v = get_knob(c, knob);
+            double best_t = std::numeric_limits<double>::infinity();
+            for (int64_t v : cand) {
+                set_knob(c, knob, v);
+                LevelPlan lp;
+                plan_level(c, g, lp);
+                double t_min = std::numeric_limits<double>::infinity();
+                for (int r = 0; r < reps; r++) {
+                    RefineData rd;
+                    rd.vol = vol;
+                    rd.cap = std::max<i64>(n, 1);
+                    state_init(c, g, objective, 1.0, rd);
+                    iota_i32(c, part, n);
+                    c.sync();
+                    const auto t0 = std::chrono::steady_clock::now();
+                    local_move(c, ctx->ms, lp, g, part, rd, true, false, nullptr, true, nullptr);
+                    c.sync();
+                    t_min = std::min(t_min, since(t0));
+                }
+                if (report && rows < cap) {
+                    report[3 * rows] = knob;
+                    report[3 * rows + 1] = v;
+                    report[3 * rows + 2] = (int64_t)(t_min * 1e6);
+                }
+                rows++;
+                if (t_min < best_t) {
+                    best_t = t_min;
+                    best_v = v;
+                }
+            }
+            set_knob(c, knob, best_v);
+        }
+        if (nrows) *nrows = rows;
     });
 }
 
@@ -942,8 +1052,7 @@ int pggc_cluster_batch(pggc_ctx* ctx, int count, const pggc_graph* graphs, int a
         }
         for (int k = 0; k < S; k++) {
             pggc_ctx* w = ctx->workers[k];
-            w->c.hub_update_div = ctx->c.hub_update_div;
-            w->c.gnn_max_sweeps = ctx->c.gnn_max_sweeps;
+            copy_knobs(w->c, ctx->c);
             if (prune) {
                 w->gnn.set_params(w->c, prm.data());
                 w->gnn.cfg = ctx->gnn.cfg;
@@ -1044,6 +1153,35 @@ int pggc_rmat(pggc_ctx* ctx, int scale, int edge_factor, double a, double b, dou
         std::unique_ptr<pggc_devgraph> h(new pggc_devgraph());
         try {
             rmat_csr(ctx->c, scale, edge_factor, a, b, c, seed, wmin, wmax, h->g);
+        } catch (...) {
+            h->g.release();
+            throw;
+        }
+        *out = h.release();
+    });
+}
+
+int pggc_lfr(pggc_ctx* ctx, int64_t n, double tau1, double kmean, int32_t kmax, double tau2, int32_t cmin,
+             int32_t cmax, double mu, int32_t wmin, int32_t wmax, uint64_t seed, int32_t* gt_out,
+             pggc_devgraph** out) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (!out) throw Error(PGGC_EINVAL, "null output");
+        LfrParams p;
+        p.n = n;
+        p.tau1 = tau1;
+        p.kmean = kmean;
+        p.kmax = kmax;
+        p.tau2 = tau2;
+        p.cmin = cmin;
+        p.cmax = cmax;
+        p.mu = mu;
+        p.wmin = wmin;
+        p.wmax = wmax;
+        p.seed = seed;
+        std::unique_ptr<pggc_devgraph> h(new pggc_devgraph());
+        try {
+            lfr_csr(ctx->c, p, h->g, gt_out);
         } catch (...) {
             h->g.release();
             throw;

@@ -71,6 +71,23 @@ struct Ctx {
     // movers while a sweep moves at most n / hub_update_div vertices and
     // rebuilt otherwise (0: always rebuilt)
     i64 hub_update_div = 32;
+    // Execution-shape knobs of the scoring kernels (results do not depend on
+    // them; pggc_autotune picks them by timing):
+    //   table_threads: threads per vertex of the LDS-table classes with
+    //     64 < degree <= 512 (64, 128 or 256);
+    //   table_spe: LDS hash-table slots per edge of a class's degree bound
+    //     (2 or 4), classes 64 < degree <= 2048;
+    //   hub_lds_max: vertices of degree 2049..hub_lds_max are scored from an
+    //     8192-slot LDS table instead of an HBM hub table (2048 or 4096);
+    //   hub_edges_item / hub_slots_item: edges per hub-table build item
+    //     (512, 1024 or 2048) and table slots per hub scan item (2048, 4096
+    //     or 8192).
+    int table_threads = 64;
+    int table_spe = 2;
+    i64 hub_lds_max = 2048;
+    i64 hub_edges_item = 1024;
+    i64 hub_slots_item = 4096;
+    u64 knob_epoch = 0;  // bumped by every knob change; level plans are rebuilt when it moves
     // the active-set predictor runs on at most this many sweeps of a
     // local-move call (0: on every eligible sweep)
     i64 gnn_max_sweeps = 0;

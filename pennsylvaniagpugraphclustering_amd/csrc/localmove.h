@@ -63,6 +63,12 @@ struct LevelPlan {
     bool packed = false;
     bool hub_live = false;  // tables match the current clustering
     i64 gslots = 0, nitems = 0, nsitems = 0, igrid = 0, sgrid = 0;
+    // the hub class is sorted by degree; its first nlds vertices (degree <=
+    // Ctx::hub_lds_max) are scored in LDS, the rest (from hub0 on) in HBM
+    // tables; he / hs: edges per build item and slots per scan item
+    u64 epoch = ~(u64)0;
+    i64 nlds = 0, hub0 = 0, nhub = 0;
+    i64 he = 1024, hs = 4096;
 };
 
 // Device scratch sized for the top level, shared by every level.

@@ -60,14 +60,14 @@ __host__ __device__ __forceinline__ int score_class(i64 d) {
     return d <= 4 ? 0 : d <= 8 ? 1 : d <= 16 ? 2 : d <= 32 ? 3 : d <= 64 ? 4 : d <= 128 ? 5 : d <= 256 ? 6
          : d <= 512 ? 7 : d <= 1024 ? 8 : d <= 2048 ? 9 : 10;
 }
-constexpr int HCH_E = 1024;  // hub edges per insert work item (1024 measured best of 512, 1024, 2048)
-constexpr int HCH_S = 4096;  // hub table slots per scan work item
-constexpr int HLDS = 2 * HCH_E;  // LDS pre-aggregation slots per insert item
-__host__ __device__ __forceinline__ i64 tslots(i64 d) {
+// LDS table of a vertex of degree d: the power of two >= spe * d slots
+__host__ __device__ __forceinline__ i64 tslots(i64 d, int spe) {
     i64 s = 2;
-    while (s < 2 * d) s <<= 1;
+    while (s < spe * d) s <<= 1;
     return s;
 }
+// vertices of the hub class scored from an LDS table (degree <= HUB_LDS_BOUND)
+constexpr i64 HUB_LDS_BOUND = 4096;
 // HBM hub tables: power of two >= 1.5 d, so at most 2/3 full (a hub has at
 // most d distinct neighbour clusters); every probe loop ends at an empty slot
 __host__ __device__ __forceinline__ i64 hub_slots(i64 d) {
@@ -81,31 +81,50 @@ __global__ void k_score_classes(const i64* rp, i64 n, i32* cls) {
     if (i >= n) return;
     cls[i] = score_class(rp[i + 1] - rp[i]);
 }
-__global__ void k_global_sizes(const i64* rp, const i32* list, i64 m, i64* sz, i64* items) {
+__global__ void k_global_sizes(const i64* rp, const i32* list, i64 m, i64 he, i64* sz, i64* items) {
     const i64 x = thread_item();
     if (x >= m) return;
     i32 i = list[x];
     i64 d = rp[i + 1] - rp[i];
     sz[x] = hub_slots(d);
-    items[x] = (d + HCH_E - 1) / HCH_E;
+    items[x] = (d + he - 1) / he;
 }
-__global__ void k_slot_items(const i64* goff, i64 m, i64* sitems) {
+__global__ void k_slot_items(const i64* goff, i64 m, i64 hs, i64* sitems) {
     const i64 x = thread_item();
     if (x >= m) return;
-    sitems[x] = (goff[x + 1] - goff[x] + HCH_S - 1) / HCH_S;
+    sitems[x] = (goff[x + 1] - goff[x] + hs - 1) / hs;
+}
+__global__ void k_hub_deg_keys(const i64* rp, const i32* list, i64 m, u64* keys, i32* vals) {
+    const i64 x = thread_item();
+    if (x >= m) return;
+    const i32 i = list[x];
+    keys[x] = (u64)(rp[i + 1] - rp[i]);
+    vals[x] = i;
+}
+// first x with degree(list[x]) > bound (list sorted by degree)
+__global__ void k_first_above(const i64* rp, const i32* list, i64 m, i64 bound, i64* out) {
+    if (threadIdx.x != 0) return;
+    i64 lo = 0, hi = m;
+    while (lo < hi) {
+        const i64 mid = (lo + hi) >> 1;
+        const i32 i = list[mid];
+        if (rp[i + 1] - rp[i] > bound) hi = mid;
+        else lo = mid + 1;
+    }
+    *out = lo;
 }
 
 // row weight of each hub (one block per edge item)
 template <class WT>
 __global__ void __launch_bounds__(256) k_hub_wsum(const i64* __restrict__ rp, WT W, const i32* __restrict__ hubs,
-                                                  i64 m, const i64* __restrict__ items, i64* hsum) {
+                                                  i64 m, const i64* __restrict__ items, i64 he, i64* hsum) {
     const i64 b = block_id();
     if (b >= items[m]) return;
     const i64 hi = upper_row(items, 0, m, b);
     const i32 i = hubs[hi];
     const i64 e = rp[i + 1];
-    const i64 j_lo = rp[i] + (b - items[hi]) * HCH_E;
-    const i64 j_hi = j_lo + HCH_E < e ? j_lo + HCH_E : e;
+    const i64 j_lo = rp[i] + (b - items[hi]) * he;
+    const i64 j_hi = j_lo + he < e ? j_lo + he : e;
     i64 acc = 0;
     for (i64 jb = j_lo; jb < j_hi; jb += 256) {
         const i64 j = jb + threadIdx.x;
@@ -167,16 +186,40 @@ __global__ void k_hub_index(const i32* hubs, i64 m, i32* hidx) {
 }
 
 void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
-    if (p.key == (const void*)g.rp && p.n == g.n) return;
+    if (p.key == (const void*)g.rp && p.n == g.n && p.epoch == c.knob_epoch) return;
     p.key = g.rp;
     p.n = g.n;
+    p.epoch = c.knob_epoch;
+    p.he = c.hub_edges_item;
+    p.hs = c.hub_slots_item;
     make_orders(c, g, p.ord);
     Buf<i32> cls(c, std::max<i64>(g.n, 1));
     p.sc_list = Buf<i32>(c, std::max<i64>(g.n, 1));
     k_score_classes<<<grid_items(g.n, 256), 256, 0, c.stream>>>(g.rp, g.n, cls);
     PGGC_LAUNCH_CHECK();
     partition_classes(c, cls, g.n, NCLS, p.sc_list, p.sc_start);
-    i64 m7 = p.sc_start[CL_HUB + 1] - p.sc_start[CL_HUB];
+    // hub class by degree (stable: ties by id), so that the hubs scored in
+    // LDS form a prefix; every hub is scored on its own, so the order within
+    // the class does not change results
+    const i64 nh = p.sc_start[CL_HUB + 1] - p.sc_start[CL_HUB];
+    p.nlds = 0;
+    if (nh > 1) {
+        i32* hl = p.sc_list.p + p.sc_start[CL_HUB];
+        Buf<u64> hk(c, nh);
+        k_hub_deg_keys<<<grid_items(nh, 256), 256, 0, c.stream>>>(g.rp, hl, nh, hk, hl);
+        PGGC_LAUNCH_CHECK();
+        radix_sort_pairs(c, hk, hl, nh, 40);
+    }
+    if (nh > 0 && c.hub_lds_max > 2048) {
+        Buf<i64> f(c, 1);
+        k_first_above<<<1, 64, 0, c.stream>>>(g.rp, p.sc_list.p + p.sc_start[CL_HUB], nh,
+                                               std::min<i64>(c.hub_lds_max, HUB_LDS_BOUND), f);
+        PGGC_LAUNCH_CHECK();
+        p.nlds = c.read1(f);
+    }
+    p.hub0 = p.sc_start[CL_HUB] + p.nlds;
+    p.nhub = nh - p.nlds;
+    const i64 m7 = p.nhub;
     p.gslots = 0;
     if (m7 > 0) {
         p.goff = Buf<i64>(c, m7 + 1);
@@ -184,25 +227,25 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
         p.sitems = Buf<i64>(c, m7 + 1);
         p.hub_pv = Buf<i64>(c, m7);
         p.hub_best = Buf<u64>(c, m7);
-        k_global_sizes<<<grid_items(m7, 256), 256, 0, c.stream>>>(g.rp, p.sc_list.p + p.sc_start[CL_HUB], m7, p.goff,
+        k_global_sizes<<<grid_items(m7, 256), 256, 0, c.stream>>>(g.rp, p.sc_list.p + p.hub0, m7, p.he, p.goff,
                                                                         p.items);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, p.goff, p.goff, m7);
         scan_i64(c, p.items, p.items, m7);
-        k_slot_items<<<grid_items(m7, 256), 256, 0, c.stream>>>(p.goff, m7, p.sitems);
+        k_slot_items<<<grid_items(m7, 256), 256, 0, c.stream>>>(p.goff, m7, p.hs, p.sitems);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, p.sitems, p.sitems, m7);
         p.gslots = c.read1(p.goff.p + m7);
         p.nitems = c.read1(p.items.p + m7);
         p.nsitems = c.read1(p.sitems.p + m7);
-        const i32* hubs = p.sc_list.p + p.sc_start[CL_HUB];
+        const i32* hubs = p.sc_list.p + p.hub0;
         // packed slots need every hub row weight below 2^32
         p.packed = true;
         if (g.wk != W_UNIT) {
             Buf<i64> hsum(c, m7);
             fill_i64(c, hsum, m7, 0);
             PGGC_WDISPATCH(g, (k_hub_wsum<<<grid_blocks(p.nitems), 256, 0, c.stream>>>(g.rp, W, hubs, m7, p.items,
-                                                                                    hsum)));
+                                                                                    p.he, hsum)));
             PGGC_LAUNCH_CHECK();
             std::vector<i64> hs(m7);
             PGGC_HIP(hipMemcpyAsync(hs.data(), hsum, m7 * sizeof(i64), hipMemcpyDeviceToHost, c.stream));
@@ -312,8 +355,9 @@ __global__ void __launch_bounds__(256) k_score_group(const i64* __restrict__ rp,
 }
 
 // One block per vertex with a hash table in LDS (or HBM for the largest
-// degrees).
-template <int BS, int SLOTS, bool GLOBAL, class WT>
+// degrees): SLOTS slots of LDS hold the table of any degree up to
+// SLOTS / SPE, each table the power of two >= SPE * degree.
+template <int BS, int SLOTS, int SPE, bool GLOBAL, class WT>
 __global__ void __launch_bounds__(BS) k_score_table(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
                                                     const i64* __restrict__ vw, const i32* __restrict__ list, i64 m,
                                                     const i32* __restrict__ part, const i64* __restrict__ vol,
@@ -330,7 +374,7 @@ __global__ void __launch_bounds__(BS) k_score_table(const i64* __restrict__ rp, 
         if (dest[i] == NO_MOVE) return;
         const i64 s = rp[i];
         const i64 d = rp[i + 1] - s;
-        const i64 S = tslots(d);
+        const i64 S = tslots(d, SPE);
         const u32 mask = (u32)(S - 1);
         i32* keys = GLOBAL ? gkeys + goff[li] : skeys;
         i64* vals = GLOBAL ? gvals + goff[li] : svals;
@@ -342,10 +386,10 @@ __global__ void __launch_bounds__(BS) k_score_table(const i64* __restrict__ rp, 
         __syncthreads();
         const i32 p = part[i];
         i64 pv = 0;
-        // LDS classes hold d <= SLOTS / 2: a thread first gathers the
+        // LDS classes hold d <= SLOTS / SPE: a thread first gathers the
         // clusters of all its (at most R) edges, so the gathers are in flight
         // together, then inserts them
-        constexpr int R = GLOBAL ? 0 : (SLOTS / 2 + BS - 1) / BS;
+        constexpr int R = GLOBAL ? 0 : (SLOTS / SPE + BS - 1) / BS;
         i64 e_done = 0;
         if (!GLOBAL) {
             i32 kk[R > 0 ? R : 1];
@@ -463,15 +507,15 @@ __global__ void __launch_bounds__(256) k_hub_raw(const i64* __restrict__ rp, con
                                                  const i64* __restrict__ vw, const i32* __restrict__ hubs, i64 m,
                                                  const i64* __restrict__ items, const i32* __restrict__ part,
                                                  const i64* __restrict__ vol, const i32* __restrict__ cons, float lam,
-                                                 float ratio, const i32* __restrict__ dest, u64* hub_best) {
+                                                 float ratio, const i32* __restrict__ dest, i64 he, u64* hub_best) {
     const i64 b = block_id();
     if (b >= items[m]) return;
     const i64 hi = upper_row(items, 0, m, b);
     const i32 i = hubs[hi];
     if (dest[i] == NO_MOVE) return;
     const i64 s = rp[i], e = rp[i + 1];
-    const i64 j_lo = s + (b - items[hi]) * HCH_E;
-    const i64 j_hi = j_lo + HCH_E < e ? j_lo + HCH_E : e;
+    const i64 j_lo = s + (b - items[hi]) * he;
+    const i64 j_hi = j_lo + he < e ? j_lo + he : e;
     const i32 p = part[i];
     const i32 cp = cons ? cons[p] : 0;
     float multi, p_conn, bth;
@@ -488,18 +532,20 @@ __global__ void __launch_bounds__(256) k_hub_raw(const i64* __restrict__ rp, con
     if (threadIdx.x == 0 && bk) atomicMax((unsigned long long*)&hub_best[hi], (unsigned long long)bk);
 }
 
-// Hubs: edge work items of HCH_E edges pre-aggregate in LDS, then insert into
+// Hubs: edge work items (he edges each) pre-aggregate in LDS, then insert into
 // the hub's HBM table (built for every hub at the first table sweep of a
 // call, own cluster included); later sweeps keep it current from the movers
 // (k_hub_update_*).  Per sweep, k_hub_pv reads the own-cluster weight, slot
 // work items scan the table and fold the packed best candidate with
 // atomicMax, and a last pass decides.
-template <bool PACKED, class WT>
+// HL LDS slots pre-aggregate an item of HL / 2 edges.
+template <bool PACKED, int HL, class WT>
 __global__ void __launch_bounds__(256) k_hub_insert(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
                                                     const i32* __restrict__ hubs, i64 m, const i64* __restrict__ items,
                                                     const i32* __restrict__ imap, i64 nmap,
                                                     const i64* __restrict__ goff, const i32* __restrict__ part,
                                                     i32* gkeys, i64* gvals, u64* gtab, u64* acct) {
+    constexpr i64 he = HL / 2;
     const i64 bb = block_id();
     if (bb >= nmap) return;
     const i64 b = imap[bb];
@@ -507,19 +553,19 @@ __global__ void __launch_bounds__(256) k_hub_insert(const i64* __restrict__ rp, 
     const i64 hi = upper_row(items, 0, m, b);
     const i32 i = hubs[hi];
     const i64 s = rp[i], e = rp[i + 1];
-    const i64 j_lo = s + (b - items[hi]) * HCH_E;
-    const i64 j_hi = j_lo + HCH_E < e ? j_lo + HCH_E : e;
-    __shared__ i32 lk[HLDS];
-    __shared__ i64 lv[HLDS];
-    lds_agg_clear<256, HLDS>(lk, lv);
+    const i64 j_lo = s + (b - items[hi]) * he;
+    const i64 j_hi = j_lo + he < e ? j_lo + he : e;
+    __shared__ i32 lk[HL];
+    __shared__ i64 lv[HL];
+    lds_agg_clear<256, HL>(lk, lv);
     for (i64 jb = j_lo; jb < j_hi; jb += 256) {  // block-uniform loop control
         const i64 j = jb + threadIdx.x;
         if (j >= j_hi) continue;
-        table_add_lds(lk, lv, HLDS - 1, part[col[j]], W(j));
+        table_add_lds(lk, lv, HL - 1, part[col[j]], W(j));
     }
     const u32 gmask = (u32)(goff[hi + 1] - goff[hi] - 1);
-    if (PACKED) lds_agg_flush_packed<256, HLDS>(lk, lv, gtab + goff[hi], gmask);
-    else lds_agg_flush<256, HLDS>(lk, lv, gkeys + goff[hi], gvals + goff[hi], gmask);
+    if (PACKED) lds_agg_flush_packed<256, HL>(lk, lv, gtab + goff[hi], gmask);
+    else lds_agg_flush<256, HL>(lk, lv, gkeys + goff[hi], gvals + goff[hi], gmask);
     // algorithmic bytes of the item: column, neighbour cluster id and weight
     // per edge, plus the hub's row bounds and cluster
     if (acct && threadIdx.x == 0)
@@ -621,7 +667,7 @@ __global__ void __launch_bounds__(256) k_hub_scan(const i32* __restrict__ hubs, 
                                                   const i32* __restrict__ dest, const i64* __restrict__ vw,
                                                   const i64* __restrict__ vol, const i32* __restrict__ cons, float lam,
                                                   float ratio, const i32* gkeys, const i64* gvals, const u64* gtab,
-                                                  const i64* __restrict__ hub_pv, u64* hub_best) {
+                                                  const i64* __restrict__ hub_pv, i64 hs, u64* hub_best) {
     const i64 bb = block_id();
     if (bb >= nmap) return;
     const i64 b = smap[bb];
@@ -629,9 +675,9 @@ __global__ void __launch_bounds__(256) k_hub_scan(const i32* __restrict__ hubs, 
     const i64 hi = upper_row(sitems, 0, m, b);
     const i32 i = hubs[hi];
     if (dest[i] == NO_MOVE) return;
-    const i64 s_lo = goff[hi] + (b - sitems[hi]) * HCH_S;
+    const i64 s_lo = goff[hi] + (b - sitems[hi]) * hs;
     const i64 s_end = goff[hi + 1];
-    const i64 s_hi = s_lo + HCH_S < s_end ? s_lo + HCH_S : s_end;
+    const i64 s_hi = s_lo + hs < s_end ? s_lo + hs : s_end;
     const i32 p = part[i];
     float multi, p_conn, bth;
     own_terms(hub_pv[hi], vol[p], vw[i], lam, ratio, multi, p_conn, bth);
@@ -686,6 +732,7 @@ __global__ void k_hub_decide(const i32* hubs, i64 m, const i32* part, const i64*
 // and destination read.  acc3[0..2] = group, table and hub classes.
 struct ScoreBounds {
     i64 s[NCLS + 1];
+    i64 hub0;  // first HBM-table hub (the hubs before it are scored in LDS)
 };
 template <int WB>
 __global__ void __launch_bounds__(256) k_score_bytes(const i64* __restrict__ rp, const i32* __restrict__ list,
@@ -695,14 +742,14 @@ __global__ void __launch_bounds__(256) k_score_bytes(const i64* __restrict__ rp,
     i64 b0 = 0, b1 = 0, b2 = 0;
     if (x < sb.s[NCLS]) {
         const i32 i = list[x];
-        const int grp = x < sb.s[CL_TABLE] ? 0 : (x < sb.s[CL_HUB] ? 1 : 2);
+        const int grp = x < sb.s[CL_TABLE] ? 0 : (x < sb.hub0 ? 1 : 2);
         i64 by = 8;
         if (dest[i] != NO_MOVE) {
             const i64 eb = (rp[i + 1] - rp[i]) * (4 + WB + (raw ? 0 : 4));
             if (grp < 2 || raw) {
                 by += 48 + eb;
             } else {
-                const i64 h = x - sb.s[CL_HUB];
+                const i64 h = x - sb.hub0;
                 by += 48 + 8 * (goff[h + 1] - goff[h]) + (build ? eb : 0);
             }
         }
@@ -735,6 +782,62 @@ __global__ void __launch_bounds__(256) k_list_bytes(const i64* __restrict__ rp, 
     if (threadIdx.x == 0 && b) atomicAdd((unsigned long long*)acc, (unsigned long long)b);
 }
 
+// LDS-table scoring of the list entries [x0, x1) (one block per vertex):
+// BS threads, SLOTS LDS slots, SPE slots per edge of the degree bound.
+template <int BS, int SLOTS, int SPE, class WT>
+static void launch_table(Ctx& c, const DGraph& g, WT W, const i32* L, i64 x0, i64 x1, const i32* part,
+                         const i64* vol, const i32* cons, float lam, float ratio, i32* dest, float* gain) {
+    const i64 m = x1 - x0;
+    if (m <= 0) return;
+    k_score_table<BS, SLOTS, SPE, false><<<grid_blocks(m), BS, 0, c.stream>>>(
+        g.rp, g.col, W, g.vw, L + x0, m, part, vol, cons, 0, lam, ratio, dest, gain, nullptr, nullptr, nullptr);
+}
+// class with degree bound D (128, 256, 512): threads per vertex and slots per
+// edge from the knobs
+template <int D, class WT>
+static void launch_table_class(Ctx& c, const DGraph& g, WT W, const i32* L, i64 x0, i64 x1, const i32* part,
+                               const i64* vol, const i32* cons, float lam, float ratio, i32* dest, float* gain) {
+    const int bs = c.table_threads, spe = c.table_spe;
+    if (bs == 64 && spe == 2) launch_table<64, 2 * D, 2>(c, g, W, L, x0, x1, part, vol, cons, lam, ratio, dest, gain);
+    else if (bs == 64) launch_table<64, 4 * D, 4>(c, g, W, L, x0, x1, part, vol, cons, lam, ratio, dest, gain);
+    else if (bs == 128 && spe == 2)
+        launch_table<128, 2 * D, 2>(c, g, W, L, x0, x1, part, vol, cons, lam, ratio, dest, gain);
+    else if (bs == 128) launch_table<128, 4 * D, 4>(c, g, W, L, x0, x1, part, vol, cons, lam, ratio, dest, gain);
+    else if (spe == 2) launch_table<256, 2 * D, 2>(c, g, W, L, x0, x1, part, vol, cons, lam, ratio, dest, gain);
+    else launch_table<256, 4 * D, 4>(c, g, W, L, x0, x1, part, vol, cons, lam, ratio, dest, gain);
+}
+
+// hub table build over all edge items, with the LDS pre-aggregation sized
+// for the plan's edges per item
+template <class WT>
+static void hub_insert_all(Ctx& c, const DGraph& g, WT W, LevelPlan& lp, const i32* hubs, i64 m, const i32* part) {
+    const dim3 grid = grid_blocks(lp.igrid);
+    i32* gk = lp.packed ? nullptr : lp.gkeys.p;
+    i64* gv = lp.packed ? nullptr : lp.gvals.p;
+    u64* gt = lp.packed ? lp.gtab.p : nullptr;
+    if (lp.packed) {
+        if (lp.he == 512)
+            k_hub_insert<true, 1024><<<grid, 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid,
+                                                                lp.goff, part, gk, gv, gt, nullptr);
+        else if (lp.he == 2048)
+            k_hub_insert<true, 4096><<<grid, 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid,
+                                                                lp.goff, part, gk, gv, gt, nullptr);
+        else
+            k_hub_insert<true, 2048><<<grid, 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid,
+                                                                lp.goff, part, gk, gv, gt, nullptr);
+    } else {
+        if (lp.he == 512)
+            k_hub_insert<false, 1024><<<grid, 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.imap,
+                                                                 lp.igrid, lp.goff, part, gk, gv, gt, nullptr);
+        else if (lp.he == 2048)
+            k_hub_insert<false, 4096><<<grid, 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.imap,
+                                                                 lp.igrid, lp.goff, part, gk, gv, gt, nullptr);
+        else
+            k_hub_insert<false, 2048><<<grid, 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.imap,
+                                                                 lp.igrid, lp.goff, part, gk, gv, gt, nullptr);
+    }
+}
+
 static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32* part, const i64* vol,
                       const i32* cons, bool raw, float lam, float ratio, i32* dest, float* gain) {
     const i64* st = lp.sc_start;
@@ -745,8 +848,9 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
         fill_i64(c, acct, 3, 0);
         ScoreBounds sb;
         for (int k = 0; k <= NCLS; k++) sb.s[k] = st[k];
+        sb.hub0 = lp.hub0;
         const int build = (!raw && !lp.hub_live) ? 1 : 0;
-        const i64* goff = st[CL_HUB + 1] > st[CL_HUB] ? lp.goff.p : nullptr;
+        const i64* goff = lp.nhub > 0 ? lp.goff.p : nullptr;
         PGGC_WDISPATCH(g, (k_score_bytes<decltype(W)::bytes><<<grid_items(g.n, 256), 256, 0, c.stream>>>(
                               g.rp, L, sb, dest, raw ? 1 : 0, build, goff, acct)));
         PGGC_LAUNCH_CHECK();
@@ -776,43 +880,37 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
             if ((m = st[CL_BIG] - st[CL_TABLE]) > 0)
                 k_score_raw<64><<<grid_blocks(m), 64, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[CL_TABLE], m,
                                                                      part, vol, cons, lam, ratio, dest, gain);
-            if ((m = st[CL_HUB] - st[CL_BIG]) > 0)
+            if ((m = lp.hub0 - st[CL_BIG]) > 0)  // big rows and the LDS-scored hubs
                 k_score_raw<256><<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[CL_BIG], m,
                                                                        part, vol, cons, lam, ratio, dest, gain);
         } else {
-            // one wave per vertex; the LDS table is sized for the class, so
-            // the many low-degree rows keep more waves resident per CU
-            if ((m = st[6] - st[5]) > 0)
-                k_score_table<64, 256, false><<<grid_blocks(m), 64, 0, c.stream>>>(
-                    g.rp, g.col, W, g.vw, L + st[5], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr,
-                    nullptr, nullptr);
-            if ((m = st[7] - st[6]) > 0)
-                k_score_table<64, 512, false><<<grid_blocks(m), 64, 0, c.stream>>>(
-                    g.rp, g.col, W, g.vw, L + st[6], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr,
-                    nullptr, nullptr);
-            if ((m = st[8] - st[7]) > 0)
-                k_score_table<64, 1024, false><<<grid_blocks(m), 64, 0, c.stream>>>(
-                    g.rp, g.col, W, g.vw, L + st[7], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr,
-                    nullptr, nullptr);
-            if ((m = st[9] - st[8]) > 0)
-                k_score_table<256, 2048, false><<<grid_blocks(m), 256, 0, c.stream>>>(
-                    g.rp, g.col, W, g.vw, L + st[8], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr,
-                    nullptr, nullptr);
-            if ((m = st[10] - st[9]) > 0)
-                k_score_table<512, 4096, false><<<grid_blocks(m), 512, 0, c.stream>>>(
-                    g.rp, g.col, W, g.vw, L + st[9], m, part, vol, cons, raw, lam, ratio, dest, gain, nullptr,
-                    nullptr, nullptr);
+            // LDS tables sized for the class, so the many low-degree rows keep
+            // more waves resident per CU; threads per vertex and slots per
+            // edge are knobs
+            launch_table_class<128>(c, g, W, L, st[5], st[6], part, vol, cons, lam, ratio, dest, gain);
+            launch_table_class<256>(c, g, W, L, st[6], st[7], part, vol, cons, lam, ratio, dest, gain);
+            launch_table_class<512>(c, g, W, L, st[7], st[8], part, vol, cons, lam, ratio, dest, gain);
+            if (c.table_spe == 2) {
+                launch_table<256, 2048, 2>(c, g, W, L, st[8], st[9], part, vol, cons, lam, ratio, dest, gain);
+                launch_table<512, 4096, 2>(c, g, W, L, st[9], st[10], part, vol, cons, lam, ratio, dest, gain);
+            } else {
+                launch_table<256, 4096, 4>(c, g, W, L, st[8], st[9], part, vol, cons, lam, ratio, dest, gain);
+                launch_table<512, 8192, 4>(c, g, W, L, st[9], st[10], part, vol, cons, lam, ratio, dest, gain);
+            }
+            // hubs of degree <= hub_lds_max: 8192 LDS slots (96 KB of LDS)
+            launch_table<512, 8192, 2>(c, g, W, L, st[CL_HUB], lp.hub0, part, vol, cons, lam, ratio, dest, gain);
         }
         c.kt_end(tk, 0.0);
         if (tk >= 0) acct_add(c, c.kt_counter(tk), acct.p + 1);
-        if ((m = st[CL_HUB + 1] - st[CL_HUB]) > 0) {
-            const i32* hubs = L + st[CL_HUB];
+        if ((m = lp.nhub) > 0) {
+            const i32* hubs = L + lp.hub0;
             tk = c.kt_begin(KS_SCORE_HUB);
             fill_i64(c, lp.hub_pv, m, 0);
             fill_i64(c, (i64*)lp.hub_best.p, m, 0);
             if (raw) {
                 k_hub_raw<<<grid_blocks(lp.nitems), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, hubs, m, lp.items, part,
-                                                                       vol, cons, lam, ratio, dest, lp.hub_best);
+                                                                       vol, cons, lam, ratio, dest, lp.he,
+                                                                       lp.hub_best);
             } else {
                 if (!lp.hub_live) {  // first table sweep of the call (or stale tables): build
                     if (lp.packed) {
@@ -821,15 +919,7 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
                         fill_i32(c, lp.gkeys, lp.gslots, EMPTY_KEY);
                         fill_i64(c, lp.gvals, lp.gslots, 0);
                     }
-                    if (lp.packed) {
-                        k_hub_insert<true><<<grid_blocks(lp.igrid), 256, 0, c.stream>>>(
-                            g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid, lp.goff, part, nullptr, nullptr,
-                            lp.gtab, nullptr);
-                    } else {
-                        k_hub_insert<false><<<grid_blocks(lp.igrid), 256, 0, c.stream>>>(
-                            g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid, lp.goff, part, lp.gkeys, lp.gvals,
-                            nullptr, nullptr);
-                    }
+                    hub_insert_all(c, g, W, lp, hubs, m, part);
                     lp.hub_live = true;
                 }
                 if (lp.packed) {
@@ -837,13 +927,13 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
                                                                             nullptr, lp.gtab, lp.hub_pv);
                     k_hub_scan<true><<<grid_blocks(lp.sgrid), 256, 0, c.stream>>>(
                         hubs, m, lp.sitems, lp.smap, lp.sgrid, lp.goff, part, dest, g.vw, vol, cons, lam, ratio,
-                        nullptr, nullptr, lp.gtab, lp.hub_pv, lp.hub_best);
+                        nullptr, nullptr, lp.gtab, lp.hub_pv, lp.hs, lp.hub_best);
                 } else {
                     k_hub_pv<false><<<grid_items(m, 256), 256, 0, c.stream>>>(hubs, m, lp.goff, part, dest, lp.gkeys,
                                                                              lp.gvals, nullptr, lp.hub_pv);
                     k_hub_scan<false><<<grid_blocks(lp.sgrid), 256, 0, c.stream>>>(
                         hubs, m, lp.sitems, lp.smap, lp.sgrid, lp.goff, part, dest, g.vw, vol, cons, lam, ratio,
-                        lp.gkeys, lp.gvals, nullptr, lp.hub_pv, lp.hub_best);
+                        lp.gkeys, lp.gvals, nullptr, lp.hub_pv, lp.hs, lp.hub_best);
                 }
             }
             k_hub_decide<<<grid_items(m, 256), 256, 0, c.stream>>>(hubs, m, part, g.vw, vol, lam, ratio, lp.hub_pv,
@@ -1273,7 +1363,7 @@ static void perform_moves(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g
     const i64 nb = std::min(k, n_massive);
     const int tk = c.kt_begin(KS_MOVES);
     fill_i64(c, s.scal.p + 3, 1, 0);
-    const i64 m7 = lp.sc_start[CL_HUB + 1] - lp.sc_start[CL_HUB];
+    const i64 m7 = lp.nhub;
     // many movers: updating costs more than rebuilding the tables next sweep
     if (lp.hub_live && m7 > 0 && (c.hub_update_div <= 0 || k * c.hub_update_div > g.n)) lp.hub_live = false;
     const bool hub_upd = lp.hub_live && m7 > 0;

@@ -195,6 +195,28 @@ def rmat_device(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=2, weights=N
     return _devgraph_tensors(h, ctx)
 
 
+def lfr_device(n, tau1=2.5, kmean=20.0, kmax=1000, tau2=1.5, cmin=20, cmax=2000, mu=0.4, weights=None, seed=3,
+               ground_truth=False, ctx=None):
+    """LFR-style benchmark graph built on the GPU by the library (pggc_lfr):
+    power-law degrees and community sizes, mixing mu, stub matching inside
+    and across communities, permuted ids, per-pair weights uniform in
+    `weights` = (lo, hi).  The oracle's generator builds the identical graph
+    on the CPU.  Returns device tensors (rp, col, w or None) and, with
+    ground_truth, the planted community of every vertex."""
+    import ctypes
+    import torch
+    from . import _lib
+    from .clustering import default_context
+    ctx = ctx or default_context()
+    wmin, wmax = (0, 0) if weights is None else weights
+    gt = torch.empty(max(n, 1), dtype=torch.int32, device="cuda") if ground_truth else None
+    h = ctypes.c_void_p()
+    _lib.check(ctx.lib.pggc_lfr(ctx.h, n, tau1, kmean, kmax, tau2, cmin, cmax, mu, wmin, wmax, seed,
+                                ctypes.c_void_p(gt.data_ptr()) if gt is not None else None, ctypes.byref(h)))
+    rp, col, w = _devgraph_tensors(h, ctx)
+    return (rp, col, w, gt[:n]) if ground_truth else (rp, col, w)
+
+
 def from_edges_device(n, src, dst, w=None, ctx=None):
     """construct_from_edgelist on the GPU (pggc_graph_from_edges): device
     int32 src/dst (and int32 weights) -> (rp, col, w) device tensors."""

@@ -70,3 +70,32 @@ def test_from_edges_rejects_bad_input(gpu_ctx):
     w = torch.tensor([1, -2, 3], dtype=torch.int32, device="cuda")
     with pytest.raises(_lib.PggcError):
         graphs.from_edges_device(6, src, dst, w)  # negative weight
+
+
+@pytest.mark.parametrize("n,kmax,cmin,cmax,mu,weights,seed", [(1, 10, 20, 200, 0.4, None, 3),
+                                                               (150, 30, 20, 100, 0.3, None, 1),
+                                                               (20000, 300, 20, 500, 0.4, (1, 100), 3),
+                                                               (200000, 1000, 20, 2000, 0.4, (1, 100), 3),
+                                                               (300000, 1000, 20, 2000, 0.6, None, 5)])
+def test_lfr_matches_oracle_generator(gpu_ctx, n, kmax, cmin, cmax, mu, weights, seed):
+    kmean = min(20.0, kmax / 3)
+    print(f"lfr n={n} kmax={kmax} c=[{cmin},{cmax}] mu={mu} w={weights} seed={seed}", flush=True)
+    rp, col, w, gt = graphs.lfr_device(n, 2.5, kmean, kmax, 1.5, cmin, cmax, mu, weights, seed, ground_truth=True)
+    orp, ocol, ow, ogt, _, _ = O.lfr(n, 2.5, kmean, kmax, 1.5, cmin, cmax, mu, weights=weights, seed=seed)
+    assert np.array_equal(npy(rp), orp)
+    assert np.array_equal(npy(col), ocol)
+    assert np.array_equal(npy(gt), ogt)
+    if weights is None:
+        assert w is None and ow is None
+    else:
+        assert np.array_equal(npy(w), ow)
+
+
+def test_lfr_rejects_bad_parameters(gpu_ctx):
+    from pennsylvaniagpugraphclustering_amd import _lib
+    with pytest.raises(_lib.PggcError):
+        graphs.lfr_device(1000, mu=1.5)
+    with pytest.raises(_lib.PggcError):
+        graphs.lfr_device(1000, kmean=5000.0)  # above kmax
+    with pytest.raises(_lib.PggcError):
+        graphs.lfr_device(1000, cmin=50, cmax=20)
