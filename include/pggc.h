@@ -253,6 +253,20 @@ int pggc_orderings(pggc_ctx* ctx, const pggc_graph* g, int32_t* order1, int32_t*
 int pggc_evaluate(pggc_ctx* ctx, const pggc_graph* g, const int32_t* labels, int64_t L, int64_t* vol_out,
                   int64_t* uncut_out, uint64_t* sq_out);
 
+/* pggc_evaluate without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as pggc_evaluate.  Its scratch (per-chunk
+ * partial sums) lives in a static device array of
+ * pggc_evaluate_noalloc_limits()->scratch_bytes (about 8 MiB) that each GPU
+ * holds until the process ends; graphs with more than max_nnz (2^32)
+ * nonzeros are rejected with PGGC_ECAPACITY.  All calls on one GPU share that
+ * array: calls that overlap on one GPU (several streams or host threads)
+ * race. */
+int pggc_evaluate_noalloc(pggc_ctx* ctx, const pggc_graph* g, const int32_t* labels, int64_t L, int64_t* vol_out,
+                          int64_t* uncut_out, uint64_t* sq_out);
+int pggc_evaluate_noalloc_limits(int64_t* max_nnz, int64_t* scratch_bytes);  /* host outputs, nullable */
+
 /* own-cluster weight of every vertex (the pvals_clone contract,
  * src/local_mover.cpp:1354-1358) */
 int pggc_own_weight(pggc_ctx* ctx, const pggc_graph* g, const int32_t* labels, int64_t* out);

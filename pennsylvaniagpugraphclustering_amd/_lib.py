@@ -93,6 +93,8 @@ SIGNATURES = {
     "pggc_state_init": (c.c_int, [P, c.POINTER(Graph), c.c_int, c.c_double, c.POINTER(State)]),
     "pggc_orderings": (c.c_int, [P, c.POINTER(Graph), P, P, I64P]),
     "pggc_evaluate": (c.c_int, [P, c.POINTER(Graph), P, c.c_int64, P, I64P, c.POINTER(c.c_uint64)]),
+    "pggc_evaluate_noalloc": (c.c_int, [P, c.POINTER(Graph), P, c.c_int64, P, I64P, c.POINTER(c.c_uint64)]),
+    "pggc_evaluate_noalloc_limits": (c.c_int, [I64P, I64P]),
     "pggc_own_weight": (c.c_int, [P, c.POINTER(Graph), P, P]),
     "pggc_contract": (c.c_int, [P, c.POINTER(Graph), P, c.c_int64, P, P, P, c.c_int64, I64P]),
     "pggc_sweep": (c.c_int, [P, c.POINTER(Graph), P, P, c.c_int64, c.c_double, c.c_float, c.c_int, P, P, P, P,

@@ -18,6 +18,7 @@
 #include "leiden.h"
 #include "localmove.h"
 #include "meme.h"
+#include "noalloc.h"
 #include "pggc.h"
 
 using namespace pggc;
@@ -401,6 +402,7 @@ int pggc_ctx_create(pggc_ctx** out, void* stream) {
         PGGC_HIP(hipGetDevice(&x->c.device));
         PGGC_HIP(hipHostMalloc((void**)&x->c.pinned, 64 * sizeof(i64), hipHostMallocDefault));
         PGGC_HIP(hipHostGetDevicePointer((void**)&x->c.pinned_dev, x->c.pinned, 0));
+        noalloc_init(x->c.device);  // resolves the evaluator scratch of this device (once per device)
         x->ms.gnn = &x->gnn;
         *out = x.release();
     });
@@ -830,6 +832,23 @@ int pggc_evaluate(pggc_ctx* ctx, const pggc_graph* gg, const int32_t* labels, in
         *uncut_out = h[0];
         sq_out[0] = (u64)h[1];
         sq_out[1] = (u64)h[2];
+    });
+}
+
+int pggc_evaluate_noalloc(pggc_ctx* ctx, const pggc_graph* gg, const int32_t* labels, int64_t L, int64_t* vol_out,
+                          int64_t* uncut_out, uint64_t* sq_out) {
+    return guard([&] {
+        CTX_CHECK(ctx);
+        if (!uncut_out || !sq_out) throw Error(PGGC_EINVAL, "uncut_out and sq_out are required");
+        DGraph g = view_of(gg);
+        evaluate_noalloc(ctx->c, g, labels, L, vol_out, uncut_out, sq_out);
+    });
+}
+
+int pggc_evaluate_noalloc_limits(int64_t* max_nnz, int64_t* scratch_bytes) {
+    return guard([&] {
+        if (max_nnz) *max_nnz = noalloc_max_nnz();
+        if (scratch_bytes) *scratch_bytes = noalloc_scratch_bytes();
     });
 }
 
