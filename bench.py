@@ -57,6 +57,10 @@ def parse():
                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
     p.add_argument("--knob", action="append", default=[], metavar="NAME=VALUE",
                    help="set a tuning knob of the HIP path before the run (repeatable)")
+    p.add_argument("--lfr-n", type=int, default=8000000,
+                   help="LFR-style leg: n vertices (power-law degrees: exponent 2.5, mean 20, max 1000; community "
+                        "sizes: exponent 1.5 in [20, 2000]; mixing 0.4; weights uniform in [1, 100]; seed 3), "
+                        "pLeiden with weighted modularity, plain and with the active-set predictor (0 skips it)")
     p.add_argument("--leiden", type=int, default=2, metavar="ITERS",
                    help="pLeiden+ leg with ex_iters=ITERS (-1: to stable, which creeps up by ~1e-8 per outer "
                         "iteration for hundreds of iterations at scale 24; 0 or less than -1 skips it)")
@@ -149,12 +153,48 @@ def roofline(kstats, scale):
     return out
 
 
+def lfr_leg(a, ctx, C, graphs, torch):
+    """The LFR-style workload: pLeiden (ex_iters 0) with weighted modularity
+    on a graph built on the GPU, timed plain (best of 2) and with the
+    active-set predictor (shipped weights and threshold)."""
+    t0 = time.perf_counter()
+    rp, col, w = graphs.lfr_device(a.lfr_n, 2.5, 20.0, 1000, 1.5, 20, 2000, 0.4, weights=(1, 100), seed=3, ctx=ctx)
+    g = C.DeviceGraph(rp, col, w, None, objective="WMod", ctx=ctx)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    times = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        q, _, info = C.cluster(g, "leiden", "WMod", 1.0, 0, seed=0, ctx=ctx)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t1)
+    out = {"n": g.n, "nnz": g.nnz, "undirected_edges": g.nnz // 2, "generate_s": gen_s, "seconds": min(times),
+           "edges_per_s": (g.nnz // 2) / min(times), "Q": q, "labels": info["labels"], "sweeps": info["sweeps"],
+           "levels": info["levels"]}
+    if not a.no_gnn:
+        from pennsylvaniagpugraphclustering_amd import predictor
+        d = predictor.enable(ctx=ctx)
+        C.gnn_reset_stats(ctx=ctx)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        qg, _, ig = C.cluster(g, "leiden", "WMod", 1.0, 0, seed=0, ctx=ctx)
+        torch.cuda.synchronize()
+        tg = time.perf_counter() - t1
+        st = C.gnn_stats(ctx=ctx)
+        C.gnn_configure("off", ctx=ctx)
+        out["gnn"] = {"tau": d["tau"], "seconds": tg, "speedup": min(times) / tg, "Q": qg, "dQ_vs_plain": qg - q,
+                      "sweeps": ig["sweeps"], "pruned_fraction": st["pruned"] / max(st["active"], 1)}
+    del g, rp, col, w
+    return out
+
+
 def batch_leg(a, ctx, C, graphs, torch):
     """Many small instances: `a.batch` planted-partition graphs with seeds
     100.., n = round(2^U(10,18)), blocks uniform in [n/256, n/32], expected
     intra degree 10 and inter degree 3 (graphs.sbm_torch), pLouvain ex_iters
     0.  Timed one by one on the context, then as one pggc_cluster_batch call
-    on a.batch_streams streams; the objectives must agree."""
+    (one disjoint union graph); the objectives must agree."""
     import numpy as np
     rng = np.random.default_rng(100)
     bg = []
@@ -175,7 +215,8 @@ def batch_leg(a, ctx, C, graphs, torch):
     tb = time.perf_counter() - t2
     ctx.set_verbose(1)
     return {"graphs": len(bg), "vertices": sum(x.n for x in bg), "edges": edges, "one_by_one_s": ts,
-            "batch_s": tb, "streams": a.batch_streams, "speedup": ts / tb, "batch_edges_per_s": edges / tb,
+            "batch_s": tb, "mode": "one disjoint union graph: each kernel launched once over all instances",
+            "speedup": ts / tb, "batch_edges_per_s": edges / tb,
             "objectives_equal": all(r[0] == q[0] for r, q in zip(res, seq)),
             "mean_Q": float(np.mean([r[0] for r in res]))}
 
@@ -219,23 +260,28 @@ def main():
         q, part, info = step()
         print(f"# warmup {w}: Q={q:.6f} info={info}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
-    ctx.set_timing(True)
     t0 = time.perf_counter()
     for s in range(a.steps):
         q, part, info = step()
         print(f"# step {s}: Q={q:.6f}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / max(a.steps, 1)
+    # one more step with the kernel timers and byte counters on (they cost
+    # time of their own, so the timed steps above run without them)
+    ctx.set_timing(True)
+    t1 = time.perf_counter()
+    step()
+    torch.cuda.synchronize()
+    instrumented_s = time.perf_counter() - t1
     kstats = ctx.kernel_stats()
     ctx.set_timing(False)
     groups = {}
     for name, (n, ms, by) in kstats.items():
         if n:
-            groups[name] = {"ms_per_step": ms / max(a.steps, 1), "launches_per_step": n / max(a.steps, 1),
-                            "GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None,
-                            "GB_per_step": by / max(a.steps, 1) / 1e9}
-            print(f"# kernel group {name}: {n} launches, {ms / max(a.steps, 1):.1f} ms/step, "
-                  f"{ms / n * 1000:.1f} us avg, {groups[name]['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
+            groups[name] = {"ms_per_step": ms, "launches_per_step": n, "GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None,
+                            "GB_per_step": by / 1e9}
+            print(f"# kernel group {name}: {n} launches, {ms:.1f} ms/step, {ms / n * 1000:.1f} us avg, "
+                  f"{groups[name]['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
 
     extra = {}
     golden = GOLDEN.format(scale=a.scale)
@@ -286,6 +332,10 @@ def main():
                         "pruned_fraction": st["pruned"] / max(st["active"], 1), "restores": st["restores"]}
         print(f"# predictor A/B: {extra['gnn']}", file=sys.stderr, flush=True)
 
+    if a.lfr_n > 0:
+        extra["lfr"] = lfr_leg(a, ctx, C, graphs, torch)
+        print(f"# LFR leg: {extra['lfr']}", file=sys.stderr, flush=True)
+
     if a.batch > 0:
         extra["batch"] = batch_leg(a, ctx, C, graphs, torch)
         print(f"# batch: {extra['batch']}", file=sys.stderr, flush=True)
@@ -321,6 +371,7 @@ def main():
         "first_full_sweep_ms": sweep_s * 1000.0 if sweep_s else None,
         "roofline": roofline(kstats, a.scale),
         "kernel_groups": groups,
+        "instrumented_step_s": instrumented_s,
         "cpu_baseline": cpu,
     }
     out.update(extra)

@@ -156,7 +156,10 @@ int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user);
  *   PGGC_KNOB_HUB_EDGES_PER_ITEM: edges per hub-table build work item:
  *     512, 1024 (default), 2048.
  *   PGGC_KNOB_HUB_SLOTS_PER_ITEM: hub-table slots per scan work item:
- *     2048, 4096 (default), 8192. */
+ *     2048, 4096 (default), 8192.
+ *   PGGC_KNOB_RAW_SWEEPS: 1 (default) scores the first sweep of a local move
+ *     from singletons on the raw adjacency (src/local_mover.cpp:1391-1395),
+ *     0 with the neighbour-cluster tables (the same result, more work). */
 #define PGGC_KNOB_HUB_UPDATE_DIV 0
 #define PGGC_KNOB_GNN_MAX_SWEEPS 1
 #define PGGC_KNOB_TABLE_THREADS 2
@@ -164,15 +167,16 @@ int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user);
 #define PGGC_KNOB_HUB_LDS_MAX_DEGREE 4
 #define PGGC_KNOB_HUB_EDGES_PER_ITEM 5
 #define PGGC_KNOB_HUB_SLOTS_PER_ITEM 6
-#define PGGC_KNOB_COUNT 7
+#define PGGC_KNOB_RAW_SWEEPS 7
+#define PGGC_KNOB_COUNT 8
 int pggc_ctx_set_knob(pggc_ctx* ctx, int knob, int64_t value);
 int pggc_ctx_get_knob(pggc_ctx* ctx, int knob, int64_t* value);
 /* Autotuner of the execution-shape knobs (no counterpart in the reference):
  * coordinate descent in the order hub LDS max degree, table threads, table
  * slots per edge, hub edges per item, hub slots per item, hub update divisor;
- * each candidate value is timed on the first local-move call on the top
- * level of g from singletons (best of reps runs, after a plan build), and the
- * fastest value is kept on ctx.  report (host, nullable): up to cap rows of
+ * each candidate value is timed on one pLouvain clustering of g
+ * (louvain_part, src/clustering_methods.cpp:139-222; best of reps runs after
+ * an untimed one), and the fastest value is kept on ctx.  report (host, nullable): up to cap rows of
  * (knob, value, microseconds); *nrows (nullable) receives the rows timed.
  * The active-set predictor must be off. */
 int pggc_autotune(pggc_ctx* ctx, const pggc_graph* g, int objective, int reps, int64_t* report, int cap,

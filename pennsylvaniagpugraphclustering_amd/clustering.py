@@ -85,7 +85,7 @@ class Context:
         return out
 
     KNOBS = {"hub_update_div": 0, "gnn_max_sweeps": 1, "table_threads": 2, "table_slots_per_edge": 3,
-             "hub_lds_max_degree": 4, "hub_edges_per_item": 5, "hub_slots_per_item": 6}
+             "hub_lds_max_degree": 4, "hub_edges_per_item": 5, "hub_slots_per_item": 6, "raw_sweeps": 7}
 
     # values the execution-shape knobs accept (results are identical for all)
     KNOB_VALUES = {"table_threads": (64, 128, 256), "table_slots_per_edge": (2, 4),
@@ -104,8 +104,8 @@ class Context:
 
     def autotune(self, g, objective="Mod", reps=2):
         """pggc_autotune: coordinate descent over the execution-shape knobs,
-        timing the first top-level local-move call of g; the fastest values
-        stay set.  Returns (chosen {knob: value}, [(knob, value, us), ...])."""
+        timing one pLouvain clustering of g per value; the fastest values stay
+        set.  Returns (chosen {knob: value}, [(knob, value, us), ...])."""
         names = {v: k for k, v in self.KNOBS.items()}
         cap = 64
         rep = (ctypes.c_int64 * (3 * cap))()
@@ -327,8 +327,11 @@ INFO_KEYS = ["labels", "uncut", "levels", "sweeps", "moves", "outer", "cands", "
 
 def cluster_batch(gs, algo="louvain", objective="Mod", mult=1.0, ex_iters=0, seeds=None, streams=8, ctx=None):
     """pggc_cluster_batch: independent runs on the graphs `gs` (or seeded
-    trials of one graph repeated in `gs`), overlapped on `streams` worker
-    streams.  Returns a list of (objective, part device tensor, info dict)."""
+    trials of one graph repeated in `gs`).  pLouvain with ex_iters 0 (and the
+    predictor off) runs the whole batch as one disjoint union graph, every
+    kernel once over all instances; other runs go to `streams` worker host
+    threads that share the context's stream.  Returns a list of (objective,
+    part device tensor, info dict), each equal to the instance run alone."""
     ctx = _ctx(ctx)
     k = len(gs)
     garr = (Graph * max(k, 1))(*[g.c() for g in gs])

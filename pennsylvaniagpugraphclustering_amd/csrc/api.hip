@@ -12,6 +12,7 @@
 #include <thread>
 #include <vector>
 
+#include "batch.h"
 #include "device.h"
 #include "gen.h"
 #include "gnn.h"
@@ -31,7 +32,7 @@ struct pggc_ctx {
     Ctx c;
     MoveScratch ms;
     Gnn gnn;
-    // worker contexts of pggc_cluster_batch (on this context's stream)
+    // worker contexts of pggc_cluster_batch, each on a stream of its own
     std::vector<pggc_ctx*> workers;
 };
 
@@ -400,7 +401,9 @@ int pggc_ctx_create(pggc_ctx** out, void* stream) {
         std::unique_ptr<pggc_ctx> x(new pggc_ctx());
         x->c.stream = (hipStream_t)stream;
         PGGC_HIP(hipGetDevice(&x->c.device));
-        PGGC_HIP(hipHostMalloc((void**)&x->c.pinned, 64 * sizeof(i64), hipHostMallocDefault));
+        // coherent (fine-grained) host memory: the readback kernel's stores
+        // reach the host without a cache writeback the host would have to wait for
+        PGGC_HIP(hipHostMalloc((void**)&x->c.pinned, 64 * sizeof(i64), hipHostMallocCoherent));
         PGGC_HIP(hipHostGetDevicePointer((void**)&x->c.pinned_dev, x->c.pinned, 0));
         noalloc_init(x->c.device);  // resolves the evaluator scratch of this device (once per device)
         x->ms.gnn = &x->gnn;
@@ -420,6 +423,7 @@ int pggc_ctx_destroy(pggc_ctx* ctx) {
         ctx->c.live.clear();
         ctx->c.trim();
         if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);
+        if (ctx->c.stage) (void)hipHostFree(ctx->c.stage);
         ctx->c.kt_free();
         delete ctx;
     });
@@ -617,6 +621,7 @@ static void set_knob(Ctx& c, int knob, int64_t value) {
         case PGGC_KNOB_HUB_LDS_MAX_DEGREE: c.hub_lds_max = value; break;
         case PGGC_KNOB_HUB_EDGES_PER_ITEM: c.hub_edges_item = value; break;
         case PGGC_KNOB_HUB_SLOTS_PER_ITEM: c.hub_slots_item = value; break;
+        case PGGC_KNOB_RAW_SWEEPS: c.raw_sweeps = value ? 1 : 0; break;
         default: throw Error(PGGC_EINVAL, "unknown knob");
     }
     if (shape) c.knob_epoch++;  // level plans depend on these
@@ -631,6 +636,7 @@ static int64_t get_knob(const Ctx& c, int knob) {
         case PGGC_KNOB_HUB_LDS_MAX_DEGREE: return c.hub_lds_max;
         case PGGC_KNOB_HUB_EDGES_PER_ITEM: return c.hub_edges_item;
         case PGGC_KNOB_HUB_SLOTS_PER_ITEM: return c.hub_slots_item;
+        case PGGC_KNOB_RAW_SWEEPS: return c.raw_sweeps;
         default: throw Error(PGGC_EINVAL, "unknown knob");
     }
 }
@@ -658,9 +664,9 @@ int pggc_ctx_get_knob(pggc_ctx* ctx, int knob, int64_t* value) {
 }
 
 // Coordinate descent over the execution-shape knobs: each knob in turn
-// takes every candidate value, the first local-move call on the top level of
-// g (from singletons, the most expensive call of a run) is timed, and the
-// fastest value is kept.  A plan is built per value before the timed runs.
+// takes every candidate value, one pLouvain clustering of g (louvain_part:
+// every level down and up, the unit of the ex_iters loop) is timed, and the
+// fastest value is kept.  A first untimed run per value builds the plans.
 int pggc_autotune(pggc_ctx* ctx, const pggc_graph* gg, int objective, int reps, int64_t* report, int cap,
                   int* nrows) {
     return guard([&] {
@@ -682,20 +688,19 @@ int pggc_autotune(pggc_ctx* ctx, const pggc_graph* gg, int objective, int reps, 
             double best_t = std::numeric_limits<double>::infinity();
             for (int64_t v : cand) {
                 set_knob(c, knob, v);
-                LevelPlan lp;
-                plan_level(c, g, lp);
+                LevelPlan top;
                 double t_min = std::numeric_limits<double>::infinity();
-                for (int r = 0; r < reps; r++) {
+                for (int r = 0; r <= reps; r++) {
                     RefineData rd;
                     rd.vol = vol;
                     rd.cap = std::max<i64>(n, 1);
                     state_init(c, g, objective, 1.0, rd);
-                    iota_i32(c, part, n);
+                    RunStats rs;
                     c.sync();
                     const auto t0 = std::chrono::steady_clock::now();
-                    local_move(c, ctx->ms, lp, g, part, rd, true, false, nullptr, true, nullptr);
+                    louvain_part(c, ctx->ms, g, rd, false, nullptr, part, rs, &top);
                     c.sync();
-                    t_min = std::min(t_min, since(t0));
+                    if (r > 0) t_min = std::min(t_min, since(t0));
                 }
                 if (report && rows < cap) {
                     report[3 * rows] = knob;
@@ -1045,6 +1050,56 @@ int pggc_cluster_batch(pggc_ctx* ctx, int count, const pggc_graph* graphs, int a
         if (count > 0 && (!graphs || !parts_out || !objectives_out))
             throw Error(PGGC_EINVAL, "graphs, parts_out and objectives_out are required");
         if (ctx->gnn.cfg.mode == GNN_TRAIN) throw Error(PGGC_EINVAL, "predictor training is not available in batches");
+        if (count == 0) return;
+        if (algo == PGGC_LOUVAIN && ex_iters == 0 && ctx->gnn.cfg.mode == GNN_OFF) {
+            // one disjoint union: every kernel runs once over all instances
+            Ctx& c = ctx->c;
+            c.sync();
+            std::vector<DGraph> gs(count);
+            std::vector<RefineData> init(count);
+            std::vector<i32*> outs(count);
+            i64 nmax = 1;
+            for (int k = 0; k < count; k++) {
+                gs[k] = view_of(&graphs[k]);
+                if (!gs[k].vw) throw Error(PGGC_EINVAL, "instance " + std::to_string(k) + ": vertex weights required");
+                if (gs[k].n > 0 && !parts_out[k])
+                    throw Error(PGGC_EINVAL, "instance " + std::to_string(k) + ": null output clustering");
+                if (objective == PGGC_OBJ_MOD || objective == PGGC_OBJ_CPM) {  // unit edge weights
+                    gs[k].w = nullptr;
+                    gs[k].wk = W_UNIT;
+                }
+                outs[k] = parts_out[k];
+                nmax = std::max(nmax, gs[k].n);
+            }
+            {
+                Buf<i64> scratch(c, nmax + 1);
+                for (int k = 0; k < count; k++) {
+                    RefineData rd;
+                    rd.vol = scratch;
+                    rd.cap = std::max<i64>(gs[k].n, 1);
+                    state_init(c, gs[k], objective, mult, rd);
+                    init[k] = rd;
+                }
+            }
+            std::vector<RefineData> fin;
+            std::vector<BatchInfo> bi;
+            louvain_batch(c, ctx->ms, gs, init, outs, fin, bi);
+            for (int k = 0; k < count; k++) {
+                objectives_out[k] = fin[k].objective();
+                if (info_out) {
+                    int64_t* o = info_out + 8 * (int64_t)k;
+                    o[0] = fin[k].label_count;
+                    o[1] = fin[k].uncut;
+                    o[2] = bi[k].levels;
+                    o[3] = bi[k].mv.sweeps;
+                    o[4] = bi[k].mv.moves;
+                    o[5] = 1;
+                    o[6] = bi[k].mv.cands;
+                    o[7] = bi[k].total_nnz;
+                }
+            }
+            return;
+        }
         const int S = std::min(streams, count);
         if (S == 0) return;
         // inputs may still be in flight on the caller's stream; the workers
@@ -1053,7 +1108,7 @@ int pggc_cluster_batch(pggc_ctx* ctx, int count, const pggc_graph* graphs, int a
         // Workers share the caller's stream: kernels run in submission order
         // while the host threads overlap their launches and readbacks.  (With
         // a stream per worker, concurrently executing instances gave
-        // run-to-run different clusterings on gfx950; see the design notes.)
+        // different clusterings from their single runs on gfx950.)
         while ((int)ctx->workers.size() < S) {
             pggc_ctx* w = nullptr;
             const int rc = pggc_ctx_create(&w, ctx->c.stream);

@@ -56,6 +56,9 @@ struct Ctx {
     size_t bytes_live = 0, bytes_cached = 0;
     i64* pinned = nullptr;      // 64 pinned int64 readback slots
     i64* pinned_dev = nullptr;  // the same slots as seen from the device
+    void* stage = nullptr;      // pinned coherent staging area for uploads
+    void* stage_dev = nullptr;
+    size_t stage_cap = 0;
     u64 seed = 0;           // leidenR seed stream
     u64 draws = 0;
     int verbose = 0;        // >0: per-iteration progress lines on stderr
@@ -87,6 +90,9 @@ struct Ctx {
     i64 hub_lds_max = 2048;
     i64 hub_edges_item = 1024;
     i64 hub_slots_item = 4096;
+    // the first sweep of a call from singletons scores the raw adjacency
+    // (every neighbour its own cluster); 0 scores it with the tables
+    int raw_sweeps = 1;
     u64 knob_epoch = 0;  // bumped by every knob change; level plans are rebuilt when it moves
     // the active-set predictor runs on at most this many sweeps of a
     // local-move call (0: on every eligible sweep)
@@ -153,6 +159,10 @@ void fill_i32(Ctx& c, i32* p, i64 n, i32 v);
 void fill_i64(Ctx& c, i64* p, i64 n, i64 v);
 void iota_i32(Ctx& c, i32* p, i64 n);
 void copy_dev(Ctx& c, void* dst, const void* src, size_t bytes);  // on c.stream
+// host -> device copy, done when it returns: the bytes go through a pinned
+// coherent staging area and a copy kernel on the stream, so later kernels
+// read them through the same cache path as data written by kernels
+void copy_h2d(Ctx& c, void* dst, const void* src, size_t bytes);
 // out[0..n] = exclusive scan of in[0..n); out[n] = total.  in may alias out.
 void scan_i64(Ctx& c, const i64* in, i64* out, i64 n);
 void scan_i32_to_i64(Ctx& c, const i32* in, i64* out, i64 n);

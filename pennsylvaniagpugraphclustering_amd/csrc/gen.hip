@@ -499,9 +499,8 @@ void lfr_csr(Ctx& c, const LfrParams& p, DevCSR& out, i32* gt) {
     const i32 K = (i32)dt.thr.size();
     Buf<i64> dcs(c, nc + 1);
     Buf<u64> dthr(c, K);
-    PGGC_HIP(hipMemcpyAsync(dcs.p, cs.data(), (nc + 1) * sizeof(i64), hipMemcpyHostToDevice, c.stream));
-    PGGC_HIP(hipMemcpyAsync(dthr.p, dt.thr.data(), K * sizeof(u64), hipMemcpyHostToDevice, c.stream));
-    c.sync();  // the host vectors are sources of the copies
+    copy_h2d(c, dcs.p, cs.data(), (nc + 1) * sizeof(i64));
+    copy_h2d(c, dthr.p, dt.thr.data(), K * sizeof(u64));
     Buf<i32> kin(c, n), kext(c, n), comm(c, n);
     k_lfr_degrees<<<grid_items(n, 256), 256, 0, c.stream>>>(n, nc, dcs, dthr, K, dt.k0, mu_m, p.seed, kin, kext, comm);
     PGGC_LAUNCH_CHECK();

@@ -654,7 +654,7 @@ void Gnn::init(Ctx& c) {
 
 void Gnn::reset_counters(Ctx& c) {
     fill_i64(c, dcnt, GC_COUNT, 0);
-    PGGC_HIP(hipMemsetAsync(dloss.p, 0, sizeof(double), c.stream));
+    fill_i64(c, (i64*)dloss.p, 1, 0);
     sweeps = fallbacks = train_steps = 0;
 }
 
@@ -689,10 +689,10 @@ void Gnn::set_params(Ctx& c, const float* host) {
     init(c);
     std::vector<u16> wb(GNN_IN * GNN_HID);
     for (int k = 0; k < GNN_IN * GNN_HID; k++) wb[k] = bf16_rne(host[GP_W1 + k]);
-    PGGC_HIP(hipMemcpyAsync(prm.p, host, GNN_P * sizeof(float), hipMemcpyHostToDevice, c.stream));
-    PGGC_HIP(hipMemcpyAsync(w1b.p, wb.data(), wb.size() * sizeof(u16), hipMemcpyHostToDevice, c.stream));
-    PGGC_HIP(hipMemsetAsync(m1.p, 0, GNN_P * sizeof(float), c.stream));
-    PGGC_HIP(hipMemsetAsync(m2.p, 0, GNN_P * sizeof(float), c.stream));
+    copy_h2d(c, prm.p, host, GNN_P * sizeof(float));
+    copy_h2d(c, w1b.p, wb.data(), wb.size() * sizeof(u16));
+    fill_i32(c, (i32*)m1.p, GNN_P, 0);
+    fill_i32(c, (i32*)m2.p, GNN_P, 0);
     c.sync();
     step = 0;
     have_params = true;

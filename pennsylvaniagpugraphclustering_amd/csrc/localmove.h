@@ -1,5 +1,7 @@
 // Local mover (src/local_mover.cpp) on one level graph.
 #pragma once
+#include <vector>
+
 #include "graph.h"
 
 namespace pggc {
@@ -103,6 +105,39 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
                 bool is_initial, bool constrained, i32* cons, bool SA, MoveStats* st);
 void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* best_part, RefineData& best,
                        bool is_initial, bool constrained, i32* cons, MoveStats* st);
+
+// ---------------------------------------------------------------- batches
+// Many independent graphs held as one disjoint union graph: instance k owns
+// the contiguous vertex range [voff[k], voff[k+1]) of every level, and the
+// label range [loff[k], loff[k] + n0[k]) at every level (n0 = its top-level
+// vertex count), so its labels are its single-run labels plus loff[k].
+// Kernels run once over the union; every per-instance decision of the
+// reference's control flow (sweep phases, early stops, best-state tracking,
+// new cluster ids, relabelling) is taken per instance, so each instance
+// ends exactly as it would alone.
+struct BatchLevel {
+    i64 K = 0;
+    std::vector<i64> voff;  // K+1 vertex offsets at this level (host)
+    Buf<i64> dvoff;
+    Buf<i32> inst;  // instance of every vertex
+    Buf<i32> loc;   // vertex id within its instance
+    void build(Ctx& c, const std::vector<i64>& offsets);
+};
+struct BatchState {
+    i64 K = 0;
+    std::vector<i64> loff;  // K+1 label offsets (host)
+    Buf<i64> dloff;
+    Buf<i32> linst;          // instance owning every label
+    Buf<float> dlam;         // float lambda per instance
+    Buf<i64> vol;            // volumes over the whole label space
+    std::vector<RefineData> rd;  // per-instance objective state (rd[k].vol unused)
+};
+
+// local_move (src/local_mover.cpp:1360-1417, SA on, unconstrained) of every
+// instance k with active[k] at once; best_part holds labels (label space).
+void local_move_batch(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const BatchLevel& bl,
+                      i32* best_part, BatchState& best, bool is_initial, const std::vector<char>& active,
+                      std::vector<MoveStats>& stats);
 
 // One scoring + afterburner sweep from a fixed state (parity entry point).
 // Returns the number of accepted moves written to moves_out (order1 order).

@@ -11,6 +11,7 @@
 //  * uncut is updated by the exact integer delta of the moved vertices
 #include <algorithm>
 #include <cfloat>
+#include <cstdio>
 #include <chrono>
 
 #include "device.h"
@@ -169,14 +170,13 @@ static void xcd_maps(Ctx& c, LevelPlan& p, i64 m) {
         xcd_of[h] = best;
         tot[best] += load(h);
     }
-    std::vector<i32> map;
-    build_map(it, xcd_of, m, map, p.igrid);
-    p.imap = Buf<i32>(c, map.size());
-    PGGC_HIP(hipMemcpyAsync(p.imap, map.data(), map.size() * sizeof(i32), hipMemcpyHostToDevice, c.stream));
-    build_map(st, xcd_of, m, map, p.sgrid);
-    p.smap = Buf<i32>(c, map.size());
-    PGGC_HIP(hipMemcpyAsync(p.smap, map.data(), map.size() * sizeof(i32), hipMemcpyHostToDevice, c.stream));
-    c.sync();  // the host vectors die here
+    std::vector<i32> imap, smap;
+    build_map(it, xcd_of, m, imap, p.igrid);
+    p.imap = Buf<i32>(c, imap.size());
+    copy_h2d(c, p.imap, imap.data(), imap.size() * sizeof(i32));
+    build_map(st, xcd_of, m, smap, p.sgrid);
+    p.smap = Buf<i32>(c, smap.size());
+    copy_h2d(c, p.smap, smap.data(), smap.size() * sizeof(i32));
 }
 
 __global__ void k_hub_index(const i32* hubs, i64 m, i32* hidx) {
@@ -290,6 +290,37 @@ __device__ __forceinline__ void decide(i32 i, float p_conn, u64 bk, i32* dest, f
     }
 }
 
+// lambda of a vertex's objective and the sweep's filter ratio: one value
+// each for a single graph, per-vertex tables for a batch of graphs held as
+// one disjoint union (each vertex carries its instance's values), whose
+// instances can be at different sweeps of their ratio schedules.  The batch
+// tables are filled per sweep by k_vertex_lam rather than looked up as
+// lam[inst[v]] inside the scoring kernels: that two-step lookup there gave
+// wrong values in about 1% of the lanes on gfx950, run to run different,
+// while the same lookup in its own kernel and the per-vertex tables were
+// exact.
+struct LamOne {
+    float v;
+    __device__ __forceinline__ float operator()(i64) const { return v; }
+    __device__ __forceinline__ float rt(i64, float r) const { return r; }
+};
+struct LamVert {
+    const float* lam;    // lambda of vertex i's instance
+    const float* ratio;  // filter ratio of vertex i's instance this sweep
+    __device__ __forceinline__ float operator()(i64 i) const { return lam[i]; }
+    __device__ __forceinline__ float rt(i64 i, float) const { return ratio[i]; }
+};
+// afterburner priority hash of a vertex (src/local_mover.cpp:319): xorshift32
+// of its id, or of its id within its instance for a batch held as one union
+// (loc[v] = v - first vertex of v's instance)
+struct PrioOne {
+    __device__ __forceinline__ i32 operator()(i32 v) const { return (i32)xorshift32((u32)v); }
+};
+struct PrioInst {
+    const i32* loc;
+    __device__ __forceinline__ i32 operator()(i32 v) const { return (i32)xorshift32((u32)loc[v]); }
+};
+
 // p_conn and the filter threshold (src/local_mover.cpp:481-487)
 __device__ __forceinline__ void own_terms(i64 pv, i64 volp, i64 wv, float lam, float ratio, float& multi,
                                           float& p_conn, float& b) {
@@ -301,11 +332,11 @@ __device__ __forceinline__ void own_terms(i64 pv, i64 volp, i64 wv, float lam, f
 
 // G lanes per vertex, one edge per lane (degree <= G): duplicate clusters
 // are merged with in-group shuffles.
-template <int G, class WT>
+template <int G, class WT, class LT>
 __global__ void __launch_bounds__(256) k_score_group(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
                                                      const i64* __restrict__ vw, const i32* __restrict__ list, i64 m,
                                                      const i32* __restrict__ part, const i64* __restrict__ vol,
-                                                     const i32* __restrict__ cons, int raw, float lam, float ratio,
+                                                     const i32* __restrict__ cons, int raw, LT lam, float ratio,
                                                      i32* __restrict__ dest, float* __restrict__ gain) {
     constexpr int GPB = 256 / G;
     const int gl = threadIdx.x & (G - 1);
@@ -343,7 +374,7 @@ __global__ void __launch_bounds__(256) k_score_group(const i64* __restrict__ rp,
             pv = group_max_i64<G>(own ? agg : (i64)0);
         }
         float multi, p_conn, b;
-        own_terms(pv, vol[p], wv, lam, ratio, multi, p_conn, b);
+        own_terms(pv, vol[p], wv, lam(i), lam.rt(i, ratio), multi, p_conn, b);
         u64 bk = 0;
         if (has && !own && agg > 0 && (float)agg >= b && (cons == nullptr || cons[key] == cons[p])) {
             float jc = fmaf(-(float)vol[key], multi, (float)agg);
@@ -357,11 +388,11 @@ __global__ void __launch_bounds__(256) k_score_group(const i64* __restrict__ rp,
 // One block per vertex with a hash table in LDS (or HBM for the largest
 // degrees): SLOTS slots of LDS hold the table of any degree up to
 // SLOTS / SPE, each table the power of two >= SPE * degree.
-template <int BS, int SLOTS, int SPE, bool GLOBAL, class WT>
+template <int BS, int SLOTS, int SPE, bool GLOBAL, class WT, class LT>
 __global__ void __launch_bounds__(BS) k_score_table(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
                                                     const i64* __restrict__ vw, const i32* __restrict__ list, i64 m,
                                                     const i32* __restrict__ part, const i64* __restrict__ vol,
-                                                    const i32* __restrict__ cons, int raw, float lam, float ratio,
+                                                    const i32* __restrict__ cons, int raw, LT lam, float ratio,
                                                     i32* __restrict__ dest, float* __restrict__ gain, i32* gkeys,
                                                     i64* gvals, const i64* __restrict__ goff) {
     __shared__ i32 skeys[GLOBAL ? 1 : SLOTS];
@@ -426,7 +457,7 @@ __global__ void __launch_bounds__(BS) k_score_table(const i64* __restrict__ rp, 
         if (GLOBAL) __threadfence();
         pv = block_sum_i64<BS>(pv);
         float multi, p_conn, b;
-        own_terms(pv, vol[p], vw[i], lam, ratio, multi, p_conn, b);
+        own_terms(pv, vol[p], vw[i], lam(i), lam.rt(i, ratio), multi, p_conn, b);
         u64 bk = 0;
         for (i64 qb = 0; qb < S; qb += BS) {  // block-uniform loop control
             const i64 q = qb + t;
@@ -474,11 +505,11 @@ __device__ __forceinline__ u64 raw_cand(const i32* __restrict__ col, WT W, const
 }
 
 // one block per vertex (64 < degree <= 2048)
-template <int BS, class WT>
+template <int BS, class WT, class LT>
 __global__ void __launch_bounds__(BS) k_score_raw(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
                                                   const i64* __restrict__ vw, const i32* __restrict__ list, i64 m,
                                                   const i32* __restrict__ part, const i64* __restrict__ vol,
-                                                  const i32* __restrict__ cons, float lam, float ratio,
+                                                  const i32* __restrict__ cons, LT lam, float ratio,
                                                   i32* __restrict__ dest, float* __restrict__ gain) {
     const i64 li = block_id();
     if (li >= m) return;
@@ -488,7 +519,7 @@ __global__ void __launch_bounds__(BS) k_score_raw(const i64* __restrict__ rp, co
     const i32 p = part[i];
     const i32 cp = cons ? cons[p] : 0;
     float multi, p_conn, b;
-    own_terms(0, vol[p], vw[i], lam, ratio, multi, p_conn, b);
+    own_terms(0, vol[p], vw[i], lam(i), lam.rt(i, ratio), multi, p_conn, b);
     u64 bk = 0;
     for (i64 jb = s; jb < e; jb += BS) {  // block-uniform loop control
         const i64 j = jb + threadIdx.x;
@@ -502,11 +533,11 @@ __global__ void __launch_bounds__(BS) k_score_raw(const i64* __restrict__ rp, co
 }
 
 // hubs: one block per edge item, folded into hub_best with atomicMax
-template <class WT>
+template <class WT, class LT>
 __global__ void __launch_bounds__(256) k_hub_raw(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
                                                  const i64* __restrict__ vw, const i32* __restrict__ hubs, i64 m,
                                                  const i64* __restrict__ items, const i32* __restrict__ part,
-                                                 const i64* __restrict__ vol, const i32* __restrict__ cons, float lam,
+                                                 const i64* __restrict__ vol, const i32* __restrict__ cons, LT lam,
                                                  float ratio, const i32* __restrict__ dest, i64 he, u64* hub_best) {
     const i64 b = block_id();
     if (b >= items[m]) return;
@@ -519,7 +550,7 @@ __global__ void __launch_bounds__(256) k_hub_raw(const i64* __restrict__ rp, con
     const i32 p = part[i];
     const i32 cp = cons ? cons[p] : 0;
     float multi, p_conn, bth;
-    own_terms(0, vol[p], vw[i], lam, ratio, multi, p_conn, bth);
+    own_terms(0, vol[p], vw[i], lam(i), lam.rt(i, ratio), multi, p_conn, bth);
     u64 bk = 0;
     for (i64 jb = j_lo; jb < j_hi; jb += 256) {  // block-uniform loop control
         const i64 j = jb + threadIdx.x;
@@ -660,12 +691,12 @@ __global__ void __launch_bounds__(1024) k_hub_update_block(const i64* __restrict
         hub_update_edge<PACKED>(col, W, hidx, goff, gkeys, gvals, gtab, j, oi, ni, stale);
 }
 
-template <bool PACKED>
+template <bool PACKED, class LT>
 __global__ void __launch_bounds__(256) k_hub_scan(const i32* __restrict__ hubs, i64 m, const i64* __restrict__ sitems,
                                                   const i32* __restrict__ smap, i64 nmap,
                                                   const i64* __restrict__ goff, const i32* __restrict__ part,
                                                   const i32* __restrict__ dest, const i64* __restrict__ vw,
-                                                  const i64* __restrict__ vol, const i32* __restrict__ cons, float lam,
+                                                  const i64* __restrict__ vol, const i32* __restrict__ cons, LT lam,
                                                   float ratio, const i32* gkeys, const i64* gvals, const u64* gtab,
                                                   const i64* __restrict__ hub_pv, i64 hs, u64* hub_best) {
     const i64 bb = block_id();
@@ -680,7 +711,7 @@ __global__ void __launch_bounds__(256) k_hub_scan(const i32* __restrict__ hubs, 
     const i64 s_hi = s_lo + hs < s_end ? s_lo + hs : s_end;
     const i32 p = part[i];
     float multi, p_conn, bth;
-    own_terms(hub_pv[hi], vol[p], vw[i], lam, ratio, multi, p_conn, bth);
+    own_terms(hub_pv[hi], vol[p], vw[i], lam(i), lam.rt(i, ratio), multi, p_conn, bth);
     u64 bk = 0;
     for (i64 qb = s_lo; qb < s_hi; qb += 256) {  // block-uniform loop control
         const i64 q = qb + threadIdx.x;
@@ -709,14 +740,15 @@ __global__ void __launch_bounds__(256) k_hub_scan(const i32* __restrict__ hubs, 
     if (threadIdx.x == 0 && bk) atomicMax((unsigned long long*)&hub_best[hi], (unsigned long long)bk);
 }
 
-__global__ void k_hub_decide(const i32* hubs, i64 m, const i32* part, const i64* vw, const i64* vol, float lam,
+template <class LT>
+__global__ void k_hub_decide(const i32* hubs, i64 m, const i32* part, const i64* vw, const i64* vol, LT lam,
                              float ratio, const i64* hub_pv, const u64* hub_best, i32* dest, float* gain) {
     const i64 hi = thread_item();
     if (hi >= m) return;
     const i32 i = hubs[hi];
     if (dest[i] == NO_MOVE) return;
     float multi, p_conn, bth;
-    own_terms(hub_pv[hi], vol[part[i]], vw[i], lam, ratio, multi, p_conn, bth);
+    own_terms(hub_pv[hi], vol[part[i]], vw[i], lam(i), lam.rt(i, ratio), multi, p_conn, bth);
     decide(i, p_conn, hub_best[hi], dest, gain);
 }
 
@@ -784,9 +816,9 @@ __global__ void __launch_bounds__(256) k_list_bytes(const i64* __restrict__ rp, 
 
 // LDS-table scoring of the list entries [x0, x1) (one block per vertex):
 // BS threads, SLOTS LDS slots, SPE slots per edge of the degree bound.
-template <int BS, int SLOTS, int SPE, class WT>
+template <int BS, int SLOTS, int SPE, class WT, class LT>
 static void launch_table(Ctx& c, const DGraph& g, WT W, const i32* L, i64 x0, i64 x1, const i32* part,
-                         const i64* vol, const i32* cons, float lam, float ratio, i32* dest, float* gain) {
+                         const i64* vol, const i32* cons, LT lam, float ratio, i32* dest, float* gain) {
     const i64 m = x1 - x0;
     if (m <= 0) return;
     k_score_table<BS, SLOTS, SPE, false><<<grid_blocks(m), BS, 0, c.stream>>>(
@@ -794,9 +826,9 @@ static void launch_table(Ctx& c, const DGraph& g, WT W, const i32* L, i64 x0, i6
 }
 // class with degree bound D (128, 256, 512): threads per vertex and slots per
 // edge from the knobs
-template <int D, class WT>
+template <int D, class WT, class LT>
 static void launch_table_class(Ctx& c, const DGraph& g, WT W, const i32* L, i64 x0, i64 x1, const i32* part,
-                               const i64* vol, const i32* cons, float lam, float ratio, i32* dest, float* gain) {
+                               const i64* vol, const i32* cons, LT lam, float ratio, i32* dest, float* gain) {
     const int bs = c.table_threads, spe = c.table_spe;
     if (bs == 64 && spe == 2) launch_table<64, 2 * D, 2>(c, g, W, L, x0, x1, part, vol, cons, lam, ratio, dest, gain);
     else if (bs == 64) launch_table<64, 4 * D, 4>(c, g, W, L, x0, x1, part, vol, cons, lam, ratio, dest, gain);
@@ -838,8 +870,9 @@ static void hub_insert_all(Ctx& c, const DGraph& g, WT W, LevelPlan& lp, const i
     }
 }
 
+template <class LT>
 static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32* part, const i64* vol,
-                      const i32* cons, bool raw, float lam, float ratio, i32* dest, float* gain) {
+                      const i32* cons, bool raw, LT lam, float ratio, i32* dest, float* gain) {
     const i64* st = lp.sc_start;
     const i32* L = lp.sc_list.p;
     Buf<i64> acct;
@@ -987,13 +1020,13 @@ static i64 gather_candidates(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph
 // afterburner_filter (src/local_mover.cpp:374-433): the change in the
 // candidate's gain if every neighbour with a larger gain (ties broken by the
 // xorshift hash) also makes its move
-template <class WT>
+template <class WT, class PT>
 __device__ __forceinline__ void ab_term(const i32* col, WT W, const AbInfo* info, i64 j, i32 p, i32 best,
-                                        float multi, float igain, i32 hi, float& acc) {
+                                        float multi, float igain, i32 hi, PT prio, float& acc) {
     const i32 v = col[j];
     const AbInfo q4 = info[v];
     const float vg = q4.gain;
-    if ((vg - igain) >= AFTERBURNER_EPS || (fabsf(vg - igain) < AFTERBURNER_EPS && (i32)xorshift32((u32)v) < hi)) {
+    if ((vg - igain) >= AFTERBURNER_EPS || (fabsf(vg - igain) < AFTERBURNER_EPS && prio(v) < hi)) {
         i32 vpart = q4.dest;
         const float q = fmaf(-multi, q4.vwf, (float)W(j));
         acc = acc - ((vpart == p) ? q : 0.0f);
@@ -1004,31 +1037,31 @@ __device__ __forceinline__ void ab_term(const i32* col, WT W, const AbInfo* info
     }
 }
 
-template <class WT>
+template <class WT, class LT, class PT>
 __global__ void __launch_bounds__(256) k_ab_serial(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
                                                    const AbInfo* __restrict__ info, const i32* __restrict__ cand,
-                                                   i64 k, float lam, i32* keep, i32* mflag) {
+                                                   i64 k, LT lam, PT prio, i32* keep, i32* mflag) {
     const i64 x = thread_item();
     if (x >= k) return;
     const i32 i = cand[x];
     const AbInfo me = info[i];
     const i32 best = me.dest, p = me.part;
-    const float multi = me.vwf * lam;
+    const float multi = me.vwf * lam(i);
     const float igain = me.gain;
-    const i32 hi = (i32)xorshift32((u32)i);
+    const i32 hi = prio(i);
     float acc = 0.0f;
     const i64 e = rp[i + 1];
 #pragma unroll 4  // independent neighbour gathers in flight; the adds keep their order
-    for (i64 j = rp[i]; j < e; j++) ab_term(col, W, info, j, p, best, multi, igain, hi, acc);
+    for (i64 j = rp[i]; j < e; j++) ab_term(col, W, info, j, p, best, multi, igain, hi, prio, acc);
     const int kp = (igain + acc >= 0.0f) ? 1 : 0;
     keep[x] = kp;
     mflag[i] = kp;  // candidates start with mflag 0: an unconditional store
 }
 
-template <class WT>
+template <class WT, class LT, class PT>
 __global__ void __launch_bounds__(256) k_ab_wave(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
                                                  const AbInfo* __restrict__ info, const i32* __restrict__ cand, i64 k0,
-                                                 i64 k1, float lam, i32* keep, i32* mflag) {
+                                                 i64 k1, LT lam, PT prio, i32* keep, i32* mflag) {
     const int lane = threadIdx.x & 63;
     const i64 x = k0 + block_id() * 4 + (threadIdx.x >> 6);
     if (x >= k1) return;
@@ -1036,15 +1069,15 @@ __global__ void __launch_bounds__(256) k_ab_wave(const i64* __restrict__ rp, con
         const i32 i = cand[x];
         const AbInfo me = info[i];
         const i32 best = me.dest, p = me.part;
-        const float multi = me.vwf * lam;
+        const float multi = me.vwf * lam(i);
         const float igain = me.gain;
-        const i32 hi = (i32)xorshift32((u32)i);
+        const i32 hi = prio(i);
         float acc = 0.0f;
         const i64 e = rp[i + 1];
 #pragma unroll 4
         for (i64 base = rp[i]; base < e; base += 64) {  // wave-uniform loop control
             const i64 j = base + lane;
-            if (j < e) ab_term(col, W, info, j, p, best, multi, igain, hi, acc);
+            if (j < e) ab_term(col, W, info, j, p, best, multi, igain, hi, prio, acc);
         }
         xlane_fence(acc);
 #pragma unroll
@@ -1057,10 +1090,10 @@ __global__ void __launch_bounds__(256) k_ab_wave(const i64* __restrict__ rp, con
     }
 }
 
-template <class WT>
+template <class WT, class LT, class PT>
 __global__ void __launch_bounds__(1024) k_ab_block(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
                                                    const AbInfo* __restrict__ info, const i32* __restrict__ cand,
-                                                   i64 k0, i64 k1, float lam, i32* keep, i32* mflag) {
+                                                   i64 k0, i64 k1, LT lam, PT prio, i32* keep, i32* mflag) {
     __shared__ float sh[1024];
     const int t = threadIdx.x;
     const i64 x = k0 + block_id();
@@ -1069,14 +1102,14 @@ __global__ void __launch_bounds__(1024) k_ab_block(const i64* __restrict__ rp, c
         const i32 i = cand[x];
         const AbInfo me = info[i];
         const i32 best = me.dest, p = me.part;
-        const float multi = me.vwf * lam;
+        const float multi = me.vwf * lam(i);
         const float igain = me.gain;
-        const i32 hi = (i32)xorshift32((u32)i);
+        const i32 hi = prio(i);
         float acc = 0.0f;
         const i64 e = rp[i + 1];
         for (i64 base = rp[i]; base < e; base += 1024) {  // block-uniform loop control
             const i64 j = base + t;
-            if (j < e) ab_term(col, W, info, j, p, best, multi, igain, hi, acc);
+            if (j < e) ab_term(col, W, info, j, p, best, multi, igain, hi, prio, acc);
         }
         sh[t] = acc;
         __syncthreads();
@@ -1112,21 +1145,22 @@ __global__ void __launch_bounds__(256) k_ab_pack(i64 n, const float* __restrict_
     info[i] = a;
 }
 
-static i64 afterburner(Ctx& c, MoveScratch& s, const DGraph& g, const i32* cand, i64 k, const i64* b, float lam,
+template <class LT, class PT>
+static i64 afterburner(Ctx& c, MoveScratch& s, const DGraph& g, const i32* cand, i64 k, const i64* b, LT lam, PT prio,
                        i32* moves) {
     const int tk = c.kt_begin(KS_AFTERBURNER);
     k_ab_pack<<<grid_items(g.n, 256), 256, 0, c.stream>>>(g.n, s.gain, s.dest, s.part, g.vw, s.abi);
     PGGC_LAUNCH_CHECK();
     PGGC_WDISPATCH(g, {
         if (b[0] > 0)
-            k_ab_serial<<<grid_items(b[0], 256), 256, 0, c.stream>>>(g.rp, g.col, W, s.abi, cand, b[0], lam, s.flag,
-                                                                     s.mflag);
+            k_ab_serial<<<grid_items(b[0], 256), 256, 0, c.stream>>>(g.rp, g.col, W, s.abi, cand, b[0], lam, prio,
+                                                                     s.flag, s.mflag);
         if (b[1] > b[0])
             k_ab_wave<<<grid_items(b[1] - b[0], 4), 256, 0, c.stream>>>(g.rp, g.col, W, s.abi, cand, b[0], b[1], lam,
-                                                                        s.flag, s.mflag);
+                                                                        prio, s.flag, s.mflag);
         if (k > b[1])
             k_ab_block<<<grid_blocks(k - b[1]), 1024, 0, c.stream>>>(g.rp, g.col, W, s.abi, cand, b[1], k, lam,
-                                                                     s.flag, s.mflag);
+                                                                     prio, s.flag, s.mflag);
     });
     c.kt_end(tk, 0.0);
     PGGC_LAUNCH_CHECK();
@@ -1357,8 +1391,11 @@ __global__ void k_set_mflag(const i32* moves, i64 k, i32* mflag) {
 // MASSIVE_CUTOFF (a wave each), then the rest (a 1024-thread block each).
 // n_large / n_massive = vertices of the level with degree >= LARGE_CUTOFF /
 // MASSIVE_CUTOFF, which bound the movers of the last two classes.
-static void perform_moves(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, RefineData& curr,
-                          const i32* moves, i64 k, i64 n_large, i64 n_massive, bool& tables) {
+// The moves themselves: hub-table updates (before part changes), the exact
+// uncut delta of every mover into du[x], volume moves and part updates.  A
+// hub table that runs out of slots sets s.scal[3].
+static void apply_moves_core(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i64* vol, const i32* moves,
+                             i64 k, i64 n_large, i64 n_massive, i64* du) {
     const i64 nw = std::min(k, n_large);
     const i64 nb = std::min(k, n_massive);
     const int tk = c.kt_begin(KS_MOVES);
@@ -1397,7 +1434,6 @@ static void perform_moves(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g
         PGGC_LAUNCH_CHECK();
     }
     {
-        Buf<i64> du(c, std::max<i64>(k, 1));
         k_mv_pack<<<grid_items(g.n, 256), 256, 0, c.stream>>>(g.n, s.part, s.dest, s.mflag, s.mvp);
         PGGC_LAUNCH_CHECK();
         PGGC_WDISPATCH(g, {
@@ -1411,9 +1447,8 @@ static void perform_moves(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g
                                                                            s.dest, s.mvp, du);
         });
         PGGC_LAUNCH_CHECK();
-        sum_i64(c, du, k, s.scal);
     }
-    k_apply_moves<<<grid_items(k, 256), 256, 0, c.stream>>>(moves, k, g.vw, s.part, s.dest, curr.vol);
+    k_apply_moves<<<grid_items(k, 256), 256, 0, c.stream>>>(moves, k, g.vw, s.part, s.dest, vol);
     c.kt_end(tk, 0.0);
     PGGC_LAUNCH_CHECK();
     if (tk >= 0) {  // timing runs only
@@ -1429,6 +1464,17 @@ static void perform_moves(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g
             g.rp, moves, k, 72, 12 + wb + (hub_upd ? 8 + wb : 0), 20 * g.n, acc);
         PGGC_LAUNCH_CHECK();
         acct_add(c, c.kt_counter(tk), acc);
+    }
+}
+
+static void perform_moves(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, RefineData& curr,
+                          const i32* moves, i64 k, i64 n_large, i64 n_massive, bool& tables) {
+    const i64 nw = std::min(k, n_large);
+    const i64 nb = std::min(k, n_massive);
+    {
+        Buf<i64> du(c, std::max<i64>(k, 1));
+        apply_moves_core(c, s, lp, g, curr.vol, moves, k, n_large, n_massive, du);
+        sum_i64(c, du, k, s.scal);
     }
     if (!tables) {
         tables = true;
@@ -1577,7 +1623,7 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
         const float fr = SA ? ratios[x] : 0.0f;
         for (int cnt = 0; cnt < limits[x]; cnt++) {
             iter++;
-            const float lam = (float)curr.lambda;
+            const LamOne lam{(float)curr.lambda};
             const bool use = G && !(is_initial && iter == 1) && kprev * 16 >= n &&
                              (c.gnn_max_sweeps <= 0 || npred < c.gnn_max_sweeps);
             const bool full = use && G->cfg.full_every > 0 && iter % G->cfg.full_every == 0;
@@ -1588,16 +1634,16 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
                 npred++;
             }
             tr.begin(s.dest, iter == 1 && !use);
-            score_all(c, s, lp, g, s.part, curr.vol, ccons, !tables, lam, fr, s.dest, s.gain);
+            score_all(c, s, lp, g, s.part, curr.vol, ccons, !tables && c.raw_sweeps, lam, fr, s.dest, s.gain);
             i64 b[2];
             i64 k = gather_candidates(c, s, lp, g, s.dest, s.cand, b);
-            i64 km = k > 0 ? afterburner(c, s, g, s.cand, k, b, lam, s.moves) : 0;
+            i64 km = k > 0 ? afterburner(c, s, g, s.cand, k, b, lam, PrioOne{}, s.moves) : 0;
             if (km == 0 && G && gnn_restore(c, *G, n, s.dest)) {
                 // nothing to move among the vertices let through: score the
                 // pruned ones too, so the call ends as the plain method does
-                score_all(c, s, lp, g, s.part, curr.vol, ccons, !tables, lam, fr, s.dest, s.gain);
+                score_all(c, s, lp, g, s.part, curr.vol, ccons, !tables && c.raw_sweeps, lam, fr, s.dest, s.gain);
                 k = gather_candidates(c, s, lp, g, s.dest, s.cand, b);
-                km = k > 0 ? afterburner(c, s, g, s.cand, k, b, lam, s.moves) : 0;
+                km = k > 0 ? afterburner(c, s, g, s.cand, k, b, lam, PrioOne{}, s.moves) : 0;
             }
             if (stats) {
                 stats->sweeps++;
@@ -1617,6 +1663,10 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
             if (iter > 1 || !is_initial) set_new_cluster_ids(c, s, g, curr, s.moves, km, constrained ? cons : nullptr);
             perform_moves(c, s, lp, g, curr, s.moves, km, g.n - lp.ord.off_large, g.n - lp.ord.off_large2, tables);
             if (stats) stats->moves += km;
+            if (c.verbose >= 3)
+                fprintf(stderr, "sweep n=%lld it=%d cands=%lld moves=%lld labels=%d uncut=%lld obj=%.17g\n",
+                        (long long)n, iter, (long long)k, (long long)km, curr.label_count, (long long)curr.uncut,
+                        curr.obj);
             if (curr.obj > best.obj) {
                 copy_state(c, best, curr);
                 copy_dev(c, best_part, s.part, n * sizeof(i32));
@@ -1707,7 +1757,7 @@ void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i
     for (int it = 0; it < 6; it++) {
         const float lam = (float)curr.lambda;
         trc.begin(s.dest, true);
-        score_all(c, s, lp, g, s.part, curr.vol, ccons, !tables, lam, 0.0f, s.dest, s.gain);
+        score_all(c, s, lp, g, s.part, curr.vol, ccons, !tables && c.raw_sweeps, LamOne{lam}, 0.0f, s.dest, s.gain);
         i64 b[2];
         i64 k = gather_candidates(c, s, lp, g, s.dest, s.moves, b);
         if (stats) {
@@ -1758,13 +1808,13 @@ i64 sweep_once(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32
     copy_dev(c, s.dest, dest, n * sizeof(i32));
     copy_dev(c, s.gain, gain, n * sizeof(float));
     const float lam = (float)lambda;
-    score_all(c, s, lp, g, s.part, vol, cons, raw, lam, ratio, s.dest, s.gain);
+    score_all(c, s, lp, g, s.part, vol, cons, raw, LamOne{lam}, ratio, s.dest, s.gain);
     i64 b[2];
     i64 k = gather_candidates(c, s, lp, g, s.dest, s.cand, b);
     *ncand = k;
     i64 km = 0;
     if (k > 0) {
-        km = afterburner(c, s, g, s.cand, k, b, lam, moves_out);
+        km = afterburner(c, s, g, s.cand, k, b, LamOne{lam}, PrioOne{}, moves_out);
         k_clear_mflag<<<grid_items(km, 256), 256, 0, c.stream>>>(moves_out, km, s.mflag);
         PGGC_LAUNCH_CHECK();
     }
@@ -1772,6 +1822,423 @@ i64 sweep_once(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32
     copy_dev(c, gain, s.gain, n * sizeof(float));
     c.sync();
     return km;
+}
+
+
+// ================================================================ batches
+// A batch of graphs as one disjoint union (see BatchLevel): the sweep
+// kernels run once over the union with per-instance lambda, ratio and
+// priority ids; the control flow of local_move runs per instance on the host
+// from per-instance counts read back once per sweep.
+__global__ void k_batch_ids(i64 n, const i64* __restrict__ voff, i64 K, i32* __restrict__ inst,
+                            i32* __restrict__ loc) {
+    const i64 v = thread_item();
+    if (v >= n) return;
+    const i64 k = upper_row(voff, 0, K, v);  // last instance starting at or before v (skips empty ones)
+    inst[v] = (i32)k;
+    loc[v] = (i32)(v - voff[k]);
+}
+
+void BatchLevel::build(Ctx& c, const std::vector<i64>& offsets) {
+    voff = offsets;
+    K = (i64)voff.size() - 1;
+    const i64 n = voff[K];
+    dvoff = Buf<i64>(c, K + 1);
+    copy_h2d(c, dvoff.p, voff.data(), (K + 1) * sizeof(i64));
+    inst = Buf<i32>(c, std::max<i64>(n, 1));
+    loc = Buf<i32>(c, std::max<i64>(n, 1));
+    if (n > 0) {
+        k_batch_ids<<<grid_items(n, 256), 256, 0, c.stream>>>(n, dvoff, K, inst, loc);
+        PGGC_LAUNCH_CHECK();
+    }
+    c.sync();
+}
+
+template <class T>
+static void upload(Ctx& c, T* dst, const std::vector<T>& src) {
+    copy_h2d(c, dst, src.data(), src.size() * sizeof(T));
+}
+template <class T>
+static void download(Ctx& c, std::vector<T>& dst, const T* src, size_t count) {
+    dst.resize(count);
+    if (count == 0) return;
+    PGGC_HIP(hipMemcpyAsync(dst.data(), src, count * sizeof(T), hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+}
+
+__global__ void k_inst_count(const i32* __restrict__ list, i64 k, const i32* __restrict__ owner, i64* cnt) {
+    const i64 x = thread_item();
+    if (x >= k) return;
+    atomicAdd((unsigned long long*)&cnt[owner[list[x]]], 1ull);
+}
+// vertices of instances outside this sweep are never scored
+__global__ void k_inst_gate(i64 n, const i32* __restrict__ inst, const i32* __restrict__ on, i32* dest) {
+    const i64 v = thread_item();
+    if (v >= n) return;
+    if (!on[inst[v]]) dest[v] = NO_MOVE;
+}
+__global__ void k_inst_fill(i64 n, const i32* __restrict__ inst, const i32* __restrict__ mask, i32* a, i32 val) {
+    const i64 v = thread_item();
+    if (v >= n) return;
+    if (mask[inst[v]]) a[v] = val;
+}
+__global__ void k_inst_add(const i32* __restrict__ moves, i64 k, const i32* __restrict__ inst,
+                           const i64* __restrict__ du, i64* acc) {
+    const i64 x = thread_item();
+    if (x >= k) return;
+    atomicAdd((unsigned long long*)&acc[inst[moves[x]]], (unsigned long long)du[x]);
+}
+// exact 128-bit sum of squared volumes over each masked instance's labels
+// [loff[k], loff[k] + L[k]); one block per instance, out[2k] = high word
+__global__ void __launch_bounds__(256) k_inst_sumsq(const i64* __restrict__ vol, const i64* __restrict__ loff,
+                                                    const i64* __restrict__ L, const i32* __restrict__ mask, i64 K,
+                                                    u64* out) {
+    const i64 k = block_id();
+    if (k >= K) return;
+    u64 lo = 0, hi = 0;
+    if (mask[k]) {
+        const i64 e = loff[k] + L[k];
+        for (i64 l = loff[k] + threadIdx.x; l < e; l += 256) {
+            const i64 v = vol[l];
+            const u64 a = (u64)(v < 0 ? -v : v);
+            const u64 plo = a * a, phi = __umul64hi(a, a);
+            lo += plo;
+            hi += phi + (lo < plo ? 1 : 0);
+        }
+    }
+    __shared__ u64 slo[256], shi[256];
+    slo[threadIdx.x] = lo;
+    shi[threadIdx.x] = hi;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            const u64 a = slo[threadIdx.x], b = slo[threadIdx.x + o];
+            slo[threadIdx.x] = a + b;
+            shi[threadIdx.x] = shi[threadIdx.x] + shi[threadIdx.x + o] + (a + b < a ? 1 : 0);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * k] = shi[0];
+        out[2 * k + 1] = slo[0];
+    }
+}
+__global__ void k_inst_copy_v(i64 n, const i32* __restrict__ inst, const i32* __restrict__ mask,
+                              const i32* __restrict__ src, i32* dst) {
+    const i64 v = thread_item();
+    if (v >= n) return;
+    if (mask[inst[v]]) dst[v] = src[v];
+}
+// labels l < loff[k] + L[k] of masked instances
+__global__ void k_inst_copy_l(i64 nl, const i32* __restrict__ linst, const i64* __restrict__ loff,
+                              const i64* __restrict__ L, const i32* __restrict__ mask, const i64* __restrict__ src,
+                              i64* dst) {
+    const i64 l = thread_item();
+    if (l >= nl) return;
+    const i32 k = linst[l];
+    if (mask[k] && l - loff[k] < L[k]) dst[l] = src[l];
+}
+__global__ void k_unused_inst(i64 nl, const i32* __restrict__ used, const i32* __restrict__ linst,
+                              const i64* __restrict__ loff, const i64* __restrict__ L, i32* flag) {
+    const i64 l = thread_item();
+    if (l >= nl) return;
+    const i32 k = linst[l];
+    flag[l] = (used[l] == 0 && l - loff[k] < L[k]) ? 1 : 0;
+}
+__global__ void k_inst_keys(const i32* __restrict__ list, i64 k, const i32* __restrict__ inst, u64* keys) {
+    const i64 x = thread_item();
+    if (x >= k) return;
+    keys[x] = (u64)(u32)inst[list[x]];
+}
+// x-th NEW_PART mover of instance q (movers grouped by instance, in move
+// order) takes the instance's x-th unused label, then labels appended after
+// its label count (src/local_mover.cpp:215-274)
+__global__ void k_assign_new_batch(const i32* __restrict__ np, i64 K_, const i32* __restrict__ inst,
+                                   const i64* __restrict__ npstart, const i64* __restrict__ ustart,
+                                   const i64* __restrict__ acnt, const i32* __restrict__ unused,
+                                   const i64* __restrict__ loff, const i64* __restrict__ L, i32* dest) {
+    const i64 x = thread_item();
+    if (x >= K_) return;
+    const i32 v = np[x];
+    const i32 q = inst[v];
+    const i64 r = x - npstart[q];
+    dest[v] = r < acnt[q] ? unused[ustart[q] + r] : (i32)(loff[q] + L[q] + (r - acnt[q]));
+}
+__global__ void k_mark_used_inst(const i32* __restrict__ part, i64 n, const i32* __restrict__ inst,
+                                 const i32* __restrict__ mask, i32* used) {
+    const i64 v = thread_item();
+    if (v >= n) return;
+    if (mask[inst[v]]) used[part[v]] = 1;
+}
+__global__ void k_relabel_vol_inst(i64 nl, const i32* __restrict__ used, const i64* __restrict__ pos,
+                                   const i32* __restrict__ linst, const i64* __restrict__ loff,
+                                   const i64* __restrict__ vol, i64* nvol) {
+    const i64 l = thread_item();
+    if (l >= nl) return;
+    if (!used[l]) return;
+    const i32 k = linst[l];
+    nvol[loff[k] + pos[l] - pos[loff[k]]] = vol[l];
+}
+__global__ void k_relabel_part_inst(i64 n, const i32* __restrict__ inst, const i32* __restrict__ mask,
+                                    const i64* __restrict__ pos, const i64* __restrict__ loff, i32* part) {
+    const i64 v = thread_item();
+    if (v >= n) return;
+    const i32 k = inst[v];
+    if (mask[k]) part[v] = (i32)(loff[k] + pos[part[v]] - pos[loff[k]]);
+}
+__global__ void k_inst_newcount(i64 K, const i64* __restrict__ pos, const i64* __restrict__ loff,
+                                const i64* __restrict__ L, i64* out) {
+    const i64 k = thread_item();
+    if (k >= K) return;
+    out[k] = pos[loff[k] + L[k]] - pos[loff[k]];
+}
+
+static std::vector<i64> label_counts(const std::vector<RefineData>& rd) {
+    std::vector<i64> L(rd.size());
+    for (size_t k = 0; k < rd.size(); k++) L[k] = rd[k].label_count;
+    return L;
+}
+
+// set_new_cluster_ids per instance over the batch's move list
+static void set_new_cluster_ids_batch(Ctx& c, MoveScratch& s, const DGraph& g, const BatchLevel& bl,
+                                      const BatchState& st, std::vector<RefineData>& cur, i64* vol,
+                                      const i32* moves, i64 k) {
+    const i64 K = bl.K, NL = st.loff[K];
+    k_new_flags<<<grid_items(k, 256), 256, 0, c.stream>>>(moves, k, s.dest, s.flag);
+    PGGC_LAUNCH_CHECK();
+    const i64 KN = compact_flags(c, s.flag, moves, k, s.tmp, s.scal);
+    if (KN == 0) return;
+    Buf<i64> Ld(c, K);
+    upload(c, Ld.p, label_counts(cur));
+    fill_i32(c, s.flag, NL, 0);
+    k_mark_used<<<grid_items(g.n, 256), 256, 0, c.stream>>>(s.part, g.n, s.flag);
+    Buf<i32> uf(c, std::max<i64>(NL, 1));
+    k_unused_inst<<<grid_items(NL, 256), 256, 0, c.stream>>>(NL, s.flag, st.linst, st.dloff, Ld, uf);
+    PGGC_LAUNCH_CHECK();
+    const i64 A = compact_flags(c, uf, nullptr, NL, s.tmp2, s.scal);
+    Buf<i64> acnt(c, K), kcnt(c, K);
+    fill_i64(c, acnt, K, 0);
+    fill_i64(c, kcnt, K, 0);
+    if (A > 0) k_inst_count<<<grid_items(A, 256), 256, 0, c.stream>>>(s.tmp2, A, st.linst, acnt);
+    k_inst_count<<<grid_items(KN, 256), 256, 0, c.stream>>>(s.tmp, KN, bl.inst, kcnt);
+    PGGC_LAUNCH_CHECK();
+    {  // group the NEW_PART movers by instance, keeping move order inside each
+        Buf<u64> keys(c, KN);
+        k_inst_keys<<<grid_items(KN, 256), 256, 0, c.stream>>>(s.tmp, KN, bl.inst, keys);
+        PGGC_LAUNCH_CHECK();
+        int bits = 1;
+        while (((i64)1 << bits) < K) bits++;
+        radix_sort_pairs(c, keys, s.tmp.p, KN, bits);
+    }
+    std::vector<i64> ha, hk;
+    download(c, ha, acnt.p, K);
+    download(c, hk, kcnt.p, K);
+    std::vector<i64> us(K), ns(K);
+    i64 u0 = 0, n0 = 0;
+    for (i64 q = 0; q < K; q++) {
+        us[q] = u0;
+        ns[q] = n0;
+        u0 += ha[q];
+        n0 += hk[q];
+    }
+    Buf<i64> dus(c, K), dns(c, K);
+    upload(c, dus.p, us);
+    upload(c, dns.p, ns);
+    k_assign_new_batch<<<grid_items(KN, 256), 256, 0, c.stream>>>(s.tmp, KN, bl.inst, dns, dus, acnt, s.tmp2,
+                                                                  st.dloff, Ld, s.dest);
+    PGGC_LAUNCH_CHECK();
+    for (i64 q = 0; q < K; q++) {
+        const i64 need = hk[q] - ha[q];
+        if (need <= 0) continue;
+        const i64 L = cur[q].label_count;
+        if (L + need > st.loff[q + 1] - st.loff[q]) throw Error(-4, "label capacity exceeded while assigning new clusters");
+        fill_i64(c, vol + st.loff[q] + L, need, 0);
+        cur[q].label_count = (i32)(L + need);
+    }
+}
+
+// relabel_contiguously (src/local_mover.cpp:100-129) of every active instance
+static void relabel_batch(Ctx& c, MoveScratch& s, const BatchLevel& bl, i32* labels, BatchState& st,
+                          const std::vector<char>& active) {
+    const i64 K = bl.K, NL = st.loff[K], n = bl.voff[K];
+    std::vector<i32> ha(K);
+    for (i64 q = 0; q < K; q++) ha[q] = active[q] ? 1 : 0;
+    Buf<i32> act(c, K);
+    upload(c, act.p, ha);
+    Buf<i64> Ld(c, K);
+    upload(c, Ld.p, label_counts(st.rd));
+    fill_i32(c, s.flag, NL, 0);
+    if (n > 0) k_mark_used_inst<<<grid_items(n, 256), 256, 0, c.stream>>>(labels, n, bl.inst, act, s.flag);
+    PGGC_LAUNCH_CHECK();
+    Buf<i64> pos(c, NL + 1), nvol(c, std::max<i64>(NL, 1)), nl(c, K);
+    scan_i32_to_i64(c, s.flag, pos, NL);
+    k_relabel_vol_inst<<<grid_items(NL, 256), 256, 0, c.stream>>>(NL, s.flag, pos, st.linst, st.dloff, st.vol, nvol);
+    k_inst_newcount<<<grid_items(K, 256), 256, 0, c.stream>>>(K, pos, st.dloff, Ld, nl);
+    k_inst_copy_l<<<grid_items(NL, 256), 256, 0, c.stream>>>(NL, st.linst, st.dloff, nl, act, nvol, st.vol);
+    if (n > 0) k_relabel_part_inst<<<grid_items(n, 256), 256, 0, c.stream>>>(n, bl.inst, act, pos, st.dloff, labels);
+    PGGC_LAUNCH_CHECK();
+    std::vector<i64> hn;
+    download(c, hn, nl.p, K);
+    for (i64 q = 0; q < K; q++)
+        if (active[q]) st.rd[q].label_count = (i32)hn[q];
+}
+
+// per-vertex lambda and filter ratio of a batch sweep (LamVert's tables)
+__global__ void k_vertex_lam(i64 n, const i32* __restrict__ inst, const float* __restrict__ lam,
+                             const float* __restrict__ ratio, float* __restrict__ vlam, float* __restrict__ vrat) {
+    const i64 v = thread_item();
+    if (v >= n) return;
+    const i32 k = inst[v];
+    vlam[v] = lam[k];
+    vrat[v] = ratio[k];
+}
+
+void local_move_batch(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const BatchLevel& bl,
+                      i32* best_part, BatchState& best, bool is_initial, const std::vector<char>& active,
+                      std::vector<MoveStats>& stats) {
+    const i64 n = g.n, K = bl.K, NL = best.loff[K];
+    s.ensure(c, std::max(n, NL));
+    plan_level(c, g, lp);
+    lp.hub_live = false;
+    Buf<i64> cvol(c, std::max<i64>(NL, 1));
+    copy_dev(c, cvol, best.vol, NL * sizeof(i64));
+    std::vector<RefineData> cur = best.rd;
+    copy_dev(c, s.part, best_part, n * sizeof(i32));
+    fill_i32(c, s.dest, n, NULL_PART);
+    Buf<i32> on(c, K), full(c, K), imp(c, K);
+    Buf<float> rat(c, K);
+    Buf<i64> cnt2(c, 2 * K), acc(c, K), Ld(c, K);
+    Buf<u64> sq(c, 2 * K);
+    const float ratios[2] = {0.75f, 0.25f};
+    const int limits[2] = {4, 2};
+    std::vector<int> phase(K, 0), cnt(K, 0), iter(K, 0);
+    std::vector<char> tables(K, is_initial ? 0 : 1), done(K);
+    for (i64 q = 0; q < K; q++) done[q] = !active[q];
+    Buf<float> vlam(c, std::max<i64>(n, 1)), vrat(c, std::max<i64>(n, 1));
+    const LamVert lam{vlam.p, vrat.p};
+    const PrioInst prio{bl.loc.p};
+    const i64 n_large = g.n - lp.ord.off_large, n_massive = g.n - lp.ord.off_large2;
+    std::vector<i32> hon(K), hfull(K), himp(K);
+    std::vector<float> hrat(K);
+    std::vector<i64> hc, hacc;
+    std::vector<u64> hsq;
+    while (true) {
+        bool any = false;
+        for (i64 q = 0; q < K; q++) {
+            hon[q] = done[q] ? 0 : 1;
+            any = any || !done[q];
+            hrat[q] = ratios[std::min(phase[q], 1)];
+        }
+        if (!any) break;
+        upload(c, on.p, hon);
+        upload(c, rat.p, hrat);
+        if (n > 0) {
+            k_inst_gate<<<grid_items(n, 256), 256, 0, c.stream>>>(n, bl.inst, on, s.dest);
+            k_vertex_lam<<<grid_items(n, 256), 256, 0, c.stream>>>(n, bl.inst, best.dlam, rat, vlam, vrat);
+            PGGC_LAUNCH_CHECK();
+        }
+        // table scoring throughout: on a singleton clustering it gives what
+        // the raw first sweep gives (every neighbour its own cluster)
+        score_all(c, s, lp, g, s.part, cvol, nullptr, false, lam, 0.0f, s.dest, s.gain);
+        i64 b[2];
+        const i64 kc = gather_candidates(c, s, lp, g, s.dest, s.cand, b);
+        i64 km = 0;
+        if (kc > 0) {
+            km = afterburner(c, s, g, s.cand, kc, b, lam, prio, s.moves);
+        }
+        fill_i64(c, cnt2, 2 * K, 0);
+        if (kc > 0) k_inst_count<<<grid_items(kc, 256), 256, 0, c.stream>>>(s.cand, kc, bl.inst, cnt2);
+        if (km > 0) k_inst_count<<<grid_items(km, 256), 256, 0, c.stream>>>(s.moves, km, bl.inst, cnt2.p + K);
+        PGGC_LAUNCH_CHECK();
+        download(c, hc, cnt2.p, 2 * K);
+        for (i64 q = 0; q < K; q++) {
+            if (!hon[q]) continue;
+            iter[q]++;
+            stats[q].sweeps++;
+            stats[q].cands += hc[q];
+        }
+        if (km > 0) {
+            set_new_cluster_ids_batch(c, s, g, bl, best, cur, cvol, s.moves, km);
+            {
+                Buf<i64> du(c, km);
+                apply_moves_core(c, s, lp, g, cvol, s.moves, km, n_large, n_massive, du);
+                fill_i64(c, acc, K, 0);
+                k_inst_add<<<grid_items(km, 256), 256, 0, c.stream>>>(s.moves, km, bl.inst, du, acc);
+                PGGC_LAUNCH_CHECK();
+            }
+            // dest := NULL_PART for movers and their neighbours, and for
+            // every vertex of an instance whose first moves these are
+            const i64 nw = std::min(km, n_large), nb = std::min(km, n_massive);
+            k_reset_affected_thread<<<grid_items(km, 256), 256, 0, c.stream>>>(g.rp, g.col, s.moves, km, s.dest,
+                                                                              s.mflag);
+            if (nw > 0)
+                k_reset_affected_wave<<<grid_items(nw, 4), 256, 0, c.stream>>>(g.rp, g.col, s.moves, km, nw, s.dest,
+                                                                               s.mflag);
+            if (nb > 0)
+                k_reset_affected_block<<<grid_blocks(nb), 1024, 0, c.stream>>>(g.rp, g.col, s.moves, km, nb, s.dest,
+                                                                               s.mflag);
+            PGGC_LAUNCH_CHECK();
+            bool anyfull = false;
+            for (i64 q = 0; q < K; q++) {
+                const bool moved = hc[K + q] > 0;
+                hfull[q] = (moved && !tables[q]) ? 1 : 0;
+                anyfull = anyfull || hfull[q];
+                if (moved) tables[q] = 1;
+                himp[q] = moved ? 1 : 0;  // instances whose objective changes
+            }
+            if (anyfull) {
+                upload(c, full.p, hfull);
+                k_inst_fill<<<grid_items(n, 256), 256, 0, c.stream>>>(n, bl.inst, full, s.dest, NULL_PART);
+                PGGC_LAUNCH_CHECK();
+            }
+            upload(c, Ld.p, label_counts(cur));
+            upload(c, imp.p, himp);
+            k_inst_sumsq<<<grid_blocks(K), 256, 0, c.stream>>>(cvol, best.dloff, Ld, imp, K, sq);
+            PGGC_LAUNCH_CHECK();
+            const i64 stale = c.read1(s.scal.p + 3);
+            if (stale) lp.hub_live = false;  // a hub table ran out of slots: rebuild next sweep
+            download(c, hacc, acc.p, K);
+            download(c, hsq, sq.p, 2 * K);
+            bool anyimp = false;
+            for (i64 q = 0; q < K; q++) {
+                const bool moved = himp[q] != 0;
+                himp[q] = 0;
+                if (!moved) continue;
+                stats[q].moves += hc[K + q];
+                cur[q].uncut += hacc[q];
+                cur[q].obj = objective_value(cur[q].uncut, cur[q].lambda, hsq[2 * q], hsq[2 * q + 1]);
+                if (c.verbose >= 3)
+                    fprintf(stderr, "sweep n=%lld it=%d cands=%lld moves=%lld labels=%d uncut=%lld obj=%.17g inst=%lld\n",
+                            (long long)(bl.voff[q + 1] - bl.voff[q]), iter[q], (long long)hc[q],
+                            (long long)hc[K + q], cur[q].label_count, (long long)cur[q].uncut, cur[q].obj,
+                            (long long)q);
+                if (cur[q].obj > best.rd[q].obj) {
+                    best.rd[q] = cur[q];
+                    himp[q] = 1;
+                    anyimp = true;
+                }
+            }
+            if (anyimp) {  // best state of the improved instances (copy_state)
+                upload(c, imp.p, himp);
+                k_inst_copy_v<<<grid_items(n, 256), 256, 0, c.stream>>>(n, bl.inst, imp, s.part, best_part);
+                k_inst_copy_l<<<grid_items(NL, 256), 256, 0, c.stream>>>(NL, best.linst, best.dloff, Ld, imp, cvol,
+                                                                        best.vol);
+                PGGC_LAUNCH_CHECK();
+            }
+        }
+        for (i64 q = 0; q < K; q++) {  // ratio schedule: 0.75 x4, then 0.25 x2; a sweep without moves ends a phase
+            if (!hon[q]) continue;
+            if (hc[q] == 0 || hc[K + q] == 0) {
+                phase[q]++;
+                cnt[q] = 0;
+            } else if (++cnt[q] == limits[phase[q]]) {
+                phase[q]++;
+                cnt[q] = 0;
+            }
+            if (phase[q] >= 2) done[q] = 1;
+        }
+    }
+    relabel_batch(c, s, bl, best_part, best, active);
 }
 
 }  // namespace pggc

@@ -1,6 +1,7 @@
 // Context allocator and data-parallel primitives: fill, scan, ordered
 // compaction, stable LSD radix sort, exact integer reductions.
 #include <algorithm>
+#include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -130,10 +131,12 @@ void Ctx::kt_free() {
 }
 
 // Small readbacks: a one-wave kernel on the context stream stores the values
-// into the pinned host slots, then the host waits for the stream.
+// into the pinned host slots (coherent host memory), makes the stores
+// visible at system scope, then the host waits for the stream.
 __global__ void k_readback(const i64* __restrict__ dev, i64* host, int count) {
     const int t = threadIdx.x;
     if (t < count) host[t] = dev[t];
+    __threadfence_system();
 }
 
 const i64* Ctx::read(const i64* dev, int count) {
@@ -186,6 +189,23 @@ void copy_dev(Ctx& c, void* dst, const void* src, size_t bytes) {
         k_copy4<<<grid_items(n, 256), 256, 0, c.stream>>>((const u32*)src, (u32*)dst, n);
     }
     PGGC_LAUNCH_CHECK();
+}
+
+void copy_h2d(Ctx& c, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return;
+    c.sync();  // the staging area is free and nothing queued still reads dst
+    if (c.stage_cap < bytes) {
+        if (c.stage) (void)hipHostFree(c.stage);
+        c.stage = nullptr;
+        c.stage_cap = 0;
+        const size_t cap = std::max<size_t>(bytes, 1 << 16);
+        PGGC_HIP(hipHostMalloc(&c.stage, cap, hipHostMallocCoherent));
+        PGGC_HIP(hipHostGetDevicePointer(&c.stage_dev, c.stage, 0));
+        c.stage_cap = cap;
+    }
+    std::memcpy(c.stage, src, bytes);
+    copy_dev(c, dst, c.stage_dev, bytes);
+    c.sync();
 }
 
 void iota_i32(Ctx& c, i32* p, i64 n) {

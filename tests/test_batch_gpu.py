@@ -80,3 +80,76 @@ def test_batch_carries_pruning(gpu_ctx):
         C().gnn_configure("off")
     for (q, p, _), (q1, p1, _) in zip(res, plain):
         assert q == q1 and np.array_equal(p.cpu().numpy(), p1.cpu().numpy())
+
+
+def weighted_graphs():
+    from tests.graphcases import hub_graph
+    out = [C().DeviceGraph.from_csr(graphs.rmat(12, 16, seed=3, weights=(1, 50)), "WMod"),
+           C().DeviceGraph.from_csr(graphs.random_graph(3000, 10, seed=4, weights=(1, 10)), "WMod"),
+           C().DeviceGraph.from_csr(hub_graph(seed=9, weights=(1, 10)), "WMod")]
+    for k in range(5):
+        out.append(C().DeviceGraph.from_csr(graphs.random_graph(500 + 300 * k, 6, seed=20 + k, weights=(1, 100)),
+                                            "WMod"))
+    return out
+
+
+@pytest.mark.parametrize("objective", ["Mod", "WMod", "CPM"])
+def test_union_batch_equals_single_runs(gpu_ctx, objective):
+    """pLouvain with ex_iters 0 runs the batch as one disjoint union graph:
+    each instance's clustering, objective and run counters equal its run
+    alone, for SBMs of uneven size, an R-MAT graph with hubs and weighted
+    graphs."""
+    if objective == "WMod":
+        gs = weighted_graphs()
+    else:
+        gs = batch_graphs()
+        if objective == "CPM":
+            gs = [C().DeviceGraph(g.rp, g.col, None, None, objective="CPM") for g in gs[:-1]]
+    print(f"union batch {objective} {len(gs)} graphs", flush=True)
+    res = C().cluster_batch(gs, "louvain", objective, 1.0, 0, seeds=list(range(len(gs))))
+    for k, (g, (q, p, info)) in enumerate(zip(gs, res)):
+        q1, p1, i1 = C().cluster(g, "louvain", objective, 1.0, 0, seed=k)
+        assert info == i1, (k, info, i1)
+        assert q == q1, k
+        assert np.array_equal(p.cpu().numpy(), p1.cpu().numpy()), k
+
+
+def test_union_batch_sbm_family_matches_oracle(gpu_ctx):
+    """Planted-partition graphs of the batched workload's family (n log-
+    uniform, blocks in [n/256, n/32], intra degree 10, inter degree 3) in
+    one union batch, each equal to the CPU oracle's run."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(7)
+    hs, gs = [], []
+    for k in range(8):
+        n = int(round(2 ** rng.uniform(10, 14)))
+        b = int(rng.integers(max(1, -(-n // 256)), max(1, n // 32) + 1))
+        h = graphs.sbm(n, b, 10, 3, seed=300 + k)
+        hs.append(h)
+        gs.append(C().DeviceGraph.from_csr(h, "Mod"))
+    print(f"union batch of {len(gs)} SBMs vs oracle", flush=True)
+    res = C().cluster_batch(gs, "louvain", "Mod", 1.0, 0)
+    for h, (q, p, info) in zip(hs, res):
+        oq, op, oi = O.cluster(h.rp, h.col, None, h.vertex_weights("Mod"), "Mod", 1.0, "louvain", 0, seed=0)
+        assert np.array_equal(p.cpu().numpy(), op)
+        assert q == oq
+        for key in ["labels", "uncut", "levels", "sweeps", "moves"]:
+            assert info[key] == oi[key], key
+
+
+def test_union_batch_many_graphs(gpu_ctx):
+    """64 graphs (n log-uniform in [2^10, 2^16]) in one union batch equal
+    their single runs."""
+    rng = np.random.default_rng(100)
+    gs = []
+    for k in range(64):
+        n = int(round(2 ** rng.uniform(10, 16)))
+        b = int(rng.integers(max(1, -(-n // 256)), max(1, n // 32) + 1))
+        rp, col = graphs.sbm_torch(n, b, 10, 3, seed=100 + k)
+        gs.append(C().DeviceGraph(rp, col, None, None, objective="Mod"))
+    print("union batch of 64 graphs", flush=True)
+    res = C().cluster_batch(gs, "louvain", "Mod", 1.0, 0, seeds=list(range(64)))
+    for k, (g, (q, p, info)) in enumerate(zip(gs, res)):
+        q1, p1, i1 = C().cluster(g, "louvain", "Mod", 1.0, 0, seed=k)
+        assert q == q1 and info == i1, k
+        assert np.array_equal(p.cpu().numpy(), p1.cpu().numpy()), k
