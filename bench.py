@@ -55,6 +55,9 @@ def parse():
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 1 GiB read + 1 GiB write elementwise kernel 5 times (the reference point "
                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
+    p.add_argument("--kernels-only", action="store_true",
+                   help="only run one step with the kernel-group timers and byte counters on and print them (the "
+                        "command the rocprofv3 kernel summaries and counter passes are taken from)")
     p.add_argument("--knob", action="append", default=[], metavar="NAME=VALUE",
                    help="set a tuning knob of the HIP path before the run (repeatable)")
     p.add_argument("--lfr-n", type=int, default=8000000,
@@ -151,6 +154,27 @@ def roofline(kstats, scale):
             out["traffic"] = t["hbm_bytes_per_launch"]
             out["traffic_source"] = {k: t[k] for k in ("commit", "profile") if k in t}
     return out
+
+
+def instrumented_step(step, ctx, torch):
+    """One step with the library's kernel-group timers (HIP events on its
+    stream) and algorithmic byte counters on; returns the raw stats, the
+    per-group summary and the step's wall time."""
+    ctx.set_timing(True)
+    t1 = time.perf_counter()
+    step()
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t1
+    kstats = ctx.kernel_stats()
+    ctx.set_timing(False)
+    groups = {}
+    for name, (n, ms, by) in kstats.items():
+        if n:
+            groups[name] = {"ms_per_step": ms, "launches_per_step": n, "GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None,
+                            "GB_per_step": by / 1e9}
+            print(f"# kernel group {name}: {n} launches, {ms:.1f} ms/step, {ms / n * 1000:.1f} us avg, "
+                  f"{groups[name]['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
+    return kstats, groups, seconds
 
 
 def lfr_leg(a, ctx, C, graphs, torch):
@@ -256,6 +280,12 @@ def main():
     def step():
         return C.cluster(g, a.algo, "Mod", 1.0, a.ex_iters, seed=0, ctx=ctx)
 
+    if a.kernels_only:
+        kstats, groups, instrumented_s = instrumented_step(step, ctx, torch)
+        print(json.dumps({"kernels_only": True, "workload": f"R-MAT scale {a.scale}, {a.algo} ex_iters={a.ex_iters}",
+                          "instrumented_step_s": instrumented_s, "roofline": roofline(kstats, a.scale),
+                          "kernel_groups": groups}))
+        return
     for w in range(a.warmup):
         q, part, info = step()
         print(f"# warmup {w}: Q={q:.6f} info={info}", file=sys.stderr, flush=True)
@@ -268,20 +298,7 @@ def main():
     dt = (time.perf_counter() - t0) / max(a.steps, 1)
     # one more step with the kernel timers and byte counters on (they cost
     # time of their own, so the timed steps above run without them)
-    ctx.set_timing(True)
-    t1 = time.perf_counter()
-    step()
-    torch.cuda.synchronize()
-    instrumented_s = time.perf_counter() - t1
-    kstats = ctx.kernel_stats()
-    ctx.set_timing(False)
-    groups = {}
-    for name, (n, ms, by) in kstats.items():
-        if n:
-            groups[name] = {"ms_per_step": ms, "launches_per_step": n, "GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None,
-                            "GB_per_step": by / 1e9}
-            print(f"# kernel group {name}: {n} launches, {ms:.1f} ms/step, {ms / n * 1000:.1f} us avg, "
-                  f"{groups[name]['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
+    kstats, groups, instrumented_s = instrumented_step(step, ctx, torch)
 
     extra = {}
     golden = GOLDEN.format(scale=a.scale)

@@ -159,7 +159,10 @@ int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user);
  *     2048, 4096 (default), 8192.
  *   PGGC_KNOB_RAW_SWEEPS: 1 (default) scores the first sweep of a local move
  *     from singletons on the raw adjacency (src/local_mover.cpp:1391-1395),
- *     0 with the neighbour-cluster tables (the same result, more work). */
+ *     0 with the neighbour-cluster tables (the same result, more work).
+ *   PGGC_KNOB_HUB_BUILD: how HBM hub tables are built: 0 (default) one LDS
+ *     slice of a table per block, written out whole; 1 edge work items
+ *     adding into the HBM table with atomics (the same tables' contents). */
 #define PGGC_KNOB_HUB_UPDATE_DIV 0
 #define PGGC_KNOB_GNN_MAX_SWEEPS 1
 #define PGGC_KNOB_TABLE_THREADS 2
@@ -168,7 +171,8 @@ int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user);
 #define PGGC_KNOB_HUB_EDGES_PER_ITEM 5
 #define PGGC_KNOB_HUB_SLOTS_PER_ITEM 6
 #define PGGC_KNOB_RAW_SWEEPS 7
-#define PGGC_KNOB_COUNT 8
+#define PGGC_KNOB_HUB_BUILD 8
+#define PGGC_KNOB_COUNT 9
 int pggc_ctx_set_knob(pggc_ctx* ctx, int knob, int64_t value);
 int pggc_ctx_get_knob(pggc_ctx* ctx, int knob, int64_t* value);
 /* Autotuner of the execution-shape knobs (no counterpart in the reference):

@@ -622,6 +622,7 @@ static void set_knob(Ctx& c, int knob, int64_t value) {
         case PGGC_KNOB_HUB_EDGES_PER_ITEM: c.hub_edges_item = value; break;
         case PGGC_KNOB_HUB_SLOTS_PER_ITEM: c.hub_slots_item = value; break;
         case PGGC_KNOB_RAW_SWEEPS: c.raw_sweeps = value ? 1 : 0; break;
+        case PGGC_KNOB_HUB_BUILD: c.hub_build = value ? 1 : 0; break;
         default: throw Error(PGGC_EINVAL, "unknown knob");
     }
     if (shape) c.knob_epoch++;  // level plans depend on these
@@ -637,6 +638,7 @@ static int64_t get_knob(const Ctx& c, int knob) {
         case PGGC_KNOB_HUB_EDGES_PER_ITEM: return c.hub_edges_item;
         case PGGC_KNOB_HUB_SLOTS_PER_ITEM: return c.hub_slots_item;
         case PGGC_KNOB_RAW_SWEEPS: return c.raw_sweeps;
+        case PGGC_KNOB_HUB_BUILD: return c.hub_build;
         default: throw Error(PGGC_EINVAL, "unknown knob");
     }
 }

@@ -93,6 +93,9 @@ struct Ctx {
     // the first sweep of a call from singletons scores the raw adjacency
     // (every neighbour its own cluster); 0 scores it with the tables
     int raw_sweeps = 1;
+    // hub tables: 0 builds each table slice in LDS and writes it out whole,
+    // 1 adds edge work items into the HBM tables with atomics
+    int hub_build = 0;
     u64 knob_epoch = 0;  // bumped by every knob change; level plans are rebuilt when it moves
     // the active-set predictor runs on at most this many sweeps of a
     // local-move call (0: on every eligible sweep)

@@ -56,14 +56,21 @@ __device__ __forceinline__ void ptable_add(u64* tab, u32 mask, i32 key, u64 w) {
     ptable_add_from(tab, mask, key, w, slot_hash(key, mask));
 }
 
+// Probe sequences of the hub tables: a table of mask + 1 slots is split into
+// slices of smask + 1 slots (smask <= mask, both powers of two minus one);
+// a key hashes over the whole table and probes linearly within its slice,
+// wrapping at the slice's end.  smask == mask is plain linear probing over
+// the table.  Slices let a build hold one slice in LDS at a time.
+__device__ __forceinline__ u32 probe_next(u32 h, u32 smask) { return (h & ~smask) | ((h + 1) & smask); }
+
 // Signed update of a table that is kept current between sweeps.  At most
-// mask+1 probes; returns false when the key is neither found nor insertable
-// (table full, or a decrement of a key the table does not hold), which
-// leaves the table stale.  A decrement never borrows across the packed key:
-// the slot holds at least the weight being moved out of it.
-__device__ __forceinline__ bool ptable_update(u64* tab, u32 mask, i32 key, i64 w) {
+// smask+1 probes; returns false when the key is neither found nor
+// insertable (slice full, or a decrement of a key the table does not hold),
+// which leaves the table stale.  A decrement never borrows across the packed
+// key: the slot holds at least the weight being moved out of it.
+__device__ __forceinline__ bool ptable_update(u64* tab, u32 mask, u32 smask, i32 key, i64 w) {
     u32 h = slot_hash(key, mask);
-    for (u32 probe = 0; probe <= mask; probe++) {
+    for (u32 probe = 0; probe <= smask; probe++) {
         const u64 cur = __hip_atomic_load(&tab[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if ((u32)(cur >> 32) == (u32)key) {
             atomicAdd((unsigned long long*)&tab[h], (unsigned long long)w);
@@ -79,13 +86,13 @@ __device__ __forceinline__ bool ptable_update(u64* tab, u32 mask, i32 key, i64 w
                 return true;
             }
         }
-        h = (h + 1) & mask;
+        h = probe_next(h, smask);
     }
     return false;
 }
-__device__ __forceinline__ bool table_update(i32* keys, i64* vals, u32 mask, i32 key, i64 w) {
+__device__ __forceinline__ bool table_update(i32* keys, i64* vals, u32 mask, u32 smask, i32 key, i64 w) {
     u32 h = slot_hash(key, mask);
-    for (u32 probe = 0; probe <= mask; probe++) {
+    for (u32 probe = 0; probe <= smask; probe++) {
         const i32 cur = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         bool hit = cur == key;
         if (!hit && cur == EMPTY_KEY) {
@@ -97,28 +104,28 @@ __device__ __forceinline__ bool table_update(i32* keys, i64* vals, u32 mask, i32
             atomicAdd((unsigned long long*)&vals[h], (unsigned long long)w);
             return true;
         }
-        h = (h + 1) & mask;
+        h = probe_next(h, smask);
     }
     return false;
 }
-// weight of `key` in a table (0 if absent), at most mask+1 probes
-__device__ __forceinline__ i64 ptable_get(const u64* tab, u32 mask, i32 key) {
+// weight of `key` in a table (0 if absent), at most smask+1 probes
+__device__ __forceinline__ i64 ptable_get(const u64* tab, u32 mask, u32 smask, i32 key) {
     u32 h = slot_hash(key, mask);
-    for (u32 probe = 0; probe <= mask; probe++) {
+    for (u32 probe = 0; probe <= smask; probe++) {
         const u64 cur = tab[h];
         if ((u32)(cur >> 32) == (u32)key) return (i64)(cur & 0xffffffffull);
         if (cur == PEMPTY) return 0;
-        h = (h + 1) & mask;
+        h = probe_next(h, smask);
     }
     return 0;
 }
-__device__ __forceinline__ i64 table_get(const i32* keys, const i64* vals, u32 mask, i32 key) {
+__device__ __forceinline__ i64 table_get(const i32* keys, const i64* vals, u32 mask, u32 smask, i32 key) {
     u32 h = slot_hash(key, mask);
-    for (u32 probe = 0; probe <= mask; probe++) {
+    for (u32 probe = 0; probe <= smask; probe++) {
         const i32 cur = keys[h];
         if (cur == key) return vals[h];
         if (cur == EMPTY_KEY) return 0;
-        h = (h + 1) & mask;
+        h = probe_next(h, smask);
     }
     return 0;
 }

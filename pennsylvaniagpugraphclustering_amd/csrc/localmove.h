@@ -52,19 +52,29 @@ struct LevelPlan {
     // runs out of slots marks the set stale and the next sweep rebuilds it.
     // packed: one u64 per slot (key << 32 | weight), used when every hub's
     // row weight fits 32 bits; otherwise separate key / int64 weight arrays.
-    // imap / smap: block -> work item, so that all items of one hub run on
-    // one XCD (blocks are dispatched round-robin over the 8 XCDs) and its
-    // table stays in that XCD's L2; -1 = idle block.
+    // A table is split into slices of at most hub_slice(packed) slots: a key
+    // probes within the slice its hash falls in, so a build holds one slice
+    // in LDS per block (soff: slice prefix per hub) and writes it out whole.
+    // A slice that cannot hold its keys (or the hub_build knob) sends its
+    // hub to the fallback build, edge items adding into the table with
+    // atomics and probing over the whole table: hflag[h] = 1, and hsm[h]
+    // (the slice mask every table user probes with) becomes the table mask.
+    // imap / smap / bmap: block -> build item, scan item or slice, so that
+    // all work of one hub runs on one XCD (blocks are dispatched round-robin
+    // over the 8 XCDs) and its table stays in that XCD's L2; -1 = idle block.
     // hidx: vertex -> hub index, -1 for other vertices.
-    Buf<i64> goff, items, sitems, hub_pv;
+    Buf<i64> goff, items, sitems, soff, hub_pv;
     Buf<u64> hub_best;
     Buf<i32> gkeys;
     Buf<i64> gvals;
     Buf<u64> gtab;
-    Buf<i32> imap, smap, hidx;
+    Buf<i32> imap, smap, bmap, hidx, hflag, hovf;
+    Buf<u32> hsm;
     bool packed = false;
     bool hub_live = false;  // tables match the current clustering
-    i64 gslots = 0, nitems = 0, nsitems = 0, igrid = 0, sgrid = 0;
+    i64 gslots = 0, nitems = 0, nsitems = 0, nslices = 0, igrid = 0, sgrid = 0, bgrid = 0;
+    i64 hs1 = 0, hs2 = 0;  // slice-class bounds (xcd_maps)
+    i64 hg = 0, ghg = 0, ihg = 0;  // first fallback-built hub, its first slot and build item
     // the hub class is sorted by degree; its first nlds vertices (degree <=
     // Ctx::hub_lds_max) are scored in LDS, the rest (from hub0 on) in HBM
     // tables; he / hs: edges per build item and slots per scan item

@@ -95,6 +95,17 @@ __global__ void k_slot_items(const i64* goff, i64 m, i64 hs, i64* sitems) {
     if (x >= m) return;
     sitems[x] = (goff[x + 1] - goff[x] + hs - 1) / hs;
 }
+// Slots per hub-table slice: one slice is built in LDS by one block, 128 KB
+// of LDS for packed u64 slots, 96 KB for separate i32 keys and i64 weights.
+constexpr i64 HUB_SLICE_PACKED = 16384, HUB_SLICE_WIDE = 8192;
+constexpr i64 HUB_MAX_SLICES = 16;  // larger tables are built by the fallback
+__host__ __device__ __forceinline__ i64 hub_slice(bool packed) { return packed ? HUB_SLICE_PACKED : HUB_SLICE_WIDE; }
+__global__ void k_slice_counts(const i64* goff, i64 m, i64 sl, i64* soff) {
+    const i64 x = thread_item();
+    if (x >= m) return;
+    const i64 S = goff[x + 1] - goff[x];
+    soff[x] = S > sl ? S / sl : 1;
+}
 __global__ void k_hub_deg_keys(const i64* rp, const i32* list, i64 m, u64* keys, i32* vals) {
     const i64 x = thread_item();
     if (x >= m) return;
@@ -140,10 +151,12 @@ __global__ void __launch_bounds__(256) k_hub_wsum(const i64* __restrict__ rp, WT
 // items); block 8k + x runs the k-th item of XCD x.
 constexpr int NXCD = 8;
 static void build_map(const std::vector<i64>& pre, const std::vector<int>& xcd_of, i64 m, std::vector<i32>& map,
-                      i64& grid) {
+                      i64& grid, i64 lo = 0, bool largest_first = false) {
     std::vector<std::vector<i32>> per(NXCD);
-    for (i64 h = 0; h < m; h++)
-        for (i64 it = pre[h]; it < pre[h + 1]; it++) per[xcd_of[h]].push_back((i32)it);
+    for (i64 k = 0; k < m; k++) {
+        const i64 h = largest_first ? m - 1 - k : k;  // hubs are sorted by degree
+        for (i64 it = std::max(pre[h], lo); it < pre[h + 1]; it++) per[xcd_of[h]].push_back((i32)it);
+    }
     size_t mx = 0;
     for (auto& v : per) mx = std::max(mx, v.size());
     grid = (i64)mx * NXCD;
@@ -153,9 +166,11 @@ static void build_map(const std::vector<i64>& pre, const std::vector<int>& xcd_o
 }
 
 static void xcd_maps(Ctx& c, LevelPlan& p, i64 m) {
-    std::vector<i64> it(m + 1), st(m + 1);
+    std::vector<i64> it(m + 1), st(m + 1), so(m + 1), go(m + 1);
+    PGGC_HIP(hipMemcpyAsync(go.data(), p.goff, (m + 1) * sizeof(i64), hipMemcpyDeviceToHost, c.stream));
     PGGC_HIP(hipMemcpyAsync(it.data(), p.items, (m + 1) * sizeof(i64), hipMemcpyDeviceToHost, c.stream));
     PGGC_HIP(hipMemcpyAsync(st.data(), p.sitems, (m + 1) * sizeof(i64), hipMemcpyDeviceToHost, c.stream));
+    PGGC_HIP(hipMemcpyAsync(so.data(), p.soff, (m + 1) * sizeof(i64), hipMemcpyDeviceToHost, c.stream));
     c.sync();
     std::vector<i64> order(m);
     for (i64 h = 0; h < m; h++) order[h] = h;
@@ -177,6 +192,28 @@ static void xcd_maps(Ctx& c, LevelPlan& p, i64 m) {
     build_map(st, xcd_of, m, smap, p.sgrid);
     p.smap = Buf<i32>(c, smap.size());
     copy_h2d(c, p.smap, smap.data(), smap.size() * sizeof(i32));
+    // slice classes (hubs are sorted by degree, so table sizes ascend):
+    // [0, hs1) tables of at most 4096 slots, [hs1, hs2) packed tables of
+    // 8192, then the tables of more slots than one block's slice class
+    // (several slices each), which go through the XCD map
+    auto first_above = [&](i64 slots) {
+        i64 h = 0;
+        while (h < m && go[h + 1] - go[h] <= slots) h++;
+        return so[h];
+    };
+    p.hs1 = first_above(4096);
+    p.hs2 = p.packed ? first_above(8192) : p.hs1;
+    // the largest hubs (more than HUB_MAX_SLICES slices) are built by the
+    // fallback: first such hub, its first table slot and first build item
+    p.hg = 0;
+    while (p.hg < m && go[p.hg + 1] - go[p.hg] <= HUB_MAX_SLICES * hub_slice(p.packed)) p.hg++;
+    p.ghg = go[p.hg];
+    p.ihg = it[p.hg];
+    std::vector<i32> bmap;
+    build_map(so, xcd_of, m, bmap, p.bgrid, p.hs2, true);
+    if (p.hs2 == so[m]) p.bgrid = 0;
+    p.bmap = Buf<i32>(c, bmap.size());
+    copy_h2d(c, p.bmap, bmap.data(), bmap.size() * sizeof(i32));
 }
 
 __global__ void k_hub_index(const i32* hubs, i64 m, i32* hidx) {
@@ -260,6 +297,14 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
             p.gkeys = Buf<i32>(c, p.gslots);
             p.gvals = Buf<i64>(c, p.gslots);
         }
+        p.soff = Buf<i64>(c, m7 + 1);
+        k_slice_counts<<<grid_items(m7, 256), 256, 0, c.stream>>>(p.goff, m7, hub_slice(p.packed), p.soff);
+        PGGC_LAUNCH_CHECK();
+        scan_i64(c, p.soff, p.soff, m7);
+        p.nslices = c.read1(p.soff.p + m7);
+        p.hflag = Buf<i32>(c, m7);
+        p.hovf = Buf<i32>(c, 1);
+        p.hsm = Buf<u32>(c, m7);
         xcd_maps(c, p, m7);
         p.hidx = Buf<i32>(c, std::max<i64>(g.n, 1));
         fill_i32(c, p.hidx, g.n, -1);
@@ -563,77 +608,257 @@ __global__ void __launch_bounds__(256) k_hub_raw(const i64* __restrict__ rp, con
     if (threadIdx.x == 0 && bk) atomicMax((unsigned long long*)&hub_best[hi], (unsigned long long)bk);
 }
 
-// Hubs: edge work items (he edges each) pre-aggregate in LDS, then insert into
-// the hub's HBM table (built for every hub at the first table sweep of a
-// call, own cluster included); later sweeps keep it current from the movers
-// (k_hub_update_*).  Per sweep, k_hub_pv reads the own-cluster weight, slot
-// work items scan the table and fold the packed best candidate with
-// atomicMax, and a last pass decides.
-// HL LDS slots pre-aggregate an item of HL / 2 edges.
+// Hubs: an HBM table per hub of its neighbour-cluster weights (own cluster
+// included), built at the first table sweep of a call and kept current from
+// the movers by later sweeps (k_hub_update_*).  Per sweep, k_hub_pv reads
+// the own-cluster weight, slot work items scan the table and fold the packed
+// best candidate with atomicMax, and a last pass decides.
+//
+// Build: one block per table slice (k_hub_build) reads all of its hub's
+// edges, keeps the (cluster, weight) pairs whose hash falls in its slice in
+// an LDS table with the slice's probe sequence, and writes the slice out
+// whole (empty slots included), so the build needs no fill, no HBM atomics
+// and only coalesced stores.  The slices of one hub run on one XCD, where
+// the hub's edges and neighbour clusters are read from L2 after the first
+// slice.  A slice whose keys do not fit flags its hub (hflag), which is then
+// cleared (k_hub_clear) and built by edge items with HBM atomics and
+// whole-table probing (k_hub_insert).
+
+// Slice masks and fallback flags before a build.  hflag 1: built by the
+// fallback from the start (hub_build mode 1: every hub, mode 2: the hubs of
+// odd index, a mixed set; mode 0: the hubs from hg on, whose tables have
+// more than HUB_MAX_SLICES slices: every slice block reads all of a hub's
+// edges, and for the largest hubs the fallback's edge items spread better
+// over the CUs).  hflag 2: a slice of the hub was full (k_hub_build), and
+// *hovf is set.
+__global__ void k_hub_sm_init(const i64* __restrict__ goff, i64 m, i64 sl, int mode, i64 hg, u32* hsm, i32* hflag,
+                              i32* hovf) {
+    const i64 h = thread_item();
+    if (h == 0) *hovf = 0;
+    if (h >= m) return;
+    const i64 S = goff[h + 1] - goff[h];
+    const bool fb = mode == 1 || (mode == 2 && (h & 1)) || (mode == 0 && h >= hg);
+    hflag[h] = fb ? 1 : 0;
+    hsm[h] = (u32)((fb || S < sl ? S : sl) - 1);
+}
+
+// LDS insert with the slice's probe sequence; false when the slice is full
+__device__ __forceinline__ bool slice_add_packed(u64* lt, u32 smask, u32 q, i32 key, u64 w) {
+    for (u32 probe = 0; probe <= smask; probe++) {
+        const u64 cur = __hip_atomic_load(&lt[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if ((u32)(cur >> 32) == (u32)key) {
+            atomicAdd((unsigned long long*)&lt[q], (unsigned long long)w);
+            return true;
+        }
+        if (cur == PEMPTY) {
+            const u64 old = atomicCAS((unsigned long long*)&lt[q], (unsigned long long)PEMPTY,
+                                      (unsigned long long)(((u64)(u32)key << 32) | w));
+            if (old == PEMPTY) return true;
+            if ((u32)(old >> 32) == (u32)key) {
+                atomicAdd((unsigned long long*)&lt[q], (unsigned long long)w);
+                return true;
+            }
+        }
+        q = (q + 1) & smask;
+    }
+    return false;
+}
+__device__ __forceinline__ bool slice_add_wide(i32* lk, i64* lv, u32 smask, u32 q, i32 key, i64 w) {
+    for (u32 probe = 0; probe <= smask; probe++) {
+        const i32 cur = __hip_atomic_load(&lk[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        bool hit = cur == key;
+        if (!hit && cur == EMPTY_KEY) {
+            const i32 old = atomicCAS(&lk[q], EMPTY_KEY, key);
+            hit = old == EMPTY_KEY || old == key;
+        }
+        if (hit) {
+            atomicAdd((unsigned long long*)&lv[q], (unsigned long long)w);
+            return true;
+        }
+        q = (q + 1) & smask;
+    }
+    return false;
+}
+
+// SL: LDS slots of the block (>= the slice size of every slice it builds),
+// BS threads.  Slice b0 + block id, or bmap[block id] when bmap is given.
+template <bool PACKED, int SL, int BS, class WT>
+__global__ void __launch_bounds__(BS) k_hub_build(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
+                                                  const i32* __restrict__ hubs, i64 m,
+                                                  const i64* __restrict__ soff, i64 b0, const i32* __restrict__ bmap,
+                                                  i64 nblk, const i64* __restrict__ goff,
+                                                  const i32* __restrict__ part, u32* hsm, i32* hflag,
+                                                  i32* hovf, i32* gkeys, i64* gvals, u64* gtab) {
+    constexpr i64 SLMAX = PACKED ? HUB_SLICE_PACKED : HUB_SLICE_WIDE;
+    __shared__ u64 lt[PACKED ? SL : 1];
+    __shared__ i32 lk[PACKED ? 1 : SL];
+    __shared__ i64 lv[PACKED ? 1 : SL];
+    __shared__ int full;
+    const i64 bb = block_id();
+    if (bb >= nblk) return;
+    const i64 b = bmap ? (i64)bmap[bb] : b0 + bb;
+    if (b < 0) return;
+    const i64 hi = upper_row(soff, 0, m, b);
+    if (hflag[hi]) return;  // built by the fallback
+    const int t = threadIdx.x;
+    const i32 i = hubs[hi];
+    const i64 S = goff[hi + 1] - goff[hi];
+    const u32 mask = (u32)(S - 1);
+    const u32 smask = (u32)((S < SLMAX ? S : SLMAX) - 1);
+    if (smask >= (u32)SL) return;  // not this instance's slices (never launched so)
+    const u32 base = (u32)(b - soff[hi]) * (smask + 1);  // first slot of the slice
+    for (u32 q = t; q <= smask; q += BS) {
+        if (PACKED) {
+            lt[q] = PEMPTY;
+        } else {
+            lk[q] = EMPTY_KEY;
+            lv[q] = 0;
+        }
+    }
+    if (t == 0) full = 0;
+    __syncthreads();
+    const i64 s = rp[i], e = rp[i + 1];
+    bool ok = true;
+    constexpr int U = 4;  // edges per thread in flight
+    for (i64 jb = s; jb < e; jb += (i64)BS * U) {  // block-uniform loop control
+        i32 kk[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const i64 j = jb + (i64)u * BS + t;
+            kk[u] = j < e ? col[j] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) kk[u] = kk[u] >= 0 ? part[kk[u]] : EMPTY_KEY;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (kk[u] == EMPTY_KEY) continue;
+            const u32 h = slot_hash(kk[u], mask);
+            if ((h & ~smask) != base) continue;
+            const i64 w = W(jb + (i64)u * BS + t);
+            ok = (PACKED ? slice_add_packed(lt, smask, h & smask, kk[u], (u64)w)
+                         : slice_add_wide(lk, lv, smask, h & smask, kk[u], w)) && ok;
+        }
+    }
+    if (!ok) full = 1;
+    __syncthreads();
+    if (full) {  // the hub goes to the fallback build
+        if (t == 0) {
+            hflag[hi] = 2;
+            hsm[hi] = mask;
+            *hovf = 1;
+        }
+        return;
+    }
+    const i64 g0 = goff[hi] + base;
+    for (u32 q = t; q <= smask; q += BS) {
+        if (PACKED) {
+            gtab[g0 + q] = lt[q];
+        } else {
+            gkeys[g0 + q] = lk[q];
+            gvals[g0 + q] = lv[q];
+        }
+    }
+}
+
+// empty every slice of a hub flagged for the fallback build
+// after a full slice (grid-stride over the slices; nothing to do unless
+// *hovf)
+template <bool PACKED>
+__global__ void __launch_bounds__(256) k_hub_clear(i64 m, const i64* __restrict__ soff, i64 nslices,
+                                                   const i64* __restrict__ goff, const i32* __restrict__ hflag,
+                                                   const i32* __restrict__ hovf, i32* gkeys, i64* gvals, u64* gtab) {
+    if (*hovf == 0) return;
+    const i64 nblk = (i64)gridDim.x * gridDim.y;
+    for (i64 b = block_id(); b < nslices; b += nblk) {
+        const i64 hi = upper_row(soff, 0, m, b);
+        if (hflag[hi] != 2) continue;
+        const i64 S = goff[hi + 1] - goff[hi];
+        const i64 ns = soff[hi + 1] - soff[hi];
+        const i64 len = S / ns;
+        const i64 g0 = goff[hi] + (b - soff[hi]) * len;
+        for (i64 q = threadIdx.x; q < len; q += 256) {
+            if (PACKED) {
+                gtab[g0 + q] = PEMPTY;
+            } else {
+                gkeys[g0 + q] = EMPTY_KEY;
+                gvals[g0 + q] = 0;
+            }
+        }
+    }
+}
+
+// Fallback build of the hubs with hflag == want (into emptied tables): edge
+// work items (he = HL / 2 edges each) pre-aggregate in LDS, then add into
+// the HBM table with atomics, probing over the whole table.  Grid-stride
+// over the items b0 + k, or imap[k] when imap is given; nothing to do when
+// gate is given and *gate is 0.
 template <bool PACKED, int HL, class WT>
 __global__ void __launch_bounds__(256) k_hub_insert(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
                                                     const i32* __restrict__ hubs, i64 m, const i64* __restrict__ items,
-                                                    const i32* __restrict__ imap, i64 nmap,
+                                                    const i32* __restrict__ imap, i64 b0, i64 nmap,
                                                     const i64* __restrict__ goff, const i32* __restrict__ part,
-                                                    i32* gkeys, i64* gvals, u64* gtab, u64* acct) {
+                                                    const i32* __restrict__ hflag, i32 want,
+                                                    const i32* __restrict__ gate, i32* gkeys, i64* gvals, u64* gtab) {
+    if (gate && *gate == 0) return;
     constexpr i64 he = HL / 2;
-    const i64 bb = block_id();
-    if (bb >= nmap) return;
-    const i64 b = imap[bb];
-    if (b < 0) return;
-    const i64 hi = upper_row(items, 0, m, b);
-    const i32 i = hubs[hi];
-    const i64 s = rp[i], e = rp[i + 1];
-    const i64 j_lo = s + (b - items[hi]) * he;
-    const i64 j_hi = j_lo + he < e ? j_lo + he : e;
     __shared__ i32 lk[HL];
     __shared__ i64 lv[HL];
-    lds_agg_clear<256, HL>(lk, lv);
-    for (i64 jb = j_lo; jb < j_hi; jb += 256) {  // block-uniform loop control
-        const i64 j = jb + threadIdx.x;
-        if (j >= j_hi) continue;
-        table_add_lds(lk, lv, HL - 1, part[col[j]], W(j));
+    const i64 nblk = (i64)gridDim.x * gridDim.y;
+    for (i64 bb = block_id(); bb < nmap; bb += nblk) {  // block-uniform loop control
+        const i64 b = imap ? (i64)imap[bb] : b0 + bb;
+        if (b < 0) continue;
+        const i64 hi = upper_row(items, 0, m, b);
+        if (hflag[hi] != want) continue;
+        const i32 i = hubs[hi];
+        const i64 s = rp[i], e = rp[i + 1];
+        const i64 j_lo = s + (b - items[hi]) * he;
+        const i64 j_hi = j_lo + he < e ? j_lo + he : e;
+        lds_agg_clear<256, HL>(lk, lv);
+        for (i64 jb = j_lo; jb < j_hi; jb += 256) {  // block-uniform loop control
+            const i64 j = jb + threadIdx.x;
+            if (j >= j_hi) continue;
+            table_add_lds(lk, lv, HL - 1, part[col[j]], W(j));
+        }
+        const u32 gmask = (u32)(goff[hi + 1] - goff[hi] - 1);
+        if (PACKED) lds_agg_flush_packed<256, HL>(lk, lv, gtab + goff[hi], gmask);
+        else lds_agg_flush<256, HL>(lk, lv, gkeys + goff[hi], gvals + goff[hi], gmask);
+        __syncthreads();  // the flush reads LDS the next item clears
     }
-    const u32 gmask = (u32)(goff[hi + 1] - goff[hi] - 1);
-    if (PACKED) lds_agg_flush_packed<256, HL>(lk, lv, gtab + goff[hi], gmask);
-    else lds_agg_flush<256, HL>(lk, lv, gkeys + goff[hi], gvals + goff[hi], gmask);
-    // algorithmic bytes of the item: column, neighbour cluster id and weight
-    // per edge, plus the hub's row bounds and cluster
-    if (acct && threadIdx.x == 0)
-        atomicAdd((unsigned long long*)acct, (unsigned long long)((j_hi - j_lo) * (8 + WT::bytes) + 20));
 }
 
 // own-cluster weight of every active hub, read from its table
 template <bool PACKED>
 __global__ void k_hub_pv(const i32* __restrict__ hubs, i64 m, const i64* __restrict__ goff,
-                         const i32* __restrict__ part, const i32* __restrict__ dest, const i32* gkeys,
-                         const i64* gvals, const u64* gtab, i64* hub_pv) {
+                         const u32* __restrict__ hsm, const i32* __restrict__ part, const i32* __restrict__ dest,
+                         const i32* gkeys, const i64* gvals, const u64* gtab, i64* hub_pv) {
     const i64 hi = thread_item();
     if (hi >= m) return;
     const i32 i = hubs[hi];
     if (dest[i] == NO_MOVE) return;
     const u32 gmask = (u32)(goff[hi + 1] - goff[hi] - 1);
-    hub_pv[hi] = PACKED ? ptable_get(gtab + goff[hi], gmask, part[i])
-                        : table_get(gkeys + goff[hi], gvals + goff[hi], gmask, part[i]);
+    hub_pv[hi] = PACKED ? ptable_get(gtab + goff[hi], gmask, hsm[hi], part[i])
+                        : table_get(gkeys + goff[hi], gvals + goff[hi], gmask, hsm[hi], part[i]);
 }
 
 // Movers' edges into hubs: weight w of edge (i, u) leaves key part[i] and
 // joins key dest[i] in hub u's table (run before the moves are applied).
 // Same degree split as the uncut delta.  A failed update sets *stale.
 template <bool PACKED, class WT>
-__device__ __forceinline__ void hub_update_edge(const i32* col, WT W, const i32* hidx, const i64* goff, i32* gkeys,
-                                                i64* gvals, u64* gtab, i64 j, i32 oi, i32 ni, i64* stale) {
+__device__ __forceinline__ void hub_update_edge(const i32* col, WT W, const i32* hidx, const i64* goff,
+                                                const u32* hsm, i32* gkeys, i64* gvals, u64* gtab, i64 j, i32 oi,
+                                                i32 ni, i64* stale) {
     const i32 h = hidx[col[j]];
     if (h < 0) return;
     const i64 w = W(j);
     const u32 gmask = (u32)(goff[h + 1] - goff[h] - 1);
+    const u32 sm = hsm[h];
     bool ok;
     if (PACKED) {
-        ok = ptable_update(gtab + goff[h], gmask, oi, -w);
-        ok = ptable_update(gtab + goff[h], gmask, ni, w) && ok;
+        ok = ptable_update(gtab + goff[h], gmask, sm, oi, -w);
+        ok = ptable_update(gtab + goff[h], gmask, sm, ni, w) && ok;
     } else {
-        ok = table_update(gkeys + goff[h], gvals + goff[h], gmask, oi, -w);
-        ok = table_update(gkeys + goff[h], gvals + goff[h], gmask, ni, w) && ok;
+        ok = table_update(gkeys + goff[h], gvals + goff[h], gmask, sm, oi, -w);
+        ok = table_update(gkeys + goff[h], gvals + goff[h], gmask, sm, ni, w) && ok;
     }
     if (!ok) *stale = 1;
 }
@@ -642,7 +867,8 @@ __global__ void __launch_bounds__(256) k_hub_update_thread(const i64* __restrict
                                                            WT W, const i32* __restrict__ moves, i64 k,
                                                            const i32* __restrict__ part, const i32* __restrict__ dest,
                                                            const i32* __restrict__ hidx, const i64* __restrict__ goff,
-                                                           i32* gkeys, i64* gvals, u64* gtab, i64* stale) {
+                                                           const u32* __restrict__ hsm, i32* gkeys, i64* gvals,
+                                                           u64* gtab, i64* stale) {
     const i64 x = thread_item();
     if (x >= k) return;
     const i32 i = moves[x];
@@ -651,14 +877,15 @@ __global__ void __launch_bounds__(256) k_hub_update_thread(const i64* __restrict
     const i32 oi = part[i], ni = dest[i];
     if (ni < 0) return;  // not applied (k_apply_moves)
 #pragma unroll 4
-    for (i64 j = s; j < e; j++) hub_update_edge<PACKED>(col, W, hidx, goff, gkeys, gvals, gtab, j, oi, ni, stale);
+    for (i64 j = s; j < e; j++) hub_update_edge<PACKED>(col, W, hidx, goff, hsm, gkeys, gvals, gtab, j, oi, ni, stale);
 }
 template <bool PACKED, class WT>
 __global__ void __launch_bounds__(256) k_hub_update_wave(const i64* __restrict__ rp, const i32* __restrict__ col,
                                                          WT W, const i32* __restrict__ moves, i64 k, i64 nw,
                                                          const i32* __restrict__ part, const i32* __restrict__ dest,
                                                          const i32* __restrict__ hidx, const i64* __restrict__ goff,
-                                                         i32* gkeys, i64* gvals, u64* gtab, i64* stale) {
+                                                         const u32* __restrict__ hsm, i32* gkeys, i64* gvals,
+                                                         u64* gtab, i64* stale) {
     const int lane = threadIdx.x & 63;
     const i64 w = block_id() * 4 + (threadIdx.x >> 6);
     if (w >= nw) return;
@@ -669,7 +896,7 @@ __global__ void __launch_bounds__(256) k_hub_update_wave(const i64* __restrict__
     if (ni < 0) return;  // not applied (k_apply_moves)
     for (i64 base = s; base < e; base += 64) {  // wave-uniform loop control
         const i64 j = base + lane;
-        if (j < e) hub_update_edge<PACKED>(col, W, hidx, goff, gkeys, gvals, gtab, j, oi, ni, stale);
+        if (j < e) hub_update_edge<PACKED>(col, W, hidx, goff, hsm, gkeys, gvals, gtab, j, oi, ni, stale);
     }
 }
 // movers of degree >= MASSIVE_CUTOFF (the last nb entries of moves at most):
@@ -679,7 +906,8 @@ __global__ void __launch_bounds__(1024) k_hub_update_block(const i64* __restrict
                                                            WT W, const i32* __restrict__ moves, i64 k, i64 nb,
                                                            const i32* __restrict__ part, const i32* __restrict__ dest,
                                                            const i32* __restrict__ hidx, const i64* __restrict__ goff,
-                                                           i32* gkeys, i64* gvals, u64* gtab, i64* stale) {
+                                                           const u32* __restrict__ hsm, i32* gkeys, i64* gvals,
+                                                           u64* gtab, i64* stale) {
     const i64 w = block_id();
     if (w >= nb) return;
     const i32 i = moves[k - 1 - w];
@@ -688,7 +916,7 @@ __global__ void __launch_bounds__(1024) k_hub_update_block(const i64* __restrict
     const i32 oi = part[i], ni = dest[i];
     if (ni < 0) return;  // not applied (k_apply_moves)
     for (i64 j = s + threadIdx.x; j < e; j += 1024)
-        hub_update_edge<PACKED>(col, W, hidx, goff, gkeys, gvals, gtab, j, oi, ni, stale);
+        hub_update_edge<PACKED>(col, W, hidx, goff, hsm, gkeys, gvals, gtab, j, oi, ni, stale);
 }
 
 template <bool PACKED, class LT>
@@ -839,35 +1067,90 @@ static void launch_table_class(Ctx& c, const DGraph& g, WT W, const i32* L, i64 
     else launch_table<256, 4 * D, 4>(c, g, W, L, x0, x1, part, vol, cons, lam, ratio, dest, gain);
 }
 
-// hub table build over all edge items, with the LDS pre-aggregation sized
-// for the plan's edges per item
+// Hub tables of the current clustering: slices built in LDS, then the
+// fallback build of the hubs flagged (by a full slice or the hub_build knob).
+// fallback insert launch over n item slots (b0 + k, or imap[k]); grid
+// capped at cap blocks (grid-stride) when cap > 0
 template <class WT>
-static void hub_insert_all(Ctx& c, const DGraph& g, WT W, LevelPlan& lp, const i32* hubs, i64 m, const i32* part) {
-    const dim3 grid = grid_blocks(lp.igrid);
+static void hub_insert_items(Ctx& c, const DGraph& g, WT W, LevelPlan& lp, const i32* hubs, i64 m, const i32* part,
+                             const i32* imap, i64 b0, i64 n, i64 cap, i32 want, const i32* gate) {
+    if (n <= 0) return;
     i32* gk = lp.packed ? nullptr : lp.gkeys.p;
     i64* gv = lp.packed ? nullptr : lp.gvals.p;
     u64* gt = lp.packed ? lp.gtab.p : nullptr;
+    const dim3 grid = grid_blocks(cap > 0 ? std::min(n, cap) : n);
+#define PGGC_HUB_INSERT(PK, HL)                                                                                   \
+    k_hub_insert<PK, HL><<<grid, 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, imap, b0, n, lp.goff, part, \
+                                                     lp.hflag, want, gate, gk, gv, gt)
     if (lp.packed) {
-        if (lp.he == 512)
-            k_hub_insert<true, 1024><<<grid, 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid,
-                                                                lp.goff, part, gk, gv, gt, nullptr);
-        else if (lp.he == 2048)
-            k_hub_insert<true, 4096><<<grid, 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid,
-                                                                lp.goff, part, gk, gv, gt, nullptr);
-        else
-            k_hub_insert<true, 2048><<<grid, 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.imap, lp.igrid,
-                                                                lp.goff, part, gk, gv, gt, nullptr);
+        if (lp.he == 512) PGGC_HUB_INSERT(true, 1024);
+        else if (lp.he == 2048) PGGC_HUB_INSERT(true, 4096);
+        else PGGC_HUB_INSERT(true, 2048);
     } else {
-        if (lp.he == 512)
-            k_hub_insert<false, 1024><<<grid, 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.imap,
-                                                                 lp.igrid, lp.goff, part, gk, gv, gt, nullptr);
-        else if (lp.he == 2048)
-            k_hub_insert<false, 4096><<<grid, 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.imap,
-                                                                 lp.igrid, lp.goff, part, gk, gv, gt, nullptr);
-        else
-            k_hub_insert<false, 2048><<<grid, 256, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.items, lp.imap,
-                                                                 lp.igrid, lp.goff, part, gk, gv, gt, nullptr);
+        if (lp.he == 512) PGGC_HUB_INSERT(false, 1024);
+        else if (lp.he == 2048) PGGC_HUB_INSERT(false, 4096);
+        else PGGC_HUB_INSERT(false, 2048);
     }
+#undef PGGC_HUB_INSERT
+    PGGC_LAUNCH_CHECK();
+}
+
+template <class WT>
+static void hub_build_all(Ctx& c, const DGraph& g, WT W, LevelPlan& lp, const i32* hubs, i64 m, const i32* part) {
+    i32* gk = lp.packed ? nullptr : lp.gkeys.p;
+    i64* gv = lp.packed ? nullptr : lp.gvals.p;
+    u64* gt = lp.packed ? lp.gtab.p : nullptr;
+    const int mode = c.hub_build;
+    k_hub_sm_init<<<grid_items(m, 256), 256, 0, c.stream>>>(lp.goff, m, hub_slice(lp.packed), mode, lp.hg, lp.hsm,
+                                                            lp.hflag, lp.hovf);
+    // empty the tables the fallback builds from the start (before the slice
+    // builds, which write the other tables whole)
+    const i64 f0 = mode == 0 ? lp.ghg : 0;
+    if (lp.gslots > f0) {
+        if (lp.packed) {
+            fill_i64(c, (i64*)lp.gtab.p + f0, lp.gslots - f0, (i64)PEMPTY);
+        } else {
+            fill_i32(c, lp.gkeys.p + f0, lp.gslots - f0, EMPTY_KEY);
+            fill_i64(c, lp.gvals.p + f0, lp.gslots - f0, 0);
+        }
+    }
+    if (mode != 1) {
+        // single-slice tables of 4096 and (packed) 8192 slots in smaller
+        // blocks, so several run per CU; the multi-slice tables through the
+        // XCD map
+        const i64 n1 = lp.hs1, n2 = lp.hs2 - lp.hs1;
+#define PGGC_HUB_BUILD(PK, SL, BS, B0, MAP, N)                                                                    \
+    k_hub_build<PK, SL, BS><<<grid_blocks(N), BS, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.soff, B0, MAP, N, lp.goff, \
+                                                                 part, lp.hsm, lp.hflag, lp.hovf, gk, gv, gt)
+        if (lp.packed) {
+            if (n1 > 0) PGGC_HUB_BUILD(true, 4096, 256, 0, nullptr, n1);
+            if (n2 > 0) PGGC_HUB_BUILD(true, 8192, 512, lp.hs1, nullptr, n2);
+            if (lp.bgrid > 0) PGGC_HUB_BUILD(true, 16384, 1024, 0, lp.bmap.p, lp.bgrid);
+        } else {
+            if (n1 > 0) PGGC_HUB_BUILD(false, 4096, 256, 0, nullptr, n1);
+            if (lp.bgrid > 0) PGGC_HUB_BUILD(false, 8192, 512, 0, lp.bmap.p, lp.bgrid);
+        }
+#undef PGGC_HUB_BUILD
+        // hubs with a full slice (normally none: the gated kernels exit)
+        constexpr i64 FB_GRID = 4096;
+        const dim3 cgrid = grid_blocks(std::min(lp.nslices, FB_GRID));
+        if (lp.packed)
+            k_hub_clear<true><<<cgrid, 256, 0, c.stream>>>(m, lp.soff, lp.nslices, lp.goff, lp.hflag, lp.hovf, gk, gv,
+                                                           gt);
+        else
+            k_hub_clear<false><<<cgrid, 256, 0, c.stream>>>(m, lp.soff, lp.nslices, lp.goff, lp.hflag, lp.hovf, gk,
+                                                            gv, gt);
+        hub_insert_items(c, g, W, lp, hubs, m, part, lp.imap.p, 0, lp.igrid, FB_GRID, 2, lp.hovf.p);
+    }
+    // the hubs built by the fallback from the start: in mode 0 the items
+    // from the first of the largest hubs on, else all items through the map
+    if (mode == 0) {
+        if (lp.nitems > lp.ihg) hub_insert_items(c, g, W, lp, hubs, m, part, nullptr, lp.ihg, lp.nitems - lp.ihg, 0, 1,
+                                                 nullptr);
+    } else {
+        hub_insert_items(c, g, W, lp, hubs, m, part, lp.imap.p, 0, lp.igrid, 0, 1, nullptr);
+    }
+    PGGC_LAUNCH_CHECK();
 }
 
 template <class LT>
@@ -946,23 +1229,17 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
                                                                        lp.hub_best);
             } else {
                 if (!lp.hub_live) {  // first table sweep of the call (or stale tables): build
-                    if (lp.packed) {
-                        fill_i64(c, (i64*)lp.gtab.p, lp.gslots, (i64)PEMPTY);
-                    } else {
-                        fill_i32(c, lp.gkeys, lp.gslots, EMPTY_KEY);
-                        fill_i64(c, lp.gvals, lp.gslots, 0);
-                    }
-                    hub_insert_all(c, g, W, lp, hubs, m, part);
+                    hub_build_all(c, g, W, lp, hubs, m, part);
                     lp.hub_live = true;
                 }
                 if (lp.packed) {
-                    k_hub_pv<true><<<grid_items(m, 256), 256, 0, c.stream>>>(hubs, m, lp.goff, part, dest, nullptr,
+                    k_hub_pv<true><<<grid_items(m, 256), 256, 0, c.stream>>>(hubs, m, lp.goff, lp.hsm, part, dest, nullptr,
                                                                             nullptr, lp.gtab, lp.hub_pv);
                     k_hub_scan<true><<<grid_blocks(lp.sgrid), 256, 0, c.stream>>>(
                         hubs, m, lp.sitems, lp.smap, lp.sgrid, lp.goff, part, dest, g.vw, vol, cons, lam, ratio,
                         nullptr, nullptr, lp.gtab, lp.hub_pv, lp.hs, lp.hub_best);
                 } else {
-                    k_hub_pv<false><<<grid_items(m, 256), 256, 0, c.stream>>>(hubs, m, lp.goff, part, dest, lp.gkeys,
+                    k_hub_pv<false><<<grid_items(m, 256), 256, 0, c.stream>>>(hubs, m, lp.goff, lp.hsm, part, dest, lp.gkeys,
                                                                              lp.gvals, nullptr, lp.hub_pv);
                     k_hub_scan<false><<<grid_blocks(lp.sgrid), 256, 0, c.stream>>>(
                         hubs, m, lp.sitems, lp.smap, lp.sgrid, lp.goff, part, dest, g.vw, vol, cons, lam, ratio,
@@ -1409,25 +1686,25 @@ static void apply_moves_core(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph
         PGGC_WDISPATCH(g, {
             if (lp.packed) {
                 k_hub_update_thread<true><<<grid_items(k, 256), 256, 0, c.stream>>>(
-                    g.rp, g.col, W, moves, k, s.part, s.dest, lp.hidx, lp.goff, nullptr, nullptr, lp.gtab, stale);
+                    g.rp, g.col, W, moves, k, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, nullptr, nullptr, lp.gtab, stale);
                 if (nw > 0)
                     k_hub_update_wave<true><<<grid_items(nw, 4), 256, 0, c.stream>>>(
-                        g.rp, g.col, W, moves, k, nw, s.part, s.dest, lp.hidx, lp.goff, nullptr, nullptr, lp.gtab,
+                        g.rp, g.col, W, moves, k, nw, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, nullptr, nullptr, lp.gtab,
                         stale);
                 if (nb > 0)
                     k_hub_update_block<true><<<grid_blocks(nb), 1024, 0, c.stream>>>(
-                        g.rp, g.col, W, moves, k, nb, s.part, s.dest, lp.hidx, lp.goff, nullptr, nullptr, lp.gtab,
+                        g.rp, g.col, W, moves, k, nb, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, nullptr, nullptr, lp.gtab,
                         stale);
             } else {
                 k_hub_update_thread<false><<<grid_items(k, 256), 256, 0, c.stream>>>(
-                    g.rp, g.col, W, moves, k, s.part, s.dest, lp.hidx, lp.goff, lp.gkeys, lp.gvals, nullptr, stale);
+                    g.rp, g.col, W, moves, k, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, lp.gkeys, lp.gvals, nullptr, stale);
                 if (nw > 0)
                     k_hub_update_wave<false><<<grid_items(nw, 4), 256, 0, c.stream>>>(
-                        g.rp, g.col, W, moves, k, nw, s.part, s.dest, lp.hidx, lp.goff, lp.gkeys, lp.gvals, nullptr,
+                        g.rp, g.col, W, moves, k, nw, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, lp.gkeys, lp.gvals, nullptr,
                         stale);
                 if (nb > 0)
                     k_hub_update_block<false><<<grid_blocks(nb), 1024, 0, c.stream>>>(
-                        g.rp, g.col, W, moves, k, nb, s.part, s.dest, lp.hidx, lp.goff, lp.gkeys, lp.gvals, nullptr,
+                        g.rp, g.col, W, moves, k, nb, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, lp.gkeys, lp.gvals, nullptr,
                         stale);
             }
         });

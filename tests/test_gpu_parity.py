@@ -247,6 +247,28 @@ def test_full_clustering_rmat16(gpu_ctx, algo):
     assert q == rq
 
 
+@pytest.mark.parametrize("hub_build", [0, 1, 2])
+@pytest.mark.parametrize("algo,ex_iters", [("louvain", -1), ("leiden+", 0)])
+def test_wide_hub_tables(gpu_ctx, hub_build, algo, ex_iters):
+    """Edge weights of 2^20..2^21 on the hub graph: the hub's row weight
+    (about 3e10) does not fit the packed 32-bit table slots, so the hub
+    tables hold separate keys and 64-bit weights; a degree-20,000 hub has a
+    table of two LDS slices.  Every build method (LDS slices, edge items with
+    HBM atomics, a mix of both) gives the oracle's clustering."""
+    g = hub_graph(seed=11, weights=(1 << 20, 1 << 21))
+    vw = g.vertex_weights("WMod")
+    print(f"cluster hub30k wide {algo} hub_build={hub_build}", flush=True)
+    try:
+        gpu_ctx.set_knob("hub_build", hub_build)
+        q, part, info = C().cluster(dgraph(g, "WMod"), algo, "WMod", 1.0, ex_iters, seed=4)
+    finally:
+        gpu_ctx.set_knob("hub_build", 0)
+    rq, rpart, rinfo = O.cluster(g.rp, g.col, g.w, vw, "WMod", 1.0, algo, ex_iters, seed=4)
+    assert np.array_equal(npy(part), rpart)
+    assert q == rq
+    assert info["sweeps"] == rinfo["sweeps"]
+
+
 def test_determinism_rmat20(gpu_ctx):
     """Size-independent property at scale 20 (1M vertices, 33M nonzeros):
     two runs give the same clustering and objective, and the objective equals
