@@ -67,8 +67,10 @@ __device__ __forceinline__ u32 probe_next(u32 h, u32 smask) { return (h & ~smask
 // smask+1 probes; returns false when the key is neither found nor
 // insertable (slice full, or a decrement of a key the table does not hold),
 // which leaves the table stale.  A decrement never borrows across the packed
-// key: the slot holds at least the weight being moved out of it.
-__device__ __forceinline__ bool ptable_update(u64* tab, u32 mask, u32 smask, i32 key, i64 w) {
+// key: the slot holds at least the weight being moved out of it.  A key new
+// to the table appends its slot to the table's occupied-slot list
+// (list[*cnt], *cnt += 1).
+__device__ __forceinline__ bool ptable_update(u64* tab, u32 mask, u32 smask, i32 key, i64 w, u32* list, i32* cnt) {
     u32 h = slot_hash(key, mask);
     for (u32 probe = 0; probe <= smask; probe++) {
         const u64 cur = __hip_atomic_load(&tab[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -80,7 +82,10 @@ __device__ __forceinline__ bool ptable_update(u64* tab, u32 mask, u32 smask, i32
             if (w < 0) return false;
             const u64 old = atomicCAS((unsigned long long*)&tab[h], (unsigned long long)PEMPTY,
                                       (unsigned long long)(((u64)(u32)key << 32) | (u64)w));
-            if (old == PEMPTY) return true;
+            if (old == PEMPTY) {
+                list[atomicAdd(cnt, 1)] = h;
+                return true;
+            }
             if ((u32)(old >> 32) == (u32)key) {
                 atomicAdd((unsigned long long*)&tab[h], (unsigned long long)w);
                 return true;
@@ -90,7 +95,8 @@ __device__ __forceinline__ bool ptable_update(u64* tab, u32 mask, u32 smask, i32
     }
     return false;
 }
-__device__ __forceinline__ bool table_update(i32* keys, i64* vals, u32 mask, u32 smask, i32 key, i64 w) {
+__device__ __forceinline__ bool table_update(i32* keys, i64* vals, u32 mask, u32 smask, i32 key, i64 w, u32* list,
+                                             i32* cnt) {
     u32 h = slot_hash(key, mask);
     for (u32 probe = 0; probe <= smask; probe++) {
         const i32 cur = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -98,6 +104,7 @@ __device__ __forceinline__ bool table_update(i32* keys, i64* vals, u32 mask, u32
         if (!hit && cur == EMPTY_KEY) {
             if (w < 0) return false;
             const i32 old = atomicCAS(&keys[h], EMPTY_KEY, key);
+            if (old == EMPTY_KEY) list[atomicAdd(cnt, 1)] = h;
             hit = old == EMPTY_KEY || old == key;
         }
         if (hit) {

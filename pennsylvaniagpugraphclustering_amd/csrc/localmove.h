@@ -70,11 +70,15 @@ struct LevelPlan {
     Buf<u64> gtab;
     Buf<i32> imap, smap, bmap, hidx, hflag, hovf;
     Buf<u32> hsm;
+    // occupied-slot list per table (slot indices from goff[h] on, hcnt[h]
+    // of them, in no particular order): scans read occupied slots only
+    Buf<u32> hlist;
+    Buf<i32> hcnt;
     bool packed = false;
     bool hub_live = false;  // tables match the current clustering
     i64 gslots = 0, nitems = 0, nsitems = 0, nslices = 0, igrid = 0, sgrid = 0, bgrid = 0;
     i64 hs1 = 0, hs2 = 0;  // slice-class bounds (xcd_maps)
-    i64 hg = 0, ghg = 0, ihg = 0;  // first fallback-built hub, its first slot and build item
+    i64 hg = 0, ghg = 0, ihg = 0, shg = 0;  // first fallback-built hub, its first slot, build item and slice
     // the hub class is sorted by degree; its first nlds vertices (degree <=
     // Ctx::hub_lds_max) are scored in LDS, the rest (from hub0 on) in HBM
     // tables; he / hs: edges per build item and slots per scan item

@@ -209,6 +209,7 @@ static void xcd_maps(Ctx& c, LevelPlan& p, i64 m) {
     while (p.hg < m && go[p.hg + 1] - go[p.hg] <= HUB_MAX_SLICES * hub_slice(p.packed)) p.hg++;
     p.ghg = go[p.hg];
     p.ihg = it[p.hg];
+    p.shg = so[p.hg];
     std::vector<i32> bmap;
     build_map(so, xcd_of, m, bmap, p.bgrid, p.hs2, true);
     if (p.hs2 == so[m]) p.bgrid = 0;
@@ -304,6 +305,8 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
         p.nslices = c.read1(p.soff.p + m7);
         p.hflag = Buf<i32>(c, m7);
         p.hovf = Buf<i32>(c, 1);
+        p.hlist = Buf<u32>(c, p.gslots);
+        p.hcnt = Buf<i32>(c, m7);
         p.hsm = Buf<u32>(c, m7);
         xcd_maps(c, p, m7);
         p.hidx = Buf<i32>(c, std::max<i64>(g.n, 1));
@@ -632,10 +635,11 @@ __global__ void __launch_bounds__(256) k_hub_raw(const i64* __restrict__ rp, con
 // over the CUs).  hflag 2: a slice of the hub was full (k_hub_build), and
 // *hovf is set.
 __global__ void k_hub_sm_init(const i64* __restrict__ goff, i64 m, i64 sl, int mode, i64 hg, u32* hsm, i32* hflag,
-                              i32* hovf) {
+                              i32* hovf, i32* hcnt) {
     const i64 h = thread_item();
     if (h == 0) *hovf = 0;
     if (h >= m) return;
+    hcnt[h] = 0;
     const i64 S = goff[h + 1] - goff[h];
     const bool fb = mode == 1 || (mode == 2 && (h & 1)) || (mode == 0 && h >= hg);
     hflag[h] = fb ? 1 : 0;
@@ -688,12 +692,13 @@ __global__ void __launch_bounds__(BS) k_hub_build(const i64* __restrict__ rp, co
                                                   const i64* __restrict__ soff, i64 b0, const i32* __restrict__ bmap,
                                                   i64 nblk, const i64* __restrict__ goff,
                                                   const i32* __restrict__ part, u32* hsm, i32* hflag,
-                                                  i32* hovf, i32* gkeys, i64* gvals, u64* gtab) {
+                                                  i32* hovf, i32* gkeys, i64* gvals, u64* gtab, u32* hlist,
+                                                  i32* hcnt) {
     constexpr i64 SLMAX = PACKED ? HUB_SLICE_PACKED : HUB_SLICE_WIDE;
     __shared__ u64 lt[PACKED ? SL : 1];
     __shared__ i32 lk[PACKED ? 1 : SL];
     __shared__ i64 lv[PACKED ? 1 : SL];
-    __shared__ int full;
+    __shared__ int full, nocc, lbase;
     const i64 bb = block_id();
     if (bb >= nblk) return;
     const i64 b = bmap ? (i64)bmap[bb] : b0 + bb;
@@ -715,7 +720,10 @@ __global__ void __launch_bounds__(BS) k_hub_build(const i64* __restrict__ rp, co
             lv[q] = 0;
         }
     }
-    if (t == 0) full = 0;
+    if (t == 0) {
+        full = 0;
+        nocc = 0;
+    }
     __syncthreads();
     const i64 s = rp[i], e = rp[i + 1];
     bool ok = true;
@@ -750,14 +758,28 @@ __global__ void __launch_bounds__(BS) k_hub_build(const i64* __restrict__ rp, co
         return;
     }
     const i64 g0 = goff[hi] + base;
+    int mine = 0;
     for (u32 q = t; q <= smask; q += BS) {
         if (PACKED) {
             gtab[g0 + q] = lt[q];
+            mine += lt[q] != PEMPTY;
         } else {
             gkeys[g0 + q] = lk[q];
             gvals[g0 + q] = lv[q];
+            mine += lk[q] != EMPTY_KEY;
         }
     }
+    // the slice's occupied slots join the hub's list (in any order)
+    if (mine) atomicAdd(&nocc, mine);
+    __syncthreads();
+    if (t == 0) {
+        lbase = nocc ? atomicAdd(&hcnt[hi], nocc) : 0;
+        nocc = 0;
+    }
+    __syncthreads();
+    u32* lst = hlist + goff[hi] + lbase;
+    for (u32 q = t; q <= smask; q += BS)
+        if (PACKED ? lt[q] != PEMPTY : lk[q] != EMPTY_KEY) lst[atomicAdd(&nocc, 1)] = base + q;
 }
 
 // empty every slice of a hub flagged for the fallback build
@@ -766,12 +788,14 @@ __global__ void __launch_bounds__(BS) k_hub_build(const i64* __restrict__ rp, co
 template <bool PACKED>
 __global__ void __launch_bounds__(256) k_hub_clear(i64 m, const i64* __restrict__ soff, i64 nslices,
                                                    const i64* __restrict__ goff, const i32* __restrict__ hflag,
-                                                   const i32* __restrict__ hovf, i32* gkeys, i64* gvals, u64* gtab) {
+                                                   const i32* __restrict__ hovf, i32* gkeys, i64* gvals, u64* gtab,
+                                                   i32* hcnt) {
     if (*hovf == 0) return;
     const i64 nblk = (i64)gridDim.x * gridDim.y;
     for (i64 b = block_id(); b < nslices; b += nblk) {
         const i64 hi = upper_row(soff, 0, m, b);
         if (hflag[hi] != 2) continue;
+        if (b == soff[hi] && threadIdx.x == 0) hcnt[hi] = 0;  // relisted after the fallback build
         const i64 S = goff[hi + 1] - goff[hi];
         const i64 ns = soff[hi + 1] - soff[hi];
         const i64 len = S / ns;
@@ -784,6 +808,44 @@ __global__ void __launch_bounds__(256) k_hub_clear(i64 m, const i64* __restrict_
                 gvals[g0 + q] = 0;
             }
         }
+    }
+}
+
+// Occupied-slot lists of the tables the fallback built: the slices
+// b0 + k (grid-stride, n of them) of hubs with hflag == want; nothing to do
+// when gate is given and *gate is 0.
+template <bool PACKED>
+__global__ void __launch_bounds__(256) k_hub_relist(i64 m, const i64* __restrict__ soff, i64 b0, i64 n,
+                                                    const i64* __restrict__ goff, const i32* __restrict__ hflag,
+                                                    i32 want, const i32* __restrict__ gate,
+                                                    const i32* __restrict__ gkeys, const u64* __restrict__ gtab,
+                                                    u32* hlist, i32* hcnt) {
+    if (gate && *gate == 0) return;
+    __shared__ int nocc, lbase;
+    const i64 nblk = (i64)gridDim.x * gridDim.y;
+    for (i64 k = block_id(); k < n; k += nblk) {  // block-uniform loop control
+        const i64 b = b0 + k;
+        const i64 hi = upper_row(soff, 0, m, b);
+        if (hflag[hi] != want) continue;
+        const i64 S = goff[hi + 1] - goff[hi];
+        const i64 len = S / (soff[hi + 1] - soff[hi]);
+        const i64 s0 = (b - soff[hi]) * len;
+        const i64 g0 = goff[hi] + s0;
+        if (threadIdx.x == 0) nocc = 0;
+        __syncthreads();
+        int mine = 0;
+        for (i64 q = threadIdx.x; q < len; q += 256) mine += PACKED ? gtab[g0 + q] != PEMPTY : gkeys[g0 + q] != EMPTY_KEY;
+        if (mine) atomicAdd(&nocc, mine);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            lbase = nocc ? atomicAdd(&hcnt[hi], nocc) : 0;
+            nocc = 0;
+        }
+        __syncthreads();
+        u32* lst = hlist + goff[hi] + lbase;
+        for (i64 q = threadIdx.x; q < len; q += 256)
+            if (PACKED ? gtab[g0 + q] != PEMPTY : gkeys[g0 + q] != EMPTY_KEY) lst[atomicAdd(&nocc, 1)] = (u32)(s0 + q);
+        __syncthreads();
     }
 }
 
@@ -845,20 +907,21 @@ __global__ void k_hub_pv(const i32* __restrict__ hubs, i64 m, const i64* __restr
 // Same degree split as the uncut delta.  A failed update sets *stale.
 template <bool PACKED, class WT>
 __device__ __forceinline__ void hub_update_edge(const i32* col, WT W, const i32* hidx, const i64* goff,
-                                                const u32* hsm, i32* gkeys, i64* gvals, u64* gtab, i64 j, i32 oi,
-                                                i32 ni, i64* stale) {
+                                                const u32* hsm, i32* gkeys, i64* gvals, u64* gtab, u32* hlist,
+                                                i32* hcnt, i64 j, i32 oi, i32 ni, i64* stale) {
     const i32 h = hidx[col[j]];
     if (h < 0) return;
     const i64 w = W(j);
-    const u32 gmask = (u32)(goff[h + 1] - goff[h] - 1);
+    const i64 g = goff[h];
+    const u32 gmask = (u32)(goff[h + 1] - g - 1);
     const u32 sm = hsm[h];
     bool ok;
     if (PACKED) {
-        ok = ptable_update(gtab + goff[h], gmask, sm, oi, -w);
-        ok = ptable_update(gtab + goff[h], gmask, sm, ni, w) && ok;
+        ok = ptable_update(gtab + g, gmask, sm, oi, -w, hlist + g, hcnt + h);
+        ok = ptable_update(gtab + g, gmask, sm, ni, w, hlist + g, hcnt + h) && ok;
     } else {
-        ok = table_update(gkeys + goff[h], gvals + goff[h], gmask, sm, oi, -w);
-        ok = table_update(gkeys + goff[h], gvals + goff[h], gmask, sm, ni, w) && ok;
+        ok = table_update(gkeys + g, gvals + g, gmask, sm, oi, -w, hlist + g, hcnt + h);
+        ok = table_update(gkeys + g, gvals + g, gmask, sm, ni, w, hlist + g, hcnt + h) && ok;
     }
     if (!ok) *stale = 1;
 }
@@ -868,7 +931,7 @@ __global__ void __launch_bounds__(256) k_hub_update_thread(const i64* __restrict
                                                            const i32* __restrict__ part, const i32* __restrict__ dest,
                                                            const i32* __restrict__ hidx, const i64* __restrict__ goff,
                                                            const u32* __restrict__ hsm, i32* gkeys, i64* gvals,
-                                                           u64* gtab, i64* stale) {
+                                                           u64* gtab, u32* hlist, i32* hcnt, i64* stale) {
     const i64 x = thread_item();
     if (x >= k) return;
     const i32 i = moves[x];
@@ -877,7 +940,7 @@ __global__ void __launch_bounds__(256) k_hub_update_thread(const i64* __restrict
     const i32 oi = part[i], ni = dest[i];
     if (ni < 0) return;  // not applied (k_apply_moves)
 #pragma unroll 4
-    for (i64 j = s; j < e; j++) hub_update_edge<PACKED>(col, W, hidx, goff, hsm, gkeys, gvals, gtab, j, oi, ni, stale);
+    for (i64 j = s; j < e; j++) hub_update_edge<PACKED>(col, W, hidx, goff, hsm, gkeys, gvals, gtab, hlist, hcnt, j, oi, ni, stale);
 }
 template <bool PACKED, class WT>
 __global__ void __launch_bounds__(256) k_hub_update_wave(const i64* __restrict__ rp, const i32* __restrict__ col,
@@ -885,7 +948,7 @@ __global__ void __launch_bounds__(256) k_hub_update_wave(const i64* __restrict__
                                                          const i32* __restrict__ part, const i32* __restrict__ dest,
                                                          const i32* __restrict__ hidx, const i64* __restrict__ goff,
                                                          const u32* __restrict__ hsm, i32* gkeys, i64* gvals,
-                                                         u64* gtab, i64* stale) {
+                                                         u64* gtab, u32* hlist, i32* hcnt, i64* stale) {
     const int lane = threadIdx.x & 63;
     const i64 w = block_id() * 4 + (threadIdx.x >> 6);
     if (w >= nw) return;
@@ -896,7 +959,7 @@ __global__ void __launch_bounds__(256) k_hub_update_wave(const i64* __restrict__
     if (ni < 0) return;  // not applied (k_apply_moves)
     for (i64 base = s; base < e; base += 64) {  // wave-uniform loop control
         const i64 j = base + lane;
-        if (j < e) hub_update_edge<PACKED>(col, W, hidx, goff, hsm, gkeys, gvals, gtab, j, oi, ni, stale);
+        if (j < e) hub_update_edge<PACKED>(col, W, hidx, goff, hsm, gkeys, gvals, gtab, hlist, hcnt, j, oi, ni, stale);
     }
 }
 // movers of degree >= MASSIVE_CUTOFF (the last nb entries of moves at most):
@@ -907,7 +970,7 @@ __global__ void __launch_bounds__(1024) k_hub_update_block(const i64* __restrict
                                                            const i32* __restrict__ part, const i32* __restrict__ dest,
                                                            const i32* __restrict__ hidx, const i64* __restrict__ goff,
                                                            const u32* __restrict__ hsm, i32* gkeys, i64* gvals,
-                                                           u64* gtab, i64* stale) {
+                                                           u64* gtab, u32* hlist, i32* hcnt, i64* stale) {
     const i64 w = block_id();
     if (w >= nb) return;
     const i32 i = moves[k - 1 - w];
@@ -916,9 +979,11 @@ __global__ void __launch_bounds__(1024) k_hub_update_block(const i64* __restrict
     const i32 oi = part[i], ni = dest[i];
     if (ni < 0) return;  // not applied (k_apply_moves)
     for (i64 j = s + threadIdx.x; j < e; j += 1024)
-        hub_update_edge<PACKED>(col, W, hidx, goff, hsm, gkeys, gvals, gtab, j, oi, ni, stale);
+        hub_update_edge<PACKED>(col, W, hidx, goff, hsm, gkeys, gvals, gtab, hlist, hcnt, j, oi, ni, stale);
 }
 
+// Scan items cover hs entries of a hub's occupied-slot list (the grid is
+// sized for full tables; items past the list's end exit at once).
 template <bool PACKED, class LT>
 __global__ void __launch_bounds__(256) k_hub_scan(const i32* __restrict__ hubs, i64 m, const i64* __restrict__ sitems,
                                                   const i32* __restrict__ smap, i64 nmap,
@@ -926,34 +991,37 @@ __global__ void __launch_bounds__(256) k_hub_scan(const i32* __restrict__ hubs, 
                                                   const i32* __restrict__ dest, const i64* __restrict__ vw,
                                                   const i64* __restrict__ vol, const i32* __restrict__ cons, LT lam,
                                                   float ratio, const i32* gkeys, const i64* gvals, const u64* gtab,
+                                                  const u32* __restrict__ hlist, const i32* __restrict__ hcnt,
                                                   const i64* __restrict__ hub_pv, i64 hs, u64* hub_best) {
     const i64 bb = block_id();
     if (bb >= nmap) return;
     const i64 b = smap[bb];
     if (b < 0) return;
     const i64 hi = upper_row(sitems, 0, m, b);
+    const i64 k_lo = (b - sitems[hi]) * hs;
+    const i64 cnt = hcnt[hi];
+    if (k_lo >= cnt) return;
     const i32 i = hubs[hi];
     if (dest[i] == NO_MOVE) return;
-    const i64 s_lo = goff[hi] + (b - sitems[hi]) * hs;
-    const i64 s_end = goff[hi + 1];
-    const i64 s_hi = s_lo + hs < s_end ? s_lo + hs : s_end;
+    const i64 k_hi = k_lo + hs < cnt ? k_lo + hs : cnt;
+    const i64 g0 = goff[hi];
+    const u32* lst = hlist + g0;
     const i32 p = part[i];
     float multi, p_conn, bth;
     own_terms(hub_pv[hi], vol[p], vw[i], lam(i), lam.rt(i, ratio), multi, p_conn, bth);
     u64 bk = 0;
-    for (i64 qb = s_lo; qb < s_hi; qb += 256) {  // block-uniform loop control
-        const i64 q = qb + threadIdx.x;
-        if (q >= s_hi) continue;
+    for (i64 kb = k_lo; kb < k_hi; kb += 256) {  // block-uniform loop control
+        const i64 k = kb + threadIdx.x;
+        if (k >= k_hi) continue;
+        const i64 q = g0 + lst[k];
         i32 key;
         i64 agg;
         if (PACKED) {
             const u64 e = gtab[q];
-            if (e == PEMPTY) continue;
             key = (i32)(e >> 32);
             agg = (i64)(e & 0xffffffffull);
         } else {
             key = gkeys[q];
-            if (key == EMPTY_KEY) continue;
             agg = gvals[q];
         }
         if (key != p && agg > 0 && (float)agg >= bth && (cons == nullptr || cons[key] == cons[p])) {
@@ -1069,6 +1137,19 @@ static void launch_table_class(Ctx& c, const DGraph& g, WT W, const i32* L, i64 
 
 // Hub tables of the current clustering: slices built in LDS, then the
 // fallback build of the hubs flagged (by a full slice or the hub_build knob).
+// occupied-slot lists of the fallback-built tables (slices b0 .. b0+n-1)
+static void hub_relist(Ctx& c, LevelPlan& lp, i64 m, i64 b0, i64 n, i64 cap, i32 want, const i32* gate) {
+    if (n <= 0) return;
+    const dim3 grid = grid_blocks(cap > 0 ? std::min(n, cap) : n);
+    if (lp.packed)
+        k_hub_relist<true><<<grid, 256, 0, c.stream>>>(m, lp.soff, b0, n, lp.goff, lp.hflag, want, gate, nullptr,
+                                                       lp.gtab, lp.hlist, lp.hcnt);
+    else
+        k_hub_relist<false><<<grid, 256, 0, c.stream>>>(m, lp.soff, b0, n, lp.goff, lp.hflag, want, gate, lp.gkeys,
+                                                        nullptr, lp.hlist, lp.hcnt);
+    PGGC_LAUNCH_CHECK();
+}
+
 // fallback insert launch over n item slots (b0 + k, or imap[k]); grid
 // capped at cap blocks (grid-stride) when cap > 0
 template <class WT>
@@ -1102,7 +1183,7 @@ static void hub_build_all(Ctx& c, const DGraph& g, WT W, LevelPlan& lp, const i3
     u64* gt = lp.packed ? lp.gtab.p : nullptr;
     const int mode = c.hub_build;
     k_hub_sm_init<<<grid_items(m, 256), 256, 0, c.stream>>>(lp.goff, m, hub_slice(lp.packed), mode, lp.hg, lp.hsm,
-                                                            lp.hflag, lp.hovf);
+                                                            lp.hflag, lp.hovf, lp.hcnt);
     // empty the tables the fallback builds from the start (before the slice
     // builds, which write the other tables whole)
     const i64 f0 = mode == 0 ? lp.ghg : 0;
@@ -1121,7 +1202,8 @@ static void hub_build_all(Ctx& c, const DGraph& g, WT W, LevelPlan& lp, const i3
         const i64 n1 = lp.hs1, n2 = lp.hs2 - lp.hs1;
 #define PGGC_HUB_BUILD(PK, SL, BS, B0, MAP, N)                                                                    \
     k_hub_build<PK, SL, BS><<<grid_blocks(N), BS, 0, c.stream>>>(g.rp, g.col, W, hubs, m, lp.soff, B0, MAP, N, lp.goff, \
-                                                                 part, lp.hsm, lp.hflag, lp.hovf, gk, gv, gt)
+                                                                 part, lp.hsm, lp.hflag, lp.hovf, gk, gv, gt, lp.hlist, \
+                                                                 lp.hcnt)
         if (lp.packed) {
             if (n1 > 0) PGGC_HUB_BUILD(true, 4096, 256, 0, nullptr, n1);
             if (n2 > 0) PGGC_HUB_BUILD(true, 8192, 512, lp.hs1, nullptr, n2);
@@ -1136,19 +1218,23 @@ static void hub_build_all(Ctx& c, const DGraph& g, WT W, LevelPlan& lp, const i3
         const dim3 cgrid = grid_blocks(std::min(lp.nslices, FB_GRID));
         if (lp.packed)
             k_hub_clear<true><<<cgrid, 256, 0, c.stream>>>(m, lp.soff, lp.nslices, lp.goff, lp.hflag, lp.hovf, gk, gv,
-                                                           gt);
+                                                           gt, lp.hcnt);
         else
             k_hub_clear<false><<<cgrid, 256, 0, c.stream>>>(m, lp.soff, lp.nslices, lp.goff, lp.hflag, lp.hovf, gk,
-                                                            gv, gt);
+                                                            gv, gt, lp.hcnt);
         hub_insert_items(c, g, W, lp, hubs, m, part, lp.imap.p, 0, lp.igrid, FB_GRID, 2, lp.hovf.p);
+        hub_relist(c, lp, m, 0, lp.nslices, FB_GRID, 2, lp.hovf.p);
     }
     // the hubs built by the fallback from the start: in mode 0 the items
     // from the first of the largest hubs on, else all items through the map
     if (mode == 0) {
-        if (lp.nitems > lp.ihg) hub_insert_items(c, g, W, lp, hubs, m, part, nullptr, lp.ihg, lp.nitems - lp.ihg, 0, 1,
-                                                 nullptr);
+        if (lp.nitems > lp.ihg) {
+            hub_insert_items(c, g, W, lp, hubs, m, part, nullptr, lp.ihg, lp.nitems - lp.ihg, 0, 1, nullptr);
+            hub_relist(c, lp, m, lp.shg, lp.nslices - lp.shg, 0, 1, nullptr);
+        }
     } else {
         hub_insert_items(c, g, W, lp, hubs, m, part, lp.imap.p, 0, lp.igrid, 0, 1, nullptr);
+        hub_relist(c, lp, m, 0, lp.nslices, 0, 1, nullptr);
     }
     PGGC_LAUNCH_CHECK();
 }
@@ -1237,13 +1323,13 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
                                                                             nullptr, lp.gtab, lp.hub_pv);
                     k_hub_scan<true><<<grid_blocks(lp.sgrid), 256, 0, c.stream>>>(
                         hubs, m, lp.sitems, lp.smap, lp.sgrid, lp.goff, part, dest, g.vw, vol, cons, lam, ratio,
-                        nullptr, nullptr, lp.gtab, lp.hub_pv, lp.hs, lp.hub_best);
+                        nullptr, nullptr, lp.gtab, lp.hlist, lp.hcnt, lp.hub_pv, lp.hs, lp.hub_best);
                 } else {
                     k_hub_pv<false><<<grid_items(m, 256), 256, 0, c.stream>>>(hubs, m, lp.goff, lp.hsm, part, dest, lp.gkeys,
                                                                              lp.gvals, nullptr, lp.hub_pv);
                     k_hub_scan<false><<<grid_blocks(lp.sgrid), 256, 0, c.stream>>>(
                         hubs, m, lp.sitems, lp.smap, lp.sgrid, lp.goff, part, dest, g.vw, vol, cons, lam, ratio,
-                        lp.gkeys, lp.gvals, nullptr, lp.hub_pv, lp.hs, lp.hub_best);
+                        lp.gkeys, lp.gvals, nullptr, lp.hlist, lp.hcnt, lp.hub_pv, lp.hs, lp.hub_best);
                 }
             }
             k_hub_decide<<<grid_items(m, 256), 256, 0, c.stream>>>(hubs, m, part, g.vw, vol, lam, ratio, lp.hub_pv,
@@ -1686,25 +1772,25 @@ static void apply_moves_core(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph
         PGGC_WDISPATCH(g, {
             if (lp.packed) {
                 k_hub_update_thread<true><<<grid_items(k, 256), 256, 0, c.stream>>>(
-                    g.rp, g.col, W, moves, k, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, nullptr, nullptr, lp.gtab, stale);
+                    g.rp, g.col, W, moves, k, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, nullptr, nullptr, lp.gtab, lp.hlist, lp.hcnt, stale);
                 if (nw > 0)
                     k_hub_update_wave<true><<<grid_items(nw, 4), 256, 0, c.stream>>>(
-                        g.rp, g.col, W, moves, k, nw, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, nullptr, nullptr, lp.gtab,
+                        g.rp, g.col, W, moves, k, nw, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, nullptr, nullptr, lp.gtab, lp.hlist, lp.hcnt,
                         stale);
                 if (nb > 0)
                     k_hub_update_block<true><<<grid_blocks(nb), 1024, 0, c.stream>>>(
-                        g.rp, g.col, W, moves, k, nb, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, nullptr, nullptr, lp.gtab,
+                        g.rp, g.col, W, moves, k, nb, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, nullptr, nullptr, lp.gtab, lp.hlist, lp.hcnt,
                         stale);
             } else {
                 k_hub_update_thread<false><<<grid_items(k, 256), 256, 0, c.stream>>>(
-                    g.rp, g.col, W, moves, k, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, lp.gkeys, lp.gvals, nullptr, stale);
+                    g.rp, g.col, W, moves, k, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, lp.gkeys, lp.gvals, nullptr, lp.hlist, lp.hcnt, stale);
                 if (nw > 0)
                     k_hub_update_wave<false><<<grid_items(nw, 4), 256, 0, c.stream>>>(
-                        g.rp, g.col, W, moves, k, nw, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, lp.gkeys, lp.gvals, nullptr,
+                        g.rp, g.col, W, moves, k, nw, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, lp.gkeys, lp.gvals, nullptr, lp.hlist, lp.hcnt,
                         stale);
                 if (nb > 0)
                     k_hub_update_block<false><<<grid_blocks(nb), 1024, 0, c.stream>>>(
-                        g.rp, g.col, W, moves, k, nb, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, lp.gkeys, lp.gvals, nullptr,
+                        g.rp, g.col, W, moves, k, nb, s.part, s.dest, lp.hidx, lp.goff, lp.hsm, lp.gkeys, lp.gvals, nullptr, lp.hlist, lp.hcnt,
                         stale);
             }
         });
