@@ -161,8 +161,10 @@ int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user);
  *     from singletons on the raw adjacency (src/local_mover.cpp:1391-1395),
  *     0 with the neighbour-cluster tables (the same result, more work).
  *   PGGC_KNOB_HUB_BUILD: how HBM hub tables are built: 0 (default) one LDS
- *     slice of a table per block, written out whole; 1 edge work items
- *     adding into the HBM table with atomics (the same tables' contents). */
+ *     slice of a table per block, written out whole (the largest tables by
+ *     edge work items); 1 every table by edge work items adding into the HBM
+ *     table with atomics; 2 the tables of odd hub index by edge items, the
+ *     rest by slices (the same tables' contents in every mode). */
 #define PGGC_KNOB_HUB_UPDATE_DIV 0
 #define PGGC_KNOB_GNN_MAX_SWEEPS 1
 #define PGGC_KNOB_TABLE_THREADS 2

@@ -622,7 +622,7 @@ static void set_knob(Ctx& c, int knob, int64_t value) {
         case PGGC_KNOB_HUB_EDGES_PER_ITEM: c.hub_edges_item = value; break;
         case PGGC_KNOB_HUB_SLOTS_PER_ITEM: c.hub_slots_item = value; break;
         case PGGC_KNOB_RAW_SWEEPS: c.raw_sweeps = value ? 1 : 0; break;
-        case PGGC_KNOB_HUB_BUILD: c.hub_build = value ? 1 : 0; break;
+        case PGGC_KNOB_HUB_BUILD: c.hub_build = (int)std::min<int64_t>(value, 2); break;
         default: throw Error(PGGC_EINVAL, "unknown knob");
     }
     if (shape) c.knob_epoch++;  // level plans depend on these
