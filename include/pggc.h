@@ -329,11 +329,15 @@ int pggc_cluster(pggc_ctx* ctx, const pggc_graph* g, int algo, int objective, do
 /* Batched runs: `count` independent pggc_cluster calls (different graphs, or
  * seeded trials of one graph as in run_clustering's trial loop,
  * app/cluster_main.cpp:189-204, or meme population members) in one call.
- * Up to `streams` worker contexts, each with a host thread of its own and
- * all on the calling context's stream, take instances in order: the host
- * work of small instances (launches, readbacks) overlaps while their kernels
- * run in submission order.  Each instance gives exactly what pggc_cluster
- * gives for it alone.  graphs[count] (host array of device graphs), seeds[count]
+ * Each instance gives exactly what pggc_cluster gives for it alone.
+ * pLouvain with ex_iters 0 and the predictor off runs the instances as one
+ * disjoint union graph: every kernel is launched once over all instances,
+ * and each instance keeps its own lambda, sweep schedule, stopping rules,
+ * best state and labels (`streams` is unused).  Other algorithms use up to
+ * `streams` worker contexts, each with a host thread of its own and all on
+ * the calling context's stream, taking instances in order: the host work of
+ * small instances (launches, readbacks) overlaps while their kernels run in
+ * submission order.  graphs[count] (host array of device graphs), seeds[count]
  * (host, nullable: instance index), parts_out[count] (host array of device
  * pointers), objectives_out[count] and info_out[8*count] (host, nullable).
  * The call first waits for the context's stream, so inputs produced on it
