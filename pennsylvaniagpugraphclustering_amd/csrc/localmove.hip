@@ -1680,16 +1680,33 @@ __global__ void __launch_bounds__(1024) k_delta_uncut_block(const i64* __restric
     if (threadIdx.x == 0) out[x] = acc;
 }
 
-__global__ void k_apply_moves(const i32* moves, i64 k, const i64* vw, i32* part, const i32* dest, i64* vol) {
+// The volume changes of a block's 256 movers are merged per cluster in an
+// LDS table first (many movers of a sweep leave or join the same clusters),
+// then each distinct cluster gets one HBM atomic; the sums are exact
+// integers, so the order does not matter.
+constexpr int AM_SLOTS = 1024;  // >= 2 keys per mover of a 256-thread block, at most half full
+__global__ void __launch_bounds__(256) k_apply_moves(const i32* moves, i64 k, const i64* vw, i32* part,
+                                                     const i32* dest, i64* vol) {
+    __shared__ i32 lk[AM_SLOTS];
+    __shared__ i64 lv[AM_SLOTS];
+    lds_agg_clear<256, AM_SLOTS>(lk, lv);
     const i64 x = thread_item();
-    if (x >= k) return;
-    const i32 i = moves[x];
-    const i32 p = part[i], b = dest[i];
-    const i64 w = vw[i];
-    if (b < 0) return;  // unresolved NEW_PART cannot reach here (p_conn is 0 for singletons)
-    atomicAdd((unsigned long long*)&vol[p], (unsigned long long)(-w));
-    atomicAdd((unsigned long long*)&vol[b], (unsigned long long)w);
-    part[i] = b;
+    if (x < k) {
+        const i32 i = moves[x];
+        const i32 p = part[i], b = dest[i];
+        const i64 w = vw[i];
+        if (b >= 0) {  // unresolved NEW_PART cannot reach here (p_conn is 0 for singletons)
+            table_add_lds(lk, lv, AM_SLOTS - 1, p, -w);
+            table_add_lds(lk, lv, AM_SLOTS - 1, b, w);
+            part[i] = b;
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < AM_SLOTS; q += 256) {
+        const i32 key = lk[q];
+        const i64 v = lv[q];
+        if (key != EMPTY_KEY && v != 0) atomicAdd((unsigned long long*)&vol[key], (unsigned long long)v);
+    }
 }
 
 // dest := NULL_PART for the movers and their neighbours (src/local_mover.cpp:717,798),
