@@ -16,6 +16,7 @@
 #include "device.h"
 #include "gen.h"
 #include "gnn.h"
+#include "hash.h"
 #include "leiden.h"
 #include "localmove.h"
 #include "meme.h"
@@ -56,10 +57,30 @@ __global__ void k_project(i64 n, const i32* coarse, i32* fine) {
     if (x >= n) return;
     fine[x] = coarse[fine[x]];
 }
-__global__ void k_down_winner(i64 n, const i32* map, i32* win) {
+// highest fine id per coarse id; a block's 256 vertices are merged per coarse
+// id in LDS first, then each distinct id gets one HBM atomicMax
+constexpr int DW_SLOTS = 512;
+__global__ void __launch_bounds__(256) k_down_winner(i64 n, const i32* map, i32* win) {
+    __shared__ i32 lk[DW_SLOTS], lm[DW_SLOTS];
+    for (int q = threadIdx.x; q < DW_SLOTS; q += 256) {
+        lk[q] = EMPTY_KEY;
+        lm[q] = -1;
+    }
+    __syncthreads();
     const i64 i = thread_item();
-    if (i >= n) return;
-    atomicMax(&win[map[i]], (i32)i);
+    if (i < n) {
+        const i32 u = map[i];
+        u32 h = slot_hash(u, DW_SLOTS - 1);
+        while (true) {  // at most 256 keys in 512 slots
+            const i32 old = atomicCAS(&lk[h], EMPTY_KEY, u);
+            if (old == EMPTY_KEY || old == u) break;
+            h = (h + 1) & (DW_SLOTS - 1);
+        }
+        atomicMax(&lm[h], (i32)i);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < DW_SLOTS; q += 256)
+        if (lk[q] != EMPTY_KEY) atomicMax(&win[lk[q]], lm[q]);
 }
 __global__ void k_down_gather(i64 nc, const i32* win, const i32* in, i32* out) {
     const i64 c = thread_item();

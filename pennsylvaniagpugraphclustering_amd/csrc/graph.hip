@@ -241,20 +241,71 @@ void coarsen_vertex_weights(Ctx& c, const DGraph& g, const i32* map, i64 nc, i64
 }
 
 // ---------------------------------------------------------------- contraction
-__global__ void k_ccount(const i64* rp, const i32* vcmap, i64 n, i64* cnt, i64* bound) {
+// members and summed degrees per coarse vertex; a block's 256 vertices are
+// merged per coarse id in LDS first (large clusters take many members of
+// every block), then each distinct id gets one pair of HBM atomics
+constexpr int CC_SLOTS = 512;
+__global__ void __launch_bounds__(256) k_ccount(const i64* rp, const i32* vcmap, i64 n, i64* cnt, i64* bound) {
+    __shared__ i32 lk[CC_SLOTS];
+    __shared__ i64 lc[CC_SLOTS], lb[CC_SLOTS];
+    for (int q = threadIdx.x; q < CC_SLOTS; q += 256) {
+        lk[q] = EMPTY_KEY;
+        lc[q] = 0;
+        lb[q] = 0;
+    }
+    __syncthreads();
     const i64 i = thread_item();
-    if (i >= n) return;
-    i32 u = vcmap[i];
-    atomicAdd((unsigned long long*)&cnt[u], 1ull);
-    atomicAdd((unsigned long long*)&bound[u], (unsigned long long)(rp[i + 1] - rp[i]));
+    if (i < n) {
+        const i32 u = vcmap[i];
+        const i64 d = rp[i + 1] - rp[i];
+        u32 h = slot_hash(u, CC_SLOTS - 1);
+        while (true) {  // at most 256 keys in 512 slots: an empty slot is always found
+            const i32 old = atomicCAS(&lk[h], EMPTY_KEY, u);
+            if (old == EMPTY_KEY || old == u) break;
+            h = (h + 1) & (CC_SLOTS - 1);
+        }
+        atomicAdd((unsigned long long*)&lc[h], 1ull);
+        atomicAdd((unsigned long long*)&lb[h], (unsigned long long)d);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < CC_SLOTS; q += 256) {
+        const i32 u = lk[q];
+        if (u == EMPTY_KEY) continue;
+        atomicAdd((unsigned long long*)&cnt[u], (unsigned long long)lc[q]);
+        atomicAdd((unsigned long long*)&bound[u], (unsigned long long)lb[q]);
+    }
 }
 
-__global__ void k_cscatter(const i32* vcmap, i64 n, i64* pos, i32* members) {
+// member lists: a block's vertices take ranks per coarse id in LDS, each
+// distinct id reserves its run of the list with one HBM atomic, and every
+// vertex writes at run start + rank (the order inside a list is free: the
+// merges that read it are order-independent)
+__global__ void __launch_bounds__(256) k_cscatter(const i32* vcmap, i64 n, i64* pos, i32* members) {
+    __shared__ i32 lk[CC_SLOTS], lc[CC_SLOTS];
+    __shared__ i64 lb[CC_SLOTS];
+    for (int q = threadIdx.x; q < CC_SLOTS; q += 256) {
+        lk[q] = EMPTY_KEY;
+        lc[q] = 0;
+    }
+    __syncthreads();
     const i64 i = thread_item();
-    if (i >= n) return;
-    i32 u = vcmap[i];
-    i64 k = (i64)atomicAdd((unsigned long long*)&pos[u], 1ull);
-    members[k] = (i32)i;
+    u32 h = 0;
+    i32 r = 0;
+    if (i < n) {
+        const i32 u = vcmap[i];
+        h = slot_hash(u, CC_SLOTS - 1);
+        while (true) {  // at most 256 keys in 512 slots
+            const i32 old = atomicCAS(&lk[h], EMPTY_KEY, u);
+            if (old == EMPTY_KEY || old == u) break;
+            h = (h + 1) & (CC_SLOTS - 1);
+        }
+        r = atomicAdd(&lc[h], 1);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < CC_SLOTS; q += 256)
+        if (lk[q] != EMPTY_KEY) lb[q] = (i64)atomicAdd((unsigned long long*)&pos[lk[q]], (unsigned long long)lc[q]);
+    __syncthreads();
+    if (i < n) members[lb[h] + r] = (i32)i;
 }
 
 constexpr i64 C_SMALL = 256;   // bound <= C_SMALL: 64-thread block, 512-slot LDS table
