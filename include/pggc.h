@@ -164,7 +164,12 @@ int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user);
  *     slice of a table per block, written out whole (the largest tables by
  *     edge work items); 1 every table by edge work items adding into the HBM
  *     table with atomics; 2 the tables of odd hub index by edge items, the
- *     rest by slices (the same tables' contents in every mode). */
+ *     rest by slices (the same tables' contents in every mode).
+ *   PGGC_KNOB_RENUMBER: 1 (default) runs the local mover of the top level
+ *     on an internal copy of the graph with vertices in descending degree
+ *     order (rows keep their entry order; labels, priorities and new cluster
+ *     ids follow the caller's ids, so results are identical), 0 on the
+ *     caller's graph.  Not used with the active-set predictor on. */
 #define PGGC_KNOB_HUB_UPDATE_DIV 0
 #define PGGC_KNOB_GNN_MAX_SWEEPS 1
 #define PGGC_KNOB_TABLE_THREADS 2
@@ -174,7 +179,8 @@ int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user);
 #define PGGC_KNOB_HUB_SLOTS_PER_ITEM 6
 #define PGGC_KNOB_RAW_SWEEPS 7
 #define PGGC_KNOB_HUB_BUILD 8
-#define PGGC_KNOB_COUNT 9
+#define PGGC_KNOB_RENUMBER 9
+#define PGGC_KNOB_COUNT 10
 int pggc_ctx_set_knob(pggc_ctx* ctx, int knob, int64_t value);
 int pggc_ctx_get_knob(pggc_ctx* ctx, int knob, int64_t* value);
 /* Autotuner of the execution-shape knobs (no counterpart in the reference):

@@ -22,6 +22,7 @@
 #include "meme.h"
 #include "noalloc.h"
 #include "pggc.h"
+#include "renum.h"
 
 using namespace pggc;
 
@@ -152,9 +153,25 @@ static Level owned_level(std::unique_ptr<OwnedGraph> og) {
 // consecutive non-contracting refinements tolerated at one level
 constexpr int LEIDEN_MAX_STALLS = 8;
 
+// The top level renumbered by degree for the local mover (results are the
+// caller's order's, bit for bit), unless the context turns it off or runs
+// the active-set predictor, whose features follow the vertex order.
+static Renum* top_renum(Ctx& c, MoveScratch& ms, const DGraph& g, Renum& r) {
+    if (!c.renumber || (ms.gnn && ms.gnn->cfg.mode != GNN_OFF)) return nullptr;
+    renum_build(c, g, r);
+    return &r;
+}
+
+// local_move at one level; rn (when set) is the top level renumbered
+static void move_level(Ctx& c, MoveScratch& ms, LevelPlan& lp, const DGraph& g, i32* part, RefineData& rfd,
+                       bool is_initial, bool constrained, i32* cons, bool SA, RunStats& rs, Renum* rn) {
+    if (rn) local_move_renum(c, ms, *rn, part, rfd, is_initial, constrained, cons, SA, &rs.mv);
+    else local_move(c, ms, lp, g, part, rfd, is_initial, constrained, cons, SA, &rs.mv);
+}
+
 // louvain_part<constrained> (src/clustering_methods.cpp:139-222)
 static void louvain_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& rfd, bool constrained,
-                         i32* constraint, i32* part_out, RunStats& rs, LevelPlan* top_plan) {
+                         i32* constraint, i32* part_out, RunStats& rs, LevelPlan* top_plan, Renum* rn) {
     if (constrained) {  // cluster_data::reset (interface/cluster_data.hpp:49-55)
         copy_dev(c, rfd.vol, top.vw, top.n * sizeof(i64));
         rfd.uncut = 0;
@@ -174,7 +191,7 @@ static void louvain_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData&
         const DGraph g = L.g;
         Buf<i32> part(c, std::max<i64>(g.n, 1));
         iota_i32(c, part, g.n);
-        local_move(c, ms, lp, g, part, rfd, true, constrained, cons, true, &rs.mv);
+        move_level(c, ms, lp, g, part, rfd, true, constrained, cons, true, rs, levels.size() == 1 ? rn : nullptr);
         if (constrained && rfd.label_count == g.n) local_move_strict(c, ms, lp, g, part, rfd, true, constrained, cons, &rs.mv);
         const i32* pp = part.p;
         parts.push_back(std::move(part));
@@ -213,7 +230,7 @@ static void louvain_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData&
         const DGraph g = levels[i].g;
         LevelPlan& lp = (i == 0 && top_plan) ? *top_plan : *levels[i].plan;
         project(c, g.n, parts[i + 1], parts[i]);
-        local_move(c, ms, lp, g, parts[i], rfd, false, false, nullptr, true, &rs.mv);
+        move_level(c, ms, lp, g, parts[i], rfd, false, false, nullptr, true, rs, i == 0 ? rn : nullptr);
     }
     copy_dev(c, part_out, parts[0].p, top.n * sizeof(i32));
     c.sync();
@@ -221,7 +238,7 @@ static void louvain_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData&
 
 // leiden_part<plus, improve> (src/clustering_methods.cpp:47-137)
 static void leiden_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& rfd, bool plus, bool improve,
-                        const i32* input, i32* part_out, RunStats& rs, LevelPlan* top_plan) {
+                        const i32* input, i32* part_out, RunStats& rs, LevelPlan* top_plan, Renum* rn) {
     std::vector<Level> levels;
     std::vector<Buf<i32>> parts;
     levels.push_back(top_level(top));
@@ -235,8 +252,9 @@ static void leiden_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& 
         LevelPlan& lp = (li == 0 && top_plan) ? *top_plan : *levels[li].plan;
         const double old_obj = rfd.obj;
         const bool init = !improve && levels.size() == 1;
-        local_move(c, ms, lp, g, part, rfd, init, false, nullptr, true, &rs.mv);
-        if (g.nnz > 100000 && old_obj == rfd.obj) local_move(c, ms, lp, g, part, rfd, init, false, nullptr, false, &rs.mv);
+        Renum* r0 = li == 0 ? rn : nullptr;
+        move_level(c, ms, lp, g, part, rfd, init, false, nullptr, true, rs, r0);
+        if (g.nnz > 100000 && old_obj == rfd.obj) move_level(c, ms, lp, g, part, rfd, init, false, nullptr, false, rs, r0);
         if (old_obj == rfd.obj) local_move_strict(c, ms, lp, g, part, rfd, init, false, nullptr, &rs.mv);
         if (rfd.label_count == g.n) {
             parts.push_back(std::move(part));
@@ -248,6 +266,7 @@ static void leiden_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& 
         const bool uni = levels.size() == 1 && g.wk == W_UNIT;
         Buf<i32> cmap(c, std::max<i64>(g.n, 1));
         auto t0 = std::chrono::steady_clock::now();
+        if (r0) plan_level(c, g, lp);  // the local mover ran on the renumbered copy's plan
         const i64 nc = leiden_refine(c, g, lp.ord, louv, rfd.vol, rfd.lambda, pv, uni, cmap);
         c.sync();
         rs.refine_s += since(t0);
@@ -290,7 +309,7 @@ static void leiden_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& 
         const DGraph g = levels[i].g;
         LevelPlan& lp = (i == 0 && top_plan) ? *top_plan : *levels[i].plan;
         project(c, g.n, parts[i + 1], parts[i]);
-        if (plus) local_move(c, ms, lp, g, parts[i], rfd, false, false, nullptr, true, &rs.mv);
+        if (plus) move_level(c, ms, lp, g, parts[i], rfd, false, false, nullptr, true, rs, i == 0 ? rn : nullptr);
     }
     copy_dev(c, part_out, parts[0].p, top.n * sizeof(i32));
     c.sync();
@@ -644,6 +663,7 @@ static void set_knob(Ctx& c, int knob, int64_t value) {
         case PGGC_KNOB_HUB_SLOTS_PER_ITEM: c.hub_slots_item = value; break;
         case PGGC_KNOB_RAW_SWEEPS: c.raw_sweeps = value ? 1 : 0; break;
         case PGGC_KNOB_HUB_BUILD: c.hub_build = (int)std::min<int64_t>(value, 2); break;
+        case PGGC_KNOB_RENUMBER: c.renumber = value ? 1 : 0; break;
         default: throw Error(PGGC_EINVAL, "unknown knob");
     }
     if (shape) c.knob_epoch++;  // level plans depend on these
@@ -660,6 +680,7 @@ static int64_t get_knob(const Ctx& c, int knob) {
         case PGGC_KNOB_HUB_SLOTS_PER_ITEM: return c.hub_slots_item;
         case PGGC_KNOB_RAW_SWEEPS: return c.raw_sweeps;
         case PGGC_KNOB_HUB_BUILD: return c.hub_build;
+        case PGGC_KNOB_RENUMBER: return c.renumber;
         default: throw Error(PGGC_EINVAL, "unknown knob");
     }
 }
@@ -712,6 +733,7 @@ int pggc_autotune(pggc_ctx* ctx, const pggc_graph* gg, int objective, int reps, 
             for (int64_t v : cand) {
                 set_knob(c, knob, v);
                 LevelPlan top;
+                Renum rtop;
                 double t_min = std::numeric_limits<double>::infinity();
                 for (int r = 0; r <= reps; r++) {
                     RefineData rd;
@@ -721,7 +743,7 @@ int pggc_autotune(pggc_ctx* ctx, const pggc_graph* gg, int objective, int reps, 
                     RunStats rs;
                     c.sync();
                     const auto t0 = std::chrono::steady_clock::now();
-                    louvain_part(c, ctx->ms, g, rd, false, nullptr, part, rs, &top);
+                    louvain_part(c, ctx->ms, g, rd, false, nullptr, part, rs, &top, top_renum(c, ctx->ms, g, rtop));
                     c.sync();
                     if (r > 0) t_min = std::min(t_min, since(t0));
                 }
@@ -967,7 +989,9 @@ int pggc_louvain(pggc_ctx* ctx, const pggc_graph* gg, pggc_state* st, int constr
         DGraph g = view_of(gg);
         RefineData rd = rd_of(st);
         RunStats rs;
-        louvain_part(ctx->c, ctx->ms, g, rd, constrained != 0, constraint, part_out, rs, nullptr);
+        Renum rtop;
+        louvain_part(ctx->c, ctx->ms, g, rd, constrained != 0, constraint, part_out, rs, nullptr,
+                     top_renum(ctx->c, ctx->ms, g, rtop));
         rd_store(rd, st);
     });
 }
@@ -979,7 +1003,9 @@ int pggc_leiden(pggc_ctx* ctx, const pggc_graph* gg, pggc_state* st, int plus, i
         DGraph g = view_of(gg);
         RefineData rd = rd_of(st);
         RunStats rs;
-        leiden_part(ctx->c, ctx->ms, g, rd, plus != 0, improve != 0, input, part_out, rs, nullptr);
+        Renum rtop;
+        leiden_part(ctx->c, ctx->ms, g, rd, plus != 0, improve != 0, input, part_out, rs, nullptr,
+                    top_renum(ctx->c, ctx->ms, g, rtop));
         rd_store(rd, st);
     });
 }
@@ -1005,6 +1031,8 @@ int pggc_cluster(pggc_ctx* ctx, const pggc_graph* gg, int algo, int objective, d
         state_init(c, g, objective, mult, rd);
         RunStats rs;
         LevelPlan top;
+        Renum rtop;
+        Renum* rn = top_renum(c, ctx->ms, g, rtop);
         // per-iteration line as printed by the reference (app/cluster_main.cpp:137,156,
         // interface/cluster_data.hpp:98-103)
         auto t_iter = std::chrono::steady_clock::now();
@@ -1022,8 +1050,8 @@ int pggc_cluster(pggc_ctx* ctx, const pggc_graph* gg, int algo, int objective, d
                     (long long)cut, rd.objective(), rd.label_count, (long long)rs.mv.sweeps);
             fflush(stderr);
         };
-        if (algo == PGGC_LOUVAIN) louvain_part(c, ctx->ms, g, rd, false, nullptr, part_out, rs, &top);
-        else leiden_part(c, ctx->ms, g, rd, algo == PGGC_LEIDEN_PLUS, false, nullptr, part_out, rs, &top);
+        if (algo == PGGC_LOUVAIN) louvain_part(c, ctx->ms, g, rd, false, nullptr, part_out, rs, &top, rn);
+        else leiden_part(c, ctx->ms, g, rd, algo == PGGC_LEIDEN_PLUS, false, nullptr, part_out, rs, &top, rn);
         report();
         double last = -std::numeric_limits<double>::infinity();
         int counter = 0;
@@ -1039,8 +1067,8 @@ int pggc_cluster(pggc_ctx* ctx, const pggc_graph* gg, int algo, int objective, d
             }
             if (term) break;
             copy_dev(c, cons, part_out, g.n * sizeof(i32));
-            if (algo == PGGC_LOUVAIN) louvain_part(c, ctx->ms, g, rd, true, cons, part_out, rs, &top);
-            else leiden_part(c, ctx->ms, g, rd, algo == PGGC_LEIDEN_PLUS, true, cons, part_out, rs, &top);
+            if (algo == PGGC_LOUVAIN) louvain_part(c, ctx->ms, g, rd, true, cons, part_out, rs, &top, rn);
+            else leiden_part(c, ctx->ms, g, rd, algo == PGGC_LEIDEN_PLUS, true, cons, part_out, rs, &top, rn);
             outer++;
             report();
         }

@@ -115,8 +115,11 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p);
 
 // local_move<constrained> (src/local_mover.cpp:1360-1417) and
 // local_move_strict (:1419-1483).  best_part and cons are device arrays.
+// ids (optional, device, n): g is a renumbering of the caller's graph and
+// ids[i] is vertex i's caller id, which then stands in for i wherever the
+// result depends on vertex ids (afterburner priorities, new cluster ids).
 void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* best_part, RefineData& best,
-                bool is_initial, bool constrained, i32* cons, bool SA, MoveStats* st);
+                bool is_initial, bool constrained, i32* cons, bool SA, MoveStats* st, const i32* ids = nullptr);
 void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* best_part, RefineData& best,
                        bool is_initial, bool constrained, i32* cons, MoveStats* st);
 

@@ -402,7 +402,7 @@ __global__ void __launch_bounds__(256) k_score_group(const i64* __restrict__ rp,
         const bool has = gl < d;
         if (has) {
             i32 v = col[s + gl];
-            key = raw ? v : part[v];
+            key = raw == 1 ? v : part[v];
             wt = W(s + gl);
         }
         i64 agg = wt;
@@ -480,7 +480,7 @@ __global__ void __launch_bounds__(BS) k_score_table(const i64* __restrict__ rp, 
                 wv[r] = 0;
                 if (e < d) {
                     const i32 v = col[s + e];
-                    kk[r] = raw ? v : part[v];
+                    kk[r] = raw == 1 ? v : part[v];
                     wv[r] = W(s + e);
                 }
             }
@@ -496,7 +496,7 @@ __global__ void __launch_bounds__(BS) k_score_table(const i64* __restrict__ rp, 
             const i64 e = eb + t;
             if (e >= d) continue;
             i32 v = col[s + e];
-            i32 key = raw ? v : part[v];
+            i32 key = raw == 1 ? v : part[v];
             i64 ww = W(s + e);
             if (!raw && key == p) pv += ww;
             else if (GLOBAL) table_add_global(keys, vals, mask, key, ww);
@@ -542,8 +542,9 @@ __global__ void __launch_bounds__(BS) k_score_table(const i64* __restrict__ rp, 
 // (gain, smallest label) order as the table path.
 template <class WT>
 __device__ __forceinline__ u64 raw_cand(const i32* __restrict__ col, WT W, const i64* __restrict__ vol,
-                                        const i32* __restrict__ cons, i32 cp, i64 j, float multi, float b) {
-    const i32 v = col[j];
+                                        const i32* __restrict__ cons, const i32* __restrict__ lab, i32 cp, i64 j,
+                                        float multi, float b) {
+    const i32 v = lab ? lab[col[j]] : col[j];  // the neighbour's (singleton) cluster
     const i64 agg = W(j);
     if (agg > 0 && (float)agg >= b && (cons == nullptr || cons[v] == cp)) {
         const float jc = fmaf(-(float)vol[v], multi, (float)agg);
@@ -557,8 +558,8 @@ template <int BS, class WT, class LT>
 __global__ void __launch_bounds__(BS) k_score_raw(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
                                                   const i64* __restrict__ vw, const i32* __restrict__ list, i64 m,
                                                   const i32* __restrict__ part, const i64* __restrict__ vol,
-                                                  const i32* __restrict__ cons, LT lam, float ratio,
-                                                  i32* __restrict__ dest, float* __restrict__ gain) {
+                                                  const i32* __restrict__ cons, const i32* __restrict__ lab, LT lam,
+                                                  float ratio, i32* __restrict__ dest, float* __restrict__ gain) {
     const i64 li = block_id();
     if (li >= m) return;
     const i32 i = list[li];
@@ -572,7 +573,7 @@ __global__ void __launch_bounds__(BS) k_score_raw(const i64* __restrict__ rp, co
     for (i64 jb = s; jb < e; jb += BS) {  // block-uniform loop control
         const i64 j = jb + threadIdx.x;
         if (j < e) {
-            const u64 k2 = raw_cand(col, W, vol, cons, cp, j, multi, b);
+            const u64 k2 = raw_cand(col, W, vol, cons, lab, cp, j, multi, b);
             bk = k2 > bk ? k2 : bk;
         }
     }
@@ -585,8 +586,9 @@ template <class WT, class LT>
 __global__ void __launch_bounds__(256) k_hub_raw(const i64* __restrict__ rp, const i32* __restrict__ col, WT W,
                                                  const i64* __restrict__ vw, const i32* __restrict__ hubs, i64 m,
                                                  const i64* __restrict__ items, const i32* __restrict__ part,
-                                                 const i64* __restrict__ vol, const i32* __restrict__ cons, LT lam,
-                                                 float ratio, const i32* __restrict__ dest, i64 he, u64* hub_best) {
+                                                 const i64* __restrict__ vol, const i32* __restrict__ cons,
+                                                 const i32* __restrict__ lab, LT lam, float ratio,
+                                                 const i32* __restrict__ dest, i64 he, u64* hub_best) {
     const i64 b = block_id();
     if (b >= items[m]) return;
     const i64 hi = upper_row(items, 0, m, b);
@@ -603,7 +605,7 @@ __global__ void __launch_bounds__(256) k_hub_raw(const i64* __restrict__ rp, con
     for (i64 jb = j_lo; jb < j_hi; jb += 256) {  // block-uniform loop control
         const i64 j = jb + threadIdx.x;
         if (j < j_hi) {
-            const u64 k2 = raw_cand(col, W, vol, cons, cp, j, multi, bth);
+            const u64 k2 = raw_cand(col, W, vol, cons, lab, cp, j, multi, bth);
             bk = k2 > bk ? k2 : bk;
         }
     }
@@ -1073,7 +1075,7 @@ __global__ void __launch_bounds__(256) k_score_bytes(const i64* __restrict__ rp,
         const int grp = x < sb.s[CL_TABLE] ? 0 : (x < sb.hub0 ? 1 : 2);
         i64 by = 8;
         if (dest[i] != NO_MOVE) {
-            const i64 eb = (rp[i + 1] - rp[i]) * (4 + WB + (raw ? 0 : 4));
+            const i64 eb = (rp[i + 1] - rp[i]) * (4 + WB + (raw == 1 ? 0 : 4));
             if (grp < 2 || raw) {
                 by += 48 + eb;
             } else {
@@ -1241,7 +1243,10 @@ static void hub_build_all(Ctx& c, const DGraph& g, WT W, LevelPlan& lp, const i3
 
 template <class LT>
 static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32* part, const i64* vol,
-                      const i32* cons, bool raw, LT lam, float ratio, i32* dest, float* gain) {
+                      const i32* cons, int raw, LT lam, float ratio, i32* dest, float* gain) {
+    // raw 1: singletons labelled by vertex id; raw 2: singletons whose labels
+    // are read from part (a renumbered graph); 0: any clustering
+    const i32* lab = raw == 2 ? part : nullptr;
     const i64* st = lp.sc_start;
     const i32* L = lp.sc_list.p;
     Buf<i64> acct;
@@ -1254,7 +1259,7 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
         const int build = (!raw && !lp.hub_live) ? 1 : 0;
         const i64* goff = lp.nhub > 0 ? lp.goff.p : nullptr;
         PGGC_WDISPATCH(g, (k_score_bytes<decltype(W)::bytes><<<grid_items(g.n, 256), 256, 0, c.stream>>>(
-                              g.rp, L, sb, dest, raw ? 1 : 0, build, goff, acct)));
+                              g.rp, L, sb, dest, raw, build, goff, acct)));
         PGGC_LAUNCH_CHECK();
     }
     PGGC_WDISPATCH(g, {
@@ -1281,10 +1286,10 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
         if (raw) {
             if ((m = st[CL_BIG] - st[CL_TABLE]) > 0)
                 k_score_raw<64><<<grid_blocks(m), 64, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[CL_TABLE], m,
-                                                                     part, vol, cons, lam, ratio, dest, gain);
+                                                                     part, vol, cons, lab, lam, ratio, dest, gain);
             if ((m = lp.hub0 - st[CL_BIG]) > 0)  // big rows and the LDS-scored hubs
                 k_score_raw<256><<<grid_blocks(m), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, L + st[CL_BIG], m,
-                                                                       part, vol, cons, lam, ratio, dest, gain);
+                                                                       part, vol, cons, lab, lam, ratio, dest, gain);
         } else {
             // LDS tables sized for the class, so the many low-degree rows keep
             // more waves resident per CU; threads per vertex and slots per
@@ -1311,7 +1316,7 @@ static void score_all(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
             fill_i64(c, (i64*)lp.hub_best.p, m, 0);
             if (raw) {
                 k_hub_raw<<<grid_blocks(lp.nitems), 256, 0, c.stream>>>(g.rp, g.col, W, g.vw, hubs, m, lp.items, part,
-                                                                       vol, cons, lam, ratio, dest, lp.he,
+                                                                       vol, cons, lab, lam, ratio, dest, lp.he,
                                                                        lp.hub_best);
             } else {
                 if (!lp.hub_live) {  // first table sweep of the call (or stale tables): build
@@ -1568,13 +1573,32 @@ __global__ void k_assign_new(const i32* newv, i64 K, const i32* avail, i64 A, i3
     if (cons) cons[l] = cons[part[i]];
 }
 
-// set_new_cluster_ids (src/local_mover.cpp:215-274)
+// key of a NEW_PART mover in the caller's move order: order class
+// (src/header/ordering.h:17-19), then the caller's id
+__global__ void k_newid_keys(const i32* newv, i64 K, const i64* rp, const i32* ids, u64* key) {
+    const i64 x = thread_item();
+    if (x >= K) return;
+    const i32 i = newv[x];
+    const i64 d = rp[i + 1] - rp[i];
+    const u64 cls = d < LARGE_CUTOFF ? 0 : (d < MASSIVE_CUTOFF ? 1 : 2);
+    key[x] = cls << 32 | (u32)ids[i];
+}
+
+// set_new_cluster_ids (src/local_mover.cpp:215-274).  ids non-null: the
+// graph is renumbered and ids[i] is vertex i's id in the caller's order; the
+// movers are then taken in the caller's move order.
 static void set_new_cluster_ids(Ctx& c, MoveScratch& s, const DGraph& g, RefineData& rd, const i32* moves, i64 k,
-                                i32* cons) {
+                                i32* cons, const i32* ids) {
     k_new_flags<<<grid_items(k, 256), 256, 0, c.stream>>>(moves, k, s.dest, s.flag);
     PGGC_LAUNCH_CHECK();
     i64 K = compact_flags(c, s.flag, moves, k, s.tmp, s.scal);
     if (K == 0) return;
+    if (ids && K > 1) {
+        Buf<u64> key(c, K);
+        k_newid_keys<<<grid_items(K, 256), 256, 0, c.stream>>>(s.tmp, K, g.rp, ids, key);
+        PGGC_LAUNCH_CHECK();
+        radix_sort_pairs(c, key, s.tmp.p, K, 34);
+    }
     const i32 curr = rd.label_count;
     fill_i32(c, s.flag, curr, 0);
     k_mark_used<<<grid_items(g.n, 256), 256, 0, c.stream>>>(s.part, g.n, s.flag);
@@ -1973,8 +1997,24 @@ static RefineData begin_curr(Ctx& c, const RefineData& best, Buf<i64>& cv, i64 n
     return curr;
 }
 
+template <class PT>
+static void local_move_t(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* best_part, RefineData& best,
+                         bool is_initial, bool constrained, i32* cons, bool SA, MoveStats* stats, PT prio,
+                         const i32* ids);
+
 void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* best_part, RefineData& best,
-                bool is_initial, bool constrained, i32* cons, bool SA, MoveStats* stats) {
+                bool is_initial, bool constrained, i32* cons, bool SA, MoveStats* stats, const i32* ids) {
+    if (ids) {
+        local_move_t(c, s, lp, g, best_part, best, is_initial, constrained, cons, SA, stats, PrioInst{ids}, ids);
+    } else {
+        local_move_t(c, s, lp, g, best_part, best, is_initial, constrained, cons, SA, stats, PrioOne{}, nullptr);
+    }
+}
+
+template <class PT>
+static void local_move_t(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* best_part, RefineData& best,
+                         bool is_initial, bool constrained, i32* cons, bool SA, MoveStats* stats, PT prio,
+                         const i32* ids) {
     const i64 n = g.n;
     s.ensure(c, n);
     plan_level(c, g, lp);
@@ -1988,6 +2028,9 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
     const int limits[2] = {4, 2};
     int iter = 0;
     const i32* ccons = constrained ? cons : nullptr;
+    // the raw first sweep (every neighbour its own cluster) reads a
+    // neighbour's cluster from part when the graph is renumbered
+    const int raw_mode = c.raw_sweeps ? (ids ? 2 : 1) : 0;
     SweepTrace tr(c, g, is_initial ? 0 : 1);
     // active-set predictor: on levels of at least min_nnz nonzeros, from the
     // second sweep of a call that starts at singletons (whose first sweep
@@ -2014,16 +2057,16 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
                 npred++;
             }
             tr.begin(s.dest, iter == 1 && !use);
-            score_all(c, s, lp, g, s.part, curr.vol, ccons, !tables && c.raw_sweeps, lam, fr, s.dest, s.gain);
+            score_all(c, s, lp, g, s.part, curr.vol, ccons, tables ? 0 : raw_mode, lam, fr, s.dest, s.gain);
             i64 b[2];
             i64 k = gather_candidates(c, s, lp, g, s.dest, s.cand, b);
-            i64 km = k > 0 ? afterburner(c, s, g, s.cand, k, b, lam, PrioOne{}, s.moves) : 0;
+            i64 km = k > 0 ? afterburner(c, s, g, s.cand, k, b, lam, prio, s.moves) : 0;
             if (km == 0 && G && gnn_restore(c, *G, n, s.dest)) {
                 // nothing to move among the vertices let through: score the
                 // pruned ones too, so the call ends as the plain method does
-                score_all(c, s, lp, g, s.part, curr.vol, ccons, !tables && c.raw_sweeps, lam, fr, s.dest, s.gain);
+                score_all(c, s, lp, g, s.part, curr.vol, ccons, tables ? 0 : raw_mode, lam, fr, s.dest, s.gain);
                 k = gather_candidates(c, s, lp, g, s.dest, s.cand, b);
-                km = k > 0 ? afterburner(c, s, g, s.cand, k, b, lam, PrioOne{}, s.moves) : 0;
+                km = k > 0 ? afterburner(c, s, g, s.cand, k, b, lam, prio, s.moves) : 0;
             }
             if (stats) {
                 stats->sweeps++;
@@ -2040,7 +2083,8 @@ void local_move(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i32* bes
                 tr.end(k, 0);
                 break;
             }
-            if (iter > 1 || !is_initial) set_new_cluster_ids(c, s, g, curr, s.moves, km, constrained ? cons : nullptr);
+            if (iter > 1 || !is_initial)
+                set_new_cluster_ids(c, s, g, curr, s.moves, km, constrained ? cons : nullptr, ids);
             perform_moves(c, s, lp, g, curr, s.moves, km, g.n - lp.ord.off_large, g.n - lp.ord.off_large2, tables);
             if (stats) stats->moves += km;
             if (c.verbose >= 3)
@@ -2148,7 +2192,7 @@ void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i
             trc.end(0, 0);
             break;
         }
-        if (it > 0 || !is_initial) set_new_cluster_ids(c, s, g, curr, s.moves, k, constrained ? cons : nullptr);
+        if (it > 0 || !is_initial) set_new_cluster_ids(c, s, g, curr, s.moves, k, constrained ? cons : nullptr, nullptr);
         fill_i64(c, s.prio, n, n + 1);
         k_prio<<<grid_items(k, 256), 256, 0, c.stream>>>(s.moves, k, s.prio);
         PGGC_LAUNCH_CHECK();
