@@ -169,6 +169,20 @@ static void move_level(Ctx& c, MoveScratch& ms, LevelPlan& lp, const DGraph& g, 
     else local_move(c, ms, lp, g, part, rfd, is_initial, constrained, cons, SA, &rs.mv);
 }
 
+// Contraction of a level; at the top level with rn set, through the
+// renumbered copy (the same coarse graph: coarse ids are the labels, coarse
+// rows are sorted and their weights are integer sums), whose rows gather
+// the map with more locality.  rn->part is the local mover's scratch and
+// free here.
+static void contract_level(Ctx& c, const DGraph& g, const i32* map, i64 nc, OwnedGraph& og, Renum* rn) {
+    if (!rn) {
+        contract(c, g, map, nc, og);
+        return;
+    }
+    renum_gather(c, *rn, map, rn->part);
+    contract(c, rn->g, rn->part, nc, og);
+}
+
 // louvain_part<constrained> (src/clustering_methods.cpp:139-222)
 static void louvain_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& rfd, bool constrained,
                          i32* constraint, i32* part_out, RunStats& rs, LevelPlan* top_plan, Renum* rn) {
@@ -199,7 +213,7 @@ static void louvain_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData&
             const i64 nc = rfd.label_count;
             std::unique_ptr<OwnedGraph> og(new OwnedGraph());
             auto t0 = std::chrono::steady_clock::now();
-            contract(c, g, pp, nc, *og);
+            contract_level(c, g, pp, nc, *og, levels.size() == 1 ? rn : nullptr);
             c.sync();
             rs.contract_s += since(t0);
             og->vw = Buf<i64>(c, std::max<i64>(nc, 1));
@@ -275,7 +289,7 @@ static void leiden_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& 
         if (nc < g.n) {
             std::unique_ptr<OwnedGraph> og(new OwnedGraph());
             auto t1 = std::chrono::steady_clock::now();
-            contract(c, g, cm, nc, *og);
+            contract_level(c, g, cm, nc, *og, r0);
             c.sync();
             rs.contract_s += since(t1);
             og->vw = Buf<i64>(c, std::max<i64>(nc, 1));

@@ -32,6 +32,10 @@ struct Renum {
 // Builds r for top (no-op when r already holds it).
 void renum_build(Ctx& c, const DGraph& top, Renum& r);
 
+// out[i'] = in[ids[i']]: a per-vertex array of the caller's order in the
+// renumbered order
+void renum_gather(Ctx& c, const Renum& r, const i32* in, i32* out);
+
 // local_move on the renumbered graph: part (caller's order) is read and
 // written in place; cons is indexed by label, so it is shared unchanged.
 void local_move_renum(Ctx& c, MoveScratch& s, Renum& r, i32* part, RefineData& best, bool is_initial,

@@ -118,13 +118,16 @@ void renum_build(Ctx& c, const DGraph& top, Renum& r) {
     r.g = g;
 }
 
+void renum_gather(Ctx& c, const Renum& r, const i32* in, i32* out) {
+    if (r.n == 0) return;
+    k_renum_gather<<<grid_items(r.n, 256), 256, 0, c.stream>>>(r.n, r.ids, in, out);
+    PGGC_LAUNCH_CHECK();
+}
+
 void local_move_renum(Ctx& c, MoveScratch& s, Renum& r, i32* part, RefineData& best, bool is_initial,
                       bool constrained, i32* cons, bool SA, MoveStats* st) {
     const i64 n = r.n;
-    if (n > 0) {
-        k_renum_gather<<<grid_items(n, 256), 256, 0, c.stream>>>(n, r.ids, part, r.part);
-        PGGC_LAUNCH_CHECK();
-    }
+    renum_gather(c, r, part, r.part);
     local_move(c, s, r.plan, r.g, r.part, best, is_initial, constrained, cons, SA, st, r.ids);
     if (n > 0) {
         k_renum_scatter<<<grid_items(n, 256), 256, 0, c.stream>>>(n, r.ids, r.part, part);
