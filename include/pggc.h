@@ -165,11 +165,14 @@ int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user);
  *     edge work items); 1 every table by edge work items adding into the HBM
  *     table with atomics; 2 the tables of odd hub index by edge items, the
  *     rest by slices (the same tables' contents in every mode).
- *   PGGC_KNOB_RENUMBER: 1 (default) runs the local mover of the top level
- *     on an internal copy of the graph with vertices in descending degree
- *     order (rows keep their entry order; labels, priorities and new cluster
- *     ids follow the caller's ids, so results are identical), 0 on the
- *     caller's graph.  Not used with the active-set predictor on. */
+ *   PGGC_KNOB_RENUMBER: where the local mover of the top level runs: on an
+ *     internal copy of the graph with vertices in descending degree order
+ *     (rows keep their entry order; labels, priorities and new cluster ids
+ *     follow the caller's ids, so results are identical) or on the caller's
+ *     graph.  1: always the copy; 2 (default): the copy when the call makes
+ *     more than one top-level pass (pggc_cluster with ex_iters != 0), which
+ *     pays for building it; 0: never.  Not used with the active-set
+ *     predictor on. */
 #define PGGC_KNOB_HUB_UPDATE_DIV 0
 #define PGGC_KNOB_GNN_MAX_SWEEPS 1
 #define PGGC_KNOB_TABLE_THREADS 2

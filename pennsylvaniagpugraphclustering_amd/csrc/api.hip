@@ -154,10 +154,12 @@ static Level owned_level(std::unique_ptr<OwnedGraph> og) {
 constexpr int LEIDEN_MAX_STALLS = 8;
 
 // The top level renumbered by degree for the local mover (results are the
-// caller's order's, bit for bit), unless the context turns it off or runs
-// the active-set predictor, whose features follow the vertex order.
-static Renum* top_renum(Ctx& c, MoveScratch& ms, const DGraph& g, Renum& r) {
-    if (!c.renumber || (ms.gnn && ms.gnn->cfg.mode != GNN_OFF)) return nullptr;
+// caller's order's, bit for bit): always with the renumber knob at 1, with
+// it at 2 (the default) when the copy serves several top-level passes
+// (reuse), never at 0 or with the active-set predictor on, whose features
+// follow the vertex order.
+static Renum* top_renum(Ctx& c, MoveScratch& ms, const DGraph& g, Renum& r, bool reuse) {
+    if (c.renumber == 0 || (c.renumber == 2 && !reuse) || (ms.gnn && ms.gnn->cfg.mode != GNN_OFF)) return nullptr;
     renum_build(c, g, r);
     return &r;
 }
@@ -280,8 +282,15 @@ static void leiden_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& 
         const bool uni = levels.size() == 1 && g.wk == W_UNIT;
         Buf<i32> cmap(c, std::max<i64>(g.n, 1));
         auto t0 = std::chrono::steady_clock::now();
-        if (r0) plan_level(c, g, lp);  // the local mover ran on the renumbered copy's plan
-        const i64 nc = leiden_refine(c, g, lp.ord, louv, rfd.vol, rfd.lambda, pv, uni, cmap);
+        const Orders* ord = &lp.ord;
+        if (r0) {  // the local mover ran on the renumbered copy's plan
+            if (!r0->ord_built) {
+                make_orders(c, g, r0->ord);
+                r0->ord_built = true;
+            }
+            ord = &r0->ord;
+        }
+        const i64 nc = leiden_refine(c, g, *ord, louv, rfd.vol, rfd.lambda, pv, uni, cmap);
         c.sync();
         rs.refine_s += since(t0);
         const i32* cm = cmap.p;
@@ -677,7 +686,7 @@ static void set_knob(Ctx& c, int knob, int64_t value) {
         case PGGC_KNOB_HUB_SLOTS_PER_ITEM: c.hub_slots_item = value; break;
         case PGGC_KNOB_RAW_SWEEPS: c.raw_sweeps = value ? 1 : 0; break;
         case PGGC_KNOB_HUB_BUILD: c.hub_build = (int)std::min<int64_t>(value, 2); break;
-        case PGGC_KNOB_RENUMBER: c.renumber = value ? 1 : 0; break;
+        case PGGC_KNOB_RENUMBER: c.renumber = (int)std::min<int64_t>(value, 2); break;
         default: throw Error(PGGC_EINVAL, "unknown knob");
     }
     if (shape) c.knob_epoch++;  // level plans depend on these
@@ -757,7 +766,7 @@ int pggc_autotune(pggc_ctx* ctx, const pggc_graph* gg, int objective, int reps, 
                     RunStats rs;
                     c.sync();
                     const auto t0 = std::chrono::steady_clock::now();
-                    louvain_part(c, ctx->ms, g, rd, false, nullptr, part, rs, &top, top_renum(c, ctx->ms, g, rtop));
+                    louvain_part(c, ctx->ms, g, rd, false, nullptr, part, rs, &top, top_renum(c, ctx->ms, g, rtop, true));
                     c.sync();
                     if (r > 0) t_min = std::min(t_min, since(t0));
                 }
@@ -1005,7 +1014,7 @@ int pggc_louvain(pggc_ctx* ctx, const pggc_graph* gg, pggc_state* st, int constr
         RunStats rs;
         Renum rtop;
         louvain_part(ctx->c, ctx->ms, g, rd, constrained != 0, constraint, part_out, rs, nullptr,
-                     top_renum(ctx->c, ctx->ms, g, rtop));
+                     top_renum(ctx->c, ctx->ms, g, rtop, false));
         rd_store(rd, st);
     });
 }
@@ -1019,7 +1028,7 @@ int pggc_leiden(pggc_ctx* ctx, const pggc_graph* gg, pggc_state* st, int plus, i
         RunStats rs;
         Renum rtop;
         leiden_part(ctx->c, ctx->ms, g, rd, plus != 0, improve != 0, input, part_out, rs, nullptr,
-                    top_renum(ctx->c, ctx->ms, g, rtop));
+                    top_renum(ctx->c, ctx->ms, g, rtop, false));
         rd_store(rd, st);
     });
 }
@@ -1046,7 +1055,7 @@ int pggc_cluster(pggc_ctx* ctx, const pggc_graph* gg, int algo, int objective, d
         RunStats rs;
         LevelPlan top;
         Renum rtop;
-        Renum* rn = top_renum(c, ctx->ms, g, rtop);
+        Renum* rn = top_renum(c, ctx->ms, g, rtop, ex_iters != 0);
         // per-iteration line as printed by the reference (app/cluster_main.cpp:137,156,
         // interface/cluster_data.hpp:98-103)
         auto t_iter = std::chrono::steady_clock::now();

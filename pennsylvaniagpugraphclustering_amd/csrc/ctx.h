@@ -96,8 +96,9 @@ struct Ctx {
     // hub tables: 0 builds each table slice in LDS and writes it out whole,
     // 1 adds edge work items into the HBM tables with atomics
     int hub_build = 0;
-    // the top level's local mover runs on a copy renumbered by degree
-    int renumber = 1;
+    // the top level's local mover runs on a copy renumbered by degree:
+    // 0 never, 1 always, 2 when a call makes several top-level passes
+    int renumber = 2;
     u64 knob_epoch = 0;  // bumped by every knob change; level plans are rebuilt when it moves
     // the active-set predictor runs on at most this many sweeps of a
     // local-move call (0: on every eligible sweep)

@@ -26,6 +26,8 @@ struct Renum {
     Buf<i64> w64;
     DGraph g;       // the renumbered graph
     LevelPlan plan;
+    Orders ord;     // orderings of the caller's graph (for the refinement), built on first use
+    bool ord_built = false;
     bool built_for(const DGraph& top) const { return key == (const void*)top.rp && n == top.n; }
 };
 

@@ -72,6 +72,7 @@ void renum_build(Ctx& c, const DGraph& top, Renum& r) {
     r.key = top.rp;
     r.n = n;
     r.plan = LevelPlan();
+    r.ord_built = false;
     r.ids = Buf<i32>(c, std::max<i64>(n, 1));
     r.part = Buf<i32>(c, std::max<i64>(n, 1));
     r.rp = Buf<i64>(c, n + 1);

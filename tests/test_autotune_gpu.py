@@ -13,7 +13,7 @@ from pennsylvaniagpugraphclustering_amd import graphs
 pytestmark = pytest.mark.gpu
 
 DEFAULTS = {"table_threads": 64, "table_slots_per_edge": 2, "hub_lds_max_degree": 2048, "hub_edges_per_item": 1024,
-            "hub_slots_per_item": 4096, "hub_update_div": 32, "hub_build": 0, "renumber": 1}
+            "hub_slots_per_item": 4096, "hub_update_div": 32, "hub_build": 0, "renumber": 2}
 
 
 @pytest.fixture(scope="module")
@@ -54,7 +54,7 @@ def test_every_knob_value_gives_identical_results(gpu_ctx, graphs_under_test, al
             # hub tables built by edge items with HBM atomics, for every hub
             # and for every other hub (the rest from LDS slices)
             variants += [{"hub_build": 1}, {"hub_build": 2}, {"hub_build": 2, "hub_update_div": 0}]
-            variants += [{"renumber": 0}]
+            variants += [{"renumber": 0}, {"renumber": 1}]
             for var in variants:
                 print(f"{name} {algo} knobs {var}", flush=True)
                 _set_all(gpu_ctx, DEFAULTS)
