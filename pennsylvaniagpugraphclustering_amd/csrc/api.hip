@@ -179,10 +179,11 @@ static void move_level(Ctx& c, MoveScratch& ms, LevelPlan& lp, const DGraph& g, 
 static void contract_level(Ctx& c, const DGraph& g, const i32* map, i64 nc, OwnedGraph& og, Renum* rn) {
     if (!rn) {
         contract(c, g, map, nc, og);
-        return;
+    } else {
+        renum_gather(c, *rn, map, rn->part);
+        contract(c, rn->g, rn->part, nc, og);
     }
-    renum_gather(c, *rn, map, rn->part);
-    contract(c, rn->g, rn->part, nc, og);
+    narrow_weights(c, g, og);
 }
 
 // louvain_part<constrained> (src/clustering_methods.cpp:139-222)
@@ -207,7 +208,9 @@ static void louvain_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData&
         const DGraph g = L.g;
         Buf<i32> part(c, std::max<i64>(g.n, 1));
         iota_i32(c, part, g.n);
+        c.mem_note("louvain: level start");
         move_level(c, ms, lp, g, part, rfd, true, constrained, cons, true, rs, levels.size() == 1 ? rn : nullptr);
+        c.mem_note("louvain: after local move");
         if (constrained && rfd.label_count == g.n) local_move_strict(c, ms, lp, g, part, rfd, true, constrained, cons, &rs.mv);
         const i32* pp = part.p;
         parts.push_back(std::move(part));
@@ -269,7 +272,9 @@ static void leiden_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& 
         const double old_obj = rfd.obj;
         const bool init = !improve && levels.size() == 1;
         Renum* r0 = li == 0 ? rn : nullptr;
+        c.mem_note("leiden: level start");
         move_level(c, ms, lp, g, part, rfd, init, false, nullptr, true, rs, r0);
+        c.mem_note("leiden: after local move");
         if (g.nnz > 100000 && old_obj == rfd.obj) move_level(c, ms, lp, g, part, rfd, init, false, nullptr, false, rs, r0);
         if (old_obj == rfd.obj) local_move_strict(c, ms, lp, g, part, rfd, init, false, nullptr, &rs.mv);
         if (rfd.label_count == g.n) {
@@ -291,6 +296,7 @@ static void leiden_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& 
             ord = &r0->ord;
         }
         const i64 nc = leiden_refine(c, g, *ord, louv, rfd.vol, rfd.lambda, pv, uni, cmap);
+        c.mem_note("leiden: after refinement");
         c.sync();
         rs.refine_s += since(t0);
         const i32* cm = cmap.p;

@@ -235,6 +235,7 @@ void louvain_batch(Ctx& c, MoveScratch& ms, const std::vector<DGraph>& gs, const
         std::unique_ptr<BLevel> nx(new BLevel());
         std::unique_ptr<OwnedGraph> og(new OwnedGraph());
         contract(c, g, lv.vcmap, nc, *og);
+        narrow_weights(c, g, *og);
         nx->bl.build(c, coff);
         og->vw = Buf<i64>(c, std::max<i64>(nc, 1));
         if (nc > 0) {

@@ -61,7 +61,7 @@ struct Ctx {
     size_t stage_cap = 0;
     u64 seed = 0;           // leidenR seed stream
     u64 draws = 0;
-    int verbose = 0;        // >0: per-iteration progress lines on stderr
+    int verbose = 0;        // >0: per-iteration progress lines on stderr; >=2: memory by stage
     // per-outer-iteration callback (iteration, seconds, cut, objective, labels)
     void (*progress)(void*, int64_t, double, int64_t, double, int32_t) = nullptr;
     void* progress_user = nullptr;
@@ -106,6 +106,8 @@ struct Ctx {
     KTimer kt;
 
     void* alloc(size_t bytes);
+    // verbose >= 2: a line on stderr with the bytes this context holds
+    void mem_note(const char* where) const;
     void release(void* p);
     void trim();
     void sync() { PGGC_HIP(hipStreamSynchronize(stream)); }

@@ -9,7 +9,8 @@ namespace pggc {
 struct OwnedGraph {
     Buf<i64> rp;
     Buf<i32> col;
-    Buf<i64> w;   // int64 coarse weights
+    Buf<i64> w;    // int64 coarse weights, or
+    Buf<i32> w32;  // the same weights in 32 bits when every one fits (narrow_weights)
     Buf<i64> vw;
     i64 n = 0, nnz = 0;
     DGraph view() const {
@@ -18,8 +19,13 @@ struct OwnedGraph {
         g.nnz = nnz;
         g.rp = rp.p;
         g.col = col.p;
-        g.w = w.p;
-        g.wk = W_I64;
+        if (w32.p) {
+            g.w = w32.p;
+            g.wk = W_I32;
+        } else {
+            g.w = w.p;
+            g.wk = W_I64;
+        }
         g.vw = vw.p;
         return g;
     }
@@ -45,6 +51,14 @@ void own_weight(Ctx& c, const DGraph& g, const i32* labels, i64* out);
 
 // Contraction (src/contract.cpp:234-336) with sorted coarse rows.
 void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out);
+
+// Replaces g's int64 coarse weights by int32 ones when every weight fits:
+// always when the level it was contracted from has unit weights and fewer
+// than 2^31 nonzeros (a coarse weight then counts fine edges), else after a
+// max reduction.  The values, and so every result, are unchanged; the
+// sweeps read 4 instead of 8 weight bytes per edge and the level holds 8
+// instead of 12 bytes per nonzero.
+void narrow_weights(Ctx& c, const DGraph& fine, OwnedGraph& g);
 
 // next.vw[map[i]] += g.vw[i]  (src/clustering_methods.cpp:33-38)
 void coarsen_vertex_weights(Ctx& c, const DGraph& g, const i32* map, i64 nc, i64* out);

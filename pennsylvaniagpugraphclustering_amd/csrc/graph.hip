@@ -1,5 +1,6 @@
 // Degree ordering, evaluator, own-cluster weights and contraction kernels.
 #include <algorithm>
+#include <vector>
 
 #include "device.h"
 #include "graph.h"
@@ -503,7 +504,9 @@ __global__ void __launch_bounds__(256) k_cbig_insert(const i64* rp, const i32* c
                                                       const i32* members, const i64* mstart, const i64* mdp,
                                                       const i32* big, i64 nbig, const i64* items, const i64* goff,
                                                       i32* gkeys, i64* gvals, u64* gtab) {
-    const i64 b = block_id();
+    // big, items and goff point at a batch of rows: items[0] and goff[0] are
+    // the batch's first work item and first table slot
+    const i64 b = block_id() + items[0];
     if (b >= items[nbig]) return;
     const i64 ui = upper_row(items, 0, nbig, b);
     const i32 u = big[ui];
@@ -530,8 +533,9 @@ __global__ void __launch_bounds__(256) k_cbig_insert(const i64* rp, const i32* c
         }
     }
     const u32 gmask = (u32)(goff[ui + 1] - goff[ui] - 1);
-    if (gtab) lds_agg_flush_packed<256, BLDS>(lk, lv, gtab + goff[ui], gmask);  // kernel-uniform
-    else lds_agg_flush<256, BLDS>(lk, lv, gkeys + goff[ui], gvals + goff[ui], gmask);
+    const i64 t0 = goff[ui] - goff[0];
+    if (gtab) lds_agg_flush_packed<256, BLDS>(lk, lv, gtab + t0, gmask);  // kernel-uniform
+    else lds_agg_flush<256, BLDS>(lk, lv, gkeys + t0, gvals + t0, gmask);
 }
 
 // gtab non-null: packed tables (key << 32 | weight), else gkeys / gvals
@@ -540,8 +544,10 @@ __global__ void __launch_bounds__(256) k_cbig_extract(const i32* big, i64 nbig, 
                                                        const i64* toff, i32* tcol, i64* tw, i64* ccnt) {
     __shared__ int s_cnt;
     __shared__ i64 s_base;
-    const i64 b = block_id();
+    // a batch of rows, as in k_cbig_insert; slot sl lives at sl - g0
+    const i64 b = block_id() + items[0];
     if (b >= items[nbig]) return;
+    const i64 g0 = goff[0];
     const i64 ui = upper_row(items, 0, nbig, b);
     const i32 u = big[ui];
     const i64 s_lo = goff[ui] + (b - items[ui]) * BCH_S;
@@ -555,10 +561,10 @@ __global__ void __launch_bounds__(256) k_cbig_extract(const i32* big, i64 nbig, 
     for (int q = 0; q < 16; q++) {
         const i64 sl = s_lo + threadIdx.x + (i64)q * 256;
         if (gtab) {
-            const u64 e = sl < s_hi ? gtab[sl] : PEMPTY;
+            const u64 e = sl < s_hi ? gtab[sl - g0] : PEMPTY;
             kk[q] = e == PEMPTY ? EMPTY_KEY : (i32)(e >> 32);
         } else {
-            kk[q] = sl < s_hi ? gkeys[sl] : EMPTY_KEY;
+            kk[q] = sl < s_hi ? gkeys[sl - g0] : EMPTY_KEY;
         }
         pos[q] = kk[q] != EMPTY_KEY ? atomicAdd(&s_cnt, 1) : -1;
     }
@@ -571,7 +577,7 @@ __global__ void __launch_bounds__(256) k_cbig_extract(const i32* big, i64 nbig, 
         if (pos[q] >= 0) {
             const i64 sl = s_lo + threadIdx.x + (i64)q * 256;
             tcol[base + pos[q]] = kk[q];
-            tw[base + pos[q]] = gtab ? (i64)(gtab[sl] & 0xffffffffull) : gvals[sl];
+            tw[base + pos[q]] = gtab ? (i64)(gtab[sl - g0] & 0xffffffffull) : gvals[sl - g0];
         }
     }
 }
@@ -629,15 +635,15 @@ __global__ void k_big_count(const i32* list, i64 m, const i64* crp, i64* cnt) {
     i32 u = list[x];
     cnt[x] = crp[u + 1] - crp[u];
 }
-// key = (big row index << cb) | column (columns < 2^cb), so the radix sort
-// needs only cb + bits(big rows) bits
+// key = (big row index in the batch << cb) | column (columns < 2^cb), so the
+// radix sort needs only cb + bits(rows in the batch) bits
 __global__ void k_big_flatten(const i32* list, i64 m, const i64* crp, const i64* boff, const i32* ocol, int cb,
                               u64* key, i64* val, i64* target) {
     const i64 x = block_id();
     if (x >= m) return;
     {
         i32 u = list[x];
-        i64 s = crp[u], e = crp[u + 1], b = boff[x];
+        i64 s = crp[u], e = crp[u + 1], b = boff[x] - boff[0];  // within the batch
         for (i64 j = s + threadIdx.x; j < e; j += blockDim.x) {
             i64 f = b + (j - s);
             key[f] = ((u64)x << cb) | (u32)ocol[j];
@@ -688,10 +694,52 @@ static void contract_bytes(Ctx& c, int tk, const i32* list, i64 m, const i64* ms
     acct_add(c, c.kt_counter(tk), acc);
 }
 
+// batch limits of the big-row merge (table slots) and sort (entries)
+constexpr i64 CBIG_BATCH_SLOTS = (i64)1 << 30;
+constexpr i64 CBIG_SORT_ENTRIES = (i64)1 << 28;
+
+static void download_i64(Ctx& c, std::vector<i64>& dst, const i64* src, i64 count) {
+    dst.resize((size_t)count);
+    c.sync();
+    PGGC_HIP(hipMemcpy(dst.data(), src, (size_t)count * sizeof(i64), hipMemcpyDeviceToHost));
+}
+
 static int bits_for(i64 v) {
     int b = 0;
     while (b < 62 && ((i64)1 << b) <= v) b++;
     return b;
+}
+
+__global__ void __launch_bounds__(256) k_max_i64(const i64* __restrict__ in, i64 n, u64* out) {
+    u64 m = 0;
+    // block-uniform loop control (the reduction below is cross-lane)
+    for (i64 base = (i64)blockIdx.x * blockDim.x; base < n; base += (i64)gridDim.x * blockDim.x) {
+        const i64 x = base + threadIdx.x;
+        if (x < n) m = (u64)in[x] > m ? (u64)in[x] : m;  // weights are positive
+    }
+    m = block_max_u64<256>(m);
+    if (threadIdx.x == 0 && m) atomicMax((unsigned long long*)out, (unsigned long long)m);
+}
+__global__ void k_narrow(const i64* __restrict__ in, i64 n, i32* __restrict__ out) {
+    for (i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (i64)gridDim.x * blockDim.x)
+        out[x] = (i32)in[x];
+}
+
+void narrow_weights(Ctx& c, const DGraph& fine, OwnedGraph& g) {
+    if (g.nnz == 0 || !g.w.p) return;
+    bool fits = fine.wk == W_UNIT && fine.nnz < ((i64)1 << 31);
+    if (!fits) {
+        Buf<u64> mx(c, 1);
+        fill_i64(c, (i64*)mx.p, 1, 0);
+        k_max_i64<<<grid_for(g.nnz, 256 * 8, 8192), 256, 0, c.stream>>>(g.w, g.nnz, mx);
+        PGGC_LAUNCH_CHECK();
+        fits = (u64)c.read1((const i64*)mx.p) <= 0x7fffffffull;
+    }
+    if (!fits) return;
+    g.w32 = Buf<i32>(c, g.nnz);
+    k_narrow<<<grid_for(g.nnz, 256 * 8, 8192), 256, 0, c.stream>>>(g.w, g.nnz, g.w32);
+    PGGC_LAUNCH_CHECK();
+    g.w.reset();
 }
 
 void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out) {
@@ -708,6 +756,7 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
     k_cscatter<<<grid_items(n, 256), 256, 0, c.stream>>>(vcmap, n, pos, members);
     PGGC_LAUNCH_CHECK();
     const i64 tb = c.read1(toff.p + nc);
+    c.mem_note("contract: before row staging");
     Buf<i32> tcol(c, std::max<i64>(tb, 1));
     Buf<i64> tw(c, std::max<i64>(tb, 1));
     Buf<i64> ccnt(c, nc + 1);
@@ -739,46 +788,67 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
     contract_bytes(c, tk, list.p + st[0], n0 + n1 + n1b, mstart, bound, ccnt, wb);
     if (n2 > 0) {
         const i32* big = list.p + st[3];
-        Buf<i64> goff(c, n2 + 1), items(c, n2 + 1), mdp(c, n + 1);
+        Buf<i64> goff(c, n2 + 1), items(c, n2 + 1), items2(c, n2 + 1), mdp(c, n + 1);
         k_goff_sizes<<<grid_items(n2, 256), 256, 0, c.stream>>>(big, n2, bound, nc, goff);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, goff, goff, n2);
-        const i64 gs = c.read1(goff.p + n2);
         // packed tables (one CAS per new key) when every coarse weight fits
         // 32 bits: with unit weights a coarse weight counts fine edges
         const bool packed = g.wk == W_UNIT && g.nnz < ((i64)1 << 32) - 1;
-        Buf<i32> gkeys(c, packed ? 1 : gs);
-        Buf<i64> gvals(c, packed ? 1 : gs);
-        Buf<u64> gtab(c, packed ? gs : 1);
-        if (packed) {
-            fill_i64(c, (i64*)gtab.p, gs, (i64)PEMPTY);
-        } else {
-            fill_i32(c, gkeys, gs, EMPTY_KEY);
-            fill_i64(c, gvals, gs, 0);
-        }
-        u64* gt = packed ? gtab.p : nullptr;
         k_member_degs<<<grid_items(n, 256), 256, 0, c.stream>>>(members, g.rp, n, mdp);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, mdp, mdp, n);
         k_items<<<grid_items(n2, 256), 256, 0, c.stream>>>(big, n2, bound, goff, nc, 0, items);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, items, items, n2);
-        const i64 I = c.read1(items.p + n2);
-        tk = c.kt_begin(KS_CONTRACT_BIG);
-        if (I > 0) {
-            PGGC_WDISPATCH(g, (k_cbig_insert<<<grid_blocks(I), 256, 0, c.stream>>>(
-                                  g.rp, g.col, W, vcmap, members, mstart, mdp, big, n2, items, goff, gkeys, gvals,
-                                  gt)));
-            PGGC_LAUNCH_CHECK();
+        k_items<<<grid_items(n2, 256), 256, 0, c.stream>>>(big, n2, bound, goff, nc, 1, items2);
+        PGGC_LAUNCH_CHECK();
+        scan_i64(c, items2, items2, n2);
+        // rows in batches whose tables hold at most CBIG_BATCH_SLOTS slots
+        // together (a larger row is a batch of its own), so the tables'
+        // memory stays bounded on graphs of billions of edges
+        std::vector<i64> hg, hi, hi2;
+        download_i64(c, hg, goff, n2 + 1);
+        download_i64(c, hi, items, n2 + 1);
+        download_i64(c, hi2, items2, n2 + 1);
+        std::vector<i64> cuts{0};
+        i64 gmax = 0;
+        while (cuts.back() < n2) {
+            const i64 r0 = cuts.back();
+            i64 r1 = r0 + 1;
+            while (r1 < n2 && hg[r1 + 1] - hg[r0] <= CBIG_BATCH_SLOTS) r1++;
+            gmax = std::max(gmax, hg[r1] - hg[r0]);
+            cuts.push_back(r1);
         }
-        k_items<<<grid_items(n2, 256), 256, 0, c.stream>>>(big, n2, bound, goff, nc, 1, items);
-        PGGC_LAUNCH_CHECK();
-        scan_i64(c, items, items, n2);
-        const i64 I2 = c.read1(items.p + n2);
-        k_cbig_extract<<<grid_blocks(I2), 256, 0, c.stream>>>(big, n2, items, goff, gkeys, gvals, gt, toff, tcol, tw,
-                                                              ccnt);
+        Buf<i32> gkeys(c, packed ? 1 : gmax);
+        Buf<i64> gvals(c, packed ? 1 : gmax);
+        Buf<u64> gtab(c, packed ? gmax : 1);
+        u64* gt = packed ? gtab.p : nullptr;
+        c.mem_note("contract: big-row tables");
+        tk = c.kt_begin(KS_CONTRACT_BIG);
+        for (size_t q = 0; q + 1 < cuts.size(); q++) {
+            const i64 r0 = cuts[q], r1 = cuts[q + 1], gsb = hg[r1] - hg[r0];
+            if (packed) {
+                fill_i64(c, (i64*)gtab.p, gsb, (i64)PEMPTY);
+            } else {
+                fill_i32(c, gkeys, gsb, EMPTY_KEY);
+                fill_i64(c, gvals, gsb, 0);
+            }
+            const i64 I = hi[r1] - hi[r0];
+            if (I > 0) {
+                PGGC_WDISPATCH(g, (k_cbig_insert<<<grid_blocks(I), 256, 0, c.stream>>>(
+                                      g.rp, g.col, W, vcmap, members, mstart, mdp, big + r0, r1 - r0, items.p + r0,
+                                      goff.p + r0, gkeys, gvals, gt)));
+                PGGC_LAUNCH_CHECK();
+            }
+            const i64 I2 = hi2[r1] - hi2[r0];
+            if (I2 > 0) {
+                k_cbig_extract<<<grid_blocks(I2), 256, 0, c.stream>>>(big + r0, r1 - r0, items2.p + r0, goff.p + r0,
+                                                                      gkeys, gvals, gt, toff, tcol, tw, ccnt);
+                PGGC_LAUNCH_CHECK();
+            }
+        }
         c.kt_end(tk, 0.0);
-        PGGC_LAUNCH_CHECK();
         contract_bytes(c, tk, big, n2, mstart, bound, ccnt, wb);
     }
     out.n = nc;
@@ -797,26 +867,45 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
         c.kt_end(tkc, bytes);
         PGGC_LAUNCH_CHECK();
     }
+    tcol.reset();  // the staged rows are copied out
+    tw.reset();
     if (n2 > 0) {
+        // the rows merged in HBM tables come out unsorted: sort each by
+        // column, in batches of at most CBIG_SORT_ENTRIES entries (a larger
+        // row is a batch of its own)
         const i32* big = list.p + st[3];
         Buf<i64> boff(c, n2 + 1);
         k_big_count<<<grid_items(n2, 256), 256, 0, c.stream>>>(big, n2, out.rp, boff);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, boff, boff, n2);
-        i64 E = c.read1(boff.p + n2);
-        if (E > 0) {
-            Buf<u64> key(c, E);
-            Buf<i64> val(c, E), target(c, E), wtmp(c, E);
+        std::vector<i64> hb;
+        download_i64(c, hb, boff, n2 + 1);
+        std::vector<i64> cuts{0};
+        i64 emax = 0;
+        while (cuts.back() < n2) {
+            const i64 r0 = cuts.back();
+            i64 r1 = r0 + 1;
+            while (r1 < n2 && hb[r1 + 1] - hb[r0] <= CBIG_SORT_ENTRIES) r1++;
+            emax = std::max(emax, hb[r1] - hb[r0]);
+            cuts.push_back(r1);
+        }
+        if (emax > 0) {
+            Buf<u64> key(c, emax);
+            Buf<i64> val(c, emax), target(c, emax), wtmp(c, emax);
             const int cb = bits_for(nc - 1);
-            k_big_flatten<<<grid_blocks(n2), 256, 0, c.stream>>>(big, n2, out.rp, boff, out.col, cb, key, val,
-                                                                         target);
-            PGGC_LAUNCH_CHECK();
-            radix_sort_pairs(c, key, val, E, cb + bits_for(n2));
-            k_big_gather<<<grid_for(E, 256 * 4, 8192), 256, 0, c.stream>>>(key, val, target, E, out.w, wtmp);
-            PGGC_LAUNCH_CHECK();
-            k_big_scatter<<<grid_for(E, 256 * 4, 8192), 256, 0, c.stream>>>(key, target, E, wtmp, cb, out.col,
-                                                                            out.w);
-            PGGC_LAUNCH_CHECK();
+            for (size_t q = 0; q + 1 < cuts.size(); q++) {
+                const i64 r0 = cuts[q], r1 = cuts[q + 1], E = hb[r1] - hb[r0];
+                if (E == 0) continue;
+                k_big_flatten<<<grid_blocks(r1 - r0), 256, 0, c.stream>>>(big + r0, r1 - r0, out.rp, boff.p + r0,
+                                                                          out.col, cb, key, val, target);
+                PGGC_LAUNCH_CHECK();
+                radix_sort_pairs(c, key, val, E, cb + bits_for(r1 - r0));
+                k_big_gather<<<grid_for(E, 256 * 4, 8192), 256, 0, c.stream>>>(key, val, target, E, out.w, wtmp);
+                PGGC_LAUNCH_CHECK();
+                k_big_scatter<<<grid_for(E, 256 * 4, 8192), 256, 0, c.stream>>>(key, target, E, wtmp, cb, out.col,
+                                                                                out.w);
+                PGGC_LAUNCH_CHECK();
+            }
         }
     }
     (void)n1;

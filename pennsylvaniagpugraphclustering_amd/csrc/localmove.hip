@@ -291,13 +291,12 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
             for (i64 x = 0; x < m7; x++)
                 if (hs[x] >= ((i64)1 << 32) - 1) p.packed = false;
         }
-        // filled at each build (hub_live is false until the first one)
-        if (p.packed) {
-            p.gtab = Buf<u64>(c, p.gslots);
-        } else {
-            p.gkeys = Buf<i32>(c, p.gslots);
-            p.gvals = Buf<i64>(c, p.gslots);
-        }
+        // the tables themselves (and their slot lists) are allocated by
+        // hub_tables_acquire when a local move starts
+        p.gtab.reset();
+        p.gkeys.reset();
+        p.gvals.reset();
+        p.hlist.reset();
         p.soff = Buf<i64>(c, m7 + 1);
         k_slice_counts<<<grid_items(m7, 256), 256, 0, c.stream>>>(p.goff, m7, hub_slice(p.packed), p.soff);
         PGGC_LAUNCH_CHECK();
@@ -305,7 +304,6 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
         p.nslices = c.read1(p.soff.p + m7);
         p.hflag = Buf<i32>(c, m7);
         p.hovf = Buf<i32>(c, 1);
-        p.hlist = Buf<u32>(c, p.gslots);
         p.hcnt = Buf<i32>(c, m7);
         p.hsm = Buf<u32>(c, m7);
         xcd_maps(c, p, m7);
@@ -314,6 +312,31 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p) {
         k_hub_index<<<grid_items(m7, 256), 256, 0, c.stream>>>(hubs, m7, p.hidx);
         PGGC_LAUNCH_CHECK();
     }
+    p.hub_live = false;
+}
+
+// Hub tables and slot lists, the bulk of a plan (about 1.5-3 slots per hub
+// edge), exist only while a local move of the level runs; they are rebuilt
+// at the first table sweep of every call anyway.  Released at the end of
+// the call (to the context's cache, which usually hands the same block back
+// at the level's next call), they leave the room to the contraction and the
+// other levels of the hierarchy on graphs of billions of edges.
+static void hub_tables_acquire(Ctx& c, LevelPlan& p) {
+    if (p.nhub == 0 || p.gslots == 0) return;
+    if (p.packed) {
+        if (!p.gtab.p) p.gtab = Buf<u64>(c, p.gslots);
+    } else if (!p.gkeys.p) {
+        p.gkeys = Buf<i32>(c, p.gslots);
+        p.gvals = Buf<i64>(c, p.gslots);
+    }
+    if (!p.hlist.p) p.hlist = Buf<u32>(c, p.gslots);
+}
+static void hub_tables_settle(Ctx& c, LevelPlan& p) {
+    (void)c;
+    p.gtab.reset();
+    p.gkeys.reset();
+    p.gvals.reset();
+    p.hlist.reset();
     p.hub_live = false;
 }
 
@@ -2018,7 +2041,9 @@ static void local_move_t(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g,
     const i64 n = g.n;
     s.ensure(c, n);
     plan_level(c, g, lp);
+    hub_tables_acquire(c, lp);
     lp.hub_live = false;  // the clustering changed since the last call
+    c.mem_note("local move: plan ready");
     Buf<i64> cv;
     RefineData curr = begin_curr(c, best, cv, n);
     copy_dev(c, s.part, best_part, n * sizeof(i32));
@@ -2099,6 +2124,7 @@ static void local_move_t(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g,
         }
     }
     relabel(c, s, n, best_part, best);
+    hub_tables_settle(c, lp);
 }
 
 // afterburner_filter_strict (src/local_mover.cpp:131-213): exact sequential
@@ -2169,6 +2195,7 @@ void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i
     const i64 n = g.n;
     s.ensure(c, n);
     plan_level(c, g, lp);
+    hub_tables_acquire(c, lp);
     lp.hub_live = false;  // the clustering changed since the last call
     Buf<i64> cv;
     RefineData curr = begin_curr(c, best, cv, n);
@@ -2218,6 +2245,7 @@ void local_move_strict(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, i
         trc.end(k, tr);
     }
     relabel(c, s, n, best_part, best);
+    hub_tables_settle(c, lp);
 }
 
 i64 sweep_once(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32* part, const i64* vol, i64 L,
@@ -2227,6 +2255,7 @@ i64 sweep_once(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32
     (void)L;
     s.ensure(c, n);
     plan_level(c, g, lp);
+    hub_tables_acquire(c, lp);
     lp.hub_live = false;
     copy_dev(c, s.part, part, n * sizeof(i32));
     copy_dev(c, s.dest, dest, n * sizeof(i32));
@@ -2245,6 +2274,7 @@ i64 sweep_once(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, const i32
     copy_dev(c, dest, s.dest, n * sizeof(i32));
     copy_dev(c, gain, s.gain, n * sizeof(float));
     c.sync();
+    hub_tables_settle(c, lp);
     return km;
 }
 
@@ -2523,6 +2553,7 @@ void local_move_batch(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
     const i64 n = g.n, K = bl.K, NL = best.loff[K];
     s.ensure(c, std::max(n, NL));
     plan_level(c, g, lp);
+    hub_tables_acquire(c, lp);
     lp.hub_live = false;
     Buf<i64> cvol(c, std::max<i64>(NL, 1));
     copy_dev(c, cvol, best.vol, NL * sizeof(i64));
@@ -2663,6 +2694,7 @@ void local_move_batch(Ctx& c, MoveScratch& s, LevelPlan& lp, const DGraph& g, co
         }
     }
     relabel_batch(c, s, bl, best_part, best, active);
+    hub_tables_settle(c, lp);
 }
 
 }  // namespace pggc

@@ -1,6 +1,7 @@
 // Context allocator and data-parallel primitives: fill, scan, ordered
 // compaction, stable LSD radix sort, exact integer reductions.
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -40,12 +41,19 @@ void* Ctx::alloc(size_t bytes) {
         trim();
         if (hipMalloc(&p, sz) != hipSuccess) {
             (void)hipGetLastError();
-            throw Error(-2, "device allocation of " + std::to_string(sz) + " bytes failed");
+            throw Error(-2, "device allocation of " + std::to_string(sz) + " bytes failed (" +
+                                std::to_string(bytes_live) + " bytes live in this context)");
         }
     }
     live[p] = sz;
     bytes_live += sz;
     return p;
+}
+
+void Ctx::mem_note(const char* where) const {
+    if (verbose < 2) return;
+    fprintf(stderr, "[mem] %s: %.2f GiB live, %.2f GiB cached\n", where, bytes_live / 1073741824.0,
+            bytes_cached / 1073741824.0);
 }
 
 void Ctx::release(void* p) {
