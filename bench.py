@@ -67,6 +67,10 @@ def parse():
     p.add_argument("--leiden", type=int, default=2, metavar="ITERS",
                    help="pLeiden+ leg with ex_iters=ITERS (-1: to stable, which creeps up by ~1e-8 per outer "
                         "iteration for hundreds of iterations at scale 24; 0 or less than -1 skips it)")
+    p.add_argument("--c5", action="store_true",
+                   help="only run the largest single-GPU workload and exit: R-MAT scale 27, edge factor 16, weights "
+                        "uniform in [1, 10], seed 4 (4.2e9 nonzeros) built on the GPU, one pLeiden+ pass and one "
+                        "pLouvain pass (ex_iters 0, weighted modularity), each checked against the evaluator")
     return p.parse_args()
 
 
@@ -245,6 +249,34 @@ def batch_leg(a, ctx, C, graphs, torch):
             "mean_Q": float(np.mean([r[0] for r in res]))}
 
 
+def c5_leg(C, graphs, torch):
+    """The largest single-GPU workload: R-MAT scale 27 with weights in
+    [1, 10] (seed 4).  pLeiden+ to stable is not run (each outer iteration
+    is a full pass of about half a minute here); one pass of pLeiden+ and of
+    pLouvain, each with its objective recomputed by the evaluator."""
+    ctx = C.default_context()
+    t0 = time.perf_counter()
+    rp, col, w = graphs.rmat_device(27, 16, seed=4, weights=(1, 10), ctx=ctx)
+    torch.cuda.synchronize()
+    out = {"workload": "R-MAT scale 27 ef16 (.57,.19,.19,.05) seed 4, weights uniform in [1, 10], WMod",
+           "n": int(rp.numel() - 1), "nnz": int(col.numel()), "generate_s": time.perf_counter() - t0, "runs": {}}
+    g = C.DeviceGraph(rp, col, w, None, objective="WMod", ctx=ctx)
+    for algo in ("leiden+", "louvain"):
+        print(f"# c5 {algo} ex_iters=0", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        q, part, info = C.cluster(g, algo, "WMod", 1.0, 0, seed=1, ctx=ctx)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t1
+        qe = C.modularity(g, part, "WMod", ctx=ctx)
+        out["runs"][algo] = {"seconds": sec, "edges_per_s": (g.nnz // 2) / sec, "Q": q, "evaluator_Q": qe,
+                             "matches_evaluator": q == qe, **info}
+        print(f"# c5 {algo}: {sec:.2f} s Q={q!r} info={info}", file=sys.stderr, flush=True)
+        del part
+    out["free_gib_after"] = torch.cuda.mem_get_info()[0] / 2 ** 30
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -261,6 +293,9 @@ def main():
         return
     from pennsylvaniagpugraphclustering_amd import graphs
     from pennsylvaniagpugraphclustering_amd import clustering as C
+    if a.c5:
+        print(json.dumps({"c5": c5_leg(C, graphs, torch)}))
+        return
 
     t0 = time.perf_counter()
     ctx = C.default_context()

@@ -12,8 +12,8 @@ offsets):
   columns without self loops, positive weights summing to nnz minus the
   uncut weight, and row sums equal to vol[a] minus the inside weight of
   cluster a;
-- a whole pLouvain run: the objective it reports is the evaluator's
-  objective of the partition it returns."""
+- a whole pLouvain run and a pLeiden+ pass: the objective each reports is
+  the evaluator's objective of the partition it returns."""
 import ctypes
 
 import pytest
@@ -123,6 +123,22 @@ def test_louvain_beyond_2_31_nonzeros(big, gpu_ctx):
     q, part, info = C.cluster(dg, "louvain", "Mod", 1.0, 0, seed=1)
     print(f"n={dg.n} nnz={dg.nnz} Q={q} info={info}", flush=True)
     assert part.numel() == dg.n
+    k = info["labels"]
+    assert int(part.min()) >= 0 and int(part.max()) < k
+    assert int(torch.unique(part).numel()) == k
+    assert q == C.modularity(dg, part)
+    assert 0.0 < q < 1.0
+
+
+def test_leiden_plus_beyond_2_31_nonzeros(big, gpu_ctx):
+    """One pLeiden+ pass (ex_iters 0) on the graph: the reported objective is
+    the evaluator's, and the labels are a partition into `labels` clusters."""
+    import torch
+    from pennsylvaniagpugraphclustering_amd import clustering as C
+    dg = big[0]
+    print("cluster leiden+", flush=True)
+    q, part, info = C.cluster(dg, "leiden+", "Mod", 1.0, 0, seed=1)
+    print(f"Q={q} info={info}", flush=True)
     k = info["labels"]
     assert int(part.min()) >= 0 and int(part.max()) < k
     assert int(torch.unique(part).numel()) == k
