@@ -5,7 +5,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <iostream>
+#include <string>
 #include <unordered_set>
 #include <vector>
 
@@ -204,9 +206,52 @@ DeviceGraph::~DeviceGraph() {
     (void)hipFree(col);
     (void)hipFree(w);
     (void)hipFree(vw);
+    if (gen) (void)pggc_devgraph_free(gen);
+}
+
+// rmat:SCALE:EDGE_FACTOR:SEED[:WMIN:WMAX] -> a graph built on the device
+static bool generate_device_graph(pggc_ctx* ctx, const BaseArgs& args, DeviceGraph& dg) {
+    std::vector<long long> f;
+    std::string spec = args.graph_file.substr(5);
+    size_t at = 0;
+    while (at <= spec.size()) {
+        const size_t nx = spec.find(':', at);
+        const std::string tok = spec.substr(at, nx == std::string::npos ? std::string::npos : nx - at);
+        char* end = nullptr;
+        const long long v = std::strtoll(tok.c_str(), &end, 10);
+        if (tok.empty() || *end != '\0') {
+            std::cerr << "FATAL ERROR: graph spec must be rmat:SCALE:EDGE_FACTOR:SEED[:WMIN:WMAX]" << std::endl;
+            return false;
+        }
+        f.push_back(v);
+        if (nx == std::string::npos) break;
+        at = nx + 1;
+    }
+    if (f.size() != 3 && f.size() != 5) {
+        std::cerr << "FATAL ERROR: graph spec must be rmat:SCALE:EDGE_FACTOR:SEED[:WMIN:WMAX]" << std::endl;
+        return false;
+    }
+    const int32_t wmin = f.size() == 5 ? (int32_t)f[3] : 0, wmax = f.size() == 5 ? (int32_t)f[4] : 0;
+    if (!check(pggc_rmat(ctx, (int)f[0], (int)f[1], 0.57, 0.19, 0.19, (uint64_t)f[2], wmin, wmax, &dg.gen),
+               "generating the graph"))
+        return false;
+    pggc_graph v{};
+    if (!check(pggc_devgraph_view(dg.gen, &v), "generating the graph")) return false;
+    std::cout << "Vertex Count: " << v.n << "; Undirected Edge Count: " << v.nnz / 2 << std::endl;
+    std::cout << std::endl;
+    dg.weighted = v.w != nullptr && !uses_unit_edges(args.obj);
+    if (!hip_ok(hipMalloc((void**)&dg.vw, (v.n > 0 ? v.n : 1) * sizeof(int64_t)), "device allocation")) return false;
+    dg.g = pggc_graph{v.n, v.nnz, v.row_ptr, v.col, dg.weighted ? v.w : nullptr, nullptr, dg.vw};
+    if (!args.vw_file.empty()) {
+        std::vector<int64_t> vw(v.n);
+        if (!check(pggc_read_ints(args.vw_file.c_str(), v.n, vw.data()), "reading the vertex weights")) return false;
+        return hip_ok(hipMemcpy(dg.vw, vw.data(), v.n * sizeof(int64_t), hipMemcpyHostToDevice), "upload");
+    }
+    return check(pggc_vertex_weights(ctx, &dg.g, objective_code(args.obj), dg.vw), "vertex weights");
 }
 
 bool load_device_graph(pggc_ctx* ctx, const BaseArgs& args, DeviceGraph& dg) {
+    if (args.graph_file.rfind("rmat:", 0) == 0) return generate_device_graph(ctx, args, dg);
     pggc_host_graph* hg = nullptr;
     int weighted = 0;
     if (!check(pggc_read_graph(args.graph_file.c_str(), uses_unit_edges(args.obj) ? 1 : 0, &hg, &weighted),

@@ -51,10 +51,14 @@ struct DeviceGraph {
     int32_t* w = nullptr;
     int64_t* vw = nullptr;
     bool weighted = false;
+    pggc_devgraph* gen = nullptr;  // a generated graph (owns the arrays g points at)
     ~DeviceGraph();
 };
 
-// Reads the graph file (unit edges if the objective says so), prints the
+// Reads the graph file, or builds the graph on the GPU when the name is
+// rmat:SCALE:EDGE_FACTOR:SEED[:WMIN:WMAX] (R-MAT with a,b,c = .57,.19,.19,
+// per-pair weights uniform in [WMIN, WMAX] when given), with unit edges if
+// the objective says so; prints the
 // vertex / undirected edge counts, uploads it and sets the vertex weights
 // (a -vtx_weights file, else degree / weighted degree / unit by objective).
 // Returns false after printing a FATAL ERROR line.

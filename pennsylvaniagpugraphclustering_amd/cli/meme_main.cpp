@@ -44,6 +44,10 @@ int main(int argc, char** argv) {
     pggc_ctx* ctx = nullptr;
     if (!cli::check(pggc_ctx_create(&ctx, nullptr), "creating the GPU context")) return -1;
     const char* seed_env = std::getenv("PGGC_SEED");
+    // PGGC_PROGRESS=1: a line on stderr after each improvement pass (long
+    // passes on large graphs otherwise print nothing for minutes)
+    const char* progress_env = std::getenv("PGGC_PROGRESS");
+    const bool progress = progress_env && progress_env[0] == '1';
     const uint64_t seed = seed_env ? std::strtoull(seed_env, nullptr, 10) : 0;
     pggc_ctx_set_seed(ctx, seed, 0);
     int rc = 0;
@@ -117,10 +121,11 @@ int main(int argc, char** argv) {
                 break;
             }
             std::cout << "Parent 1 obj: " << pop[p1].obj << "; Parent 2 obj: " << pop[p2].obj
-                      << "; Offspring obj: " << objective_of(st);
+                      << "; Offspring obj: " << objective_of(st) << std::flush;
             for (int x = 0; x < 5 && rc == 0; x++) {
                 if (!cli::check(pggc_leiden(ctx, g, &st, 1, 1, c3, c3b), "pLeiden+ improvement")) rc = -1;
                 std::swap(c3, c3b);
+                if (progress) std::cerr << "meme: improvement pass " << x + 1 << " of 5: " << objective_of(st) << std::endl;
             }
             if (rc) break;
             const double o3 = objective_of(st);

@@ -162,3 +162,22 @@ def test_meme_tool(tmp_path):
     vol, uncut, sq = O.evaluate(g.rp, g.col, None, g.vertex_weights("Mod"), part, int(part.max()) + 1)
     q = (uncut - sq / g.nnz) / g.nnz
     assert abs(q - final) < 1e-8
+
+
+@pytest.mark.parametrize("spec,weights,objective", [("rmat:14:16:2", None, "Mod"),
+                                                    ("rmat:13:16:3:1:10", (1, 10), "WMod")])
+def test_generated_graph_spec(tmp_path, spec, weights, objective):
+    """-i rmat:SCALE:EDGE_FACTOR:SEED[:WMIN:WMAX] builds the graph on the GPU
+    (the library's R-MAT generator, bit-identical to the oracle's): the tool's
+    partition equals the oracle's run on the oracle-generated graph."""
+    f = [int(x) for x in spec.split(":")[1:4]]
+    rp, col, w = O.rmat(f[0], f[1], seed=f[2], weights=weights)
+    out_p = tmp_path / "part.txt"
+    out = run("pleiden+", "-i", spec, "-objective", objective, "-o", str(out_p), "-ex_iters", "1")
+    vw = np.diff(rp).astype(np.int64) if objective == "Mod" else np.bincount(
+        np.repeat(np.arange(len(rp) - 1), np.diff(rp)), weights=w.astype(np.float64), minlength=len(rp) - 1
+    ).astype(np.int64)
+    q, part, info = O.cluster(rp, col, None if objective == "Mod" else w, vw, objective, 1.0, "leiden+", 1, seed=0)
+    assert f"Vertex Count: {len(rp) - 1}; Undirected Edge Count: {int(rp[-1]) // 2}" in out
+    assert np.array_equal(read_part(out_p), part)
+    assert f"{best_objective(out):.9g}" == f"{q:.9g}"
