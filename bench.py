@@ -71,6 +71,10 @@ def parse():
                    help="only run the largest single-GPU workload and exit: R-MAT scale 27, edge factor 16, weights "
                         "uniform in [1, 10], seed 4 (4.2e9 nonzeros) built on the GPU, one pLeiden+ pass and one "
                         "pLouvain pass (ex_iters 0, weighted modularity), each checked against the evaluator")
+    p.add_argument("--c4b", action="store_true",
+                   help="only run the seeded-trials workload and exit: 128 pLeiden+ trials (ex_iters 0, seeds "
+                        "0..127) on one R-MAT scale-22 graph (edge factor 16, seed 2), one by one and as one "
+                        "pggc_cluster_batch call, every trial checked against its single run")
     return p.parse_args()
 
 
@@ -277,6 +281,31 @@ def c5_leg(C, graphs, torch):
     return out
 
 
+def c4b_leg(C, graphs, torch, trials=128):
+    """Seeded pLeiden+ trials on one graph (the reference's -trials loop,
+    app/cluster_main.cpp:189-204): one by one, then as one batch call, whose
+    workers share the context's stream (kernels of one trial at a time)."""
+    ctx = C.default_context()
+    rp, col, _ = graphs.rmat_device(22, 16, seed=2, ctx=ctx)
+    g = C.DeviceGraph(rp, col, None, None, objective="Mod", ctx=ctx)
+    seeds = list(range(trials))
+    C.cluster(g, "leiden+", "Mod", 1.0, 0, seed=0, ctx=ctx)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    single = [C.cluster(g, "leiden+", "Mod", 1.0, 0, seed=s, ctx=ctx)[0] for s in seeds]
+    torch.cuda.synchronize()
+    one_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = C.cluster_batch([g] * trials, "leiden+", "Mod", 1.0, 0, seeds=seeds, streams=8, ctx=ctx)
+    torch.cuda.synchronize()
+    batch_s = time.perf_counter() - t0
+    return {"workload": f"{trials} pLeiden+ trials (ex_iters 0, seeds 0..{trials - 1}) on R-MAT scale 22 ef16 seed 2",
+            "n": g.n, "nnz": g.nnz, "one_by_one_s": one_s, "batch_s": batch_s, "speedup": one_s / batch_s,
+            "mode": "worker host threads sharing the context's stream",
+            "objectives_equal": all(r[0] == q for r, q in zip(res, single)),
+            "best_Q": max(single), "mean_Q": sum(single) / len(single)}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -295,6 +324,9 @@ def main():
     from pennsylvaniagpugraphclustering_amd import clustering as C
     if a.c5:
         print(json.dumps({"c5": c5_leg(C, graphs, torch)}))
+        return
+    if a.c4b:
+        print(json.dumps({"c4b": c4b_leg(C, graphs, torch)}))
         return
 
     t0 = time.perf_counter()
