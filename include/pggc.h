@@ -172,7 +172,12 @@ int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user);
  *     graph.  1: always the copy; 2 (default): the copy when the call makes
  *     more than one top-level pass (pggc_cluster with ex_iters != 0), which
  *     pays for building it; 0: never.  Not used with the active-set
- *     predictor on. */
+ *     predictor on.
+ *   PGGC_KNOB_CONTRACT_BATCH: the contraction merges its largest coarse rows
+ *     in HBM hash tables and sorts them in batches of rows holding at most
+ *     this many table slots (default 2^30) and a quarter as many sort
+ *     entries, which bounds its memory on graphs of billions of edges; the
+ *     coarse graph is the same for every value >= 1. */
 #define PGGC_KNOB_HUB_UPDATE_DIV 0
 #define PGGC_KNOB_GNN_MAX_SWEEPS 1
 #define PGGC_KNOB_TABLE_THREADS 2
@@ -183,7 +188,8 @@ int pggc_ctx_set_sweep_trace(pggc_ctx* ctx, pggc_sweep_fn fn, void* user);
 #define PGGC_KNOB_RAW_SWEEPS 7
 #define PGGC_KNOB_HUB_BUILD 8
 #define PGGC_KNOB_RENUMBER 9
-#define PGGC_KNOB_COUNT 10
+#define PGGC_KNOB_CONTRACT_BATCH 10
+#define PGGC_KNOB_COUNT 11
 int pggc_ctx_set_knob(pggc_ctx* ctx, int knob, int64_t value);
 int pggc_ctx_get_knob(pggc_ctx* ctx, int knob, int64_t* value);
 /* Autotuner of the execution-shape knobs (no counterpart in the reference):

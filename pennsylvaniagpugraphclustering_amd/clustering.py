@@ -86,7 +86,7 @@ class Context:
 
     KNOBS = {"hub_update_div": 0, "gnn_max_sweeps": 1, "table_threads": 2, "table_slots_per_edge": 3,
              "hub_lds_max_degree": 4, "hub_edges_per_item": 5, "hub_slots_per_item": 6, "raw_sweeps": 7,
-             "hub_build": 8, "renumber": 9}
+             "hub_build": 8, "renumber": 9, "contract_batch": 10}
 
     # values the execution-shape knobs accept (results are identical for all)
     KNOB_VALUES = {"table_threads": (64, 128, 256), "table_slots_per_edge": (2, 4),

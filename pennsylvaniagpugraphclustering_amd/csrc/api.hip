@@ -693,6 +693,7 @@ static void set_knob(Ctx& c, int knob, int64_t value) {
         case PGGC_KNOB_RAW_SWEEPS: c.raw_sweeps = value ? 1 : 0; break;
         case PGGC_KNOB_HUB_BUILD: c.hub_build = (int)std::min<int64_t>(value, 2); break;
         case PGGC_KNOB_RENUMBER: c.renumber = (int)std::min<int64_t>(value, 2); break;
+        case PGGC_KNOB_CONTRACT_BATCH: c.contract_batch = std::max<int64_t>(value, 1); break;
         default: throw Error(PGGC_EINVAL, "unknown knob");
     }
     if (shape) c.knob_epoch++;  // level plans depend on these
@@ -710,6 +711,7 @@ static int64_t get_knob(const Ctx& c, int knob) {
         case PGGC_KNOB_RAW_SWEEPS: return c.raw_sweeps;
         case PGGC_KNOB_HUB_BUILD: return c.hub_build;
         case PGGC_KNOB_RENUMBER: return c.renumber;
+        case PGGC_KNOB_CONTRACT_BATCH: return c.contract_batch;
         default: throw Error(PGGC_EINVAL, "unknown knob");
     }
 }

@@ -99,6 +99,9 @@ struct Ctx {
     // the top level's local mover runs on a copy renumbered by degree:
     // 0 never, 1 always, 2 when a call makes several top-level passes
     int renumber = 2;
+    // big-row contraction batches: table slots per batch (sort entries per
+    // batch: a quarter of it)
+    i64 contract_batch = (i64)1 << 30;
     u64 knob_epoch = 0;  // bumped by every knob change; level plans are rebuilt when it moves
     // the active-set predictor runs on at most this many sweeps of a
     // local-move call (0: on every eligible sweep)

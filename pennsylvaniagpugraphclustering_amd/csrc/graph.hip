@@ -694,9 +694,6 @@ static void contract_bytes(Ctx& c, int tk, const i32* list, i64 m, const i64* ms
     acct_add(c, c.kt_counter(tk), acc);
 }
 
-// batch limits of the big-row merge (table slots) and sort (entries)
-constexpr i64 CBIG_BATCH_SLOTS = (i64)1 << 30;
-constexpr i64 CBIG_SORT_ENTRIES = (i64)1 << 28;
 
 static void download_i64(Ctx& c, std::vector<i64>& dst, const i64* src, i64 count) {
     dst.resize((size_t)count);
@@ -804,9 +801,10 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
         k_items<<<grid_items(n2, 256), 256, 0, c.stream>>>(big, n2, bound, goff, nc, 1, items2);
         PGGC_LAUNCH_CHECK();
         scan_i64(c, items2, items2, n2);
-        // rows in batches whose tables hold at most CBIG_BATCH_SLOTS slots
+        // rows in batches whose tables hold at most c.contract_batch slots
         // together (a larger row is a batch of its own), so the tables'
         // memory stays bounded on graphs of billions of edges
+        const i64 slot_cap = std::max<i64>(c.contract_batch, 1);
         std::vector<i64> hg, hi, hi2;
         download_i64(c, hg, goff, n2 + 1);
         download_i64(c, hi, items, n2 + 1);
@@ -816,7 +814,7 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
         while (cuts.back() < n2) {
             const i64 r0 = cuts.back();
             i64 r1 = r0 + 1;
-            while (r1 < n2 && hg[r1 + 1] - hg[r0] <= CBIG_BATCH_SLOTS) r1++;
+            while (r1 < n2 && hg[r1 + 1] - hg[r0] <= slot_cap) r1++;
             gmax = std::max(gmax, hg[r1] - hg[r0]);
             cuts.push_back(r1);
         }
@@ -871,8 +869,9 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
     tw.reset();
     if (n2 > 0) {
         // the rows merged in HBM tables come out unsorted: sort each by
-        // column, in batches of at most CBIG_SORT_ENTRIES entries (a larger
-        // row is a batch of its own)
+        // column, in batches of at most c.contract_batch / 4 entries (a
+        // larger row is a batch of its own)
+        const i64 sort_cap = std::max<i64>(c.contract_batch / 4, 1);
         const i32* big = list.p + st[3];
         Buf<i64> boff(c, n2 + 1);
         k_big_count<<<grid_items(n2, 256), 256, 0, c.stream>>>(big, n2, out.rp, boff);
@@ -885,7 +884,7 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
         while (cuts.back() < n2) {
             const i64 r0 = cuts.back();
             i64 r1 = r0 + 1;
-            while (r1 < n2 && hb[r1 + 1] - hb[r0] <= CBIG_SORT_ENTRIES) r1++;
+            while (r1 < n2 && hb[r1 + 1] - hb[r0] <= sort_cap) r1++;
             emax = std::max(emax, hb[r1] - hb[r0]);
             cuts.push_back(r1);
         }

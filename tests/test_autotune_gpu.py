@@ -1,7 +1,7 @@
 """The execution-shape knobs (threads per vertex and slots per edge of the
 LDS-table scoring, the LDS/HBM cut-off for hubs, hub build and scan item
 sizes, the hub-table update divisor, the hub-table build method, the
-top-level renumbering) change time only: every value gives
+top-level renumbering, the contraction's batch size) change time only: every value gives
 the same clustering, bit for bit, as the defaults.  The autotuner times each
 candidate and keeps the fastest, so it cannot change results either."""
 import numpy as np
@@ -13,7 +13,8 @@ from pennsylvaniagpugraphclustering_amd import graphs
 pytestmark = pytest.mark.gpu
 
 DEFAULTS = {"table_threads": 64, "table_slots_per_edge": 2, "hub_lds_max_degree": 2048, "hub_edges_per_item": 1024,
-            "hub_slots_per_item": 4096, "hub_update_div": 32, "hub_build": 0, "renumber": 2}
+            "hub_slots_per_item": 4096, "hub_update_div": 32, "hub_build": 0, "renumber": 2,
+            "contract_batch": 1 << 30}
 
 
 @pytest.fixture(scope="module")
@@ -54,7 +55,7 @@ def test_every_knob_value_gives_identical_results(gpu_ctx, graphs_under_test, al
             # hub tables built by edge items with HBM atomics, for every hub
             # and for every other hub (the rest from LDS slices)
             variants += [{"hub_build": 1}, {"hub_build": 2}, {"hub_build": 2, "hub_update_div": 0}]
-            variants += [{"renumber": 0}, {"renumber": 1}]
+            variants += [{"renumber": 0}, {"renumber": 1}, {"contract_batch": 4096}]
             for var in variants:
                 print(f"{name} {algo} knobs {var}", flush=True)
                 _set_all(gpu_ctx, DEFAULTS)

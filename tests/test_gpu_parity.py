@@ -315,3 +315,26 @@ def test_sbm_131072_planted_partition(gpu_ctx):
     assert q == rq
     for k in ["labels", "uncut", "sweeps", "moves", "outer"]:
         assert info[k] == rinfo[k], k
+
+
+@pytest.mark.parametrize("batch", [1, 4096, 1 << 16])
+@pytest.mark.parametrize("name,nc", [("rmat12w", 3), ("hub30k", 40), ("rmat12", 17)])
+def test_contraction_in_batches(gpu_ctx, name, nc, batch):
+    """The big coarse rows (bound > 2048) merged and sorted in batches of at
+    most `batch` table slots (a quarter as many sort entries; 1 puts every
+    row in a batch of its own): bit-identical to the oracle, as with the
+    default single batch."""
+    g, obj = CASES[name]
+    rng = np.random.default_rng(nc + batch)
+    vc = rng.integers(0, nc, g.n).astype(np.int32)
+    vc[:nc] = np.arange(nc)
+    print(f"contract {name} nc={nc} batch={batch}", flush=True)
+    gpu_ctx.set_knob("contract_batch", batch)
+    try:
+        rp, col, w = C().build_coarse_graph(dgraph(g, obj), vc, nc)
+    finally:
+        gpu_ctx.set_knob("contract_batch", 1 << 30)
+    rrp, rcol, rw = O.contract(g.rp, g.col, weights_for(g, obj), vc, nc)
+    assert np.array_equal(npy(rp), rrp)
+    assert np.array_equal(npy(col), rcol)
+    assert np.array_equal(npy(w), rw)
