@@ -142,6 +142,16 @@ static Level top_level(const DGraph& g) {
     l.plan.reset(new LevelPlan());
     return l;
 }
+// On the way back up the hierarchy a level is done once its clustering is
+// projected onto the finer one: its graph, plan and clustering are freed,
+// so the finer levels' local moves have that memory (the whole hierarchy of
+// a billion-edge graph does not fit next to the top level's hub tables).
+static void release_level(Level& l, Buf<i32>& part) {
+    l.own.reset();
+    l.plan.reset();
+    part.reset();
+}
+
 static Level owned_level(std::unique_ptr<OwnedGraph> og) {
     Level l;
     l.g = og->view();
@@ -249,6 +259,7 @@ static void louvain_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData&
         const DGraph g = levels[i].g;
         LevelPlan& lp = (i == 0 && top_plan) ? *top_plan : *levels[i].plan;
         project(c, g.n, parts[i + 1], parts[i]);
+        release_level(levels[i + 1], parts[i + 1]);  // done with the coarser level
         move_level(c, ms, lp, g, parts[i], rfd, false, false, nullptr, true, rs, i == 0 ? rn : nullptr);
     }
     copy_dev(c, part_out, parts[0].p, top.n * sizeof(i32));
@@ -338,6 +349,7 @@ static void leiden_part(Ctx& c, MoveScratch& ms, const DGraph& top, RefineData& 
         const DGraph g = levels[i].g;
         LevelPlan& lp = (i == 0 && top_plan) ? *top_plan : *levels[i].plan;
         project(c, g.n, parts[i + 1], parts[i]);
+        release_level(levels[i + 1], parts[i + 1]);  // done with the coarser level
         if (plus) move_level(c, ms, lp, g, parts[i], rfd, false, false, nullptr, true, rs, i == 0 ? rn : nullptr);
     }
     copy_dev(c, part_out, parts[0].p, top.n * sizeof(i32));
