@@ -395,6 +395,11 @@ def main():
             if gr:
                 extra["leiden_plus"]["matches_oracle"] = (sha(partl) == gr["sha_part"] and
                                                           float(ql).hex() == gr["objective_hex"])
+        # one more pLeiden+ run with the kernel timers and byte counters on:
+        # the refinement's group (leiden) and the others of this algorithm
+        _, lgroups, _ = instrumented_step(lambda: C.cluster(g, "leiden+", "Mod", 1.0, a.leiden, seed=0, ctx=ctx),
+                                          ctx, torch)
+        extra["leiden_plus"]["kernel_groups"] = lgroups
         print(f"# pLeiden+: {extra['leiden_plus']}", file=sys.stderr, flush=True)
 
     if not a.no_gnn:

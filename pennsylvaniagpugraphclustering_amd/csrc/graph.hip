@@ -125,25 +125,7 @@ void make_orders(Ctx& c, const DGraph& g, Orders& o) {
 }
 
 // ------------------------------------------------- edge-chunk row reductions
-// Each block owns a contiguous range of EC edges; each thread owns EPT
-// consecutive edges and walks their rows with a monotone row pointer.
-constexpr int EB = 256, EPT = 16, EC = EB * EPT;
-
-struct ChunkRows {
-    i64 eb, ee, rlo, rhi;
-};
-__device__ __forceinline__ void chunk_rows(const i64* rp, i64 n, i64 nnz, ChunkRows& cr) {
-    __shared__ i64 s_lo, s_hi;
-    cr.eb = (i64)blockIdx.x * EC;
-    cr.ee = cr.eb + EC < nnz ? cr.eb + EC : nnz;
-    if (threadIdx.x == 0) {
-        s_lo = upper_row(rp, 0, n, cr.eb);
-        s_hi = upper_row(rp, 0, n, cr.ee - 1) + 1;
-    }
-    __syncthreads();
-    cr.rlo = s_lo;
-    cr.rhi = s_hi;
-}
+// (ChunkRows / chunk_rows in device.h)
 
 template <class WT>
 __global__ void __launch_bounds__(EB) k_eval_uncut(const i64* rp, const i32* col, WT W, const i32* lab, i64 n,

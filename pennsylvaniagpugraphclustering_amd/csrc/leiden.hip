@@ -19,10 +19,13 @@ __global__ void k_well(const i64* vw, const i32* louv, const i64* vol, const i64
 
 // :296-393 (unit weights): first satisfactory neighbour cyclically from a
 // hashed start; 1/1000 of the vertices keep themselves.
-__global__ void k_hn_uniform(const i64* rp, const i32* col, const i64* vw, const i32* louv, const i32* well, i64 n,
-                             float gamma, i32 seed, i32* hn) {
-    const i64 i = thread_item();
-    if (i >= n) return;
+// list: the vertices of degree < 128 (order1 prefix); larger degrees take
+// k_hn_uniform_wave
+__global__ void k_hn_uniform(const i64* rp, const i32* col, const i64* vw, const i32* louv, const i32* well,
+                             const i32* list, i64 m, float gamma, i32 seed, i32* hn) {
+    const i64 x = thread_item();
+    if (x >= m) return;
+    const i32 i = list[x];
     const i64 s = rp[i], e = rp[i + 1], width = e - s;
     const float multi = gamma * (float)vw[i];
     const u32 h = murmur3_i32((i32)((u32)seed + (u32)i));
@@ -46,6 +49,49 @@ __global__ void k_hn_uniform(const i64* rp, const i32* col, const i64* vw, const
         if (1.0f >= multi * (float)vw[v]) { pick = v; found = true; }
     }
     hn[i] = pick;
+}
+
+// the same choice with one wave per vertex: the lanes test 64 consecutive
+// positions of the cyclic scan from the hashed start at a time, and the
+// first passing position (lowest lane of the first chunk with one) is the
+// serial scan's pick
+__global__ void __launch_bounds__(256) k_hn_uniform_wave(const i64* rp, const i32* col, const i64* vw,
+                                                         const i32* louv, const i32* well, const i32* list, i64 m,
+                                                         float gamma, i32 seed, i32* hn) {
+    const int lane = threadIdx.x & 63;
+    const i64 x = block_id() * 4 + (threadIdx.x >> 6);
+    if (x >= m) return;
+    const i32 i = list[x];
+    const i64 s = rp[i], e = rp[i + 1], width = e - s;
+    const float multi = gamma * (float)vw[i];
+    const u32 h = murmur3_i32((i32)((u32)seed + (u32)i));
+    const i32 jx32 = (i32)h;
+    if (jx32 % 1000 == 0 || well[i] == 0 || width == 0) {
+        if (lane == 0) hn[i] = (i32)i;
+        return;
+    }
+    const i64 off = (i64)(h % (u32)width);
+    const i32 ci = louv[i];
+    for (i64 base = 0; base < width; base += 64) {  // wave-uniform loop control
+        const i64 t = base + lane;
+        i32 ok = 0;
+        i32 v = -1;
+        if (t < width) {
+            const i64 q = off + t;
+            v = col[s + (q < width ? q : q - width)];
+            ok = ci == louv[v] && well[v] != 0 && 1.0f >= multi * (float)vw[v];
+        }
+        xlane_fence(ok);  // the values below cross lanes
+        xlane_fence(v);
+        const u64 bal = __ballot(ok != 0);
+        if (bal) {
+            const int first = __ffsll((unsigned long long)bal) - 1;
+            const i32 pick = __shfl(v, first);
+            if (lane == 0) hn[i] = pick;
+            return;
+        }
+    }
+    if (lane == 0) hn[i] = (i32)i;
 }
 
 // :395-474 (weighted): argmax over neighbours of rand * (w - gamma w_i w_v)^4
@@ -184,23 +230,34 @@ __global__ void k_gstarts(i64 n, const i32* ids, const i32* vcmap, i64* cbeg, co
     wsorted[x] = vw[i];
 }
 
+// inner[i] = weight of i's edges to earlier members (by order) of its
+// refined cluster; edge chunks (most rows are short), per-row partial sums
+// added with atomics into the zeroed array (exact integer sums)
 template <class WT>
-__global__ void __launch_bounds__(256) k_inner(const i64* rp, const i32* col, WT W, const i32* vcmap,
-                                               const i32* order, i64 n, i64* inner) {
-    const int lane = threadIdx.x & 63;
-    const i64 i = block_id() * 4 + (threadIdx.x >> 6);
-    if (i >= n) return;
-    const i32 ci = vcmap[i], oi = order[i];
+__global__ void __launch_bounds__(EB) k_inner(const i64* rp, const i32* col, WT W, const i32* vcmap,
+                                              const i32* order, i64 n, i64 nnz, i64* inner) {
+    ChunkRows cr;
+    chunk_rows(rp, n, nnz, cr);
+    const i64 j0 = cr.eb + (i64)threadIdx.x * EPT;
+    if (j0 >= cr.ee) return;
+    i64 r = upper_row(rp, cr.rlo, cr.rhi, j0);
+    i64 nxt = rp[r + 1];
+    i32 ci = vcmap[r], oi = order[r];
     i64 acc = 0;
-    const i64 e = rp[i + 1];
-    for (i64 base = rp[i]; base < e; base += 64) {
-        const i64 j = base + lane;
-        if (j >= e) continue;
-        i32 v = col[j];
+    for (int k = 0; k < EPT; k++) {
+        const i64 j = j0 + k;
+        if (j >= cr.ee) break;
+        if (j >= nxt) {
+            if (acc) atomicAdd((unsigned long long*)&inner[r], (unsigned long long)acc);
+            acc = 0;
+            r = advance_row(rp, r, cr.rhi, j, nxt);
+            ci = vcmap[r];
+            oi = order[r];
+        }
+        const i32 v = col[j];
         if (vcmap[v] == ci && order[v] < oi) acc += W(j);
     }
-    acc = wave_sum_i64(acc);
-    if (lane == 0) inner[i] = acc;
+    if (acc) atomicAdd((unsigned long long*)&inner[r], (unsigned long long)acc);
 }
 
 __global__ void k_outer_vals(i64 n, const i32* ids, const i64* pv, const i64* inner, i64* ov) {
@@ -269,7 +326,14 @@ i64 leiden_refine(Ctx& c, const DGraph& g, const Orders& ord, const i32* louv, c
     PGGC_LAUNCH_CHECK();
     if (uniform) {
         const i32 seed = (i32)c.next_draw();
-        k_hn_uniform<<<G, 256, 0, c.stream>>>(g.rp, g.col, g.vw, louv, well, n, gamma, seed, hn);
+        const i64 big = ord.off_large;  // order1: degree < 128 first
+        if (big > 0)
+            k_hn_uniform<<<grid_items(big, 256), 256, 0, c.stream>>>(g.rp, g.col, g.vw, louv, well, ord.o1, big,
+                                                                     gamma, seed, hn);
+        if (n > big)
+            k_hn_uniform_wave<<<grid_items(n - big, 4), 256, 0, c.stream>>>(g.rp, g.col, g.vw, louv, well,
+                                                                            ord.o1.p + big, n - big, gamma, seed,
+                                                                            hn);
         PGGC_LAUNCH_CHECK();
     } else {
         const i32 seed = (i32)c.next_draw();
@@ -305,9 +369,12 @@ i64 leiden_refine(Ctx& c, const DGraph& g, const Orders& ord, const i32* louv, c
     k_gstarts<<<G, 256, 0, c.stream>>>(n, ids, vcmap, cbeg, g.vw, wsorted);
     PGGC_LAUNCH_CHECK();
     scan_i64(c, wsorted, tsz, n);
-    PGGC_WDISPATCH(g, (k_inner<<<grid_items(n, 4), 256, 0, c.stream>>>(g.rp, g.col, W, vcmap, order, n,
-                                                                               inner)));
-    PGGC_LAUNCH_CHECK();
+    fill_i64(c, inner, n, 0);
+    if (g.nnz > 0) {
+        PGGC_WDISPATCH(g, (k_inner<<<(int)ceil_div(g.nnz, EC), EB, 0, c.stream>>>(g.rp, g.col, W, vcmap, order, n,
+                                                                                g.nnz, inner)));
+        PGGC_LAUNCH_CHECK();
+    }
     k_outer_vals<<<G, 256, 0, c.stream>>>(n, ids, pv, inner, ov);
     PGGC_LAUNCH_CHECK();
     scan_i64(c, ov, osc, n);
