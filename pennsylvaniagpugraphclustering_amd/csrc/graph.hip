@@ -619,8 +619,8 @@ __global__ void k_big_count(const i32* list, i64 m, const i64* crp, i64* cnt) {
 }
 // key = (big row index in the batch << cb) | column (columns < 2^cb), so the
 // radix sort needs only cb + bits(rows in the batch) bits
-__global__ void k_big_flatten(const i32* list, i64 m, const i64* crp, const i64* boff, const i32* ocol, int cb,
-                              u64* key, i64* val, i64* target) {
+__global__ void k_big_flatten(const i32* list, i64 m, const i64* crp, const i64* boff, const i32* ocol,
+                              const i64* ow, int cb, u64* key, i64* val, i64* target) {
     const i64 x = block_id();
     if (x >= m) return;
     {
@@ -629,23 +629,20 @@ __global__ void k_big_flatten(const i32* list, i64 m, const i64* crp, const i64*
         for (i64 j = s + threadIdx.x; j < e; j += blockDim.x) {
             i64 f = b + (j - s);
             key[f] = ((u64)x << cb) | (u32)ocol[j];
-            val[f] = f;  // 64-bit: big rows can hold 2^31 or more entries
+            val[f] = ow[j];  // the weight travels with its key through the sort
             target[f] = j;
         }
     }
 }
-__global__ void k_big_gather(const u64* key, const i64* val, const i64* target, i64 E, const i64* ow_in,
-                             i64* wtmp) {
-    for (i64 f = (i64)blockIdx.x * blockDim.x + threadIdx.x; f < E; f += (i64)gridDim.x * blockDim.x)
-        wtmp[f] = ow_in[target[val[f]]];
-}
-__global__ void k_big_scatter(const u64* key, const i64* target, i64 E, const i64* wtmp, int cb, i32* ocol,
+// the f-th smallest key of the batch goes to the f-th position of its rows
+// in row order (rows keep their order and their lengths through the sort)
+__global__ void k_big_scatter(const u64* key, const i64* target, i64 E, const i64* wsorted, int cb, i32* ocol,
                               i64* ow) {
     const u64 mask = ((u64)1 << cb) - 1;
     for (i64 f = (i64)blockIdx.x * blockDim.x + threadIdx.x; f < E; f += (i64)gridDim.x * blockDim.x) {
         i64 j = target[f];
         ocol[j] = (i32)(key[f] & mask);
-        ow[j] = wtmp[f];
+        ow[j] = wsorted[f];
     }
 }
 
@@ -872,18 +869,16 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
         }
         if (emax > 0) {
             Buf<u64> key(c, emax);
-            Buf<i64> val(c, emax), target(c, emax), wtmp(c, emax);
+            Buf<i64> val(c, emax), target(c, emax);
             const int cb = bits_for(nc - 1);
             for (size_t q = 0; q + 1 < cuts.size(); q++) {
                 const i64 r0 = cuts[q], r1 = cuts[q + 1], E = hb[r1] - hb[r0];
                 if (E == 0) continue;
                 k_big_flatten<<<grid_blocks(r1 - r0), 256, 0, c.stream>>>(big + r0, r1 - r0, out.rp, boff.p + r0,
-                                                                          out.col, cb, key, val, target);
+                                                                          out.col, out.w, cb, key, val, target);
                 PGGC_LAUNCH_CHECK();
                 radix_sort_pairs(c, key, val, E, cb + bits_for(r1 - r0));
-                k_big_gather<<<grid_for(E, 256 * 4, 8192), 256, 0, c.stream>>>(key, val, target, E, out.w, wtmp);
-                PGGC_LAUNCH_CHECK();
-                k_big_scatter<<<grid_for(E, 256 * 4, 8192), 256, 0, c.stream>>>(key, target, E, wtmp, cb, out.col,
+                k_big_scatter<<<grid_for(E, 256 * 4, 8192), 256, 0, c.stream>>>(key, target, E, val, cb, out.col,
                                                                                 out.w);
                 PGGC_LAUNCH_CHECK();
             }
