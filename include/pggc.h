@@ -90,7 +90,9 @@ int pggc_ctx_destroy(pggc_ctx* ctx);
 int pggc_ctx_set_seed(pggc_ctx* ctx, uint64_t seed, uint64_t draws);
 int pggc_ctx_get_draws(pggc_ctx* ctx, uint64_t* draws);
 /* level > 0: pggc_cluster prints one line per outer iteration to stderr, as
- * the reference CLI does (app/cluster_main.cpp:137,156) */
+ * the reference CLI does (app/cluster_main.cpp:137,156); level >= 2 also
+ * prints the device memory the context holds at each stage of a clustering
+ * call (local moves, refinement, contraction) */
 int pggc_ctx_set_verbose(pggc_ctx* ctx, int level);
 /* Kernel timing (measurement hook, no reference counterpart): when on, HIP
  * events are recorded on the context stream around the launches of each slot
