@@ -44,3 +44,14 @@ def test_null_context_is_rejected():
     rc = lib.pggc_ctx_set_seed(None, 1, 0)
     assert rc == _lib.EINVAL
     assert b"null context" in lib.pggc_last_error()
+
+
+def test_knob_ids_match_the_header():
+    """The Python mirror's knob names map to the header's PGGC_KNOB_* ids,
+    and every id below PGGC_KNOB_COUNT has a name."""
+    from pennsylvaniagpugraphclustering_amd.clustering import Context
+    src = open(os.path.join(ROOT, "include", "pggc.h")).read()
+    ids = {m.group(1).lower(): int(m.group(2)) for m in re.finditer(r"#define PGGC_KNOB_(\w+) (\d+)", src)}
+    count = ids.pop("count")
+    assert sorted(ids.values()) == list(range(count))
+    assert Context.KNOBS == ids
