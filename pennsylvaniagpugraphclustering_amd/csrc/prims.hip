@@ -175,8 +175,8 @@ void fill_i64(Ctx& c, i64* p, i64 n, i64 v) {
     PGGC_LAUNCH_CHECK();
 }
 // device-to-device copy as a kernel on the context stream (16-byte words
-// when both ends are 16-byte aligned and the size is a multiple of 16, else
-// 4-byte words; every copy in the library is a multiple of 4 bytes)
+// when both ends are 16-byte aligned, else 4-byte words; every copy in the
+// library is a multiple of 4 bytes)
 __global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, i64 n) {
     const i64 i = thread_item();
     if (i < n) dst[i] = src[i];
@@ -189,13 +189,22 @@ void copy_dev(Ctx& c, void* dst, const void* src, size_t bytes) {
     if (bytes == 0 || dst == src) return;
     if (bytes % 4 != 0 || ((uintptr_t)dst | (uintptr_t)src) % 4 != 0)
         throw Error(-5, "copy_dev: size and addresses must be multiples of 4 bytes");
-    if (bytes % 16 == 0 && ((uintptr_t)dst | (uintptr_t)src) % 16 == 0) {
-        const i64 n = (i64)(bytes / 16);
-        k_copy16<<<grid_items(n, 256), 256, 0, c.stream>>>((const uint4*)src, (uint4*)dst, n);
-    } else {
-        const i64 n = (i64)(bytes / 4);
-        k_copy4<<<grid_items(n, 256), 256, 0, c.stream>>>((const u32*)src, (u32*)dst, n);
+    if (((uintptr_t)dst | (uintptr_t)src) % 16 == 0) {
+        // 16-byte words for the bulk, 4-byte words for a tail of up to 12 bytes
+        const i64 n16 = (i64)(bytes / 16);
+        if (n16 > 0) {
+            k_copy16<<<grid_items(n16, 256), 256, 0, c.stream>>>((const uint4*)src, (uint4*)dst, n16);
+            PGGC_LAUNCH_CHECK();
+        }
+        const i64 n4 = (i64)(bytes % 16) / 4;
+        if (n4 > 0) {
+            k_copy4<<<1, 64, 0, c.stream>>>((const u32*)src + 4 * n16, (u32*)dst + 4 * n16, n4);
+            PGGC_LAUNCH_CHECK();
+        }
+        return;
     }
+    const i64 n = (i64)(bytes / 4);
+    k_copy4<<<grid_items(n, 256), 256, 0, c.stream>>>((const u32*)src, (u32*)dst, n);
     PGGC_LAUNCH_CHECK();
 }
 
