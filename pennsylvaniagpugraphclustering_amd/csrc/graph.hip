@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "batching.h"
 #include "device.h"
 #include "graph.h"
 #include "hash.h"
@@ -788,15 +789,8 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
         download_i64(c, hg, goff, n2 + 1);
         download_i64(c, hi, items, n2 + 1);
         download_i64(c, hi2, items2, n2 + 1);
-        std::vector<i64> cuts{0};
         i64 gmax = 0;
-        while (cuts.back() < n2) {
-            const i64 r0 = cuts.back();
-            i64 r1 = r0 + 1;
-            while (r1 < n2 && hg[r1 + 1] - hg[r0] <= slot_cap) r1++;
-            gmax = std::max(gmax, hg[r1] - hg[r0]);
-            cuts.push_back(r1);
-        }
+        const std::vector<i64> cuts = batch_cuts(hg, slot_cap, &gmax);
         Buf<i32> gkeys(c, packed ? 1 : gmax);
         Buf<i64> gvals(c, packed ? 1 : gmax);
         Buf<u64> gtab(c, packed ? gmax : 1);
@@ -858,15 +852,8 @@ void contract(Ctx& c, const DGraph& g, const i32* vcmap, i64 nc, OwnedGraph& out
         scan_i64(c, boff, boff, n2);
         std::vector<i64> hb;
         download_i64(c, hb, boff, n2 + 1);
-        std::vector<i64> cuts{0};
         i64 emax = 0;
-        while (cuts.back() < n2) {
-            const i64 r0 = cuts.back();
-            i64 r1 = r0 + 1;
-            while (r1 < n2 && hb[r1 + 1] - hb[r0] <= sort_cap) r1++;
-            emax = std::max(emax, hb[r1] - hb[r0]);
-            cuts.push_back(r1);
-        }
+        const std::vector<i64> cuts = batch_cuts(hb, sort_cap, &emax);
         if (emax > 0) {
             Buf<u64> key(c, emax);
             Buf<i64> val(c, emax), target(c, emax);
